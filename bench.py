@@ -1,0 +1,139 @@
+"""Throughput benchmark of the MI355X diffuse-react engine.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+                    [--workload NAME] [--no-cpu-baseline]
+
+A step is one diffuse-react iteration over every molecule of the workload,
+with the molecules resident in HBM.  Prints one JSON line with the metric
+(molecule-timesteps/s), the roofline of the dominant kernel measured with HIP
+events on the engine's stream, and a CPU baseline: the oracle (a restatement
+of the reference's FP64 CPU path, ISAAC64 stream, one core) timed on a bounded
+sample of the same kind of work on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+# algorithmic bytes per volume molecule-timestep: SoA state read once and
+# written once (positions + species/flags word + subpartition word)
+B_ALG = {"fp64": 64.0, "fp32": 40.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--workload", default="free_diffusion_replicas")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--mols-per-replica", type=int, default=150000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def build_workload(args):
+    from mcell_amd import workloads as W
+    if args.workload == "free_diffusion_replicas":
+        wl = W.free_diffusion_replicas(args.replicas, args.mols_per_replica, iterations=args.steps)
+        cfg = {"workload": "free diffusion, %d replicas x %d A in a shared reflective 1 um cube (12 walls), "
+                           "0.1 um subpartitions, dt 1e-6 s" % (args.replicas, args.mols_per_replica),
+               "replicas": args.replicas, "molecules": wl.num_molecules, "walls": int(len(wl.walls))}
+    elif args.workload == "free_diffusion_box":
+        wl = W.free_diffusion_box(iterations=args.steps)
+        cfg = {"workload": "free diffusion, 10,000 A in a reflective 1 um cube, 0.1 um subpartitions",
+               "molecules": wl.num_molecules, "walls": int(len(wl.walls))}
+    else:
+        raise SystemExit("unknown workload %s" % args.workload)
+    return wl, cfg
+
+
+def cpu_baseline(seconds):
+    """The oracle in reference mode (ISAAC64 global stream, sequential FP64)
+    on the reference's free-diffusion case, for about `seconds` of CPU time."""
+    from mcell_amd import workloads as W
+    from oracle import pyoracle as po
+    wl = W.free_diffusion_box()
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_ISAAC)
+    done, secs = 0, 0.0
+    chunk = 50
+    while secs < seconds and done < 20000:
+        secs += w.run(chunk)
+        done += chunk
+    rate = wl.num_molecules * done / secs
+    return {"value": rate, "unit": "molecule-timesteps/s", "cores": 1, "kind": "port",
+            "sample": "oracle (FP64 restatement of the reference CPU path, ISAAC64 stream), "
+                      "free diffusion of 10,000 A in the reflective 1 um cube, %d iterations, %.1f s" % (done, secs)}
+
+
+def add_all_molecules(eng, wl):
+    """Uploads each replica's molecules (replicas are contiguous runs)."""
+    import numpy as np
+    bounds = np.searchsorted(wl.replica, np.arange(wl.num_replicas + 1))
+    for r in range(wl.num_replicas):
+        a, b = bounds[r], bounds[r + 1]
+        eng.add_volume_molecules(r, wl.ids[a:b], wl.mol_species[a:b], wl.positions[a:b])
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    from mcell_amd.engine import DiffuseReactEngine
+    wl, cfg = build_workload(args)
+    eng = DiffuseReactEngine(wl.part, precision=args.precision, seed=wl.config.seed,
+                             num_replicas=wl.num_replicas, capacity=wl.num_molecules)
+    eng.set_geometry(wl.vertices, wl.walls, wl.wall_class)
+    eng.set_species(wl.species_steps)
+    add_all_molecules(eng, wl)
+    n = wl.num_molecules
+    print("bench: %s, %d molecules, precision %s, warmup %d, steps %d" % (args.workload, n, args.precision,
+                                                                          args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+    for it in range(args.warmup):
+        eng.step(it, 1)
+    eng.sync()
+    t0 = time.perf_counter()
+    st = eng.step(args.warmup, args.steps)
+    eng.sync()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = n * args.steps / secs
+    avg_ms, launches = eng.last_kernel_time()
+    achieved = B_ALG[args.precision] * n / (avg_ms * 1e-3) / 1e9
+    eng.close()
+    out = {
+        "metric": "molecule-timesteps/s on 1 MI355X (% HBM peak); FP32+fallback vs FP64 speedup",
+        "value": value,
+        "unit": "molecule-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": cfg,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_diffuse_reflect64" if args.precision == "fp64" else "k_diffuse_reflect32",
+                     "kernel_avg_ms": avg_ms, "bytes_per_molecule_step": B_ALG[args.precision]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

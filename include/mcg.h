@@ -1,0 +1,156 @@
+/*
+ * mcg -- MI355X diffuse-react engine for MCell4: the C ABI.
+ *
+ * These entry points are what a GPU replacement of MCell4's DiffuseReactEvent
+ * (src4/diffuse_react_event.h:108-339) calls.  World, Partition, the species,
+ * reaction and geometry definitions and the libmcell Model API stay on the
+ * host; the event hands their flattened data to the engine once, keeps the
+ * molecules resident in HBM, and calls mcg_step once per iteration (or for a
+ * run of iterations between two scheduler barriers).
+ *
+ * All functions return MCG_OK (0) or an error status; mcg_last_error() gives
+ * the text of the last error on the calling thread.  Lengths are in the
+ * reference's internal units (length_unit = 1/sqrt(surface_grid_density) um,
+ * time_unit = time_step s), exactly as Partition and SimulationConfig hold them.
+ *
+ * Threading: a context is used by one host thread at a time.  The engine runs
+ * its kernels on its own HIP stream (default priority).
+ */
+#ifndef MCG_H_
+#define MCG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCG_ABI_VERSION 1
+
+typedef struct mcg_ctx mcg_ctx;
+
+enum {
+  MCG_OK = 0,
+  MCG_E_INVALID = 1,     /* bad argument */
+  MCG_E_HIP = 2,         /* HIP runtime error */
+  MCG_E_CAPACITY = 3,    /* molecule capacity exceeded */
+  MCG_E_ESCAPED = 4,     /* a molecule left the partition (reference: fatal error) */
+  MCG_E_UNSUPPORTED = 5, /* feature not on this path */
+  MCG_E_NOMEM = 6,       /* device allocation failed */
+  MCG_E_LIMIT = 7,       /* a per-step safety limit was hit (reflections, re-draws, walk) */
+  MCG_E_STATE = 8        /* call out of order (e.g. step before geometry) */
+};
+
+enum {
+  MCG_PREC_FP64 = 0,          /* plain FP64: positions, ray-triangle test, reflection */
+  MCG_PREC_FP32_FALLBACK = 1  /* FP32 positions and ray test, FP64 near edges / planes */
+};
+
+/* SimulationConfig + Partition geometry (src4/simulation_config.h:28-135,
+ * src4/partition.h:1112-1204), internal units. */
+typedef struct mcg_config {
+  double time_unit;              /* SimulationConfig::time_unit [s] */
+  double length_unit;            /* SimulationConfig::length_unit [um] */
+  double origin[3];              /* Partition::origin_corner */
+  double partition_edge_length;  /* SimulationConfig::partition_edge_length */
+  double subpart_edge_length;    /* SimulationConfig::subpart_edge_length */
+  double subpart_edge_length_rcp;
+  uint32_t num_subparts_per_edge;
+  int32_t use_expanded_list;     /* SimulationConfig::use_expanded_list */
+  double rxn_radius_3d;          /* SimulationConfig::rxn_radius_3d */
+  int32_t precision;             /* MCG_PREC_* */
+  uint32_t num_replicas;         /* independent instances sharing the mesh (>= 1) */
+  uint64_t seed;                 /* key of the per-molecule counter-based streams */
+  uint64_t molecule_capacity;    /* volume molecules over all replicas (incl. products) */
+  uint32_t flags;                /* MCG_FLAG_* */
+  uint32_t reserved;
+} mcg_config;
+
+#define MCG_FLAG_TRACE_HITS 1u /* record the wall indices each molecule hits (parity tests) */
+
+/* BNG::Species fields on this path (space_step/time_step from
+ * Species::update_space_and_time_step, MCell3 src/mcell_species.c:237-300). */
+typedef struct mcg_species {
+  double space_step;   /* internal length units */
+  double time_step;    /* internal time units */
+  uint32_t flags;      /* MCG_SPF_* */
+  uint32_t reserved;
+} mcg_species;
+
+#define MCG_SPF_CAN_VOLVOL 1u
+#define MCG_SPF_CAN_VOLWALL 2u
+#define MCG_SPF_CAN_VOLSURF 4u
+#define MCG_SPF_TARGET_ONLY 8u
+
+/* Per-step counters mirroring SimulationStats (src4/simulation_stats.h). */
+typedef struct mcg_step_stats {
+  uint64_t molecule_steps;       /* volume molecules diffused, summed over iterations */
+  uint64_t wall_hits;            /* reflections */
+  uint64_t reactions;
+  double kernel_ms;              /* device time of the diffuse kernels (HIP events) */
+  double wall_ms;                /* host wall time of the call */
+  uint32_t kernel_launches;
+  uint32_t reserved;
+} mcg_step_stats;
+
+const char* mcg_last_error(void);
+int mcg_abi_version(void);
+
+/* Replaces `new DiffuseReactEvent(world)` at src4/world.cpp:280-282: creates
+ * an engine on the current HIP device for the partition described by cfg. */
+int mcg_create(const mcg_config* cfg, mcg_ctx** out);
+int mcg_destroy(mcg_ctx* ctx);
+
+/* Partition geometry: vertices (Partition::get_geometry_vertex, internal
+ * units, xyz interleaved) and walls (vertex index triples, the wall index is
+ * the position in the array).  The engine recomputes the wall constants
+ * (Wall::initialize_wall_constants, src4/wall.cpp:281-342) and the per
+ * subpartition wall lists (Partition::finalize_walls, src4/partition.cpp:91-118).
+ * wall_class may be NULL (all walls reflective). */
+int mcg_set_geometry(mcg_ctx* ctx, const double* vertices, uint32_t num_vertices, const uint32_t* walls,
+                     uint32_t num_walls, const uint32_t* wall_class);
+
+/* Species table, indexed by species id (< 256). */
+int mcg_set_species(mcg_ctx* ctx, const mcg_species* species, uint32_t num_species);
+
+/* Volume molecules of one replica (Partition::add_volume_molecule semantics:
+ * each position must lie in the partition).  ids are the molecule ids that
+ * key their random streams. */
+int mcg_add_volume_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids, const uint32_t* species,
+                             const double* positions, uint64_t count);
+
+/* Runs num_iterations diffuse-react iterations starting at first_iteration
+ * (DiffuseReactEvent::step with event_time = iteration and a barrier at the
+ * end of every iteration).  stats may be NULL. */
+int mcg_step(mcg_ctx* ctx, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats);
+
+/* Waits for all queued work. */
+int mcg_sync(mcg_ctx* ctx);
+
+int mcg_num_molecules(mcg_ctx* ctx, uint64_t* count);
+
+/* Copies the live volume molecules back (for Partition at a barrier).  Any
+ * output pointer may be NULL.  Molecules come back in engine order. */
+int mcg_get_volume_molecules(mcg_ctx* ctx, uint32_t* ids, uint32_t* species, uint32_t* replica, double* positions,
+                             uint32_t* subparts, uint64_t capacity, uint64_t* count);
+
+/* Parity/inspection: wall constants (16 doubles per wall: normal, unit_u,
+ * unit_v, distance_to_origin, uv_vert1_u, uv_vert2.u, uv_vert2.v, area, 0, 0)
+ * and the per-subpartition wall lists as CSR (offsets has n^3 + 1 entries).
+ * Any output pointer may be NULL; *num_entries gets the CSR length. */
+int mcg_get_walls(mcg_ctx* ctx, double* wall_constants, uint32_t* offsets, uint32_t* indices, uint64_t* num_entries);
+
+/* Parity: wall hits recorded in the last iteration when MCG_FLAG_TRACE_HITS
+ * is set: for molecule k (engine order), counts[k] hits, walls[k*8 .. k*8+7]. */
+int mcg_get_hit_trace(mcg_ctx* ctx, uint32_t* walls, uint32_t* counts, uint64_t capacity);
+
+/* Average device time of the dominant (diffuse) kernel over the last
+ * mcg_step call, measured with HIP events on the engine's stream. */
+int mcg_last_kernel_time(mcg_ctx* ctx, double* avg_ms, uint32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCG_H_ */

@@ -1,0 +1,118 @@
+"""ctypes binding of the engine's C ABI (include/mcg.h).
+
+The library is loaded from mcell_amd/lib/libmcg.so (built in place by
+``__graft_entry__.build()`` / ``make -C mcell_amd/csrc``).  There is no
+fallback: if the library is missing the import fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmcg.so")
+
+MCG_OK = 0
+MCG_PREC_FP64 = 0
+MCG_PREC_FP32_FALLBACK = 1
+MCG_FLAG_TRACE_HITS = 1
+
+MCG_SPF_CAN_VOLVOL = 1
+MCG_SPF_CAN_VOLWALL = 2
+MCG_SPF_CAN_VOLSURF = 4
+MCG_SPF_TARGET_ONLY = 8
+
+TRACE_HITS = 8
+
+# every symbol declared in include/mcg.h
+EXPORTED = [
+    "mcg_last_error", "mcg_abi_version", "mcg_create", "mcg_destroy", "mcg_set_geometry", "mcg_set_species",
+    "mcg_add_volume_molecules", "mcg_step", "mcg_sync", "mcg_num_molecules", "mcg_get_volume_molecules",
+    "mcg_get_walls", "mcg_get_hit_trace", "mcg_last_kernel_time",
+]
+
+
+class McgConfig(ctypes.Structure):
+    _fields_ = [
+        ("time_unit", ctypes.c_double),
+        ("length_unit", ctypes.c_double),
+        ("origin", ctypes.c_double * 3),
+        ("partition_edge_length", ctypes.c_double),
+        ("subpart_edge_length", ctypes.c_double),
+        ("subpart_edge_length_rcp", ctypes.c_double),
+        ("num_subparts_per_edge", ctypes.c_uint32),
+        ("use_expanded_list", ctypes.c_int32),
+        ("rxn_radius_3d", ctypes.c_double),
+        ("precision", ctypes.c_int32),
+        ("num_replicas", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("molecule_capacity", ctypes.c_uint64),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class McgSpecies(ctypes.Structure):
+    _fields_ = [
+        ("space_step", ctypes.c_double),
+        ("time_step", ctypes.c_double),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class McgStepStats(ctypes.Structure):
+    _fields_ = [
+        ("molecule_steps", ctypes.c_uint64),
+        ("wall_hits", ctypes.c_uint64),
+        ("reactions", ctypes.c_uint64),
+        ("kernel_ms", ctypes.c_double),
+        ("wall_ms", ctypes.c_double),
+        ("kernel_launches", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class McgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("mcg status %d: %s" % (status, msg))
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Loads libmcg.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libmcg.so not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                          "or `make -C mcell_amd/csrc` (%s missing)" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    lib.mcg_last_error.restype = ctypes.c_char_p
+    lib.mcg_abi_version.restype = ctypes.c_int
+    lib.mcg_create.argtypes = [ctypes.POINTER(McgConfig), ctypes.POINTER(P)]
+    lib.mcg_destroy.argtypes = [P]
+    lib.mcg_set_geometry.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_uint32, P]
+    lib.mcg_set_species.argtypes = [P, ctypes.POINTER(McgSpecies), ctypes.c_uint32]
+    lib.mcg_add_volume_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, ctypes.c_uint64]
+    lib.mcg_step.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(McgStepStats)]
+    lib.mcg_sync.argtypes = [P]
+    lib.mcg_num_molecules.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
+    lib.mcg_get_volume_molecules.argtypes = [P, P, P, P, P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    lib.mcg_get_walls.argtypes = [P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]
+    lib.mcg_get_hit_trace.argtypes = [P, P, P, ctypes.c_uint64]
+    lib.mcg_last_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32)]
+    for name in EXPORTED:
+        if not name.startswith("mcg_last_error") and name != "mcg_abi_version":
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != MCG_OK:
+        raise McgError(status, load().mcg_last_error().decode())

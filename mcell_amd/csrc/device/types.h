@@ -1,0 +1,87 @@
+// Device-side data layout of the diffuse-react engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcg {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+// Per-wall constants in FP64 (Wall::initialize_wall_constants,
+// src4/wall.cpp:281-342) plus the three vertices, 24 doubles = 192 B.
+struct WallRec64 {
+  double nx, ny, nz, d;        // normal, distance_to_origin
+  double ux, uy, uz, uv1u;     // unit_u, uv_vert1_u
+  double vx, vy, vz, uv2u;     // unit_v, uv_vert2.u
+  double p0x, p0y, p0z, uv2v;  // vertex 0, uv_vert2.v
+  double p1x, p1y, p1z, area;  // vertex 1
+  double p2x, p2y, p2z, pad;   // vertex 2
+};
+
+// The same in FP32 for the FP32 ray-triangle test, 16 floats = 64 B (one
+// cache line).  Vertex 0 is stored relative to nothing: positions stay in
+// partition units, which are bounded by the partition edge (<= 3e4 units).
+struct WallRec32 {
+  float nx, ny, nz, d;
+  float ux, uy, uz, uv1u;
+  float vx, vy, vz, uv2u;
+  float p0x, p0y, p0z, uv2v;
+};
+
+// SimulationConfig / Partition geometry in internal units.
+struct PartitionParams {
+  double ox, oy, oz;     // origin_corner
+  double qx, qy, qz;     // opposite_corner
+  double edge;           // partition_edge_length
+  double sp_len;         // subpart_edge_length
+  double sp_rcp;         // subpart_edge_length_rcp
+  double rxn_radius;     // rxn_radius_3d
+  uint32_t n;            // num_subparts_per_partition_edge
+  uint32_t n2;
+  uint32_t use_expanded_list;
+  uint32_t pad;
+};
+
+struct SpeciesRec {
+  double space_step;
+  double time_step;
+  uint32_t flags;
+  uint32_t pad;
+};
+
+enum : uint32_t {
+  SPF_CAN_VOLVOL = 1u << 0,
+  SPF_CAN_VOLWALL = 1u << 1,
+  SPF_CAN_VOLSURF = 1u << 2,
+  SPF_TARGET_ONLY = 1u << 3,
+};
+
+// molecule tag word: species (8 bits) | flags (8 bits) | replica (16 bits)
+__device__ __host__ __forceinline__ uint32_t tag_species(uint32_t t) { return t & 0xFFu; }
+__device__ __host__ __forceinline__ uint32_t tag_flags(uint32_t t) { return (t >> 8) & 0xFFu; }
+__device__ __host__ __forceinline__ uint32_t tag_replica(uint32_t t) { return t >> 16; }
+__device__ __host__ __forceinline__ uint32_t make_tag(uint32_t s, uint32_t f, uint32_t r) {
+  return (s & 0xFFu) | ((f & 0xFFu) << 8) | (r << 16);
+}
+enum : uint32_t { MF_DEFUNCT = 1u << 0 };
+
+// error bits reported through a device word
+enum : uint32_t {
+  ERR_ESCAPED = 1u << 0,     // a molecule left the partition (diffuse_react_event.cpp:592-612)
+  ERR_SEGMENTS = 1u << 1,    // more than kMaxSegments reflections in one step
+  ERR_REDO = 1u << 2,        // more than kMaxRedo edge re-draws for one wall search
+  ERR_DDA = 1u << 3,         // subpartition walk did not terminate
+  ERR_TRACE = 1u << 4,       // hit trace buffer overflow (trace only)
+};
+
+constexpr int kMaxSegments = 1024;
+constexpr int kMaxRedo = 4096;
+constexpr int kTraceHits = 8;
+
+struct HitTrace {
+  uint32_t* wall;     // [n * kTraceHits]
+  uint32_t* count;    // [n]
+  uint32_t enabled;
+};
+
+}  // namespace mcg

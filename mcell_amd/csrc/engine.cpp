@@ -1,0 +1,346 @@
+// The engine context and the C ABI declared in include/mcg.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mcg.h"
+#include "geometry.h"
+#include "kernels.h"
+
+using namespace mcg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(MCG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct mcg_ctx {
+  mcg_config cfg{};
+  PartitionParams P{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // geometry
+  std::vector<WallRec64> walls_host;
+  std::vector<uint32_t> off_host, idx_host;
+  DevBuf<WallRec64> walls64;
+  DevBuf<WallRec32> walls32;
+  DevBuf<uint32_t> off, idx;
+  bool have_geometry = false;
+
+  // species
+  DevBuf<SpeciesRec> species;
+  uint32_t num_species = 0;
+
+  // molecules (structure of arrays)
+  uint64_t n = 0;
+  DevBuf<double> x, y, z;
+  DevBuf<uint32_t> id, tag;
+
+  // diagnostics
+  DevBuf<uint32_t> err;
+  DevBuf<uint32_t> trace_wall, trace_count;
+  double last_kernel_ms = 0;
+  uint32_t last_launches = 0;
+
+  ~mcg_ctx() {
+    walls64.release(); walls32.release(); off.release(); idx.release(); species.release();
+    x.release(); y.release(); z.release(); id.release(); tag.release(); err.release();
+    trace_wall.release(); trace_count.release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+const char* mcg_last_error(void) { return g_err.c_str(); }
+int mcg_abi_version(void) { return MCG_ABI_VERSION; }
+
+int mcg_create(const mcg_config* cfg, mcg_ctx** out) {
+  if (!cfg || !out) return fail(MCG_E_INVALID, "null argument");
+  *out = nullptr;
+  if (cfg->num_subparts_per_edge == 0 || cfg->num_subparts_per_edge > 1024)
+    return fail(MCG_E_INVALID, "num_subparts_per_edge out of range");
+  if (!(cfg->subpart_edge_length > 0) || !(cfg->partition_edge_length > 0))
+    return fail(MCG_E_INVALID, "non-positive partition or subpartition length");
+  if (cfg->precision != MCG_PREC_FP64 && cfg->precision != MCG_PREC_FP32_FALLBACK)
+    return fail(MCG_E_INVALID, "unknown precision");
+  if (cfg->num_replicas == 0 || cfg->num_replicas > 65536) return fail(MCG_E_INVALID, "num_replicas out of range");
+  if (cfg->molecule_capacity == 0 || cfg->molecule_capacity >= 0xFFFFFFFFull)
+    return fail(MCG_E_INVALID, "molecule_capacity out of range");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (ndev == 0) return fail(MCG_E_HIP, "no HIP device");
+  auto* c = new mcg_ctx();
+  c->cfg = *cfg;
+  PartitionParams& P = c->P;
+  P.ox = cfg->origin[0]; P.oy = cfg->origin[1]; P.oz = cfg->origin[2];
+  P.edge = cfg->partition_edge_length;
+  P.qx = P.ox + P.edge; P.qy = P.oy + P.edge; P.qz = P.oz + P.edge;  // partition.cpp:57
+  P.sp_len = cfg->subpart_edge_length;
+  P.sp_rcp = cfg->subpart_edge_length_rcp;
+  P.rxn_radius = cfg->rxn_radius_3d;
+  P.n = cfg->num_subparts_per_edge;
+  P.n2 = P.n * P.n;
+  P.use_expanded_list = cfg->use_expanded_list ? 1u : 0u;
+  auto cleanup_fail = [&](int code) {
+    delete c;
+    return code;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
+    return cleanup_fail(fail(MCG_E_HIP, std::string("stream/event creation: ") + hipGetErrorString(e)));
+  size_t cap = cfg->molecule_capacity;
+  if (c->x.alloc(cap) || c->y.alloc(cap) || c->z.alloc(cap) || c->id.alloc(cap) || c->tag.alloc(cap) ||
+      c->err.alloc(1))
+    return cleanup_fail(fail(MCG_E_NOMEM, "molecule buffers"));
+  if (cfg->flags & MCG_FLAG_TRACE_HITS) {
+    if (c->trace_wall.alloc(cap * kTraceHits) || c->trace_count.alloc(cap))
+      return cleanup_fail(fail(MCG_E_NOMEM, "trace buffers"));
+  }
+  if ((e = hipMemsetAsync(c->err.p, 0, sizeof(uint32_t), c->stream)) != hipSuccess)
+    return cleanup_fail(fail(MCG_E_HIP, hipGetErrorString(e)));
+  *out = c;
+  return MCG_OK;
+}
+
+int mcg_destroy(mcg_ctx* ctx) {
+  if (!ctx) return MCG_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return MCG_OK;
+}
+
+int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint32_t* walls, uint32_t nw,
+                     const uint32_t* wall_class) {
+  if (!c || (nv && !vertices) || (nw && !walls)) return fail(MCG_E_INVALID, "null argument");
+  for (uint32_t i = 0; i < nw; ++i) {
+    if (walls[3 * i] >= nv || walls[3 * i + 1] >= nv || walls[3 * i + 2] >= nv)
+      return fail(MCG_E_INVALID, "wall vertex index out of range");
+    if (wall_class && wall_class[i] != 0) return fail(MCG_E_UNSUPPORTED, "only reflective walls on this path");
+  }
+  const PartitionParams& P = c->P;
+  for (uint32_t i = 0; i < nv; ++i) {
+    const double* v = vertices + 3 * i;
+    if (!(v[0] >= P.ox && v[1] >= P.oy && v[2] >= P.oz && v[0] < P.qx && v[1] < P.qy && v[2] < P.qz))
+      return fail(MCG_E_INVALID, "vertex outside the partition");
+  }
+  c->walls_host.resize(nw);
+  for (uint32_t i = 0; i < nw; ++i)
+    c->walls_host[i] = wall_constants(vertices + 3 * walls[3 * i], vertices + 3 * walls[3 * i + 1],
+                                      vertices + 3 * walls[3 * i + 2]);
+  bin_walls(P, c->walls_host, c->off_host, c->idx_host);
+  std::vector<WallRec32> w32(nw);
+  for (uint32_t i = 0; i < nw; ++i) w32[i] = wall_to_fp32(c->walls_host[i]);
+  if (c->walls64.alloc(std::max<size_t>(nw, 1)) || c->walls32.alloc(std::max<size_t>(nw, 1)) ||
+      c->off.alloc(c->off_host.size()) || c->idx.alloc(std::max<size_t>(c->idx_host.size(), 1)))
+    return fail(MCG_E_NOMEM, "wall buffers");
+  if (nw) {
+    HIP_TRY(hipMemcpy(c->walls64.p, c->walls_host.data(), nw * sizeof(WallRec64), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->walls32.p, w32.data(), nw * sizeof(WallRec32), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemcpy(c->off.p, c->off_host.data(), c->off_host.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (!c->idx_host.empty())
+    HIP_TRY(hipMemcpy(c->idx.p, c->idx_host.data(), c->idx_host.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  c->have_geometry = true;
+  return MCG_OK;
+}
+
+int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
+  if (!c || !sp || ns == 0 || ns > 256) return fail(MCG_E_INVALID, "bad species table");
+  std::vector<SpeciesRec> h(ns);
+  for (uint32_t i = 0; i < ns; ++i) {
+    if (sp[i].flags & (MCG_SPF_CAN_VOLVOL | MCG_SPF_CAN_VOLWALL | MCG_SPF_CAN_VOLSURF))
+      return fail(MCG_E_UNSUPPORTED, "reactive species are not on this path yet");
+    if (!(sp[i].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
+    h[i].space_step = sp[i].space_step;
+    h[i].time_step = sp[i].time_step;
+    h[i].flags = sp[i].flags;
+  }
+  if (c->species.alloc(ns)) return fail(MCG_E_NOMEM, "species buffer");
+  HIP_TRY(hipMemcpy(c->species.p, h.data(), ns * sizeof(SpeciesRec), hipMemcpyHostToDevice));
+  c->num_species = ns;
+  return MCG_OK;
+}
+
+int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, const uint32_t* species,
+                             const double* pos, uint64_t count) {
+  if (!c || (count && (!ids || !species || !pos))) return fail(MCG_E_INVALID, "null argument");
+  if (replica >= c->cfg.num_replicas) return fail(MCG_E_INVALID, "replica out of range");
+  if (c->n + count > c->cfg.molecule_capacity) return fail(MCG_E_CAPACITY, "molecule capacity exceeded");
+  const PartitionParams& P = c->P;
+  std::vector<double> hx(count), hy(count), hz(count);
+  std::vector<uint32_t> ht(count);
+  for (uint64_t i = 0; i < count; ++i) {
+    const double* p = pos + 3 * i;
+    if (!(p[0] >= P.ox && p[1] >= P.oy && p[2] >= P.oz && p[0] < P.qx && p[1] < P.qy && p[2] < P.qz))
+      return fail(MCG_E_INVALID, "molecule outside the partition (partition.h:559-563)");
+    if (species[i] >= c->num_species) return fail(MCG_E_INVALID, "species id out of range");
+    hx[i] = p[0]; hy[i] = p[1]; hz[i] = p[2];
+    ht[i] = make_tag(species[i], 0, replica);
+  }
+  if (count) {
+    size_t o = c->n;
+    HIP_TRY(hipMemcpyAsync(c->x.p + o, hx.data(), count * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->y.p + o, hy.data(), count * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->z.p + o, hz.data(), count * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->id.p + o, ids, count * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tag.p + o, ht.data(), count * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  c->n += count;
+  return MCG_OK;
+}
+
+int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  if (!c->have_geometry || c->num_species == 0) return fail(MCG_E_STATE, "geometry and species must be set first");
+  auto t0 = std::chrono::steady_clock::now();
+  MolView64 m{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p};
+  WallView64 W{c->walls64.p, c->off.p, c->idx.p};
+  HitTrace tr{c->trace_wall.p, c->trace_count.p, (c->cfg.flags & MCG_FLAG_TRACE_HITS) ? 1u : 0u};
+  uint32_t n = (uint32_t)c->n;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t k = 0; k < num_iterations; ++k) {
+    HIP_TRY(launch_diffuse_reflect64(c->stream, m, n, c->species.p, c->P, W, c->cfg.seed, first_iteration + k,
+                                     c->err.p, tr));
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  uint32_t err = 0;
+  HIP_TRY(hipMemcpyAsync(&err, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_kernel_ms = num_iterations ? ms / num_iterations : 0;
+  c->last_launches = num_iterations;
+  if (stats) {
+    stats->molecule_steps = (uint64_t)n * num_iterations;
+    stats->wall_hits = 0;
+    stats->reactions = 0;
+    stats->kernel_ms = ms;
+    stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats->kernel_launches = num_iterations;
+  }
+  if (err) {
+    HIP_TRY(hipMemsetAsync(c->err.p, 0, 4, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err & ERR_ESCAPED) return fail(MCG_E_ESCAPED, "a molecule escaped the partition");
+    return fail(MCG_E_LIMIT, "per-step limit exceeded (error bits " + std::to_string(err) + ")");
+  }
+  return MCG_OK;
+}
+
+int mcg_sync(mcg_ctx* c) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCG_OK;
+}
+
+int mcg_num_molecules(mcg_ctx* c, uint64_t* count) {
+  if (!c || !count) return fail(MCG_E_INVALID, "null argument");
+  *count = c->n;
+  return MCG_OK;
+}
+
+int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, double* pos,
+                             uint32_t* subparts, uint64_t capacity, uint64_t* count) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  uint64_t n = c->n;
+  if (count) *count = n;
+  if (capacity < n) return fail(MCG_E_CAPACITY, "output capacity too small");
+  std::vector<double> hx(n), hy(n), hz(n);
+  std::vector<uint32_t> ht(n);
+  if (n) {
+    HIP_TRY(hipMemcpyAsync(hx.data(), c->x.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hy.data(), c->y.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hz.data(), c->z.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(ht.data(), c->tag.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (ids) HIP_TRY(hipMemcpyAsync(ids, c->id.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  const PartitionParams& P = c->P;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (species) species[i] = tag_species(ht[i]);
+    if (replica) replica[i] = tag_replica(ht[i]);
+    if (pos) { pos[3 * i] = hx[i]; pos[3 * i + 1] = hy[i]; pos[3 * i + 2] = hz[i]; }
+    if (subparts) {
+      int ix = (int)((hx[i] - P.ox) * P.sp_rcp), iy = (int)((hy[i] - P.oy) * P.sp_rcp),
+          iz = (int)((hz[i] - P.oz) * P.sp_rcp);
+      subparts[i] = (uint32_t)ix + (uint32_t)iy * P.n + (uint32_t)iz * P.n2;
+    }
+  }
+  return MCG_OK;
+}
+
+int mcg_get_walls(mcg_ctx* c, double* wc, uint32_t* offsets, uint32_t* indices, uint64_t* num_entries) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  if (wc)
+    for (size_t i = 0; i < c->walls_host.size(); ++i) {
+      const WallRec64& w = c->walls_host[i];
+      double* o = wc + 16 * i;
+      double v[16] = {w.nx, w.ny, w.nz, w.ux, w.uy, w.uz, w.vx, w.vy, w.vz, w.d, w.uv1u, w.uv2u, w.uv2v, w.area, 0, 0};
+      std::memcpy(o, v, sizeof(v));
+    }
+  if (offsets) std::memcpy(offsets, c->off_host.data(), c->off_host.size() * 4);
+  if (indices) std::memcpy(indices, c->idx_host.data(), c->idx_host.size() * 4);
+  if (num_entries) *num_entries = c->idx_host.size();
+  return MCG_OK;
+}
+
+int mcg_get_hit_trace(mcg_ctx* c, uint32_t* walls, uint32_t* counts, uint64_t capacity) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  if (!(c->cfg.flags & MCG_FLAG_TRACE_HITS)) return fail(MCG_E_STATE, "created without MCG_FLAG_TRACE_HITS");
+  if (capacity < c->n) return fail(MCG_E_CAPACITY, "output capacity too small");
+  if (c->n) {
+    if (walls) HIP_TRY(hipMemcpy(walls, c->trace_wall.p, c->n * kTraceHits * 4, hipMemcpyDeviceToHost));
+    if (counts) HIP_TRY(hipMemcpy(counts, c->trace_count.p, c->n * 4, hipMemcpyDeviceToHost));
+  }
+  return MCG_OK;
+}
+
+int mcg_last_kernel_time(mcg_ctx* c, double* avg_ms, uint32_t* launches) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  if (avg_ms) *avg_ms = c->last_kernel_ms;
+  if (launches) *launches = c->last_launches;
+  return MCG_OK;
+}
+
+}  // extern "C"

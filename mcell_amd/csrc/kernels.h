@@ -1,0 +1,32 @@
+// Launch wrappers of the diffuse-react kernels (host-visible declarations).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "device/types.h"
+
+namespace mcg {
+
+// Structure-of-arrays molecule state in HBM (FP64 positions).
+struct MolView64 {
+  double* x;
+  double* y;
+  double* z;
+  uint32_t* id;
+  uint32_t* tag;
+};
+
+struct WallView64 {
+  const WallRec64* walls;
+  const uint32_t* off;  // CSR offsets, n^3 + 1
+  const uint32_t* idx;  // wall indices, ascending within a subpartition
+};
+
+hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, const SpeciesRec* species,
+                                    const PartitionParams& P, const WallView64& W, uint64_t seed, uint32_t iteration,
+                                    uint32_t* err, const HitTrace& trace);
+
+hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
+
+}  // namespace mcg

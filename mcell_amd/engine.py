@@ -1,0 +1,139 @@
+"""Python side of the diffuse-react engine: a thin owner of an ``mcg_ctx``.
+
+This plays the role MCell4's DiffuseReactEvent plays for World: it is created
+once per partition (src4/world.cpp:280-282), receives the flattened
+geometry/species tables, and is stepped once per iteration
+(DiffuseReactEvent::step, src4/diffuse_react_event.cpp:53-64).  All compute
+runs in libmcg.so on the GPU; nothing here computes positions.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .model import PartitionConfig
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DiffuseReactEngine:
+    def __init__(self, part: PartitionConfig, precision: str = "fp64", seed: int = 1, num_replicas: int = 1,
+                 capacity: int = 1 << 20, trace: bool = False):
+        self.lib = _lib.load()
+        self.part = part
+        cfg = _lib.McgConfig()
+        cfg.time_unit = part.time_unit
+        cfg.length_unit = part.length_unit
+        for k in range(3):
+            cfg.origin[k] = part.origin[k]
+        cfg.partition_edge_length = part.partition_edge_length
+        cfg.subpart_edge_length = part.subpart_edge_length
+        cfg.subpart_edge_length_rcp = part.subpart_edge_length_rcp
+        cfg.num_subparts_per_edge = part.num_subparts_per_edge
+        cfg.use_expanded_list = part.use_expanded_list
+        cfg.rxn_radius_3d = part.rxn_radius_3d
+        cfg.precision = {"fp64": _lib.MCG_PREC_FP64, "fp32": _lib.MCG_PREC_FP32_FALLBACK}[precision]
+        cfg.num_replicas = num_replicas
+        cfg.seed = seed
+        cfg.molecule_capacity = capacity
+        cfg.flags = _lib.MCG_FLAG_TRACE_HITS if trace else 0
+        self.ctx = ctypes.c_void_p()
+        _lib.check(self.lib.mcg_create(ctypes.byref(cfg), ctypes.byref(self.ctx)))
+        self.trace = trace
+
+    def close(self):
+        if self.ctx:
+            self.lib.mcg_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------- set-up --
+    def set_geometry(self, vertices: np.ndarray, walls: np.ndarray, wall_class: Optional[np.ndarray] = None):
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        w = np.ascontiguousarray(walls, dtype=np.uint32).reshape(-1, 3)
+        c = None if wall_class is None else np.ascontiguousarray(wall_class, dtype=np.uint32)
+        _lib.check(self.lib.mcg_set_geometry(self.ctx, _ptr(v), len(v), _ptr(w), len(w), _ptr(c)))
+
+    def set_species(self, species: Sequence[tuple]):
+        """species: (space_step, time_step, flags) per species id."""
+        arr = (_lib.McgSpecies * len(species))()
+        for i, (ss, ts, fl) in enumerate(species):
+            arr[i].space_step = ss
+            arr[i].time_step = ts
+            arr[i].flags = fl
+        _lib.check(self.lib.mcg_set_species(self.ctx, arr, len(species)))
+
+    def add_volume_molecules(self, replica: int, ids: np.ndarray, species: np.ndarray, positions: np.ndarray):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        sp = np.ascontiguousarray(species, dtype=np.uint32)
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        assert len(ids) == len(sp) == len(pos)
+        _lib.check(self.lib.mcg_add_volume_molecules(self.ctx, replica, _ptr(ids), _ptr(sp), _ptr(pos), len(ids)))
+
+    # --------------------------------------------------------------- run --
+    def step(self, first_iteration: int, num_iterations: int = 1) -> _lib.McgStepStats:
+        st = _lib.McgStepStats()
+        _lib.check(self.lib.mcg_step(self.ctx, first_iteration, num_iterations, ctypes.byref(st)))
+        return st
+
+    def sync(self):
+        _lib.check(self.lib.mcg_sync(self.ctx))
+
+    def last_kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_uint32()
+        _lib.check(self.lib.mcg_last_kernel_time(self.ctx, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    # ----------------------------------------------------------- readback --
+    def num_molecules(self) -> int:
+        n = ctypes.c_uint64()
+        _lib.check(self.lib.mcg_num_molecules(self.ctx, ctypes.byref(n)))
+        return n.value
+
+    def molecules(self):
+        n = self.num_molecules()
+        ids = np.zeros(n, np.uint32)
+        sp = np.zeros(n, np.uint32)
+        rep = np.zeros(n, np.uint32)
+        pos = np.zeros((n, 3), np.float64)
+        sub = np.zeros(n, np.uint32)
+        cnt = ctypes.c_uint64()
+        _lib.check(self.lib.mcg_get_volume_molecules(self.ctx, _ptr(ids), _ptr(sp), _ptr(rep), _ptr(pos), _ptr(sub),
+                                                     n, ctypes.byref(cnt)))
+        return {"id": ids, "species": sp, "replica": rep, "pos": pos, "subpart": sub}
+
+    def walls(self):
+        cnt = ctypes.c_uint64()
+        _lib.check(self.lib.mcg_get_walls(self.ctx, None, None, None, ctypes.byref(cnt)))
+        nsub = self.part.num_subparts_per_edge ** 3
+        off = np.zeros(nsub + 1, np.uint32)
+        idx = np.zeros(max(cnt.value, 1), np.uint32)
+        return off, idx, cnt.value
+
+    def wall_data(self, num_walls: int):
+        consts = np.zeros((num_walls, 16), np.float64)
+        cnt = ctypes.c_uint64()
+        _lib.check(self.lib.mcg_get_walls(self.ctx, None, None, None, ctypes.byref(cnt)))
+        nsub = self.part.num_subparts_per_edge ** 3
+        off = np.zeros(nsub + 1, np.uint32)
+        idx = np.zeros(max(cnt.value, 1), np.uint32)
+        _lib.check(self.lib.mcg_get_walls(self.ctx, _ptr(consts), _ptr(off), _ptr(idx), ctypes.byref(cnt)))
+        return consts, off, idx[:cnt.value]
+
+    def hit_trace(self):
+        n = self.num_molecules()
+        walls = np.zeros(n * _lib.TRACE_HITS, np.uint32)
+        counts = np.zeros(n, np.uint32)
+        _lib.check(self.lib.mcg_get_hit_trace(self.ctx, _ptr(walls), _ptr(counts), n))
+        return walls.reshape(n, _lib.TRACE_HITS), counts

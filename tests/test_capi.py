@@ -1,0 +1,55 @@
+"""The C ABI library loads and exports every entry point include/mcg.h
+declares (no compute calls: these run without a GPU)."""
+import ctypes
+import os
+import re
+
+from mcell_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "mcg.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mcg_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    assert sorted(_lib.EXPORTED) == declared_functions()
+
+
+def test_abi_version_and_struct_sizes(tmp_path):
+    import shutil
+    import subprocess
+    lib = _lib.load()
+    assert lib.mcg_abi_version() == 1
+    sizes = (ctypes.sizeof(_lib.McgConfig), ctypes.sizeof(_lib.McgSpecies), ctypes.sizeof(_lib.McgStepStats))
+    if shutil.which("gcc") is None:
+        assert sizes == (112, 24, 48)
+        return
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include "mcg.h"\nint main(){printf("%zu %zu %zu", sizeof(mcg_config), '
+                   'sizeof(mcg_species), sizeof(mcg_step_stats));return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    c_sizes = tuple(int(v) for v in subprocess.check_output([str(exe)]).split())
+    assert sizes == c_sizes
+
+
+def test_create_rejects_bad_config_without_touching_a_device():
+    lib = _lib.load()
+    cfg = _lib.McgConfig()
+    ctx = ctypes.c_void_p()
+    cfg.num_subparts_per_edge = 0  # invalid before any HIP call
+    assert lib.mcg_create(ctypes.byref(cfg), ctypes.byref(ctx)) == 1
+    assert b"num_subparts_per_edge" in lib.mcg_last_error()
+    assert not ctx.value
