@@ -36,7 +36,26 @@ def parse():
     ap.add_argument("--mols-per-replica", type=int, default=150000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device-to-device copy (known HBM bytes) to scale rocprof counters")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain copy of known size: 1 GiB read + 1 GiB written per launch."""
+    import torch
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    secs = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "launches": reps,
+                      "GBps": 2 * (1 << 30) / secs / 1e9}), flush=True)
 
 
 def build_workload(args):
@@ -84,6 +103,9 @@ def add_all_molecules(eng, wl):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
@@ -128,7 +150,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_diffuse_reflect64" if args.precision == "fp64" else "k_diffuse_reflect32",
-                     "kernel_avg_ms": avg_ms, "bytes_per_molecule_step": B_ALG[args.precision]},
+                     "kernel_avg_ms": avg_ms, "bytes_per_molecule_step": B_ALG[args.precision],
+                     "slow_path_fraction": st.slow_path_last / max(n, 1)},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

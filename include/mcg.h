@@ -91,7 +91,7 @@ typedef struct mcg_step_stats {
   double kernel_ms;              /* device time of the diffuse kernels (HIP events) */
   double wall_ms;                /* host wall time of the call */
   uint32_t kernel_launches;
-  uint32_t reserved;
+  uint32_t slow_path_last;       /* molecules that needed the full ray trace in the last iteration */
 } mcg_step_stats;
 
 const char* mcg_last_error(void);

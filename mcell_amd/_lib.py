@@ -69,7 +69,7 @@ class McgStepStats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double),
         ("wall_ms", ctypes.c_double),
         ("kernel_launches", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("slow_path_last", ctypes.c_uint32),
     ]
 
 

@@ -1,0 +1,97 @@
+// The common case of a diffusion step, kept register-light: draw the
+// displacement, and if the straight move cannot touch any wall, commit it.
+//
+// A move touches only subpartitions inside the index box spanned by its start
+// and end subpartitions (the segment lies inside the bounding box of its end
+// points), and every wall is listed in every subpartition it meets, so
+// testing the walls of that index box (at most 2x2x2 boxes for moves shorter
+// than a subpartition) decides whether any wall can be hit.  If none can, the
+// reference's ray trace would find no collision and commit pos + disp
+// (diffuse_react_event.cpp:774-777); otherwise the molecule is handed to the
+// full ray-trace kernel, which re-draws the same displacement from the same
+// counter-based stream.
+#pragma once
+#include "filter32.h"
+#include "path64.h"
+
+namespace mcg {
+
+// The slow-path queue is split into kQueueShards shards; workgroup b appends
+// to shard b % kQueueShards, so the per-wave atomics of different workgroups
+// hit different counters (one shared counter serialises them in L2).  A
+// shard holds the slots of all workgroups that map to it, so it cannot
+// overflow: shard_cap = 256 * ceil(num_workgroups / kQueueShards).
+constexpr uint32_t kQueueShards = 2048;
+
+// Wave-aggregated append of the `want` lanes' value to a shard: one atomic
+// per wave, slot order by lane.
+__device__ __forceinline__ void wave_append(bool want, uint32_t value, uint32_t* __restrict__ shard_slots,
+                                            uint32_t* __restrict__ shard_count) {
+  const uint64_t mask = __ballot(want);
+  if (mask == 0) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(shard_count, (uint32_t)__popcll(mask));
+  base = __shfl(base, (int)leader);
+  if (want) {
+    const uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    shard_slots[base + (uint32_t)__popcll(below)] = value;
+  }
+}
+
+// Index box of the move; false if the end point leaves the partition.
+__device__ __forceinline__ bool move_index_box(const PartitionParams& P, D3 pos, D3 end, int lo[3], int hi[3]) {
+  if (!in_partition(P, end)) return false;
+  int a[3], b[3];
+  subpart_3d(P, pos, a[0], a[1], a[2]);
+  subpart_3d(P, end, b[0], b[1], b[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = a[k] < b[k] ? a[k] : b[k];
+    hi[k] = a[k] < b[k] ? b[k] : a[k];
+    if (lo[k] < 0 || hi[k] >= (int)P.n) return false;
+  }
+  return true;
+}
+
+// FP64: every wall of the index box is an exact WALL_MISS of collide_wall
+// (no hit, no re-draw).  Exact in the reference's arithmetic.
+// `ent` holds the wall records in list order (ent[k] = walls[idx[k]]).
+__device__ __forceinline__ bool all_walls_miss64(const PartitionParams& P, const WallRec64* __restrict__ ent,
+                                                 const uint32_t* __restrict__ off, D3 pos, D3 disp,
+                                                 const int lo[3], const int hi[3]) {
+  for (int z = lo[2]; z <= hi[2]; ++z)
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int x = lo[0]; x <= hi[0]; ++x) {
+        const uint32_t s = subpart_of(P, x, y, z);
+        const uint32_t k1 = off[s + 1];
+        for (uint32_t k = off[s]; k < k1; ++k)
+          if (!wall_certain_miss64(ent[k], pos, disp)) return false;
+      }
+  return true;
+}
+
+// FP32 filter over the index box; *near_plane set when the FP32 end point is
+// within `band` of some wall plane.
+__device__ __forceinline__ bool all_walls_miss32(const PartitionParams& P, const FilterWall* __restrict__ ent,
+                                                 const uint32_t* __restrict__ off, const FilterParams& F,
+                                                 float band, D3 pos, D3 disp,
+                                                 const int lo[3], const int hi[3], bool& near_plane) {
+  const float px = (float)pos.x, py = (float)pos.y, pz = (float)pos.z;
+  const float mx = (float)disp.x, my = (float)disp.y, mz = (float)disp.z;
+  for (int z = lo[2]; z <= hi[2]; ++z)
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int x = lo[0]; x <= hi[0]; ++x) {
+        const uint32_t s = subpart_of(P, x, y, z);
+        const uint32_t k1 = off[s + 1];
+        for (uint32_t k = off[s]; k < k1; ++k) {
+          float de;
+          if (!certain_miss32(ent[k], F, px, py, pz, mx, my, mz, &de)) return false;
+          if (fabs_f(de) < band) near_plane = true;
+        }
+      }
+  return true;
+}
+
+}  // namespace mcg

@@ -224,6 +224,50 @@ __device__ __forceinline__ int collide_wall64(const WallRec64& w, D3 pos, D3& mo
   return WALL_MISS;
 }
 
+// collide_wall evaluated without moving anything: true exactly when the
+// reference returns WALL_MISS (a hit or a re-draw case returns false and
+// consumes no random draw).
+__device__ __forceinline__ bool wall_certain_miss64(const WallRec64& w, D3 pos, D3 move) {
+  D3 n = {w.nx, w.ny, w.nz};
+  double dp = dot3(n, pos);
+  double dv = dot3(n, move);
+  double dd = dp - w.d;
+  double d_eps;
+  if (dd > 0) {
+    d_eps = POS_EPS;
+    if (dd < d_eps) d_eps = 0.5 * dd;
+    if (dd + dv > d_eps) return true;
+  } else {
+    d_eps = -POS_EPS;
+    if (dd > d_eps) d_eps = 0.5 * dd;
+    if (dd < 0 && dd + dv < d_eps) return true;
+  }
+  if (dd == 0) return dv != 0;
+  double a = 1.0 / dv;
+  a *= -dd;
+  D3 hit = {pos.x + a * move.x, pos.y + a * move.y, pos.z + a * move.z};
+  D3 local = {hit.x - w.p0x, hit.y - w.p0y, hit.z - w.p0z};
+  double b = dot3(local, d3(w.ux, w.uy, w.uz));
+  double c = dot3(local, d3(w.vx, w.vy, w.vz));
+  double f;
+  if (w.uv2v < 0) {
+    c = -c;
+    f = -w.uv2v;
+  } else {
+    f = w.uv2v;
+  }
+  if (c > 0) {
+    double g = b * f;
+    double h = c * w.uv2u;
+    if (g > h) {
+      if (c * w.uv1u + g < h + w.uv1u * w.uv2v) return false;
+      return distinguishable(c * w.uv1u + g, h + w.uv1u * w.uv2v, POS_EPS);
+    }
+    return distinguishable(g, h, POS_EPS);
+  }
+  return distinguishable(c, 0.0, POS_EPS);
+}
+
 struct WallHit {
   uint32_t wall;
   int type;
@@ -301,12 +345,18 @@ __device__ __forceinline__ void diffuse_reflect64(const PartitionParams& P, cons
     WallHit hit;
     bool found = closest_wall_in_subpart64(P, walls, off, idx, start, last_hit, pos, disp, rng, hit, err);
     if (!found) {
-      SubpartWalk64 walk;
-      walk.init(P, pos, start, part_disp);
-      while (walk.next(P, err)) {
-        if (closest_wall_in_subpart64(P, walls, off, idx, walk.cur, last_hit, pos, disp, rng, hit, err)) {
-          found = true;
-          break;
+      // the walk only runs when the move leaves the start subpartition
+      D3 dst = {pos.x + part_disp.x, pos.y + part_disp.y, pos.z + part_disp.z};
+      int dx, dy, dz;
+      subpart_3d(P, dst, dx, dy, dz);
+      if (subpart_of(P, dx, dy, dz) != start) {
+        SubpartWalk64 walk;
+        walk.init(P, pos, start, part_disp);
+        while (walk.next(P, err)) {
+          if (closest_wall_in_subpart64(P, walls, off, idx, walk.cur, last_hit, pos, disp, rng, hit, err)) {
+            found = true;
+            break;
+          }
         }
       }
     }

@@ -27,6 +27,7 @@ struct WallRec32 {
   float vx, vy, vz, uv2u;
   float p0x, p0y, p0z, uv2v;
 };
+static_assert(sizeof(WallRec32) == 64, "one cache line per FP32 wall");
 
 // SimulationConfig / Partition geometry in internal units.
 struct PartitionParams {

@@ -61,26 +61,34 @@ struct mcg_ctx {
   DevBuf<WallRec64> walls64;
   DevBuf<WallRec32> walls32;
   DevBuf<uint32_t> off, idx;
+  DevBuf<WallRec64> ent64;  // wall records in list (entry) order
+  DevBuf<WallRec32> ent32;
   bool have_geometry = false;
 
   // species
   DevBuf<SpeciesRec> species;
   uint32_t num_species = 0;
 
-  // molecules (structure of arrays)
+  // molecules (structure of arrays): FP64 or FP32 positions by precision
   uint64_t n = 0;
   DevBuf<double> x, y, z;
+  DevBuf<float> xf, yf, zf;
   DevBuf<uint32_t> id, tag;
+  float max_abs_coord = 0;
+  bool fp32() const { return cfg.precision == MCG_PREC_FP32_FALLBACK; }
 
   // diagnostics
   DevBuf<uint32_t> err;
+  DevBuf<uint32_t> queue, qcount;  // hard-path queue of molecule indices
   DevBuf<uint32_t> trace_wall, trace_count;
   double last_kernel_ms = 0;
   uint32_t last_launches = 0;
 
   ~mcg_ctx() {
     walls64.release(); walls32.release(); off.release(); idx.release(); species.release();
-    x.release(); y.release(); z.release(); id.release(); tag.release(); err.release();
+    ent64.release(); ent32.release();
+    x.release(); y.release(); z.release(); xf.release(); yf.release(); zf.release();
+    id.release(); tag.release(); err.release(); queue.release(); qcount.release();
     trace_wall.release(); trace_count.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -129,9 +137,14 @@ int mcg_create(const mcg_config* cfg, mcg_ctx** out) {
       (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
     return cleanup_fail(fail(MCG_E_HIP, std::string("stream/event creation: ") + hipGetErrorString(e)));
   size_t cap = cfg->molecule_capacity;
-  if (c->x.alloc(cap) || c->y.alloc(cap) || c->z.alloc(cap) || c->id.alloc(cap) || c->tag.alloc(cap) ||
-      c->err.alloc(1))
+  bool pos_fail = c->fp32() ? (c->xf.alloc(cap) || c->yf.alloc(cap) || c->zf.alloc(cap))
+                            : (c->x.alloc(cap) || c->y.alloc(cap) || c->z.alloc(cap));
+  if (pos_fail || c->id.alloc(cap) || c->tag.alloc(cap) || c->err.alloc(1) ||
+      c->queue.alloc((size_t)kNumQueueShards * shard_capacity((uint32_t)cap)) || c->qcount.alloc(kNumQueueShards))
     return cleanup_fail(fail(MCG_E_NOMEM, "molecule buffers"));
+  double m = 0;
+  for (double v : {P.ox, P.oy, P.oz, P.qx, P.qy, P.qz}) m = std::max(m, std::fabs(v));
+  c->max_abs_coord = (float)m;
   if (cfg->flags & MCG_FLAG_TRACE_HITS) {
     if (c->trace_wall.alloc(cap * kTraceHits) || c->trace_count.alloc(cap))
       return cleanup_fail(fail(MCG_E_NOMEM, "trace buffers"));
@@ -180,6 +193,18 @@ int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint
   HIP_TRY(hipMemcpy(c->off.p, c->off_host.data(), c->off_host.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   if (!c->idx_host.empty())
     HIP_TRY(hipMemcpy(c->idx.p, c->idx_host.data(), c->idx_host.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  // wall records repeated in list order: a subpartition's walls are one
+  // contiguous read, without the index indirection
+  size_t ne = c->idx_host.size();
+  std::vector<WallRec64> e64(std::max<size_t>(ne, 1));
+  std::vector<WallRec32> e32(std::max<size_t>(ne, 1));
+  for (size_t k = 0; k < ne; ++k) {
+    e64[k] = c->walls_host[c->idx_host[k]];
+    e32[k] = w32[c->idx_host[k]];
+  }
+  if (c->ent64.alloc(e64.size()) || c->ent32.alloc(e32.size())) return fail(MCG_E_NOMEM, "wall entry buffers");
+  HIP_TRY(hipMemcpy(c->ent64.p, e64.data(), e64.size() * sizeof(WallRec64), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->ent32.p, e32.data(), e32.size() * sizeof(WallRec32), hipMemcpyHostToDevice));
   c->have_geometry = true;
   return MCG_OK;
 }
@@ -217,11 +242,26 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
     hx[i] = p[0]; hy[i] = p[1]; hz[i] = p[2];
     ht[i] = make_tag(species[i], 0, replica);
   }
+  std::vector<float> fx, fy, fz;
+  if (c->fp32()) {
+    fx.resize(count); fy.resize(count); fz.resize(count);
+    for (uint64_t i = 0; i < count; ++i) {
+      fx[i] = (float)hx[i]; fy[i] = (float)hy[i]; fz[i] = (float)hz[i];
+      if (!(fx[i] < (float)P.qx && fy[i] < (float)P.qy && fz[i] < (float)P.qz))
+        return fail(MCG_E_INVALID, "molecule rounds onto the partition boundary in FP32");
+    }
+  }
   if (count) {
     size_t o = c->n;
-    HIP_TRY(hipMemcpyAsync(c->x.p + o, hx.data(), count * 8, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->y.p + o, hy.data(), count * 8, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->z.p + o, hz.data(), count * 8, hipMemcpyHostToDevice, c->stream));
+    if (c->fp32()) {
+      HIP_TRY(hipMemcpyAsync(c->xf.p + o, fx.data(), count * 4, hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->yf.p + o, fy.data(), count * 4, hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->zf.p + o, fz.data(), count * 4, hipMemcpyHostToDevice, c->stream));
+    } else {
+      HIP_TRY(hipMemcpyAsync(c->x.p + o, hx.data(), count * 8, hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->y.p + o, hy.data(), count * 8, hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->z.p + o, hz.data(), count * 8, hipMemcpyHostToDevice, c->stream));
+    }
     HIP_TRY(hipMemcpyAsync(c->id.p + o, ids, count * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->tag.p + o, ht.data(), count * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -235,18 +275,28 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
   if (!c->have_geometry || c->num_species == 0) return fail(MCG_E_STATE, "geometry and species must be set first");
   auto t0 = std::chrono::steady_clock::now();
   MolView64 m{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p};
-  WallView64 W{c->walls64.p, c->off.p, c->idx.p};
+  MolView32 mf{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p};
+  WallView64 W{c->walls64.p, c->off.p, c->idx.p, c->ent64.p, c->ent32.p};
   HitTrace tr{c->trace_wall.p, c->trace_count.p, (c->cfg.flags & MCG_FLAG_TRACE_HITS) ? 1u : 0u};
   uint32_t n = (uint32_t)c->n;
+  SlowQueue Q{c->queue.p, c->qcount.p, 0};
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   for (uint32_t k = 0; k < num_iterations; ++k) {
-    HIP_TRY(launch_diffuse_reflect64(c->stream, m, n, c->species.p, c->P, W, c->cfg.seed, first_iteration + k,
-                                     c->err.p, tr));
+    if (c->fp32())
+      HIP_TRY(launch_diffuse_reflect32(c->stream, mf, n, c->species.p, c->P, W, c->walls32.p, c->max_abs_coord,
+                                       c->cfg.seed, first_iteration + k, Q, c->err.p, tr));
+    else
+      HIP_TRY(launch_diffuse_reflect64(c->stream, m, n, c->species.p, c->P, W, c->cfg.seed, first_iteration + k,
+                                       Q, c->err.p, tr));
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
-  uint32_t err = 0;
+  uint32_t err = 0, slow = 0;
+  std::vector<uint32_t> shard_counts(kNumQueueShards, 0);
   HIP_TRY(hipMemcpyAsync(&err, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
+  if (num_iterations && n)
+    HIP_TRY(hipMemcpyAsync(shard_counts.data(), c->qcount.p, kNumQueueShards * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  for (uint32_t v : shard_counts) slow += v;
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_kernel_ms = num_iterations ? ms / num_iterations : 0;
@@ -257,7 +307,8 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
     stats->reactions = 0;
     stats->kernel_ms = ms;
     stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    stats->kernel_launches = num_iterations;
+    stats->kernel_launches = 3 * num_iterations;
+    stats->slow_path_last = slow;
   }
   if (err) {
     HIP_TRY(hipMemsetAsync(c->err.p, 0, 4, c->stream));
@@ -288,10 +339,20 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
   if (capacity < n) return fail(MCG_E_CAPACITY, "output capacity too small");
   std::vector<double> hx(n), hy(n), hz(n);
   std::vector<uint32_t> ht(n);
+  if (n && c->fp32()) {
+    std::vector<float> fx(n), fy(n), fz(n);
+    HIP_TRY(hipMemcpyAsync(fx.data(), c->xf.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(fy.data(), c->yf.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(fz.data(), c->zf.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (uint64_t i = 0; i < n; ++i) { hx[i] = fx[i]; hy[i] = fy[i]; hz[i] = fz[i]; }
+  }
   if (n) {
-    HIP_TRY(hipMemcpyAsync(hx.data(), c->x.p, n * 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(hy.data(), c->y.p, n * 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(hz.data(), c->z.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (!c->fp32()) {
+      HIP_TRY(hipMemcpyAsync(hx.data(), c->x.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(hy.data(), c->y.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(hz.data(), c->z.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
     HIP_TRY(hipMemcpyAsync(ht.data(), c->tag.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     if (ids) HIP_TRY(hipMemcpyAsync(ids, c->id.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

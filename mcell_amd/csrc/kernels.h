@@ -17,14 +17,39 @@ struct MolView64 {
   uint32_t* tag;
 };
 
+// Structure-of-arrays molecule state in HBM (FP32 positions).
+struct MolView32 {
+  float* x;
+  float* y;
+  float* z;
+  uint32_t* id;
+  uint32_t* tag;
+};
+
 struct WallView64 {
   const WallRec64* walls;
-  const uint32_t* off;  // CSR offsets, n^3 + 1
-  const uint32_t* idx;  // wall indices, ascending within a subpartition
+  const uint32_t* off;    // CSR offsets, n^3 + 1
+  const uint32_t* idx;    // wall indices, ascending within a subpartition
+  const WallRec64* ent;   // walls[idx[k]] for every list entry k
+  const WallRec32* ent32; // its FP32 form
 };
+
+// Slow-path queue: kQueueShards counters and shards of shard_cap slots.
+struct SlowQueue {
+  uint32_t* slots;
+  uint32_t* count;
+  uint32_t shard_cap;
+};
+constexpr uint32_t kNumQueueShards = 2048;
+uint32_t shard_capacity(uint32_t n);
 
 hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, const SpeciesRec* species,
                                     const PartitionParams& P, const WallView64& W, uint64_t seed, uint32_t iteration,
+                                    SlowQueue Q, uint32_t* err, const HitTrace& trace);
+
+hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t n, const SpeciesRec* species,
+                                    const PartitionParams& P, const WallView64& W, const WallRec32* w32,
+                                    float max_abs_coord, uint64_t seed, uint32_t iteration, SlowQueue Q,
                                     uint32_t* err, const HitTrace& trace);
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);

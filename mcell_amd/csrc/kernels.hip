@@ -1,6 +1,17 @@
 // HIP kernels of the diffuse-react engine (gfx950).
+//
+// One iteration of non-reacting volume molecules is two launches:
+//   k_light*  every molecule: draw the displacement; if the straight move
+//             certainly touches no wall, commit it; otherwise append the
+//             molecule to the hard queue (one atomic per wave).
+//   k_hard*   queued molecules: the full ray trace with reflections, which
+//             re-draws the same displacement from the molecule's stream.
+// The light kernel holds few registers (high occupancy for the HBM stream);
+// the register-heavy trace runs only for the few molecules near walls.
 #include <hip/hip_runtime.h>
 
+#include "device/light.h"
+#include "device/path32.h"
 #include "device/path64.h"
 #include "device/rng.h"
 #include "device/types.h"
@@ -10,7 +21,7 @@ namespace mcg {
 
 __constant__ ZigguratTables c_zig = MCG_ZIGGURAT_TABLES_INIT;
 
-// Stages the Ziggurat tables (3 KB) into LDS so that the per-lane strip
+// Stages the Ziggurat tables (2.5 KB) into LDS so that the per-lane strip
 // lookups of a wave are LDS gathers rather than scattered cache reads.
 __device__ __forceinline__ void stage_ziggurat(ZigguratTables& T) {
   const uint32_t* src = reinterpret_cast<const uint32_t*>(&c_zig);
@@ -20,63 +31,201 @@ __device__ __forceinline__ void stage_ziggurat(ZigguratTables& T) {
   __syncthreads();
 }
 
-// One iteration of every live volume molecule that has no volume reactions:
-// draw the displacement (pick_vol_displacement, diffusion_utils.inl:113-119),
-// ray-trace it through the subpartition grid and reflect off walls.
-__global__ __launch_bounds__(256) void k_diffuse_reflect64(MolView64 m, uint32_t n,
-                                                           const SpeciesRec* __restrict__ species,
-                                                           PartitionParams P, WallView64 W, uint64_t seed,
-                                                           uint32_t iteration, uint32_t* __restrict__ err_out,
-                                                           HitTrace trace) {
+// pick_vol_displacement (diffusion_utils.inl:113-119) for a full step
+__device__ __forceinline__ D3 draw_displacement(PhiloxStream& rng, const ZigguratTables& T, double scale) {
+  D3 d;
+  d.x = scale * gauss(rng, T) * 0.70710678118654752440;
+  d.y = scale * gauss(rng, T) * 0.70710678118654752440;
+  d.z = scale * gauss(rng, T) * 0.70710678118654752440;
+  return d;
+}
+
+__global__ void k_reset_counters(uint32_t* c, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = 0;
+}
+
+// ------------------------------------------------------------------- FP64 --
+__global__ __launch_bounds__(256) void k_light64(MolView64 m, uint32_t n, const SpeciesRec* __restrict__ species,
+                                                 PartitionParams P, WallView64 W, uint64_t seed, uint32_t iteration,
+                                                 SlowQueue Q, HitTrace trace) {
   __shared__ ZigguratTables T;
   stage_ziggurat(T);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t tag = m.tag[i];
-  if (tag_flags(tag) & MF_DEFUNCT) return;
-  const SpeciesRec sp = species[tag_species(tag)];
-  if (!(sp.space_step > 0)) return;
-  PhiloxStream rng;
-  rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
-  D3 pos = {m.x[i], m.y[i], m.z[i]};
-  const double scale = sp.space_step;
-  D3 disp;
-  disp.x = scale * gauss(rng, T) * 0.70710678118654752440;
-  disp.y = scale * gauss(rng, T) * 0.70710678118654752440;
-  disp.z = scale * gauss(rng, T) * 0.70710678118654752440;
-  uint32_t err = 0;
-  uint32_t tn = 0;
-  uint32_t* tw = trace.enabled ? trace.wall + (size_t)i * kTraceHits : nullptr;
-  diffuse_reflect64(P, W.walls, W.off, W.idx, pos, disp, rng, err, tw, tn);
-  m.x[i] = pos.x;
-  m.y[i] = pos.y;
-  m.z[i] = pos.z;
-  if (trace.enabled) trace.count[i] = tn;
-  if (err) atomicOr(err_out, err);
+  bool hard = false;
+  if (i < n) {
+    const uint32_t tag = m.tag[i];
+    const SpeciesRec sp = species[tag_species(tag)];
+    if (!(tag_flags(tag) & MF_DEFUNCT) && sp.space_step > 0) {
+      PhiloxStream rng;
+      rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
+      const D3 pos = {m.x[i], m.y[i], m.z[i]};
+      const D3 disp = draw_displacement(rng, T, sp.space_step);
+      const D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
+      int lo[3], hi[3];
+      if (move_index_box(P, pos, end, lo, hi) && all_walls_miss64(P, W.ent, W.off, pos, disp, lo, hi)) {
+        m.x[i] = end.x;
+        m.y[i] = end.y;
+        m.z[i] = end.z;
+        if (trace.enabled) trace.count[i] = 0;
+      } else {
+        hard = true;
+      }
+    }
+  }
+  const uint32_t shard = blockIdx.x % kQueueShards;
+  wave_append(hard, i, Q.slots + (size_t)shard * Q.shard_cap, Q.count + shard);
 }
 
-// Copy kernel of known size: calibrates the HBM byte counters and gives the
-// achievable stream bandwidth on this device.
+__global__ __launch_bounds__(256) void k_hard64(MolView64 m, const SpeciesRec* __restrict__ species,
+                                                PartitionParams P, WallView64 W, uint64_t seed, uint32_t iteration,
+                                                SlowQueue Q, uint32_t* __restrict__ err_out, HitTrace trace) {
+  __shared__ ZigguratTables T;
+  stage_ziggurat(T);
+  // workgroup k drains shard k
+  const uint32_t count = Q.count[blockIdx.x];
+  const uint32_t* slots = Q.slots + (size_t)blockIdx.x * Q.shard_cap;
+  for (uint32_t q = threadIdx.x; q < count; q += blockDim.x) {
+    const uint32_t i = slots[q];
+    const uint32_t tag = m.tag[i];
+    const SpeciesRec sp = species[tag_species(tag)];
+    PhiloxStream rng;
+    rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
+    D3 pos = {m.x[i], m.y[i], m.z[i]};
+    const D3 disp = draw_displacement(rng, T, sp.space_step);
+    uint32_t err = 0, tn = 0;
+    uint32_t* tw = trace.enabled ? trace.wall + (size_t)i * kTraceHits : nullptr;
+    diffuse_reflect64(P, W.walls, W.off, W.idx, pos, disp, rng, err, tw, tn);
+    m.x[i] = pos.x;
+    m.y[i] = pos.y;
+    m.z[i] = pos.z;
+    if (trace.enabled) trace.count[i] = tn;
+    if (err) atomicOr(err_out, err);
+  }
+}
+
+// ------------------------------------------------------------------- FP32 --
+__global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, const SpeciesRec* __restrict__ species,
+                                                 PartitionParams P, Walls32View W, FilterParams F, float band,
+                                                 uint64_t seed, uint32_t iteration, SlowQueue Q, HitTrace trace) {
+  __shared__ ZigguratTables T;
+  stage_ziggurat(T);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool hard = false;
+  if (i < n) {
+    const uint32_t tag = m.tag[i];
+    const SpeciesRec sp = species[tag_species(tag)];
+    if (!(tag_flags(tag) & MF_DEFUNCT) && sp.space_step > 0) {
+      PhiloxStream rng;
+      rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
+      const D3 pos = {(double)m.x[i], (double)m.y[i], (double)m.z[i]};
+      const D3 disp = draw_displacement(rng, T, sp.space_step);
+      const D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
+      int lo[3], hi[3];
+      bool near_plane = false;
+      if (move_index_box(P, pos, end, lo, hi) &&
+          all_walls_miss32(P, W.ent32, W.off, F, band, pos, disp, lo, hi, near_plane) && !near_plane) {
+        m.x[i] = (float)end.x;
+        m.y[i] = (float)end.y;
+        m.z[i] = (float)end.z;
+        if (trace.enabled) trace.count[i] = 0;
+      } else {
+        hard = true;
+      }
+    }
+  }
+  const uint32_t shard = blockIdx.x % kQueueShards;
+  wave_append(hard, i, Q.slots + (size_t)shard * Q.shard_cap, Q.count + shard);
+}
+
+__global__ __launch_bounds__(256) void k_hard32(MolView32 m, const SpeciesRec* __restrict__ species,
+                                                PartitionParams P, Walls32View W, FilterParams F, float band,
+                                                uint64_t seed, uint32_t iteration, SlowQueue Q,
+                                                uint32_t* __restrict__ err_out, HitTrace trace) {
+  __shared__ ZigguratTables T;
+  stage_ziggurat(T);
+  // workgroup k drains shard k
+  const uint32_t count = Q.count[blockIdx.x];
+  const uint32_t* slots = Q.slots + (size_t)blockIdx.x * Q.shard_cap;
+  for (uint32_t q = threadIdx.x; q < count; q += blockDim.x) {
+    const uint32_t i = slots[q];
+    const uint32_t tag = m.tag[i];
+    const SpeciesRec sp = species[tag_species(tag)];
+    PhiloxStream rng;
+    rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
+    D3 pos = {(double)m.x[i], (double)m.y[i], (double)m.z[i]};
+    const D3 disp = draw_displacement(rng, T, sp.space_step);
+    uint32_t err = 0, tn = 0;
+    uint32_t* tw = trace.enabled ? trace.wall + (size_t)i * kTraceHits : nullptr;
+    bool near_wall = diffuse_reflect32(P, W, F, band, pos, disp, rng, err, tw, tn);
+    float fx, fy, fz;
+    if (near_wall) {
+      round_to_fp32_same_side(P, W.w64, W.off, W.idx, pos.x, pos.y, pos.z, &fx, &fy, &fz);
+    } else {
+      fx = (float)pos.x;
+      fy = (float)pos.y;
+      fz = (float)pos.z;
+    }
+    m.x[i] = fx;
+    m.y[i] = fy;
+    m.z[i] = fz;
+    if (trace.enabled) trace.count[i] = tn;
+    if (err) atomicOr(err_out, err);
+  }
+}
+
+// Copy kernel of known size (device-to-device bandwidth reference).
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// -------------------------------------------------------------- launchers --
+static_assert(kNumQueueShards == kQueueShards, "host and device shard counts differ");
+static_assert(kQueueShards % 256 == 0, "counter reset grid");
+
+uint32_t shard_capacity(uint32_t n) {
+  uint32_t blocks = (n + 255) / 256;
+  return 256u * ((blocks + kQueueShards - 1) / kQueueShards);
+}
+
+
 hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, const SpeciesRec* species,
                                     const PartitionParams& P, const WallView64& W, uint64_t seed, uint32_t iteration,
+                                    SlowQueue Q, uint32_t* err, const HitTrace& trace) {
+  if (n == 0) return hipSuccess;
+  Q.shard_cap = shard_capacity(n);
+  hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
+  hipLaunchKernelGGL(k_light64, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W, seed, iteration, Q,
+                     trace);
+  hipLaunchKernelGGL(k_hard64, dim3(kQueueShards), dim3(256), 0, s, m, species, P, W, seed, iteration, Q, err, trace);
+  return hipGetLastError();
+}
+
+hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t n, const SpeciesRec* species,
+                                    const PartitionParams& P, const WallView64& W, const WallRec32* w32,
+                                    float max_abs_coord, uint64_t seed, uint32_t iteration, SlowQueue Q,
                                     uint32_t* err, const HitTrace& trace) {
   if (n == 0) return hipSuccess;
-  dim3 block(256), grid((n + 255) / 256);
-  hipLaunchKernelGGL(k_diffuse_reflect64, grid, block, 0, s, m, n, species, P, W, seed, iteration, err, trace);
+  FilterParams F = make_filter_params(max_abs_coord);
+  float band = 2.0f * F.e_dd + 8.0f * F.eps * max_abs_coord;
+  Walls32View W32{W.walls, reinterpret_cast<const FilterWall*>(w32), W.off, W.idx,
+                  reinterpret_cast<const FilterWall*>(W.ent32)};
+  Q.shard_cap = shard_capacity(n);
+  hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
+  hipLaunchKernelGGL(k_light32, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W32, F, band, seed,
+                     iteration, Q, trace);
+  hipLaunchKernelGGL(k_hard32, dim3(kQueueShards), dim3(256), 0, s, m, species, P, W32, F, band, seed, iteration, Q,
+                     err, trace);
   return hipGetLastError();
 }
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
   size_t n = bytes / sizeof(float4);
   if (n == 0) return hipSuccess;
-  dim3 block(256), grid(2048);
-  hipLaunchKernelGGL(k_copy, grid, block, 0, s, static_cast<const float4*>(src), static_cast<float4*>(dst), n);
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, static_cast<const float4*>(src), static_cast<float4*>(dst),
+                     n);
   return hipGetLastError();
 }
 

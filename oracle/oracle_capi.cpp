@@ -263,6 +263,19 @@ int orc_collide_wall(void* wp, const double* pos, uint32_t wall, const double* m
   return (int)ct;
 }
 
+// collide_wall (no move update) for n rays against one wall; types[i] gets
+// the CollType (3 = REDO case, no draw consumed)
+void orc_collide_wall_batch(void* wp, uint32_t wall, uint64_t n, const double* pos, const double* move,
+                            int32_t* types) {
+  World* w = static_cast<World*>(wp);
+  for (uint64_t i = 0; i < n; ++i) {
+    V3 mv{move[3 * i], move[3 * i + 1], move[3 * i + 2]};
+    V3 h{0, 0, 0};
+    double tt = 0;
+    types[i] = (int32_t)w->collide_wall({pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, wall, true, mv, tt, h);
+  }
+}
+
 // subpartitions crossed by a move (walls list, ordered); returns count
 uint32_t orc_crossed_subparts(void* wp, const double* pos, const double* disp, uint32_t* out, uint32_t cap) {
   World* w = static_cast<World*>(wp);

@@ -106,6 +106,78 @@ def test_msd_matches_6Dt(gpu_available):
     eng.close()
 
 
+def _fp32_start(wl):
+    """Positions rounded to FP32 so that both forms start from the same point."""
+    wl.positions = wl.positions.astype(np.float32).astype(np.float64)
+    return wl
+
+
+def test_fp32_fallback_takes_the_fp64_wall_hits(gpu_available):
+    """From the same start, the FP32+fallback form hits exactly the walls the
+    FP64 form hits, and its position is the FP64 one rounded to FP32 (moved by
+    at most a few ulps where the end point sits on a wall plane)."""
+    wl = _fp32_start(W.free_diffusion_box(scale=5.0))
+    e64 = _engine(wl, "fp64")
+    e32 = _engine(wl, "fp32")
+    e64.step(0, 1)
+    e32.step(0, 1)
+    w64, c64 = e64.hit_trace()
+    w32, c32 = e32.hit_trace()
+    assert np.array_equal(c64, c32)
+    assert c64.sum() > 50
+    for k in np.nonzero(c64)[0]:
+        assert np.array_equal(w64[k, :c64[k]], w32[k, :c32[k]])
+    p64 = e64.molecules()["pos"]
+    p32 = e32.molecules()["pos"]
+    ulp = np.spacing(np.abs(p64).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(p32 - p64) <= 4 * ulp)
+    assert np.mean(p32 == p64.astype(np.float32).astype(np.float64)) > 0.999
+    e64.close()
+    e32.close()
+
+
+def test_fp32_fallback_long_run_agreement_and_containment(gpu_available):
+    """Over 200 iterations the FP32+fallback trajectories stay inside the cube
+    and take the FP64 wall-hit sequence for nearly every molecule-step; the
+    disagreement rate and the position RMS against FP64 are reported."""
+    wl = _fp32_start(W.free_diffusion_box())
+    e64 = _engine(wl, "fp64")
+    e32 = _engine(wl, "fp32")
+    hits = disagree = 0
+    for it in range(200):
+        e64.step(it, 1)
+        e32.step(it, 1)
+        w64, c64 = e64.hit_trace()
+        w32, c32 = e32.hit_trace()
+        same = (c64 == c32) & np.all((w64 == w32) | (np.arange(8)[None, :] >= c64[:, None]), axis=1)
+        hits += int(c64.sum())
+        disagree += int((~same & (c64 + c32 > 0)).sum())
+    p64 = e64.molecules()["pos"]
+    p32 = e32.molecules()["pos"]
+    rms = float(np.sqrt(np.mean((p32 - p64) ** 2)))
+    print("fp32 vs fp64 over 200 iterations: %d wall hits, %d molecule-steps with a different hit sequence "
+          "(rate %.2e), position RMS %.3e lu" % (hits, disagree, disagree / max(hits, 1), rms))
+    assert np.all(np.abs(p32) < 50.0)
+    assert disagree / max(hits, 1) < 1e-3
+    assert rms < 1e-3
+    e64.close()
+    e32.close()
+
+
+def test_fp32_msd_matches_6Dt(gpu_available):
+    wl = W.free_diffusion_box(scale=10.0)
+    eng = _engine(wl, "fp32", trace=False)
+    p0 = wl.positions.astype(np.float32).astype(np.float64)
+    eng.step(0, 10)
+    p = eng.molecules()["pos"]
+    inner = np.all(np.abs(p0) < 35.0, axis=1)
+    d2 = ((p - p0) ** 2).sum(axis=1)[inner]
+    expected = 6 * 1e-6 * 10 * 1e-6 / (wl.part.length_unit * 1e-4) ** 2
+    se = expected * np.sqrt(2.0 / 3.0) / np.sqrt(inner.sum())
+    assert abs(d2.mean() - expected) < 5 * se
+    eng.close()
+
+
 def test_replicas_are_independent_streams(gpu_available):
     """Two replicas with identical molecules but different replica ids take
     different paths; each replica matches its own oracle run."""
