@@ -65,6 +65,11 @@ def build_workload(args):
         cfg = {"workload": "free diffusion, %d replicas x %d A in a shared reflective 1 um cube (12 walls), "
                            "0.1 um subpartitions, dt 1e-6 s" % (args.replicas, args.mols_per_replica),
                "replicas": args.replicas, "molecules": wl.num_molecules, "walls": int(len(wl.walls))}
+    elif args.workload == "bimol_box":
+        wl = W.bimol_box(iterations=args.steps)
+        cfg = {"workload": "A + B -> C (1e8 /M/s) in a reflective 1 um cube (12 walls), 50,000 A + 50,000 B "
+                           "(D 1e-6 cm^2/s), C with D 5e-7, 0.05 um subpartitions, counts every 10 iterations, "
+                           "dt 1e-6 s", "molecules": wl.num_molecules, "walls": int(len(wl.walls)), "replicas": 1}
     elif args.workload == "free_diffusion_box":
         wl = W.free_diffusion_box(iterations=args.steps)
         cfg = {"workload": "free diffusion, 10,000 A in a reflective 1 um cube, 0.1 um subpartitions",
@@ -74,31 +79,36 @@ def build_workload(args):
     return wl, cfg
 
 
-def cpu_baseline(seconds):
+def cpu_baseline(seconds, workload):
     """The oracle in reference mode (ISAAC64 global stream, sequential FP64)
-    on the reference's free-diffusion case, for about `seconds` of CPU time."""
+    for about `seconds` of CPU time: the reference's free-diffusion case, or
+    for the reaction workloads the single-instance A + B -> C cube."""
     from mcell_amd import workloads as W
     from oracle import pyoracle as po
-    wl = W.free_diffusion_box()
+    if workload.startswith("free"):
+        wl = W.free_diffusion_box()
+        what = "free diffusion of 10,000 A in the reflective 1 um cube"
+    else:
+        wl = W.bimol_box()
+        what = "A + B -> C, 50,000 A + 50,000 B in the reflective 1 um cube"
     w = po.world_from_workload(wl, draw_mode=po.DRAW_ISAAC)
-    done, secs = 0, 0.0
-    chunk = 50
+    w.set_barrier_interval(wl.barrier_interval)
+    done, secs, steps = 0, 0.0, 0
+    chunk = 10
     while secs < seconds and done < 20000:
         secs += w.run(chunk)
         done += chunk
-    rate = wl.num_molecules * done / secs
+    steps = w.stats()["diffuse_3d_calls"]
+    rate = steps / secs
     return {"value": rate, "unit": "molecule-timesteps/s", "cores": 1, "kind": "port",
-            "sample": "oracle (FP64 restatement of the reference CPU path, ISAAC64 stream), "
-                      "free diffusion of 10,000 A in the reflective 1 um cube, %d iterations, %.1f s" % (done, secs)}
+            "sample": "oracle (FP64 restatement of the reference CPU path, ISAAC64 stream), %s, %d iterations, "
+                      "%.1f s" % (what, done, secs)}
 
 
-def add_all_molecules(eng, wl):
-    """Uploads each replica's molecules (replicas are contiguous runs)."""
-    import numpy as np
-    bounds = np.searchsorted(wl.replica, np.arange(wl.num_replicas + 1))
-    for r in range(wl.num_replicas):
-        a, b = bounds[r], bounds[r + 1]
-        eng.add_volume_molecules(r, wl.ids[a:b], wl.mol_species[a:b], wl.positions[a:b])
+def kernel_name(args, wl):
+    if wl.rxns:
+        return "k_move<double>" if args.precision == "fp64" else "k_move<float>"
+    return "k_diffuse_reflect64" if args.precision == "fp64" else "k_diffuse_reflect32"
 
 
 def main():
@@ -112,11 +122,8 @@ def main():
         sys.exit(2)
     from mcell_amd.engine import DiffuseReactEngine
     wl, cfg = build_workload(args)
-    eng = DiffuseReactEngine(wl.part, precision=args.precision, seed=wl.config.seed,
-                             num_replicas=wl.num_replicas, capacity=wl.num_molecules)
-    eng.set_geometry(wl.vertices, wl.walls, wl.wall_class)
-    eng.set_species(wl.species_steps)
-    add_all_molecules(eng, wl)
+    eng = DiffuseReactEngine.from_workload(wl, precision=args.precision,
+                                           capacity=wl.num_molecules * (2 if wl.rxns else 1))
     n = wl.num_molecules
     print("bench: %s, %d molecules, precision %s, warmup %d, steps %d" % (args.workload, n, args.precision,
                                                                           args.warmup, args.steps),
@@ -129,9 +136,11 @@ def main():
     eng.sync()
     t1 = time.perf_counter()
     secs = t1 - t0
-    value = n * args.steps / secs
+    steps_done = st.molecule_steps if wl.rxns else n * args.steps
+    value = steps_done / secs
     avg_ms, launches = eng.last_kernel_time()
-    achieved = B_ALG[args.precision] * n / (avg_ms * 1e-3) / 1e9
+    per_step = steps_done / args.steps
+    achieved = B_ALG[args.precision] * per_step / (avg_ms * 1e-3) / 1e9
     eng.close()
     out = {
         "metric": "molecule-timesteps/s on 1 MI355X (% HBM peak); FP32+fallback vs FP64 speedup",
@@ -149,12 +158,16 @@ def main():
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_diffuse_reflect64" if args.precision == "fp64" else "k_diffuse_reflect32",
-                     "kernel_avg_ms": avg_ms, "bytes_per_molecule_step": B_ALG[args.precision],
-                     "slow_path_fraction": st.slow_path_last / max(n, 1)},
+                     "kernel": kernel_name(args, wl), "kernel_avg_ms": avg_ms,
+                     "bytes_per_molecule_step": B_ALG[args.precision]},
     }
+    if wl.rxns:
+        out["reactions"] = st.reactions
+        out["settle_rounds"] = st.slow_path_last
+    else:
+        out["roofline"]["slow_path_fraction"] = st.slow_path_last / max(n, 1)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, args.workload)
     print(json.dumps(out), flush=True)
 
 

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 1
+#define MCG_ABI_VERSION 2
 
 typedef struct mcg_ctx mcg_ctx;
 
@@ -64,10 +64,16 @@ typedef struct mcg_config {
   uint64_t seed;                 /* key of the per-molecule counter-based streams */
   uint64_t molecule_capacity;    /* volume molecules over all replicas (incl. products) */
   uint32_t flags;                /* MCG_FLAG_* */
-  uint32_t reserved;
+  uint32_t barrier_interval;     /* iterations between scheduler barriers (count events); 0 = every
+                                    iteration.  Bounds each molecule's time step as
+                                    Calendar::get_time_up_to_next_barrier does (scheduler.cpp:212-242),
+                                    at most 100 iterations ahead (diffuse_react_event.h:37). */
 } mcg_config;
 
-#define MCG_FLAG_TRACE_HITS 1u /* record the wall indices each molecule hits (parity tests) */
+#define MCG_FLAG_TRACE_HITS 1u    /* record the wall indices each molecule hits (parity tests) */
+#define MCG_FLAG_PROPOSE_ONLY 2u  /* parity mode: every molecule moves against the iteration-start
+                                     snapshot and successful reaction tests are logged, not carried
+                                     out (a reacting molecule stops where its last ray segment began) */
 
 /* BNG::Species fields on this path (space_step/time_step from
  * Species::update_space_and_time_step, MCell3 src/mcell_species.c:237-300). */
@@ -82,6 +88,33 @@ typedef struct mcg_species {
 #define MCG_SPF_CAN_VOLWALL 2u
 #define MCG_SPF_CAN_VOLSURF 4u
 #define MCG_SPF_TARGET_ONLY 8u
+
+/* A volume+volume reaction class (BNG::RxnClass for two volume reactants
+ * as test_bimolecular and outcome_bimolecular use it, rxn_utils.inl:336-414,
+ * diffuse_react_event.cpp:1833-1895).  Probabilities are per collision,
+ * cumulative over pathways (RxnClass::cum_probs; the last entry is
+ * max_fixed_p).  Every product is a new volume molecule created at the
+ * collision point; both reactants are destroyed (classes that keep a reactant
+ * are refused with MCG_E_UNSUPPORTED). */
+typedef struct mcg_bimol_rxn {
+  uint32_t reactant_a, reactant_b; /* species ids */
+  uint32_t num_pathways;           /* 1..4 */
+  uint32_t reserved;
+  double cum_probs[4];
+  uint32_t num_products[4];        /* per pathway, 0..3 */
+  uint32_t products[4][3];         /* product species per pathway */
+} mcg_bimol_rxn;
+
+/* One reaction that happened (or, with MCG_FLAG_PROPOSE_ONLY, was proposed). */
+typedef struct mcg_rxn_event {
+  double time;                /* absolute time in iterations */
+  double pos[3];              /* collision point */
+  uint32_t iteration, replica;
+  uint32_t mover_id, partner_id;
+  uint32_t rxn, pathway;      /* index into the mcg_set_bimol_rxns table, pathway */
+  uint32_t first_product_id;  /* products get ids first_product_id + k */
+  uint32_t num_products;
+} mcg_rxn_event;
 
 /* Per-step counters mirroring SimulationStats (src4/simulation_stats.h). */
 typedef struct mcg_step_stats {
@@ -114,6 +147,24 @@ int mcg_set_geometry(mcg_ctx* ctx, const double* vertices, uint32_t num_vertices
 /* Species table, indexed by species id (< 256). */
 int mcg_set_species(mcg_ctx* ctx, const mcg_species* species, uint32_t num_species);
 
+/* Volume-volume reaction classes.  Reactant species get MCG_SPF_CAN_VOLVOL.
+ * Molecules of reactive species must stay within the box of subpartitions
+ * spanned by the geometry and the molecules added (one subpartition of
+ * margin); leaving it is reported as MCG_E_LIMIT. */
+int mcg_set_bimol_rxns(mcg_ctx* ctx, const mcg_bimol_rxn* rxns, uint32_t num_rxns);
+
+/* Keeps up to `capacity` reaction events of each mcg_step call (0 = off). */
+int mcg_set_rxn_log(mcg_ctx* ctx, uint64_t capacity);
+
+/* Events of the last mcg_step call in (iteration, replica, mover id) order
+ * within each settling phase; *count gets the number recorded (which may
+ * exceed what the log kept). */
+int mcg_get_rxn_log(mcg_ctx* ctx, mcg_rxn_event* events, uint64_t capacity, uint64_t* count);
+
+/* Live volume molecules per (replica, species): counts[r * num_species + s]
+ * (MolOrRxnCountEvent's molecule counts over the whole partition). */
+int mcg_count_species(mcg_ctx* ctx, uint64_t* counts);
+
 /* Volume molecules of one replica (Partition::add_volume_molecule semantics:
  * each position must lie in the partition).  ids are the molecule ids that
  * key their random streams. */
@@ -121,17 +172,27 @@ int mcg_add_volume_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids
                              const double* positions, uint64_t count);
 
 /* Runs num_iterations diffuse-react iterations starting at first_iteration
- * (DiffuseReactEvent::step with event_time = iteration and a barrier at the
- * end of every iteration).  stats may be NULL. */
+ * (DiffuseReactEvent::step with event_time = iteration; barriers every
+ * cfg.barrier_interval iterations).  stats may be NULL.
+ *
+ * Without reaction classes each molecule's step is the reference's step with
+ * the molecule's own random stream.  With reaction classes, the molecules of
+ * an iteration move at once against the positions at the iteration's start;
+ * a molecule whose reaction test succeeds claims itself and its partner, the
+ * proposal with the smaller mover id wins a contested molecule, and losers
+ * repeat their step without the consumed molecules.  Products move for the
+ * rest of the step afterwards, as the reference's new_diffuse_actions do. */
 int mcg_step(mcg_ctx* ctx, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats);
 
 /* Waits for all queued work. */
 int mcg_sync(mcg_ctx* ctx);
 
+/* Live volume molecules over all replicas. */
 int mcg_num_molecules(mcg_ctx* ctx, uint64_t* count);
 
 /* Copies the live volume molecules back (for Partition at a barrier).  Any
- * output pointer may be NULL.  Molecules come back in engine order. */
+ * output pointer may be NULL.  Molecules come back in engine order (which
+ * changes every iteration when reactions are on). */
 int mcg_get_volume_molecules(mcg_ctx* ctx, uint32_t* ids, uint32_t* species, uint32_t* replica, double* positions,
                              uint32_t* subparts, uint64_t capacity, uint64_t* count);
 

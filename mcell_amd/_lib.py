@@ -16,6 +16,7 @@ MCG_OK = 0
 MCG_PREC_FP64 = 0
 MCG_PREC_FP32_FALLBACK = 1
 MCG_FLAG_TRACE_HITS = 1
+MCG_FLAG_PROPOSE_ONLY = 2
 
 MCG_SPF_CAN_VOLVOL = 1
 MCG_SPF_CAN_VOLWALL = 2
@@ -28,7 +29,8 @@ TRACE_HITS = 8
 EXPORTED = [
     "mcg_last_error", "mcg_abi_version", "mcg_create", "mcg_destroy", "mcg_set_geometry", "mcg_set_species",
     "mcg_add_volume_molecules", "mcg_step", "mcg_sync", "mcg_num_molecules", "mcg_get_volume_molecules",
-    "mcg_get_walls", "mcg_get_hit_trace", "mcg_last_kernel_time",
+    "mcg_get_walls", "mcg_get_hit_trace", "mcg_last_kernel_time", "mcg_set_bimol_rxns", "mcg_set_rxn_log",
+    "mcg_get_rxn_log", "mcg_count_species",
 ]
 
 
@@ -48,7 +50,7 @@ class McgConfig(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("molecule_capacity", ctypes.c_uint64),
         ("flags", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("barrier_interval", ctypes.c_uint32),
     ]
 
 
@@ -59,6 +61,39 @@ class McgSpecies(ctypes.Structure):
         ("flags", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
     ]
+
+
+class McgBimolRxn(ctypes.Structure):
+    _fields_ = [
+        ("reactant_a", ctypes.c_uint32),
+        ("reactant_b", ctypes.c_uint32),
+        ("num_pathways", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("cum_probs", ctypes.c_double * 4),
+        ("num_products", ctypes.c_uint32 * 4),
+        ("products", (ctypes.c_uint32 * 3) * 4),
+    ]
+
+
+class McgRxnEvent(ctypes.Structure):
+    _fields_ = [
+        ("time", ctypes.c_double),
+        ("pos", ctypes.c_double * 3),
+        ("iteration", ctypes.c_uint32),
+        ("replica", ctypes.c_uint32),
+        ("mover_id", ctypes.c_uint32),
+        ("partner_id", ctypes.c_uint32),
+        ("rxn", ctypes.c_uint32),
+        ("pathway", ctypes.c_uint32),
+        ("first_product_id", ctypes.c_uint32),
+        ("num_products", ctypes.c_uint32),
+    ]
+
+
+# numpy view of mcg_rxn_event
+RXN_EVENT_DTYPE = [("time", "<f8"), ("pos", "<f8", (3,)), ("iteration", "<u4"), ("replica", "<u4"),
+                   ("mover_id", "<u4"), ("partner_id", "<u4"), ("rxn", "<u4"), ("pathway", "<u4"),
+                   ("first_product_id", "<u4"), ("num_products", "<u4")]
 
 
 class McgStepStats(ctypes.Structure):
@@ -106,6 +141,10 @@ def load():
     lib.mcg_get_walls.argtypes = [P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]
     lib.mcg_get_hit_trace.argtypes = [P, P, P, ctypes.c_uint64]
     lib.mcg_last_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32)]
+    lib.mcg_set_bimol_rxns.argtypes = [P, ctypes.POINTER(McgBimolRxn), ctypes.c_uint32]
+    lib.mcg_set_rxn_log.argtypes = [P, ctypes.c_uint64]
+    lib.mcg_get_rxn_log.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    lib.mcg_count_species.argtypes = [P, P]
     for name in EXPORTED:
         if not name.startswith("mcg_last_error") and name != "mcg_abi_version":
             getattr(lib, name).restype = ctypes.c_int

@@ -21,9 +21,9 @@ struct Walls32View {
   const FilterWall* __restrict__ ent32;  // w32[idx[k]] for every list entry k
 };
 
-// The FP64 test of one candidate wall; kept out of line so that the register
-// budget of the FP32 hot loop is not sized by the rare FP64 path.
-__device__ __noinline__ int collide_wall64_ool(const WallRec64* __restrict__ walls, uint32_t wi, double px, double py,
+// The FP64 test of one candidate wall (inlined: an out-of-line call forces
+// the caller's live registers through scratch memory).
+__device__ __forceinline__ int collide_wall64_ool(const WallRec64* __restrict__ walls, uint32_t wi, double px, double py,
                                                double pz, double* mx, double* my, double* mz, double* t, double* hx,
                                                double* hy, double* hz, PhiloxStream* rng) {
   D3 pos = {px, py, pz};
@@ -61,7 +61,7 @@ __device__ __forceinline__ bool closest_wall_in_subpart32(const PartitionParams&
       uint32_t wi = W.idx[k];
       if (wi == last_hit) continue;
       float de;
-      if (certain_miss32(W.w32[wi], F, px, py, pz, mx, my, mz, &de)) {
+      if (certain_miss32(W.ent32[k], F, px, py, pz, mx, my, mz, &de)) {
         if (fabs_f(de) < band) end_band = true;
         continue;
       }
@@ -96,7 +96,7 @@ __device__ __forceinline__ bool closest_wall_in_subpart32(const PartitionParams&
 
 // Rounds the FP64 end point to FP32 keeping it on the same side of every wall
 // plane of its subpartition that it is within a few ulps of.
-__device__ __noinline__ void round_to_fp32_same_side(const PartitionParams& P, const WallRec64* __restrict__ walls,
+__device__ __forceinline__ void round_to_fp32_same_side(const PartitionParams& P, const WallRec64* __restrict__ walls,
                                                      const uint32_t* __restrict__ off,
                                                      const uint32_t* __restrict__ idx, double px, double py,
                                                      double pz, float* ox, float* oy, float* oz) {

@@ -1,0 +1,165 @@
+// Volume-volume reactions on the GPU: the molecule-collision half of
+// ray_trace_vol (diffuse_react_event.cpp:713-772), collide_mol
+// (collision_utils.inl:464-515), exact_disk (exact_disk_utils.inl:840-1145)
+// and test_bimolecular (rxn_utils.inl:301-414), against a snapshot of the
+// start-of-iteration molecules held in cell-sorted order.
+#pragma once
+#include "path64.h"
+#include "types.h"
+
+namespace mcg {
+
+constexpr int kMaxPathways = 4;
+constexpr int kMaxProducts = 3;
+
+// One bimolecular reaction class (both reactants volume molecules).
+struct RxnRec {
+  double max_fixed_p;
+  double cum_probs[kMaxPathways];
+  float max_p_f;  // rxn_utils.inl:367 holds max_p in a float
+  uint32_t num_pathways;
+  uint32_t reactant_a, reactant_b;
+  uint32_t num_products[kMaxPathways];
+  uint32_t products[kMaxPathways][kMaxProducts];
+};
+
+// Cell grid over the region molecules can occupy: subpartition index box
+// [lo, lo + dim) per replica.  start[] is the CSR over the cell-sorted
+// snapshot, per_replica = dim.x * dim.y * dim.z.
+struct CellGrid {
+  int lo[3];
+  int dim[3];
+  uint32_t per_replica;
+  const uint32_t* __restrict__ start;
+};
+
+__device__ __forceinline__ bool cell_of(const CellGrid& G, int gx, int gy, int gz, uint32_t replica, uint32_t& c) {
+  const int x = gx - G.lo[0], y = gy - G.lo[1], z = gz - G.lo[2];
+  if (x < 0 || y < 0 || z < 0 || x >= G.dim[0] || y >= G.dim[1] || z >= G.dim[2]) return false;
+  c = replica * G.per_replica + (uint32_t)x + (uint32_t)G.dim[0] * ((uint32_t)y + (uint32_t)G.dim[1] * (uint32_t)z);
+  return true;
+}
+
+// Snapshot of the start-of-iteration molecules, cell-sorted (FP64 positions
+// or FP32 positions promoted on load).
+template <typename PosT>
+struct Snapshot {
+  const PosT* __restrict__ x;
+  const PosT* __restrict__ y;
+  const PosT* __restrict__ z;
+  const uint32_t* __restrict__ id;
+  const uint32_t* __restrict__ tag;
+  const uint32_t* __restrict__ consumed;  // nonzero: taken by an accepted reaction this iteration
+};
+
+// A small set of subpartition indices (the reference's SubpartIndicesSet).
+struct SubpartSet {
+  static constexpr int kCap = 48;
+  uint32_t v[kCap];
+  int n;
+  bool overflow;
+  __device__ __forceinline__ void clear() {
+    n = 0;
+    overflow = false;
+  }
+  __device__ __forceinline__ void insert(uint32_t s) {
+    for (int k = 0; k < n; ++k)
+      if (v[k] == s) return;
+    if (n < kCap) v[n++] = s;
+    else overflow = true;
+  }
+};
+
+// collision_utils_subparts.inl:38-122
+__device__ __forceinline__ void collect_neighbors(const PartitionParams& P, D3 pos, const int idx[3], double r,
+                                                  SubpartSet& out) {
+  const double part_len = P.edge;
+  D3 rel = {pos.x - P.ox, pos.y - P.oy, pos.z - P.oz};
+  D3 plus = {rel.x + r, rel.y + r, rel.z + r};
+  D3 minus = {rel.x - r, rel.y - r, rel.z - r};
+  D3 b = {(double)idx[0] * P.sp_len, (double)idx[1] * P.sp_len, (double)idx[2] * P.sp_len};
+  int xd = 0, yd = 0, zd = 0;
+  if (minus.x < b.x && minus.x > 0.0) { out.insert(subpart_of(P, idx[0] - 1, idx[1], idx[2])); xd = -1; }
+  else if (plus.x > b.x + P.sp_len && plus.x < part_len) { out.insert(subpart_of(P, idx[0] + 1, idx[1], idx[2])); xd = 1; }
+  if (minus.y < b.y && minus.y > 0.0) { out.insert(subpart_of(P, idx[0], idx[1] - 1, idx[2])); yd = -1; }
+  else if (plus.y > b.y + P.sp_len && plus.y < part_len) { out.insert(subpart_of(P, idx[0], idx[1] + 1, idx[2])); yd = 1; }
+  if (minus.z < b.z && minus.z > 0.0) { out.insert(subpart_of(P, idx[0], idx[1], idx[2] - 1)); zd = -1; }
+  else if (plus.z > b.z + P.sp_len && plus.z < part_len) { out.insert(subpart_of(P, idx[0], idx[1], idx[2] + 1)); zd = 1; }
+  if (xd && yd) out.insert(subpart_of(P, idx[0] + xd, idx[1] + yd, idx[2]));
+  if (xd && zd) out.insert(subpart_of(P, idx[0] + xd, idx[1], idx[2] + zd));
+  if (yd && zd) out.insert(subpart_of(P, idx[0], idx[1] + yd, idx[2] + zd));
+  if (xd && yd && zd) out.insert(subpart_of(P, idx[0] + xd, idx[1] + yd, idx[2] + zd));
+}
+
+// collect_crossed_subparts with collect_for_molecules (and not for walls):
+// the start subpartition, every subpartition the walk enters, and with the
+// expanded list the neighbours within rxn_radius*sqrt(2) of the start, of
+// each crossing point and of the end point.
+__device__ __forceinline__ void collect_molecule_subparts(const PartitionParams& P, D3 pos, uint32_t start,
+                                                          D3 disp, SubpartSet& out, uint32_t& err) {
+  out.clear();
+  out.insert(start);
+  const double r_nb = P.rxn_radius * 1.41421356238;  // POS_SQRT2, defines.h:148
+  int src[3] = {(int)(start % P.n), (int)((start / P.n) % P.n), (int)((start / P.n2) % P.n)};
+  if (P.use_expanded_list) collect_neighbors(P, pos, src, r_nb, out);
+  SubpartWalk64 walk;
+  walk.init(P, pos, start, disp);
+  while (walk.next(P, err)) {
+    out.insert(walk.cur);
+    if (P.use_expanded_list) collect_neighbors(P, walk.cp, walk.ci, r_nb, out);
+  }
+  if (P.use_expanded_list) {
+    D3 dst = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
+    int d[3];
+    subpart_3d(P, dst, d[0], d[1], d[2]);
+    collect_neighbors(P, dst, d, r_nb, out);
+  }
+  if (out.overflow) err |= ERR_CELLS;
+}
+
+// collide_mol (collision_utils.inl:464-515) without the defunct/self checks
+__device__ __forceinline__ bool collide_mol64(D3 pos, D3 disp, D3 other, double sigma2, double& t, D3& cpos) {
+  D3 dir = {other.x - pos.x, other.y - pos.y, other.z - pos.z};
+  double d = dot3(dir, disp);
+  if (d < 0) return false;
+  double movelen2 = dot3(disp, disp);
+  if (d > movelen2) return false;
+  double dirlen2 = dot3(dir, dir);
+  if (movelen2 * dirlen2 - d * d > movelen2 * sigma2) return false;
+  t = d / movelen2;
+  cpos = {pos.x + t * disp.x, pos.y + t * disp.y, pos.z + t * disp.z};
+  return true;
+}
+
+// rxn_utils.inl:301-320
+__device__ __forceinline__ int binary_search_double(const double* A, double match, int max_idx, double mult) {
+  int min_idx = 0;
+  while (max_idx - min_idx > 1) {
+    int mid = (max_idx + min_idx) / 2;
+    if (match > (A[mid] * mult)) min_idx = mid;
+    else max_idx = mid;
+  }
+  if (match > A[min_idx] * mult) return max_idx;
+  return min_idx;
+}
+
+// test_bimolecular (rxn_utils.inl:336-414) with local_prob_factor == 0;
+// -1 = no reaction.  One uniform draw.
+__device__ __forceinline__ int test_bimolecular(const RxnRec& rx, double scaling, PhiloxStream& rng) {
+  double prob;
+  if (rx.max_fixed_p < scaling) {
+    prob = rng.dbl() * scaling;
+    if (prob >= rx.max_fixed_p) return -1;
+  } else {
+    const float max_p = rx.max_p_f;
+    if (max_p >= scaling) {
+      prob = rng.dbl() * max_p;
+    } else {
+      prob = rng.dbl() * scaling;
+      if (prob >= max_p) return -1;
+    }
+  }
+  return binary_search_double(rx.cum_probs, prob, (int)rx.num_pathways - 1, 1);
+}
+
+}  // namespace mcg

@@ -1,0 +1,226 @@
+// One diffusion step of a volume molecule that can react with volume
+// molecules: the loop of diffuse_vol_molecule (diffuse_react_event.cpp:
+// 367-618) with ray_trace_vol's wall and molecule collisions, processed in
+// time order, against the cell-sorted snapshot of the iteration's start.
+// The first successful reaction test ends the molecule's step with a
+// proposal; whether it happens is decided afterwards (one reactant can be
+// claimed by several movers).
+#pragma once
+#include "displace.h"
+#include "exact_disk.h"
+#include "filter32.h"
+#include "path32.h"
+#include "react.h"
+
+namespace mcg {
+
+struct ReactParams {
+  PartitionParams P;
+  const WallRec64* __restrict__ walls;
+  const uint32_t* __restrict__ off;
+  const uint32_t* __restrict__ idx;
+  const FilterWall* __restrict__ ent32;  // FP32 records in list order (FP32 form)
+  CellGrid G;
+  const RxnRec* __restrict__ rxns;
+  const int32_t* __restrict__ pair_rxn;  // [ns * ns], -1 = none
+  uint32_t num_species;
+  uint64_t seed;
+  uint32_t iteration;
+  FilterParams F;
+  float band;
+};
+
+struct Proposal {
+  double time;    // absolute time of the collision
+  double x, y, z; // collision point
+  uint32_t mover_slot, partner_slot;
+  uint32_t mover_id, partner_id;
+  uint32_t rxn, pathway;
+  uint32_t replica, pad;
+};
+
+// Closest wall in one subpartition, exact FP64 or FP32-filtered.
+template <bool kFilter32>
+__device__ __forceinline__ bool closest_wall_any(const ReactParams& R, uint32_t s, uint32_t last_hit, D3 pos,
+                                                 D3& disp, PhiloxStream& rng, WallHit& hit, uint32_t& err,
+                                                 bool& end_band) {
+  if (kFilter32) {
+    Walls32View W{R.walls, nullptr, R.off, R.idx, R.ent32};
+    return closest_wall_in_subpart32(R.P, W, R.F, s, last_hit, pos, disp, rng, hit, err, R.band, end_band);
+  }
+  return closest_wall_in_subpart64(R.P, R.walls, R.off, R.idx, s, last_hit, pos, disp, rng, hit, err);
+}
+
+// The next molecule collision after key (t_last, id_last) in time order
+// (ties: larger partner id first, diffuse_react_event.cpp:354-358), strictly
+// before t_limit.  Scans the partner lists of the collected subpartitions.
+template <typename PosT>
+__device__ __forceinline__ bool next_collision(const ReactParams& R, const Snapshot<PosT>& S, const SubpartSet& set,
+                                               uint32_t replica, uint32_t my_species, uint32_t my_id, D3 pos,
+                                               D3 disp, double t_limit, double t_last, uint32_t id_last,
+                                               bool first, double& t_best, uint32_t& slot_best, D3& pos_best,
+                                               int32_t& rxn_best) {
+  const double sigma2 = R.P.rxn_radius * R.P.rxn_radius;
+  bool found = false;
+  uint32_t id_best = 0;
+  for (int k = 0; k < set.n; ++k) {
+    const uint32_t s = set.v[k];
+    const int gx = (int)(s % R.P.n), gy = (int)((s / R.P.n) % R.P.n), gz = (int)((s / R.P.n2) % R.P.n);
+    uint32_t c;
+    if (!cell_of(R.G, gx, gy, gz, replica, c)) continue;  // no molecules outside the grid
+    const uint32_t j1 = R.G.start[c + 1];
+    for (uint32_t j = R.G.start[c]; j < j1; ++j) {
+      const int32_t rx = R.pair_rxn[my_species * R.num_species + tag_species(S.tag[j])];
+      if (rx < 0) continue;
+      double t;
+      D3 cp;
+      if (!collide_mol64(pos, disp, D3{(double)S.x[j], (double)S.y[j], (double)S.z[j]}, sigma2, t, cp)) continue;
+      if (!(t < t_limit)) continue;
+      const uint32_t pid = S.id[j];
+      if (pid == my_id || S.consumed[j]) continue;
+      // strictly after the last processed key
+      if (!first && !(t > t_last || (t == t_last && pid < id_last))) continue;
+      if (!found || t < t_best || (t == t_best && pid > id_best)) {
+        found = true;
+        t_best = t;
+        id_best = pid;
+        slot_best = j;
+        pos_best = cp;
+        rxn_best = rx;
+      }
+    }
+  }
+  return found;
+}
+
+// Result of one mover's step.
+struct MoverEnd {
+  D3 pos;          // end position; with a proposal, the start of the segment it was found in
+  double t_steps;  // time the step covered (units of the time step), before reflections
+  bool near_plane; // FP32 form: the end point lies near a wall plane
+};
+
+// The step of one mover (diffuse_vol_molecule, diffuse_react_event.cpp:367-
+// 618) against the snapshot: displacement clipped to max_time = barrier -
+// toff, ray segments with wall reflections, and for species with volume
+// reactions the molecule collisions of each segment in time order, each
+// reaction test drawing from the mover's stream.  The first successful test
+// ends the step with a proposal (return true).
+template <typename PosT>
+__device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot<PosT>& S, const SpeciesRec& sp,
+                                            const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
+                                            double barrier, Proposal& prop, MoverEnd& out, uint32_t& err) {
+  constexpr bool kF32 = sizeof(PosT) == 4;  // FP32 positions: FP32 wall filter
+  const uint32_t rep = tag_replica(tag), spc = tag_species(tag);
+  PhiloxStream rng;
+  rng.init(R.seed, idv, R.iteration, rep, 0);
+  double t_steps;
+  D3 disp;
+  const double r_rate_factor = clipped_displacement(rng, T, sp.space_step, sp.time_step, barrier - toff, t_steps, disp);
+  out.t_steps = t_steps;
+  out.near_plane = false;
+  uint32_t last_hit = kInvalid;
+  double elapsed = (double)R.iteration + toff;
+  const bool can_react = (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY);
+  SubpartSet set;
+  int seg;
+  for (seg = 0; seg < kMaxSegments; ++seg) {
+    bool near_plane = false;
+    const D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
+    const D3 part_disp = in_partition(R.P, end) ? disp : displacement_up_to_partition_boundary(R.P, pos, disp);
+    int sx, sy, sz;
+    subpart_3d(R.P, pos, sx, sy, sz);
+    if (sx < 0 || sy < 0 || sz < 0 || sx >= (int)R.P.n || sy >= (int)R.P.n || sz >= (int)R.P.n) {
+      err |= ERR_ESCAPED;
+      out.pos = pos;
+      return false;
+    }
+    const uint32_t start = subpart_of(R.P, sx, sy, sz);
+    const D3 dst = {pos.x + part_disp.x, pos.y + part_disp.y, pos.z + part_disp.z};
+    int dx, dy, dz;
+    subpart_3d(R.P, dst, dx, dy, dz);
+    const uint32_t dest = subpart_of(R.P, dx, dy, dz);
+    WallHit hit;
+    uint32_t hit_sp = start;
+    bool found = closest_wall_any<kF32>(R, start, last_hit, pos, disp, rng, hit, err, near_plane);
+    if (!found && dest != start) {
+      SubpartWalk64 walk;
+      walk.init(R.P, pos, start, part_disp);
+      while (walk.next(R.P, err)) {
+        if (closest_wall_any<kF32>(R, walk.cur, last_hit, pos, disp, rng, hit, err, near_plane)) {
+          found = true;
+          hit_sp = walk.cur;
+          break;
+        }
+      }
+    }
+    if (can_react) {
+      // ray_trace_vol (diffuse_react_event.cpp:735-772): the subpartitions of
+      // the move up to the wall hit (the whole move when the hit is in the
+      // destination subpartition), then every partner registered in them
+      const D3 mol_disp = (found && hit_sp != dest) ? D3{hit.pos.x - pos.x, hit.pos.y - pos.y, hit.pos.z - pos.z}
+                                                    : part_disp;
+      collect_molecule_subparts(R.P, pos, start, mol_disp, set, err);
+      const double t_limit = found ? hit.t : 1e300;
+      double t_last = 0;
+      uint32_t id_last = 0;
+      bool first = true;
+      for (int guard = 0; guard < 4096; ++guard) {
+        double tc;
+        uint32_t slot;
+        D3 cp;
+        int32_t rx;
+        if (!next_collision(R, S, set, rep, spc, idv, pos, disp, t_limit, t_last, id_last, first, tc, slot, cp, rx))
+          break;
+        first = false;
+        t_last = tc;
+        id_last = S.id[slot];
+        if (tc < STIME_EPS) continue;  // is_immediate_collision (collision_utils.inl:814-816)
+        bool full = false;
+        const D3 target = {(double)S.x[slot], (double)S.y[slot], (double)S.z[slot]};
+        const double factor = exact_disk_dev(R.P, R.walls, R.off, R.idx, cp, disp, R.P.rxn_radius, target, &full);
+        if (full) err |= ERR_EXD;
+        if (factor < 0) continue;  // the disk is occluded
+        const int pw = test_bimolecular(R.rxns[rx], factor * r_rate_factor, rng);
+        if (pw >= 0) {
+          prop.time = elapsed + t_steps * tc;
+          prop.x = cp.x;
+          prop.y = cp.y;
+          prop.z = cp.z;
+          prop.partner_slot = slot;
+          prop.mover_id = idv;
+          prop.partner_id = id_last;
+          prop.rxn = (uint32_t)rx;
+          prop.pathway = (uint32_t)pw;
+          prop.replica = rep;
+          // ray_trace_vol commits the move before the collisions are
+          // processed when the segment hit no wall (diffuse_react_event.cpp:773-777)
+          out.pos = found ? pos : D3{pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
+          return true;
+        }
+      }
+    }
+    if (!found) {
+      // disp as the wall search left it (an edge re-draw may have moved it)
+      pos = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
+      out.near_plane = near_plane;
+      break;
+    }
+    // reflect_from_wall (collision_utils.inl:1711-1747)
+    elapsed += t_steps * hit.t;
+    const WallRec64& w = R.walls[hit.wall];
+    const double t_reflect = hit.t;
+    pos = hit.pos;
+    t_steps *= (1.0 - t_reflect);
+    last_hit = hit.wall;
+    const double rf = -2.0 * dot3(disp, d3(w.nx, w.ny, w.nz));
+    const double s = 1.0 - t_reflect;
+    disp = {(disp.x + rf * w.nx) * s, (disp.y + rf * w.ny) * s, (disp.z + rf * w.nz) * s};
+  }
+  if (seg == kMaxSegments) err |= ERR_SEGMENTS;
+  if (!in_partition(R.P, pos)) err |= ERR_ESCAPED;
+  out.pos = pos;
+  return false;
+}
+
+}  // namespace mcg

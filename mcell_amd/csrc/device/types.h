@@ -73,6 +73,10 @@ enum : uint32_t {
   ERR_REDO = 1u << 2,        // more than kMaxRedo edge re-draws for one wall search
   ERR_DDA = 1u << 3,         // subpartition walk did not terminate
   ERR_TRACE = 1u << 4,       // hit trace buffer overflow (trace only)
+  ERR_CELLS = 1u << 5,       // more subpartitions around one move than the collision set holds
+  ERR_GRID = 1u << 6,        // a reactive molecule outside the cell grid
+  ERR_EXD = 1u << 7,         // exact_disk vertex pool exhausted
+  ERR_QUEUE = 1u << 8,       // proposal / product capacity exceeded
 };
 
 constexpr int kMaxSegments = 1024;

@@ -7,13 +7,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mcg.h"
+#include "engine.h"
 #include "geometry.h"
-#include "kernels.h"
 
 using namespace mcg;
 
-namespace {
+namespace mcg {
 
 thread_local std::string g_err;
 
@@ -22,79 +21,16 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                          \
-  do {                                                                                         \
-    hipError_t e_ = (expr);                                                                    \
-    if (e_ != hipSuccess) return fail(MCG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+}  // namespace mcg
 
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  hipError_t alloc(size_t count) {
-    release();
-    if (count == 0) return hipSuccess;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
-    if (e == hipSuccess) n = count;
-    else p = nullptr;
-    return e;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-};
-
-}  // namespace
-
-struct mcg_ctx {
-  mcg_config cfg{};
-  PartitionParams P{};
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
-  // geometry
-  std::vector<WallRec64> walls_host;
-  std::vector<uint32_t> off_host, idx_host;
-  DevBuf<WallRec64> walls64;
-  DevBuf<WallRec32> walls32;
-  DevBuf<uint32_t> off, idx;
-  DevBuf<WallRec64> ent64;  // wall records in list (entry) order
-  DevBuf<WallRec32> ent32;
-  bool have_geometry = false;
-
-  // species
-  DevBuf<SpeciesRec> species;
-  uint32_t num_species = 0;
-
-  // molecules (structure of arrays): FP64 or FP32 positions by precision
-  uint64_t n = 0;
-  DevBuf<double> x, y, z;
-  DevBuf<float> xf, yf, zf;
-  DevBuf<uint32_t> id, tag;
-  float max_abs_coord = 0;
-  bool fp32() const { return cfg.precision == MCG_PREC_FP32_FALLBACK; }
-
-  // diagnostics
-  DevBuf<uint32_t> err;
-  DevBuf<uint32_t> queue, qcount;  // hard-path queue of molecule indices
-  DevBuf<uint32_t> trace_wall, trace_count;
-  double last_kernel_ms = 0;
-  uint32_t last_launches = 0;
-
-  ~mcg_ctx() {
-    walls64.release(); walls32.release(); off.release(); idx.release(); species.release();
-    ent64.release(); ent32.release();
-    x.release(); y.release(); z.release(); xf.release(); yf.release(); zf.release();
-    id.release(); tag.release(); err.release(); queue.release(); qcount.release();
-    trace_wall.release(); trace_count.release();
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
+mcg_ctx::~mcg_ctx() {
+  if (rx.hdr_host) (void)hipHostFree(rx.hdr_host);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (evm0) (void)hipEventDestroy(evm0);
+  if (evm1) (void)hipEventDestroy(evm1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
 
 extern "C" {
 
@@ -134,7 +70,8 @@ int mcg_create(const mcg_config* cfg, mcg_ctx** out) {
   };
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess)
+      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+      (e = hipEventCreate(&c->evm0)) != hipSuccess || (e = hipEventCreate(&c->evm1)) != hipSuccess)
     return cleanup_fail(fail(MCG_E_HIP, std::string("stream/event creation: ") + hipGetErrorString(e)));
   size_t cap = cfg->molecule_capacity;
   bool pos_fail = c->fp32() ? (c->xf.alloc(cap) || c->yf.alloc(cap) || c->zf.alloc(cap))
@@ -176,6 +113,8 @@ int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint
     if (!(v[0] >= P.ox && v[1] >= P.oy && v[2] >= P.oz && v[0] < P.qx && v[1] < P.qy && v[2] < P.qz))
       return fail(MCG_E_INVALID, "vertex outside the partition");
   }
+  c->vert_host.assign(vertices, vertices + 3 * (size_t)nv);
+  c->rx.grid_dirty = true;
   c->walls_host.resize(nw);
   for (uint32_t i = 0; i < nw; ++i)
     c->walls_host[i] = wall_constants(vertices + 3 * walls[3 * i], vertices + 3 * walls[3 * i + 1],
@@ -213,15 +152,18 @@ int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
   if (!c || !sp || ns == 0 || ns > 256) return fail(MCG_E_INVALID, "bad species table");
   std::vector<SpeciesRec> h(ns);
   for (uint32_t i = 0; i < ns; ++i) {
-    if (sp[i].flags & (MCG_SPF_CAN_VOLVOL | MCG_SPF_CAN_VOLWALL | MCG_SPF_CAN_VOLSURF))
-      return fail(MCG_E_UNSUPPORTED, "reactive species are not on this path yet");
+    if (sp[i].flags & MCG_SPF_CAN_VOLSURF)
+      return fail(MCG_E_UNSUPPORTED, "volume-surface reactions are not on this path yet");
+    if (!(sp[i].space_step >= 0)) return fail(MCG_E_INVALID, "negative space step");
     if (!(sp[i].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
     h[i].space_step = sp[i].space_step;
     h[i].time_step = sp[i].time_step;
     h[i].flags = sp[i].flags;
   }
+  if (c->rx.on) return fail(MCG_E_STATE, "species must be set before the reaction classes");
   if (c->species.alloc(ns)) return fail(MCG_E_NOMEM, "species buffer");
   HIP_TRY(hipMemcpy(c->species.p, h.data(), ns * sizeof(SpeciesRec), hipMemcpyHostToDevice));
+  c->species_host = h;
   c->num_species = ns;
   return MCG_OK;
 }
@@ -241,6 +183,25 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
     if (species[i] >= c->num_species) return fail(MCG_E_INVALID, "species id out of range");
     hx[i] = p[0]; hy[i] = p[1]; hz[i] = p[2];
     ht[i] = make_tag(species[i], 0, replica);
+  }
+  if (c->rx.on && count) {
+    // the device holds the id counters once steps have run
+    if (!c->rx.next_id_dirty)
+      HIP_TRY(hipMemcpy(c->rx.next_id_host.data(), c->rx.next_id.p, c->rx.next_id_host.size() * 4,
+                        hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemsetAsync(c->rx.toff.p + c->n, 0, count * sizeof(double), c->stream));
+  }
+  if (c->rx.next_id_host.size() != c->cfg.num_replicas) c->rx.next_id_host.assign(c->cfg.num_replicas, 0);
+  for (uint64_t i = 0; i < count; ++i) {
+    c->rx.next_id_host[replica] = std::max(c->rx.next_id_host[replica], ids[i] + 1);
+    for (int a = 0; a < 3; ++a) {
+      c->mol_lo[a] = std::min(c->mol_lo[a], pos[3 * i + a]);
+      c->mol_hi[a] = std::max(c->mol_hi[a], pos[3 * i + a]);
+    }
+  }
+  if (count) {
+    c->rx.next_id_dirty = true;
+    c->rx.grid_dirty = true;
   }
   std::vector<float> fx, fy, fz;
   if (c->fp32()) {
@@ -273,6 +234,7 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
 int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats) {
   if (!c) return fail(MCG_E_INVALID, "null context");
   if (!c->have_geometry || c->num_species == 0) return fail(MCG_E_STATE, "geometry and species must be set first");
+  if (c->rx.on) return react_step(c, first_iteration, num_iterations, stats);
   auto t0 = std::chrono::steady_clock::now();
   MolView64 m{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p};
   MolView32 mf{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p};
@@ -327,18 +289,25 @@ int mcg_sync(mcg_ctx* c) {
 
 int mcg_num_molecules(mcg_ctx* c, uint64_t* count) {
   if (!c || !count) return fail(MCG_E_INVALID, "null argument");
-  *count = c->n;
+  if (!c->rx.on || c->num_species == 0) {
+    *count = c->n;
+    return MCG_OK;
+  }
+  std::vector<uint64_t> counts((size_t)c->cfg.num_replicas * c->num_species);
+  int st = count_species(c, counts.data());
+  if (st != MCG_OK) return st;
+  uint64_t t = 0;
+  for (uint64_t v : counts) t += v;
+  *count = t;
   return MCG_OK;
 }
 
 int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, double* pos,
                              uint32_t* subparts, uint64_t capacity, uint64_t* count) {
   if (!c) return fail(MCG_E_INVALID, "null context");
-  uint64_t n = c->n;
-  if (count) *count = n;
-  if (capacity < n) return fail(MCG_E_CAPACITY, "output capacity too small");
+  const uint64_t n = c->n;
   std::vector<double> hx(n), hy(n), hz(n);
-  std::vector<uint32_t> ht(n);
+  std::vector<uint32_t> ht(n), hid(n);
   if (n && c->fp32()) {
     std::vector<float> fx(n), fy(n), fz(n);
     HIP_TRY(hipMemcpyAsync(fx.data(), c->xf.p, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -354,19 +323,27 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
       HIP_TRY(hipMemcpyAsync(hz.data(), c->z.p, n * 8, hipMemcpyDeviceToHost, c->stream));
     }
     HIP_TRY(hipMemcpyAsync(ht.data(), c->tag.p, n * 4, hipMemcpyDeviceToHost, c->stream));
-    if (ids) HIP_TRY(hipMemcpyAsync(ids, c->id.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hid.data(), c->id.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
+  uint64_t live = 0;
+  for (uint64_t i = 0; i < n; ++i) live += (tag_flags(ht[i]) & MF_DEFUNCT) ? 0 : 1;
+  if (count) *count = live;
+  if (capacity < live) return fail(MCG_E_CAPACITY, "output capacity too small");
   const PartitionParams& P = c->P;
+  uint64_t k = 0;
   for (uint64_t i = 0; i < n; ++i) {
-    if (species) species[i] = tag_species(ht[i]);
-    if (replica) replica[i] = tag_replica(ht[i]);
-    if (pos) { pos[3 * i] = hx[i]; pos[3 * i + 1] = hy[i]; pos[3 * i + 2] = hz[i]; }
+    if (tag_flags(ht[i]) & MF_DEFUNCT) continue;
+    if (ids) ids[k] = hid[i];
+    if (species) species[k] = tag_species(ht[i]);
+    if (replica) replica[k] = tag_replica(ht[i]);
+    if (pos) { pos[3 * k] = hx[i]; pos[3 * k + 1] = hy[i]; pos[3 * k + 2] = hz[i]; }
     if (subparts) {
       int ix = (int)((hx[i] - P.ox) * P.sp_rcp), iy = (int)((hy[i] - P.oy) * P.sp_rcp),
           iz = (int)((hz[i] - P.oz) * P.sp_rcp);
-      subparts[i] = (uint32_t)ix + (uint32_t)iy * P.n + (uint32_t)iz * P.n2;
+      subparts[k] = (uint32_t)ix + (uint32_t)iy * P.n + (uint32_t)iz * P.n2;
     }
+    ++k;
   }
   return MCG_OK;
 }
@@ -389,6 +366,7 @@ int mcg_get_walls(mcg_ctx* c, double* wc, uint32_t* offsets, uint32_t* indices, 
 int mcg_get_hit_trace(mcg_ctx* c, uint32_t* walls, uint32_t* counts, uint64_t capacity) {
   if (!c) return fail(MCG_E_INVALID, "null context");
   if (!(c->cfg.flags & MCG_FLAG_TRACE_HITS)) return fail(MCG_E_STATE, "created without MCG_FLAG_TRACE_HITS");
+  if (c->rx.on) return fail(MCG_E_UNSUPPORTED, "hit traces are recorded without reaction classes only");
   if (capacity < c->n) return fail(MCG_E_CAPACITY, "output capacity too small");
   if (c->n) {
     if (walls) HIP_TRY(hipMemcpy(walls, c->trace_wall.p, c->n * kTraceHits * 4, hipMemcpyDeviceToHost));

@@ -1,0 +1,130 @@
+// The engine context behind the C ABI (include/mcg.h); internal to libmcg.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mcg.h"
+#include "kernels.h"
+#include "kernels_react.h"
+
+namespace mcg {
+
+// Records the error text for mcg_last_error and returns `code`.
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) return ::mcg::fail(MCG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// Buffers of the reactive path (allocated by mcg_set_bimol_rxns).
+struct ReactState {
+  bool on = false;
+  std::vector<RxnRec> rxns;
+  std::vector<int32_t> pair;  // species x species -> rxn index or -1
+  DevBuf<RxnRec> rxns_d;
+  DevBuf<int32_t> pair_d;
+  DevBuf<double> toff;                       // molecules' diffusion-time offsets
+  DevBuf<double> sx, sy, sz;                 // snapshot positions (FP64 form)
+  DevBuf<float> sxf, syf, szf;               // snapshot positions (FP32 form)
+  DevBuf<uint32_t> sid, stag;
+  DevBuf<double> stoff;
+  DevBuf<uint32_t> key, rank, consumed, owner;
+  DevBuf<Proposal> prop;
+  DevBuf<uint32_t> plist, alist, alist_sorted, rlist0, rlist1, mlist;
+  DevBuf<uint64_t> keys_a, keys_b;
+  DevBuf<uint32_t> cnt, pre, head, next_id;
+  DevBuf<uint32_t> hdr;
+  DevBuf<uint32_t> cell_count, cell_start;
+  DevBuf<unsigned char> scan_tmp;
+  CellGrid grid{};
+  uint32_t num_cells = 0;
+  bool grid_dirty = true;
+  std::vector<uint32_t> next_id_host;  // per replica: 1 + largest id seen
+  bool next_id_dirty = true;
+  uint32_t* hdr_host = nullptr;        // pinned mirror of hdr
+  // reaction log of the last mcg_step call
+  DevBuf<RxnEvent> log;
+  uint64_t log_cap = 0;
+  uint64_t log_count = 0;
+};
+
+}  // namespace mcg
+
+struct mcg_ctx {
+  mcg_config cfg{};
+  mcg::PartitionParams P{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evm0 = nullptr, evm1 = nullptr;
+
+  // geometry
+  std::vector<mcg::WallRec64> walls_host;
+  std::vector<uint32_t> off_host, idx_host;
+  std::vector<double> vert_host;
+  mcg::DevBuf<mcg::WallRec64> walls64;
+  mcg::DevBuf<mcg::WallRec32> walls32;
+  mcg::DevBuf<uint32_t> off, idx;
+  mcg::DevBuf<mcg::WallRec64> ent64;  // wall records in list (entry) order
+  mcg::DevBuf<mcg::WallRec32> ent32;
+  bool have_geometry = false;
+
+  // species
+  mcg::DevBuf<mcg::SpeciesRec> species;
+  std::vector<mcg::SpeciesRec> species_host;
+  uint32_t num_species = 0;
+
+  // molecules (structure of arrays): FP64 or FP32 positions by precision.
+  // n counts array slots; with reactions on, consumed molecules stay as
+  // defunct slots until the next iteration's sort drops them.
+  uint64_t n = 0;
+  mcg::DevBuf<double> x, y, z;
+  mcg::DevBuf<float> xf, yf, zf;
+  mcg::DevBuf<uint32_t> id, tag;
+  float max_abs_coord = 0;
+  bool fp32() const { return cfg.precision == MCG_PREC_FP32_FALLBACK; }
+  double mol_lo[3] = {1e300, 1e300, 1e300}, mol_hi[3] = {-1e300, -1e300, -1e300};
+
+  // diagnostics
+  mcg::DevBuf<uint32_t> err;
+  mcg::DevBuf<uint32_t> queue, qcount;  // hard-path queue of molecule indices
+  mcg::DevBuf<uint32_t> trace_wall, trace_count;
+  double last_kernel_ms = 0;
+  uint32_t last_launches = 0;
+
+  mcg::ReactState rx;
+
+  ~mcg_ctx();
+};
+
+namespace mcg {
+// The reactive iteration loop (engine_react.cpp).
+int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats);
+// Live molecules per (replica, species) into counts (size num_replicas * num_species).
+int count_species(mcg_ctx* c, uint64_t* counts);
+}  // namespace mcg

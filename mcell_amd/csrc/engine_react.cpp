@@ -1,0 +1,364 @@
+// Volume-volume reactions: the reaction-class ABI and the host loop of a
+// reactive iteration (kernels in kernels_react.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "engine.h"
+
+using namespace mcg;
+
+namespace mcg {
+
+namespace {
+
+// Box of subpartitions that reactive molecules may occupy: the geometry's
+// and the added molecules' bounding box, one subpartition of margin.
+void setup_grid(mcg_ctx* c) {
+  const PartitionParams& P = c->P;
+  double lo[3] = {c->mol_lo[0], c->mol_lo[1], c->mol_lo[2]};
+  double hi[3] = {c->mol_hi[0], c->mol_hi[1], c->mol_hi[2]};
+  for (size_t v = 0; v + 2 < c->vert_host.size(); v += 3)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], c->vert_host[v + a]);
+      hi[a] = std::max(hi[a], c->vert_host[v + a]);
+    }
+  const double o[3] = {P.ox, P.oy, P.oz};
+  CellGrid& G = c->rx.grid;
+  for (int a = 0; a < 3; ++a) {
+    int l = 0, h = (int)P.n - 1;
+    if (lo[a] <= hi[a]) {
+      l = std::max(0, (int)((lo[a] - o[a]) * P.sp_rcp) - 1);
+      h = std::min((int)P.n - 1, (int)((hi[a] - o[a]) * P.sp_rcp) + 1);
+    }
+    G.lo[a] = l;
+    G.dim[a] = h - l + 1;
+  }
+  G.per_replica = (uint32_t)G.dim[0] * (uint32_t)G.dim[1] * (uint32_t)G.dim[2];
+  c->rx.num_cells = G.per_replica * c->cfg.num_replicas;
+}
+
+template <typename PosT>
+ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* sx, PosT* sy, PosT* sz) {
+  ReactState& r = c->rx;
+  ReactPipeline<PosT> p{};
+  p.R.P = c->P;
+  p.R.walls = c->walls64.p;
+  p.R.off = c->off.p;
+  p.R.idx = c->idx.p;
+  p.R.ent32 = reinterpret_cast<const FilterWall*>(c->ent32.p);
+  p.R.G = r.grid;
+  p.R.G.start = r.cell_start.p;
+  p.R.rxns = r.rxns_d.p;
+  p.R.pair_rxn = r.pair_d.p;
+  p.R.num_species = c->num_species;
+  p.R.seed = c->cfg.seed;
+  p.R.F = make_filter_params(c->max_abs_coord);
+  p.R.band = 2.0f * p.R.F.e_dd + 8.0f * p.R.F.eps * c->max_abs_coord;
+  p.species = c->species.p;
+  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p};
+  p.Sm = RMolView<PosT>{sx, sy, sz, r.sid.p, r.stag.p, r.stoff.p};
+  p.key = r.key.p;
+  p.rank = r.rank.p;
+  p.cell_count = r.cell_count.p;
+  p.cell_start = r.cell_start.p;
+  p.consumed = r.consumed.p;
+  p.owner = r.owner.p;
+  p.prop = r.prop.p;
+  p.plist = r.plist.p;
+  p.alist = r.alist.p;
+  p.alist_sorted = r.alist_sorted.p;
+  p.rlist0 = r.rlist0.p;
+  p.rlist1 = r.rlist1.p;
+  p.mlist = r.mlist.p;
+  p.keys_a = r.keys_a.p;
+  p.keys_b = r.keys_b.p;
+  p.cnt = r.cnt.p;
+  p.pre = r.pre.p;
+  p.head = r.head.p;
+  p.next_id = r.next_id.p;
+  p.hdr = r.hdr.p;
+  p.scan_tmp = r.scan_tmp.p;
+  p.scan_tmp_bytes = r.scan_tmp.n;
+  return p;
+}
+
+int read_hdr(mcg_ctx* c) {
+  HIP_TRY(hipMemcpyAsync(c->rx.hdr_host, c->rx.hdr.p, kHdrWords * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCG_OK;
+}
+
+// Claims, acceptance and repeated moves until every proposal of the phase
+// is settled.
+template <typename PosT>
+int settle_rounds(mcg_ctx* c, ReactPipeline<PosT>& p, bool snapshot_src, double barrier, uint32_t* err,
+                  uint32_t& rounds) {
+  uint32_t* h = c->rx.hdr_host;
+  for (uint32_t r = 0;; ++r) {
+    int st = read_hdr(c);
+    if (st) return st;
+    const uint32_t np = h[kHdrProposals];
+    if (np == 0) return MCG_OK;
+    if (r > 100000) return fail(MCG_E_LIMIT, "reaction settling did not converge");
+    uint32_t* rl = (r & 1) ? p.rlist1 : p.rlist0;
+    uint32_t* rc = p.hdr + ((r & 1) ? kHdrRerun1 : kHdrRerun0);
+    HIP_TRY(p.settle(c->stream, np, rl, rc));
+    st = read_hdr(c);
+    if (st) return st;
+    const uint32_t nr = h[(r & 1) ? kHdrRerun1 : kHdrRerun0];
+    ++rounds;
+    if (nr == 0) return MCG_OK;
+    HIP_TRY(p.move(c->stream, snapshot_src, rl, rc, nr, barrier, false, err, nullptr));
+  }
+}
+
+template <typename PosT>
+int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t num, mcg_step_stats* stats,
+                   double& move_ms) {
+  ReactState& r = c->rx;
+  uint32_t* h = r.hdr_host;
+  const bool propose_only = (c->cfg.flags & MCG_FLAG_PROPOSE_ONLY) != 0;
+  const uint64_t cap = c->cfg.molecule_capacity;
+  uint64_t reactions = 0, steps = 0;
+  uint32_t rounds = 0;
+  r.log_count = 0;
+  HIP_TRY(launch_fill_u32(c->stream, p.hdr, 0u, kHdrWords));
+  move_ms = 0;
+  for (uint32_t k = 0; k < num; ++k) {
+    const uint32_t it = first + k;
+    // Calendar::get_time_up_to_next_barrier (scheduler.cpp:212-242)
+    const uint64_t b = c->cfg.barrier_interval ? c->cfg.barrier_interval : 1;
+    const uint64_t next_barrier = ((uint64_t)it / b + 1) * b;
+    const double barrier = std::min((double)(next_barrier - it), 100.0);
+    p.R.iteration = it;
+    const uint32_t n_in = (uint32_t)c->n;
+    HIP_TRY(p.sort(c->stream, n_in, r.num_cells, c->err.p));
+    HIP_TRY(launch_copy_u32(c->stream, p.cell_start + r.num_cells, p.hdr + kHdrNumOut));
+    HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
+    HIP_TRY(hipEventRecord(c->evm0, c->stream));
+    HIP_TRY(p.move(c->stream, true, nullptr, p.cell_start + r.num_cells, n_in, barrier, propose_only, c->err.p,
+                   p.hdr + kHdrMoved));
+    HIP_TRY(hipEventRecord(c->evm1, c->stream));
+    if (propose_only) {
+      HIP_TRY(p.log_proposals(c->stream, n_in, it, r.log.p, p.hdr + kHdrLog, (uint32_t)r.log_cap));
+    } else {
+      bool snapshot_src = true;
+      for (;;) {
+        int st = settle_rounds(c, p, snapshot_src, barrier, c->err.p, rounds);
+        if (st) return st;
+        const uint32_t na = h[kHdrAccepted];
+        if (na == 0) break;
+        reactions += na;
+        const uint64_t log_base = std::min<uint64_t>(r.log_count, 0xFFFFFFFFull);
+        HIP_TRY(p.products(c->stream, na, (uint32_t)cap, r.log.p, (uint32_t)log_base, (uint32_t)r.log_cap,
+                           c->err.p));
+        r.log_count += na;
+        HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
+        st = read_hdr(c);
+        if (st) return st;
+        const uint32_t nm = h[kHdrProductMovers];
+        if (nm == 0) break;
+        snapshot_src = false;
+        HIP_TRY(p.move(c->stream, false, p.mlist, p.hdr + kHdrProductMovers, nm, barrier, false, c->err.p,
+                       p.hdr + kHdrMoved));
+      }
+    }
+    int st = read_hdr(c);
+    if (st) return st;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->evm0, c->evm1));
+    move_ms += ms;
+    c->n = h[kHdrNumOut];
+    steps += h[kHdrMoved];
+    HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrMoved, 0u, 1u));
+    uint32_t e = 0;
+    HIP_TRY(hipMemcpy(&e, c->err.p, 4, hipMemcpyDeviceToHost));
+    if (e) {
+      HIP_TRY(hipMemset(c->err.p, 0, 4));
+      if (e & ERR_ESCAPED) return fail(MCG_E_ESCAPED, "a molecule escaped the partition");
+      if (e & ERR_GRID) return fail(MCG_E_LIMIT, "a molecule left the subpartition box of the reaction grid");
+      if (e & ERR_QUEUE) return fail(MCG_E_CAPACITY, "molecule capacity exceeded by reaction products");
+      return fail(MCG_E_LIMIT, "per-step limit exceeded (error bits " + std::to_string(e) + ")");
+    }
+  }
+  if (propose_only) r.log_count = h[kHdrLog];
+  if (stats) {
+    stats->molecule_steps = steps;
+    stats->reactions = reactions;
+    stats->slow_path_last = rounds;
+  }
+  return MCG_OK;
+}
+
+}  // namespace
+
+int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats) {
+  if (c->cfg.flags & MCG_FLAG_TRACE_HITS) return fail(MCG_E_UNSUPPORTED, "hit traces need a context without reactions");
+  ReactState& r = c->rx;
+  auto t0 = std::chrono::steady_clock::now();
+  if (r.grid_dirty) {
+    setup_grid(c);
+    if (r.cell_count.alloc((size_t)r.num_cells + 1) || r.cell_start.alloc((size_t)r.num_cells + 1))
+      return fail(MCG_E_NOMEM, "cell grid");
+    size_t tmp = ReactPipeline<double>::scan_bytes(r.num_cells, (uint32_t)c->cfg.molecule_capacity);
+    if (tmp > r.scan_tmp.n && r.scan_tmp.alloc(tmp)) return fail(MCG_E_NOMEM, "scan scratch");
+    r.grid_dirty = false;
+  }
+  if (r.next_id_dirty) {
+    HIP_TRY(hipMemcpy(r.next_id.p, r.next_id_host.data(), r.next_id_host.size() * 4, hipMemcpyHostToDevice));
+    r.next_id_dirty = false;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  double move_ms = 0;
+  int st;
+  if (c->fp32()) {
+    auto p = make_pipeline<float>(c, c->xf.p, c->yf.p, c->zf.p, r.sxf.p, r.syf.p, r.szf.p);
+    st = run_iterations(c, p, first_iteration, num_iterations, stats, move_ms);
+  } else {
+    auto p = make_pipeline<double>(c, c->x.p, c->y.p, c->z.p, r.sx.p, r.sy.p, r.sz.p);
+    st = run_iterations(c, p, first_iteration, num_iterations, stats, move_ms);
+  }
+  if (st) return st;
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_kernel_ms = num_iterations ? move_ms / num_iterations : 0;
+  c->last_launches = num_iterations;
+  if (stats) {
+    stats->kernel_ms = ms;
+    stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats->kernel_launches = num_iterations;
+  }
+  return MCG_OK;
+}
+
+int count_species(mcg_ctx* c, uint64_t* counts) {
+  const size_t nc = (size_t)c->cfg.num_replicas * c->num_species;
+  DevBuf<uint64_t> d;
+  if (d.alloc(std::max<size_t>(nc, 1))) return fail(MCG_E_NOMEM, "count buffer");
+  HIP_TRY(hipMemsetAsync(d.p, 0, nc * 8, c->stream));
+  if (c->n) {
+    if (c->fp32()) {
+      ReactPipeline<float> p{};
+      p.M = RMolView<float>{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p, nullptr};
+      HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, d.p));
+    } else {
+      ReactPipeline<double> p{};
+      p.M = RMolView<double>{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p, nullptr};
+      HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, d.p));
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(counts, d.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCG_OK;
+}
+
+}  // namespace mcg
+
+extern "C" {
+
+int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
+  if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
+  if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  if (c->rx.on) return fail(MCG_E_STATE, "reaction classes are already set");
+  if (n == 0) return MCG_OK;
+  const uint32_t ns = c->num_species;
+  std::vector<RxnRec> h(n);
+  std::vector<int32_t> pair((size_t)ns * ns, -1);
+  std::vector<SpeciesRec> sp = c->species_host;
+  for (uint32_t i = 0; i < n; ++i) {
+    const mcg_bimol_rxn& x = rxns[i];
+    if (x.reactant_a >= ns || x.reactant_b >= ns) return fail(MCG_E_INVALID, "reactant species out of range");
+    if (x.num_pathways < 1 || x.num_pathways > (uint32_t)kMaxPathways) return fail(MCG_E_INVALID, "1..4 pathways");
+    if (pair[x.reactant_a * ns + x.reactant_b] >= 0) return fail(MCG_E_INVALID, "two classes for one reactant pair");
+    RxnRec& r = h[i];
+    std::memset(&r, 0, sizeof(r));
+    double last = 0;
+    for (uint32_t k = 0; k < x.num_pathways; ++k) {
+      if (!(x.cum_probs[k] >= last)) return fail(MCG_E_INVALID, "cumulative probabilities must not decrease");
+      last = x.cum_probs[k];
+      r.cum_probs[k] = x.cum_probs[k];
+      if (x.num_products[k] > (uint32_t)kMaxProducts) return fail(MCG_E_INVALID, "at most 3 products per pathway");
+      r.num_products[k] = x.num_products[k];
+      for (uint32_t q = 0; q < x.num_products[k]; ++q) {
+        const uint32_t ps = x.products[k][q];
+        if (ps >= ns) return fail(MCG_E_INVALID, "product species out of range");
+        if (ps == x.reactant_a || ps == x.reactant_b)
+          return fail(MCG_E_UNSUPPORTED, "reaction classes that keep a reactant are not on this path yet");
+        r.products[k][q] = ps;
+      }
+    }
+    r.num_pathways = x.num_pathways;
+    r.max_fixed_p = r.cum_probs[x.num_pathways - 1];
+    r.max_p_f = (float)r.max_fixed_p;
+    r.reactant_a = x.reactant_a;
+    r.reactant_b = x.reactant_b;
+    pair[x.reactant_a * ns + x.reactant_b] = (int32_t)i;
+    pair[x.reactant_b * ns + x.reactant_a] = (int32_t)i;
+    sp[x.reactant_a].flags |= SPF_CAN_VOLVOL;
+    sp[x.reactant_b].flags |= SPF_CAN_VOLVOL;
+  }
+  for (uint32_t s = 0; s < ns; ++s)
+    if (!(sp[s].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
+  ReactState& r = c->rx;
+  const size_t cap = c->cfg.molecule_capacity;
+  bool bad = r.rxns_d.alloc(n) || r.pair_d.alloc(pair.size()) || r.toff.alloc(cap) || r.sid.alloc(cap) ||
+             r.stag.alloc(cap) || r.stoff.alloc(cap) || r.key.alloc(cap) || r.rank.alloc(cap) ||
+             r.consumed.alloc(cap) || r.owner.alloc(cap) || r.prop.alloc(cap) || r.plist.alloc(cap) ||
+             r.alist.alloc(cap) || r.alist_sorted.alloc(cap) || r.rlist0.alloc(cap) || r.rlist1.alloc(cap) ||
+             r.mlist.alloc(cap) || r.keys_a.alloc(cap) || r.keys_b.alloc(cap) || r.cnt.alloc(cap + 1) ||
+             r.pre.alloc(cap + 1) || r.head.alloc(c->cfg.num_replicas) || r.next_id.alloc(c->cfg.num_replicas) ||
+             r.hdr.alloc(kHdrWords);
+  if (!bad)
+    bad = c->fp32() ? (r.sxf.alloc(cap) || r.syf.alloc(cap) || r.szf.alloc(cap))
+                    : (r.sx.alloc(cap) || r.sy.alloc(cap) || r.sz.alloc(cap));
+  if (bad) return fail(MCG_E_NOMEM, "reaction buffers");
+  if (!r.hdr_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.hdr_host), kHdrWords * 4, hipHostMallocDefault));
+  HIP_TRY(hipMemcpy(r.rxns_d.p, h.data(), n * sizeof(RxnRec), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(r.pair_d.p, pair.data(), pair.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->species.p, sp.data(), ns * sizeof(SpeciesRec), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(r.toff.p, 0, cap * sizeof(double)));
+  c->species_host = sp;
+  r.rxns = h;
+  r.pair = pair;
+  if (r.next_id_host.size() != c->cfg.num_replicas) r.next_id_host.assign(c->cfg.num_replicas, 0);
+  r.next_id_dirty = true;
+  r.grid_dirty = true;
+  r.on = true;
+  return MCG_OK;
+}
+
+int mcg_set_rxn_log(mcg_ctx* c, uint64_t capacity) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  if (capacity > 0xFFFFFFFFull) return fail(MCG_E_INVALID, "log capacity too large");
+  if (c->rx.log.alloc(capacity)) return fail(MCG_E_NOMEM, "reaction log");
+  c->rx.log_cap = capacity;
+  c->rx.log_count = 0;
+  return MCG_OK;
+}
+
+int mcg_get_rxn_log(mcg_ctx* c, mcg_rxn_event* out, uint64_t capacity, uint64_t* count) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  static_assert(sizeof(mcg_rxn_event) == sizeof(RxnEvent), "event layout");
+  const uint64_t kept = std::min(c->rx.log_count, c->rx.log_cap);
+  if (count) *count = c->rx.log_count;
+  if (out && kept) {
+    if (capacity < kept) return fail(MCG_E_CAPACITY, "output capacity too small");
+    HIP_TRY(hipMemcpy(out, c->rx.log.p, kept * sizeof(RxnEvent), hipMemcpyDeviceToHost));
+  }
+  return MCG_OK;
+}
+
+int mcg_count_species(mcg_ctx* c, uint64_t* counts) {
+  if (!c || !counts) return fail(MCG_E_INVALID, "null argument");
+  if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  return count_species(c, counts);
+}
+
+}  // extern "C"

@@ -10,6 +10,7 @@
 // the register-heavy trace runs only for the few molecules near walls.
 #include <hip/hip_runtime.h>
 
+#include "device/displace.h"
 #include "device/light.h"
 #include "device/path32.h"
 #include "device/path64.h"
@@ -19,7 +20,7 @@
 
 namespace mcg {
 
-__constant__ ZigguratTables c_zig = MCG_ZIGGURAT_TABLES_INIT;
+static __constant__ ZigguratTables c_zig = MCG_ZIGGURAT_TABLES_INIT;
 
 // Stages the Ziggurat tables (2.5 KB) into LDS so that the per-lane strip
 // lookups of a wave are LDS gathers rather than scattered cache reads.
@@ -29,15 +30,6 @@ __device__ __forceinline__ void stage_ziggurat(ZigguratTables& T) {
   constexpr int words = sizeof(ZigguratTables) / 4;
   for (int k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
   __syncthreads();
-}
-
-// pick_vol_displacement (diffusion_utils.inl:113-119) for a full step
-__device__ __forceinline__ D3 draw_displacement(PhiloxStream& rng, const ZigguratTables& T, double scale) {
-  D3 d;
-  d.x = scale * gauss(rng, T) * 0.70710678118654752440;
-  d.y = scale * gauss(rng, T) * 0.70710678118654752440;
-  d.z = scale * gauss(rng, T) * 0.70710678118654752440;
-  return d;
 }
 
 __global__ void k_reset_counters(uint32_t* c, uint32_t n) {

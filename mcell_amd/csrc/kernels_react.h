@@ -1,0 +1,80 @@
+// Host-side view of the reaction pipeline (kernels_react.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "device/react_step.h"
+
+namespace mcg {
+
+// Molecule arrays of the reactive path: positions, id, tag and the offset of
+// the molecule's diffusion time past the start of its iteration (non-zero
+// only for products created inside a step).
+template <typename PosT>
+struct RMolView {
+  PosT* x;
+  PosT* y;
+  PosT* z;
+  uint32_t* id;
+  uint32_t* tag;
+  double* toff;
+};
+
+// One carried-out (or, in the snapshot mode, proposed) reaction: 64 bytes,
+// the layout of mcg_rxn_event in the C ABI.
+struct RxnEvent {
+  double time;  // absolute time (iterations)
+  double x, y, z;
+  uint32_t iteration, replica;
+  uint32_t mover_id, partner_id;
+  uint32_t rxn, pathway;
+  uint32_t first_product_id, num_products;
+};
+static_assert(sizeof(RxnEvent) == 64, "event layout");
+
+// device counters
+enum : uint32_t {
+  kHdrNumOut = 0,         // used output slots (snapshot size + products)
+  kHdrProposals = 1,      // proposals of the current round
+  kHdrAccepted = 2,       // accepted reactions of the current phase
+  kHdrProductMovers = 3,  // products that move in this iteration
+  kHdrRerun0 = 4,         // movers queued for another round (two lists)
+  kHdrRerun1 = 5,
+  kHdrMoved = 6,          // molecule steps taken
+  kHdrLog = 7,            // events logged (snapshot mode)
+  kHdrWords = 8,
+};
+
+template <typename PosT>
+struct ReactPipeline {
+  ReactParams R;
+  const SpeciesRec* species;
+  RMolView<PosT> M;   // molecules: input of the sort, output of the moves
+  RMolView<PosT> Sm;  // cell-sorted snapshot
+  uint32_t *key, *rank, *cell_count, *cell_start;
+  uint32_t *consumed, *owner;
+  Proposal* prop;
+  uint32_t *plist, *alist, *alist_sorted, *rlist0, *rlist1, *mlist;
+  uint64_t *keys_a, *keys_b;
+  uint32_t *cnt, *pre, *head, *next_id;
+  uint32_t* hdr;
+  void* scan_tmp;
+  size_t scan_tmp_bytes;
+
+  static size_t scan_bytes(uint32_t num_cells, uint32_t capacity);
+  hipError_t sort(hipStream_t s, uint32_t n_in, uint32_t num_cells, uint32_t* err);
+  hipError_t move(hipStream_t s, bool snapshot_src, const uint32_t* list, const uint32_t* count, uint32_t max_movers,
+                  double barrier, bool propose_only, uint32_t* err, uint32_t* moved);
+  hipError_t settle(hipStream_t s, uint32_t max_props, uint32_t* rlist, uint32_t* rcount);
+  hipError_t products(hipStream_t s, uint32_t n_acc, uint32_t capacity, RxnEvent* log, uint32_t log_base,
+                      uint32_t log_cap, uint32_t* err);
+  hipError_t log_proposals(hipStream_t s, uint32_t max_props, uint32_t iteration, RxnEvent* log, uint32_t* log_count,
+                           uint32_t log_cap);
+  hipError_t count_species(hipStream_t s, uint32_t n, uint32_t num_species, uint32_t num_replicas, uint64_t* counts);
+};
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n);
+hipError_t launch_copy_u32(hipStream_t s, const uint32_t* src, uint32_t* dst);
+
+}  // namespace mcg

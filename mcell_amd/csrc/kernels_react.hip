@@ -1,0 +1,498 @@
+// HIP kernels of one diffuse-react iteration with volume-volume reactions.
+//
+// The reference moves molecules one after another, each seeing the others
+// where they are at that moment (diffuse_react_event.cpp:53-161).  Here all
+// molecules of an iteration move at once against a snapshot of the
+// iteration's start, and reactions are settled afterwards:
+//
+//   k_cell_keys / scan / k_scatter   counting sort of the live molecules by
+//                                    (replica, subpartition) into the snapshot
+//   k_move                           every mover's step (react_step.h); a mover
+//                                    whose reaction test succeeds stops with a
+//                                    proposal and claims itself and its partner
+//                                    (atomicMin of its id)
+//   k_accept                         a proposal is accepted when it holds both
+//                                    claims; its reactants are consumed
+//   k_requeue                        claims are released; movers whose proposal
+//                                    lost and that were not consumed move again
+//                                    (same stream, consumed partners skipped)
+//   products                         accepted reactions in (replica, mover id)
+//                                    order get product ids per replica; products
+//                                    move for the rest of the step as new movers
+//
+// The smallest mover id among open proposals always wins its claims, so every
+// round settles at least one proposal.  Molecules leave the snapshot order in
+// the output arrays (slot j of the snapshot writes slot j of the output);
+// consumed molecules stay as defunct holes until the next sort drops them.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "device/displace.h"
+#include "device/react_step.h"
+#include "kernels_react.h"
+
+namespace mcg {
+
+static __constant__ ZigguratTables c_zig_r = MCG_ZIGGURAT_TABLES_INIT;
+
+__device__ __forceinline__ bool cmp_eq_d(double a, double b, double eps) { return fabs(a - b) < eps; }
+__device__ __forceinline__ bool cmp_lt_d(double a, double b, double eps) { return a < b && !cmp_eq_d(a, b, eps); }
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------- cell sort --
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_cell_keys(RMolView<PosT> m, uint32_t n, PartitionParams P, CellGrid G,
+                                                   uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                   uint32_t* __restrict__ cell_count, uint32_t* __restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t tag = m.tag[i];
+  uint32_t c = kInvalid;
+  if (!(tag_flags(tag) & MF_DEFUNCT)) {
+    int sx, sy, sz;
+    subpart_3d(P, D3{(double)m.x[i], (double)m.y[i], (double)m.z[i]}, sx, sy, sz);
+    if (!cell_of(G, sx, sy, sz, tag_replica(tag), c)) {
+      atomicOr(err, ERR_GRID);
+      c = kInvalid;
+    }
+  }
+  key[i] = c;
+  if (c != kInvalid) rank[i] = atomicAdd(cell_count + c, 1u);
+}
+
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, const uint32_t* __restrict__ key,
+                                                 const uint32_t* __restrict__ rank,
+                                                 const uint32_t* __restrict__ cell_start, RMolView<PosT> s,
+                                                 uint32_t* __restrict__ consumed, uint32_t* __restrict__ owner) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = key[i];
+  if (c == kInvalid) return;
+  const uint32_t j = cell_start[c] + rank[i];
+  s.x[j] = m.x[i];
+  s.y[j] = m.y[i];
+  s.z[j] = m.z[i];
+  s.id[j] = m.id[i];
+  s.tag[j] = m.tag[i];
+  s.toff[j] = m.toff[i];
+  consumed[j] = 0;
+  owner[j] = kInvalid;
+}
+
+// ----------------------------------------------------------------- moves --
+// FP32 form: the end point in FP32, kept on the FP64 point's side of nearby
+// wall planes.
+__device__ __forceinline__ void store_pos(const ReactParams& R, double* x, double* y, double* z, uint32_t j, D3 p,
+                                          bool) {
+  x[j] = p.x;
+  y[j] = p.y;
+  z[j] = p.z;
+}
+__device__ __forceinline__ void store_pos(const ReactParams& R, float* x, float* y, float* z, uint32_t j, D3 p,
+                                          bool near_plane) {
+  float fx, fy, fz;
+  if (near_plane) {
+    round_to_fp32_same_side(R.P, R.walls, R.off, R.idx, p.x, p.y, p.z, &fx, &fy, &fz);
+  } else {
+    fx = (float)p.x;
+    fy = (float)p.y;
+    fz = (float)p.z;
+  }
+  x[j] = fx;
+  y[j] = fy;
+  z[j] = fz;
+}
+
+// diffuse_single_molecule's time bookkeeping (diffuse_react_event.cpp:
+// 201-337): diffusion_time += t_steps, then rounded to a whole iteration when
+// within SQRT_EPS of one.  Returns the offset past the next iteration start.
+__device__ __forceinline__ double next_toff(uint32_t iteration, double toff, double t_steps) {
+  double t = ((double)iteration + toff) + t_steps;
+  const double r = rint(t);
+  if (cmp_eq_d(t, r, 1e-6)) t = r;
+  return t - (double)(iteration + 1);
+}
+
+// One mover per thread.  Movers are snapshot slots [0, *count) when list is
+// null, else slots list[0, *count).  src holds the movers' start state (the
+// snapshot, or the output arrays for products); out receives end states.
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, RMolView<PosT> src,
+                                              RMolView<PosT> out, const uint32_t* __restrict__ list,
+                                              const uint32_t* __restrict__ count,
+                                              const SpeciesRec* __restrict__ species, double barrier,
+                                              uint32_t propose_only, Proposal* __restrict__ prop,
+                                              uint32_t* __restrict__ plist, uint32_t* __restrict__ pcount,
+                                              uint32_t* __restrict__ owner, uint32_t* __restrict__ err_out,
+                                              uint32_t* __restrict__ moved) {
+  __shared__ ZigguratTables T;
+  {
+    const uint32_t* zs = reinterpret_cast<const uint32_t*>(&c_zig_r);
+    uint32_t* zd = reinterpret_cast<uint32_t*>(&T);
+    for (int k = threadIdx.x; k < (int)(sizeof(ZigguratTables) / 4); k += blockDim.x) zd[k] = zs[k];
+    __syncthreads();
+  }
+  const uint32_t n = *count;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  bool proposed = false;
+  uint32_t j = 0;
+  if (q < n) {
+    j = list ? list[q] : q;
+    const uint32_t tag = src.tag[j];
+    const uint32_t idv = src.id[j];
+    const double toff = src.toff[j];
+    const SpeciesRec sp = species[tag_species(tag)];
+    D3 pos = {(double)src.x[j], (double)src.y[j], (double)src.z[j]};
+    uint32_t err = 0;
+    MoverEnd e;
+    e.pos = pos;
+    e.t_steps = 0;
+    e.near_plane = false;
+    Proposal p;
+    bool moves = sp.space_step > 0;
+    if (moves) proposed = react_mover(R, S, sp, T, pos, idv, tag, toff, barrier, p, e, err);
+    if (proposed) {
+      p.mover_slot = j;
+      p.pad = 0;
+      prop[j] = p;
+      if (!propose_only) {
+        atomicMin(owner + j, idv);
+        atomicMin(owner + p.partner_slot, idv);
+      }
+    }
+    if (!proposed || propose_only) {
+      store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
+      out.id[j] = idv;
+      out.tag[j] = tag;
+      out.toff[j] = moves ? next_toff(R.iteration, toff, e.t_steps) : toff;
+    }
+    if (err) atomicOr(err_out, err);
+  }
+  // proposals are rare: one atomic per wave that has any
+  const uint64_t mask = __ballot(proposed);
+  if (mask) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(pcount, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (proposed) plist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
+  }
+  if (moved && q < n) {
+    const uint64_t all = __ballot(true);
+    if (__lane_id() == __ffsll((long long)all) - 1) atomicAdd(moved, (uint32_t)__popcll(all));
+  }
+}
+
+// A proposal whose mover and partner both carry its claim is accepted.
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ plist, const uint32_t* __restrict__ pcount,
+                                                Proposal* __restrict__ prop, const uint32_t* __restrict__ owner,
+                                                uint32_t* __restrict__ consumed, uint32_t* __restrict__ out_tag,
+                                                const RxnRec* __restrict__ rxns, uint32_t* __restrict__ alist,
+                                                uint32_t* __restrict__ acount) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  bool acc = false;
+  uint32_t j = 0;
+  if (q < *pcount) {
+    j = plist[q];
+    const Proposal& p = prop[j];
+    if (owner[j] == p.mover_id && owner[p.partner_slot] == p.mover_id) {
+      acc = true;
+      // outcome_bimolecular (diffuse_react_event.cpp:1833-1895): both
+      // reactants are consumed (reactions keeping a reactant are refused at
+      // set-up)
+      consumed[j] = 1;
+      consumed[p.partner_slot] = 1;
+      out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
+      out_tag[p.partner_slot] = out_tag[p.partner_slot] | (MF_DEFUNCT << 8);
+    }
+  }
+  const uint64_t mask = __ballot(acc);
+  if (mask) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(acount, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (acc) alist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
+  }
+}
+
+// Releases the claims of every proposal of the round and queues the movers
+// of lost proposals that are still alive.
+__global__ __launch_bounds__(256) void k_requeue(const uint32_t* __restrict__ plist,
+                                                 const uint32_t* __restrict__ pcount,
+                                                 const Proposal* __restrict__ prop, uint32_t* __restrict__ owner,
+                                                 const uint32_t* __restrict__ consumed, uint32_t* __restrict__ rlist,
+                                                 uint32_t* __restrict__ rcount) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  bool again = false;
+  uint32_t j = 0;
+  if (q < *pcount) {
+    j = plist[q];
+    owner[j] = kInvalid;
+    owner[prop[j].partner_slot] = kInvalid;
+    again = consumed[j] == 0;
+  }
+  const uint64_t mask = __ballot(again);
+  if (mask) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(rcount, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (again) rlist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
+  }
+}
+
+// ---------------------------------------------------------------- products --
+__global__ __launch_bounds__(256) void k_acc_keys(const uint32_t* __restrict__ alist, uint32_t n,
+                                                  const Proposal* __restrict__ prop, uint64_t* __restrict__ keys) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const Proposal& p = prop[alist[k]];
+  keys[k] = ((uint64_t)p.replica << 32) | p.mover_id;
+}
+
+__device__ __forceinline__ uint32_t new_products(const RxnRec& r, uint32_t pathway) {
+  return r.num_products[pathway];
+}
+
+__global__ __launch_bounds__(256) void k_prod_counts(const uint32_t* __restrict__ sorted_slots, uint32_t n,
+                                                     const Proposal* __restrict__ prop,
+                                                     const RxnRec* __restrict__ rxns, uint32_t* __restrict__ cnt) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > n) return;
+  if (k == n) {
+    cnt[k] = 0;
+    return;
+  }
+  const Proposal& p = prop[sorted_slots[k]];
+  cnt[k] = new_products(rxns[p.rxn], p.pathway);
+}
+
+__global__ __launch_bounds__(256) void k_seg_heads(const uint64_t* __restrict__ keys, uint32_t n,
+                                                   uint32_t* __restrict__ head) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t rep = (uint32_t)(keys[k] >> 32);
+  if (k == 0 || (uint32_t)(keys[k - 1] >> 32) != rep) head[rep] = k;
+}
+
+// Products of accepted reaction k (in (replica, mover id) order) get ids
+// next_id[replica] + (products of earlier reactions of that replica) + i and
+// output slots base + pre[k] + i.  Products created before the iteration's
+// end move in this iteration (diffuse_react_event.cpp:2883-2886).
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t* __restrict__ sorted_slots,
+                                                  const uint64_t* __restrict__ keys, uint32_t n,
+                                                  const Proposal* __restrict__ prop, const uint32_t* __restrict__ pre,
+                                                  const uint32_t* __restrict__ head,
+                                                  const uint32_t* __restrict__ next_id, const uint32_t* __restrict__ hdr,
+                                                  uint32_t capacity, RMolView<PosT> out, uint32_t* __restrict__ consumed,
+                                                  uint32_t* __restrict__ owner, uint32_t* __restrict__ mlist,
+                                                  uint32_t* __restrict__ mcount, RxnEvent* __restrict__ log,
+                                                  uint32_t log_base, uint32_t log_cap, uint32_t* __restrict__ err) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const Proposal& p = prop[sorted_slots[k]];
+  const RxnRec& rx = R.rxns[p.rxn];
+  const uint32_t rep = (uint32_t)(keys[k] >> 32);
+  const uint32_t base = hdr[kHdrNumOut];
+  const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
+  const uint32_t np = rx.num_products[p.pathway];
+  const bool moves_now = cmp_lt_d(p.time, (double)R.iteration + 1.0, 1e-12);
+  for (uint32_t i = 0; i < np; ++i) {
+    const uint32_t slot = base + pre[k] + i;
+    if (slot >= capacity) {
+      atomicOr(err, ERR_QUEUE);
+      break;
+    }
+    store_pos(R, out.x, out.y, out.z, slot, D3{p.x, p.y, p.z}, true);
+    out.id[slot] = first_id + i;
+    out.tag[slot] = make_tag(rx.products[p.pathway][i], 0, rep);
+    out.toff[slot] = p.time - (double)(R.iteration + (moves_now ? 0u : 1u));
+    consumed[slot] = 0;
+    owner[slot] = kInvalid;
+    if (moves_now) mlist[atomicAdd(mcount, 1u)] = slot;
+  }
+  if (log && log_base + k < log_cap) {
+    RxnEvent& e = log[log_base + k];
+    e.time = p.time;
+    e.x = p.x;
+    e.y = p.y;
+    e.z = p.z;
+    e.iteration = R.iteration;
+    e.replica = rep;
+    e.mover_id = p.mover_id;
+    e.partner_id = p.partner_id;
+    e.rxn = p.rxn;
+    e.pathway = p.pathway;
+    e.first_product_id = first_id;
+    e.num_products = np;
+  }
+}
+
+__global__ void k_bump_ids(const uint64_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ pre,
+                           const uint32_t* __restrict__ head, uint32_t* __restrict__ next_id,
+                           uint32_t* __restrict__ hdr) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t rep = (uint32_t)(keys[k] >> 32);
+  if (k == n - 1 || (uint32_t)(keys[k + 1] >> 32) != rep) next_id[rep] += pre[k + 1] - pre[head[rep]];
+  if (k == n - 1) hdr[kHdrNumOut] += pre[n];
+}
+
+// Proposals logged without being carried out (the snapshot parity mode).
+__global__ __launch_bounds__(256) void k_log_proposals(const uint32_t* __restrict__ plist,
+                                                       const uint32_t* __restrict__ pcount,
+                                                       const Proposal* __restrict__ prop, uint32_t iteration,
+                                                       RxnEvent* __restrict__ log, uint32_t* __restrict__ log_count,
+                                                       uint32_t log_cap) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= *pcount) return;
+  const Proposal& p = prop[plist[q]];
+  const uint32_t k = atomicAdd(log_count, 1u);
+  if (k >= log_cap) return;
+  RxnEvent& e = log[k];
+  e.time = p.time;
+  e.x = p.x;
+  e.y = p.y;
+  e.z = p.z;
+  e.iteration = iteration;
+  e.replica = p.replica;
+  e.mover_id = p.mover_id;
+  e.partner_id = p.partner_id;
+  e.rxn = p.rxn;
+  e.pathway = p.pathway;
+  e.first_product_id = kInvalid;
+  e.num_products = 0;
+}
+
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_count_species(RMolView<PosT> m, uint32_t n, uint32_t num_species,
+                                                       uint32_t num_replicas, uint64_t* __restrict__ counts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t tag = m.tag[i];
+  if (tag_flags(tag) & MF_DEFUNCT) return;
+  const uint32_t s = tag_species(tag), r = tag_replica(tag);
+  if (s < num_species && r < num_replicas) atomicAdd((unsigned long long*)(counts + (size_t)r * num_species + s), 1ull);
+}
+
+// --------------------------------------------------------------- launchers --
+static inline dim3 blocks_for(uint32_t n) { return dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+
+template <typename PosT>
+hipError_t ReactPipeline<PosT>::sort(hipStream_t s, uint32_t n_in, uint32_t num_cells, uint32_t* err) {
+  hipLaunchKernelGGL(k_fill_u32, blocks_for(num_cells + 1), dim3(256), 0, s, cell_count, 0u, num_cells + 1);
+  if (n_in)
+    hipLaunchKernelGGL(k_cell_keys<PosT>, blocks_for(n_in), dim3(256), 0, s, M, n_in, R.P, R.G, key, rank, cell_count,
+                       err);
+  size_t tmp = scan_tmp_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cell_count, cell_start, num_cells + 1, s);
+  if (e != hipSuccess) return e;
+  if (n_in)
+    hipLaunchKernelGGL(k_scatter<PosT>, blocks_for(n_in), dim3(256), 0, s, M, n_in, key, rank, cell_start, Sm, consumed,
+                       owner);
+  return hipGetLastError();
+}
+
+template <typename PosT>
+size_t ReactPipeline<PosT>::scan_bytes(uint32_t num_cells, uint32_t capacity) {
+  size_t a = 0, b = 0, c = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, a, (uint32_t*)nullptr, (uint32_t*)nullptr, num_cells + 1);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr, capacity + 1);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, c, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, capacity);
+  return std::max(a, std::max(b, c));
+}
+
+template <typename PosT>
+hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uint32_t* list, const uint32_t* count,
+                                     uint32_t max_movers, double barrier, bool propose_only, uint32_t* err,
+                                     uint32_t* moved) {
+  if (max_movers == 0) return hipSuccess;
+  Snapshot<PosT> S{Sm.x, Sm.y, Sm.z, Sm.id, Sm.tag, consumed};
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrProposals, 0u, 1u);
+  hipLaunchKernelGGL(k_move<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, snapshot_src ? Sm : M, M, list,
+                     count, species, barrier, propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, err,
+                     moved);
+  return hipGetLastError();
+}
+
+template <typename PosT>
+hipError_t ReactPipeline<PosT>::settle(hipStream_t s, uint32_t max_props, uint32_t* rlist, uint32_t* rcount) {
+  hipLaunchKernelGGL(k_accept<PosT>, blocks_for(max_props), dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
+                     consumed, M.tag, R.rxns, alist, hdr + kHdrAccepted);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, rcount, 0u, 1u);
+  hipLaunchKernelGGL(k_requeue, blocks_for(max_props), dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
+                     consumed, rlist, rcount);
+  return hipGetLastError();
+}
+
+template <typename PosT>
+hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t capacity, RxnEvent* log,
+                                         uint32_t log_base, uint32_t log_cap, uint32_t* err) {
+  if (n_acc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_acc_keys, blocks_for(n_acc), dim3(256), 0, s, alist, n_acc, prop, keys_a);
+  size_t tmp = scan_tmp_bytes;
+  hipError_t e =
+      hipcub::DeviceRadixSort::SortPairs(scan_tmp, tmp, keys_a, keys_b, alist, alist_sorted, n_acc, 0, 64, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prod_counts, blocks_for(n_acc + 1), dim3(256), 0, s, alist_sorted, n_acc, prop, R.rxns, cnt);
+  tmp = scan_tmp_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt, pre, n_acc + 1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_seg_heads, blocks_for(n_acc), dim3(256), 0, s, keys_b, n_acc, head);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrProductMovers, 0u, 1u);
+  hipLaunchKernelGGL(k_products<PosT>, blocks_for(n_acc), dim3(256), 0, s, R, alist_sorted, keys_b, n_acc, prop, pre,
+                     head, next_id, hdr, capacity, M, consumed, owner, mlist, hdr + kHdrProductMovers, log, log_base,
+                     log_cap, err);
+  hipLaunchKernelGGL(k_bump_ids, blocks_for(n_acc), dim3(256), 0, s, keys_b, n_acc, pre, head, next_id, hdr);
+  return hipGetLastError();
+}
+
+template <typename PosT>
+hipError_t ReactPipeline<PosT>::log_proposals(hipStream_t s, uint32_t max_props, uint32_t iteration, RxnEvent* log,
+                                              uint32_t* log_count, uint32_t log_cap) {
+  if (max_props == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_log_proposals, blocks_for(max_props), dim3(256), 0, s, plist, hdr + kHdrProposals, prop,
+                     iteration, log, log_count, log_cap);
+  return hipGetLastError();
+}
+
+template <typename PosT>
+hipError_t ReactPipeline<PosT>::count_species(hipStream_t s, uint32_t n, uint32_t num_species, uint32_t num_replicas,
+                                              uint64_t* counts) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_species<PosT>, blocks_for(n), dim3(256), 0, s, M, n, num_species, num_replicas, counts);
+  return hipGetLastError();
+}
+
+template struct ReactPipeline<double>;
+template struct ReactPipeline<float>;
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, blocks_for(n), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+hipError_t launch_copy_u32(hipStream_t s, const uint32_t* src, uint32_t* dst) {
+  hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(64), 0, s, src, dst);
+  return hipGetLastError();
+}
+
+}  // namespace mcg

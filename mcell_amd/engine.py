@@ -23,7 +23,8 @@ def _ptr(a: Optional[np.ndarray]):
 
 class DiffuseReactEngine:
     def __init__(self, part: PartitionConfig, precision: str = "fp64", seed: int = 1, num_replicas: int = 1,
-                 capacity: int = 1 << 20, trace: bool = False):
+                 capacity: int = 1 << 20, trace: bool = False, barrier_interval: int = 1,
+                 propose_only: bool = False):
         self.lib = _lib.load()
         self.part = part
         cfg = _lib.McgConfig()
@@ -41,10 +42,33 @@ class DiffuseReactEngine:
         cfg.num_replicas = num_replicas
         cfg.seed = seed
         cfg.molecule_capacity = capacity
-        cfg.flags = _lib.MCG_FLAG_TRACE_HITS if trace else 0
+        cfg.flags = (_lib.MCG_FLAG_TRACE_HITS if trace else 0) | (_lib.MCG_FLAG_PROPOSE_ONLY if propose_only else 0)
+        cfg.barrier_interval = barrier_interval
         self.ctx = ctypes.c_void_p()
         _lib.check(self.lib.mcg_create(ctypes.byref(cfg), ctypes.byref(self.ctx)))
         self.trace = trace
+
+    @classmethod
+    def from_workload(cls, wl, precision: str = "fp64", seed: Optional[int] = None, capacity: Optional[int] = None,
+                      propose_only: bool = False, log_capacity: int = 0, trace: bool = False):
+        """An engine holding the workload's geometry, species, reaction
+        classes and molecules (every replica)."""
+        from .workloads import rxn_table
+        cap = capacity if capacity is not None else max(2 * wl.num_molecules, 1024)
+        eng = cls(wl.part, precision=precision, seed=wl.config.seed if seed is None else seed,
+                  num_replicas=wl.num_replicas, capacity=cap, trace=trace, barrier_interval=wl.barrier_interval,
+                  propose_only=propose_only)
+        eng.set_geometry(wl.vertices, wl.walls, wl.wall_class)
+        eng.set_species(wl.species_steps)
+        if wl.rxns:
+            eng.set_bimol_rxns(rxn_table(wl))
+        if log_capacity:
+            eng.set_rxn_log(log_capacity)
+        for r in range(wl.num_replicas):
+            sel = wl.replica == r
+            if sel.any():
+                eng.add_volume_molecules(r, wl.ids[sel], wl.mol_species[sel], wl.positions[sel])
+        return eng
 
     def close(self):
         if self.ctx:
@@ -72,6 +96,37 @@ class DiffuseReactEngine:
             arr[i].time_step = ts
             arr[i].flags = fl
         _lib.check(self.lib.mcg_set_species(self.ctx, arr, len(species)))
+
+    def set_bimol_rxns(self, rxns):
+        """rxns: (reactant_a, reactant_b, cum_probs, products per pathway)."""
+        arr = (_lib.McgBimolRxn * max(len(rxns), 1))()
+        for i, (a, b, cum, prods) in enumerate(rxns):
+            arr[i].reactant_a = a
+            arr[i].reactant_b = b
+            arr[i].num_pathways = len(cum)
+            for k, p in enumerate(cum):
+                arr[i].cum_probs[k] = p
+                arr[i].num_products[k] = len(prods[k])
+                for q, ps in enumerate(prods[k]):
+                    arr[i].products[k][q] = ps
+        _lib.check(self.lib.mcg_set_bimol_rxns(self.ctx, arr, len(rxns)))
+
+    def set_rxn_log(self, capacity: int):
+        _lib.check(self.lib.mcg_set_rxn_log(self.ctx, capacity))
+
+    def rxn_log(self) -> np.ndarray:
+        """Events of the last step call (structured array, mcg_rxn_event)."""
+        cnt = ctypes.c_uint64()
+        _lib.check(self.lib.mcg_get_rxn_log(self.ctx, None, 0, ctypes.byref(cnt)))
+        out = np.zeros(cnt.value, dtype=_lib.RXN_EVENT_DTYPE)
+        if cnt.value:
+            _lib.check(self.lib.mcg_get_rxn_log(self.ctx, _ptr(out), cnt.value, ctypes.byref(cnt)))
+        return out
+
+    def count_species(self, num_replicas: int, num_species: int) -> np.ndarray:
+        out = np.zeros((num_replicas, num_species), np.uint64)
+        _lib.check(self.lib.mcg_count_species(self.ctx, _ptr(out)))
+        return out
 
     def add_volume_molecules(self, replica: int, ids: np.ndarray, species: np.ndarray, positions: np.ndarray):
         ids = np.ascontiguousarray(ids, dtype=np.uint32)

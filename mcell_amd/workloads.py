@@ -41,6 +41,7 @@ class Workload:
     num_replicas: int = 1
     config: M.Config = None
     objects: List[M.GeometryObject] = field(default_factory=list)
+    barrier_interval: int = 1     # iterations between count barriers
 
     @property
     def num_molecules(self):
@@ -56,6 +57,16 @@ def _species_table(species, part, rxns):
     for s, f in zip(species, flags):
         ss, ts = s.steps(part)
         out.append((ss, ts, f))
+    return out
+
+
+def rxn_table(wl: Workload):
+    """The reaction classes as the engine takes them: (a, b, cum_probs,
+    products per pathway), with P = rate * pb_factor (one pathway)."""
+    out = []
+    for r in wl.rxns:
+        pb = r.pb_factor(wl.part, wl.species[r.reactant_a], wl.species[r.reactant_b])
+        out.append((r.reactant_a, r.reactant_b, [r.rate * pb], [list(r.products)]))
     return out
 
 
@@ -88,3 +99,25 @@ def free_diffusion_replicas(num_replicas: int, mols_per_replica: int, iterations
     return Workload("free_diffusion_replicas", part, verts, walls, classes, species,
                     _species_table(species, part, []), ids, np.zeros(n, np.uint32), pos, rep, iterations,
                     num_replicas=num_replicas, config=cfg, objects=[box])
+
+
+def bimol_box(n_a: int = 50000, n_b: int = 50000, iterations: int = 2000, seed: int = 2,
+              num_replicas: int = 1, rate: float = 1e8, subpartition_dimension: float = 0.05,
+              barrier_interval: int = 10) -> Workload:
+    """A + B -> C in the 1 um reflective cube: A and B with D = 1e-6 cm^2/s,
+    C with D = 5e-7, uniform placement, counts every ``barrier_interval``
+    iterations.  Replica r is placed with seed ``seed + r`` (replica 0 is the
+    single-instance workload); ids are 0..n_a-1 for A, then B."""
+    cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
+    box = M.create_box("box", edge_dimension=1.0)
+    part = M.derive_partition(cfg, M.bounding_box([box]), has_bimol_vol_rxns=True)
+    verts, walls, classes = M.vertices_internal([box], part)
+    species = [M.Species("A", 1e-6), M.Species("B", 1e-6), M.Species("C", 5e-7)]
+    rxns = [M.BimolReaction(0, 1, rate, [2])]
+    n = n_a + n_b
+    pos = np.concatenate([M.place_uniform_in_box(n, 0.5, seed + r, part) for r in range(num_replicas)])
+    sp1 = np.concatenate([np.zeros(n_a, np.uint32), np.ones(n_b, np.uint32)])
+    return Workload("bimol_box", part, verts, walls, classes, species, _species_table(species, part, rxns),
+                    np.tile(np.arange(n, dtype=np.uint32), num_replicas), np.tile(sp1, num_replicas), pos,
+                    np.repeat(np.arange(num_replicas, dtype=np.uint32), n), iterations, rxns=rxns,
+                    num_replicas=num_replicas, config=cfg, objects=[box], barrier_interval=barrier_interval)
