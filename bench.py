@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--workload", default="free_diffusion_replicas")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--mols-per-replica", type=int, default=150000)
+    ap.add_argument("--subpart", type=float, default=None,
+                    help="subpartition edge in um (default: the workload's, or the autotuner's for replicas)")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -64,6 +67,13 @@ def build_workload(args):
         wl = W.free_diffusion_replicas(args.replicas, args.mols_per_replica, iterations=args.steps)
         cfg = {"workload": "free diffusion, %d replicas x %d A in a shared reflective 1 um cube (12 walls), "
                            "0.1 um subpartitions, dt 1e-6 s" % (args.replicas, args.mols_per_replica),
+               "replicas": args.replicas, "molecules": wl.num_molecules, "walls": int(len(wl.walls))}
+    elif args.workload == "replicas_sphere":
+        wl = W.replicas_sphere_in_cube(num_replicas=args.replicas, iterations=args.steps)
+        cfg = {"workload": "%d replicas of A + B -> C (1e8 /M/s) over one mesh: reflective 2 um cube + icosphere "
+                           "r 0.5 um (5,120 triangles); per replica N_A ~ U[1e3, 2e5], N_B = N_A/2, "
+                           "D ~ U[1e-7, 5e-6] cm^2/s (seed 4 + i), C with D 5e-7, counts every 10 iterations, "
+                           "dt 1e-6 s" % args.replicas,
                "replicas": args.replicas, "molecules": wl.num_molecules, "walls": int(len(wl.walls))}
     elif args.workload == "bimol_box":
         wl = W.bimol_box(iterations=args.steps)
@@ -121,7 +131,17 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from mcell_amd.engine import DiffuseReactEngine
+    from mcell_amd import workloads as W
     wl, cfg = build_workload(args)
+    if args.subpart is not None:
+        wl = W.with_subpartition(wl, args.subpart)
+    elif wl.num_replicas > 1 and not args.no_autotune:
+        from mcell_amd import autotune
+        best, rates = autotune.choose_subpartition(
+            wl, args.precision, log=lambda m: print(m, file=sys.stderr, flush=True))
+        wl = W.with_subpartition(wl, best)
+        cfg["autotune"] = {"subpartition_um": best, "rates": {str(k): v for k, v in rates.items()}}
+    cfg["subpartition_um"] = wl.config.subpartition_dimension
     eng = DiffuseReactEngine.from_workload(wl, precision=args.precision,
                                            capacity=wl.num_molecules * (2 if wl.rxns else 1))
     n = wl.num_molecules

@@ -140,9 +140,25 @@ int mcg_destroy(mcg_ctx* ctx);
  * the position in the array).  The engine recomputes the wall constants
  * (Wall::initialize_wall_constants, src4/wall.cpp:281-342) and the per
  * subpartition wall lists (Partition::finalize_walls, src4/partition.cpp:91-118).
- * wall_class may be NULL (all walls reflective). */
+ * wall_class may be NULL (all walls reflective); otherwise per wall 0 (no
+ * surface class) or a 1-based class of mcg_set_surface_classes. */
 int mcg_set_geometry(mcg_ctx* ctx, const double* vertices, uint32_t num_vertices, const uint32_t* walls,
                      uint32_t num_walls, const uint32_t* wall_class);
+
+/* Surface classes (the reference's surface-class species with reflective,
+ * transparent and absorptive properties, mcell4_converter.cpp:552-560, acting
+ * through collide_and_react_with_walls, diffuse_react_event.cpp:991-1067).
+ * Walls carry a class through mcg_set_geometry's wall_class (0 = none, else
+ * 1..num_classes).  action[(c - 1) * num_species + s] says what class c does
+ * to species s hit from the wall's front (bits 0-1: the move runs against the
+ * normal) and from its back (bits 2-3): MCG_WALL_REFLECT, MCG_WALL_TRANSPARENT
+ * (cross_transparent_wall, :3007-3099) or MCG_WALL_ABSORB (destroyed after
+ * test_intersect's two draws).  Species with any other action than reflect get
+ * MCG_SPF_CAN_VOLWALL.  Call after mcg_set_geometry and mcg_set_species. */
+#define MCG_WALL_REFLECT 0u
+#define MCG_WALL_TRANSPARENT 1u
+#define MCG_WALL_ABSORB 2u
+int mcg_set_surface_classes(mcg_ctx* ctx, uint32_t num_classes, const uint8_t* action);
 
 /* Species table, indexed by species id (< 256). */
 int mcg_set_species(mcg_ctx* ctx, const mcg_species* species, uint32_t num_species);
@@ -152,6 +168,16 @@ int mcg_set_species(mcg_ctx* ctx, const mcg_species* species, uint32_t num_speci
  * spanned by the geometry and the molecules added (one subpartition of
  * margin); leaving it is reported as MCG_E_LIMIT. */
 int mcg_set_bimol_rxns(mcg_ctx* ctx, const mcg_bimol_rxn* rxns, uint32_t num_rxns);
+
+/* Per-replica parameters for batches of instances that differ in diffusion
+ * constants or rates (parameter sweeps over one mesh): replica `replica`
+ * moves with these species parameters (num_species entries; the flags stay
+ * the base table's) and reacts with these classes (the base table's
+ * reactants, pathways and products; only cum_probs may differ).  rxns may be
+ * NULL when no reaction classes are set.  Call after mcg_set_species and
+ * mcg_set_bimol_rxns; replicas never set keep the base tables.  No
+ * counterpart in MCell4, where each instance is its own process. */
+int mcg_set_replica_params(mcg_ctx* ctx, uint32_t replica, const mcg_species* species, const mcg_bimol_rxn* rxns);
 
 /* Keeps up to `capacity` reaction events of each mcg_step call (0 = off). */
 int mcg_set_rxn_log(mcg_ctx* ctx, uint64_t capacity);

@@ -23,6 +23,10 @@ MCG_SPF_CAN_VOLWALL = 2
 MCG_SPF_CAN_VOLSURF = 4
 MCG_SPF_TARGET_ONLY = 8
 
+MCG_WALL_REFLECT = 0
+MCG_WALL_TRANSPARENT = 1
+MCG_WALL_ABSORB = 2
+
 TRACE_HITS = 8
 
 # every symbol declared in include/mcg.h
@@ -30,7 +34,7 @@ EXPORTED = [
     "mcg_last_error", "mcg_abi_version", "mcg_create", "mcg_destroy", "mcg_set_geometry", "mcg_set_species",
     "mcg_add_volume_molecules", "mcg_step", "mcg_sync", "mcg_num_molecules", "mcg_get_volume_molecules",
     "mcg_get_walls", "mcg_get_hit_trace", "mcg_last_kernel_time", "mcg_set_bimol_rxns", "mcg_set_rxn_log",
-    "mcg_get_rxn_log", "mcg_count_species",
+    "mcg_get_rxn_log", "mcg_count_species", "mcg_set_replica_params", "mcg_set_surface_classes",
 ]
 
 
@@ -145,6 +149,8 @@ def load():
     lib.mcg_set_rxn_log.argtypes = [P, ctypes.c_uint64]
     lib.mcg_get_rxn_log.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
     lib.mcg_count_species.argtypes = [P, P]
+    lib.mcg_set_surface_classes.argtypes = [P, ctypes.c_uint32, P]
+    lib.mcg_set_replica_params.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(McgSpecies), P]
     for name in EXPORTED:
         if not name.startswith("mcg_last_error") and name != "mcg_abi_version":
             getattr(lib, name).restype = ctypes.c_int

@@ -132,13 +132,16 @@ __device__ __forceinline__ void round_to_fp32_same_side(const PartitionParams& P
 
 // diffuse_reflect64 with the FP32 filter; returns whether the end point is
 // within `band` of a wall plane.
+// `alive` becomes false when a surface class absorbs the molecule.
 __device__ __forceinline__ bool diffuse_reflect32(const PartitionParams& P, const Walls32View& W,
-                                                  const FilterParams& F, float band, D3& pos, D3 disp,
-                                                  PhiloxStream& rng, uint32_t& err, uint32_t* trace_walls,
-                                                  uint32_t& trace_n) {
+                                                  const FilterParams& F, float band, const SurfaceClasses& SC,
+                                                  uint32_t species, D3& pos, D3 disp, PhiloxStream& rng,
+                                                  uint32_t& err, uint32_t* trace_walls, uint32_t& trace_n,
+                                                  bool& alive) {
   uint32_t last_hit = kInvalid;
   int seg;
   bool end_band = false;
+  alive = true;
   for (seg = 0; seg < kMaxSegments; ++seg) {
     end_band = false;
     D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
@@ -172,18 +175,30 @@ __device__ __forceinline__ bool diffuse_reflect32(const PartitionParams& P, cons
       pos = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
       break;
     }
-    const WallRec64& w = W.w64[hit.wall];
-    double t_reflect = hit.t;
-    pos = hit.pos;
-    last_hit = hit.wall;
-    double rf = -2.0 * dot3(disp, d3(w.nx, w.ny, w.nz));
-    double s = 1.0 - t_reflect;
-    disp = {(disp.x + rf * w.nx) * s, (disp.y + rf * w.ny) * s, (disp.z + rf * w.nz) * s};
     if (trace_walls) {
       if (trace_n < (uint32_t)kTraceHits) trace_walls[trace_n] = hit.wall;
       else err |= ERR_TRACE;
       ++trace_n;
     }
+    const uint32_t act = SC.of(hit.wall, species, hit.type == WALL_FRONT);
+    if (act == SC_ABSORB) {
+      (void)rng.dbl();
+      (void)rng.dbl();
+      pos = hit.pos;
+      alive = false;
+      return false;
+    }
+    const WallRec64& w = W.w64[hit.wall];
+    double t_hit = hit.t;
+    pos = hit.pos;
+    last_hit = hit.wall;
+    double s = 1.0 - t_hit;
+    if (act == SC_TRANSPARENT) {
+      disp = {disp.x * s, disp.y * s, disp.z * s};
+      continue;
+    }
+    double rf = -2.0 * dot3(disp, d3(w.nx, w.ny, w.nz));
+    disp = {(disp.x + rf * w.nx) * s, (disp.y + rf * w.ny) * s, (disp.z + rf * w.nz) * s};
   }
   if (seg == kMaxSegments) err |= ERR_SEGMENTS;
   if (!in_partition(P, pos)) err |= ERR_ESCAPED;

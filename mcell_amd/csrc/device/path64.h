@@ -325,11 +325,13 @@ __device__ __forceinline__ bool closest_wall_in_subpart64(const PartitionParams&
 // the ray-trace/reflect loop of diffuse_vol_molecule
 // (diffuse_react_event.cpp:367-618 with only reflective walls), for a full
 // time step (t_steps = 1 internal unit, the molecule's own time_step).
-__device__ __forceinline__ void diffuse_reflect64(const PartitionParams& P, const WallRec64* __restrict__ walls,
+// Walls act by the species' surface classes: reflect, cross (transparent) or
+// absorb; returns false when the molecule was absorbed.
+__device__ __forceinline__ bool diffuse_reflect64(const PartitionParams& P, const WallRec64* __restrict__ walls,
                                                   const uint32_t* __restrict__ off,
-                                                  const uint32_t* __restrict__ idx, D3& pos, D3 disp,
-                                                  PhiloxStream& rng, uint32_t& err, uint32_t* trace_walls,
-                                                  uint32_t& trace_n) {
+                                                  const uint32_t* __restrict__ idx, const SurfaceClasses& SC,
+                                                  uint32_t species, D3& pos, D3 disp, PhiloxStream& rng,
+                                                  uint32_t& err, uint32_t* trace_walls, uint32_t& trace_n) {
   uint32_t last_hit = kInvalid;
   int seg;
   for (seg = 0; seg < kMaxSegments; ++seg) {
@@ -339,7 +341,7 @@ __device__ __forceinline__ void diffuse_reflect64(const PartitionParams& P, cons
     subpart_3d(P, pos, sx, sy, sz);
     if (sx < 0 || sy < 0 || sz < 0 || sx >= (int)P.n || sy >= (int)P.n || sz >= (int)P.n) {
       err |= ERR_ESCAPED;  // never index the wall lists outside the grid
-      return;
+      return true;
     }
     uint32_t start = subpart_of(P, sx, sy, sz);
     WallHit hit;
@@ -364,22 +366,36 @@ __device__ __forceinline__ void diffuse_reflect64(const PartitionParams& P, cons
       pos = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
       break;
     }
-    // reflect_from_wall (collision_utils.inl:1711-1747)
-    const WallRec64& w = walls[hit.wall];
-    double t_reflect = hit.t;
-    pos = hit.pos;
-    last_hit = hit.wall;
-    double rf = -2.0 * dot3(disp, d3(w.nx, w.ny, w.nz));
-    double s = 1.0 - t_reflect;
-    disp = {(disp.x + rf * w.nx) * s, (disp.y + rf * w.ny) * s, (disp.z + rf * w.nz) * s};
     if (trace_walls) {
       if (trace_n < (uint32_t)kTraceHits) trace_walls[trace_n] = hit.wall;
       else err |= ERR_TRACE;
       ++trace_n;
     }
+    const uint32_t act = SC.of(hit.wall, species, hit.type == WALL_FRONT);
+    if (act == SC_ABSORB) {
+      // test_intersect's two draws (rxn_utils.inl:593-626), then destroyed
+      (void)rng.dbl();
+      (void)rng.dbl();
+      pos = hit.pos;
+      return false;
+    }
+    const WallRec64& w = walls[hit.wall];
+    const double t_hit = hit.t;
+    pos = hit.pos;
+    last_hit = hit.wall;
+    const double s = 1.0 - t_hit;
+    if (act == SC_TRANSPARENT) {
+      // cross_transparent_wall (diffuse_react_event.cpp:3071-3094)
+      disp = {disp.x * s, disp.y * s, disp.z * s};
+      continue;
+    }
+    // reflect_from_wall (collision_utils.inl:1711-1747)
+    double rf = -2.0 * dot3(disp, d3(w.nx, w.ny, w.nz));
+    disp = {(disp.x + rf * w.nx) * s, (disp.y + rf * w.ny) * s, (disp.z + rf * w.nz) * s};
   }
   if (seg == kMaxSegments) err |= ERR_SEGMENTS;
   if (!in_partition(P, pos)) err |= ERR_ESCAPED;
+  return true;
 }
 
 }  // namespace mcg

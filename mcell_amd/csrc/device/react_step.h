@@ -22,12 +22,14 @@ struct ReactParams {
   const FilterWall* __restrict__ ent32;  // FP32 records in list order (FP32 form)
   CellGrid G;
   const RxnRec* __restrict__ rxns;
+  uint32_t rxn_stride;                   // reaction classes per replica table (0: one shared table)
   const int32_t* __restrict__ pair_rxn;  // [ns * ns], -1 = none
   uint32_t num_species;
   uint64_t seed;
   uint32_t iteration;
   FilterParams F;
   float band;
+  SurfaceClasses SC;
 };
 
 struct Proposal {
@@ -98,6 +100,7 @@ struct MoverEnd {
   D3 pos;          // end position; with a proposal, the start of the segment it was found in
   double t_steps;  // time the step covered (units of the time step), before reflections
   bool near_plane; // FP32 form: the end point lies near a wall plane
+  bool absorbed;   // destroyed by an absorptive surface class
 };
 
 // The step of one mover (diffuse_vol_molecule, diffuse_react_event.cpp:367-
@@ -119,6 +122,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   const double r_rate_factor = clipped_displacement(rng, T, sp.space_step, sp.time_step, barrier - toff, t_steps, disp);
   out.t_steps = t_steps;
   out.near_plane = false;
+  out.absorbed = false;
   uint32_t last_hit = kInvalid;
   double elapsed = (double)R.iteration + toff;
   const bool can_react = (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY);
@@ -181,7 +185,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
         const double factor = exact_disk_dev(R.P, R.walls, R.off, R.idx, cp, disp, R.P.rxn_radius, target, &full);
         if (full) err |= ERR_EXD;
         if (factor < 0) continue;  // the disk is occluded
-        const int pw = test_bimolecular(R.rxns[rx], factor * r_rate_factor, rng);
+        const int pw = test_bimolecular(R.rxns[rep * R.rxn_stride + rx], factor * r_rate_factor, rng);
         if (pw >= 0) {
           prop.time = elapsed + t_steps * tc;
           prop.x = cp.x;
@@ -206,15 +210,30 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
       out.near_plane = near_plane;
       break;
     }
-    // reflect_from_wall (collision_utils.inl:1711-1747)
+    // collide_and_react_with_walls (diffuse_react_event.cpp:529-561, 991-1067)
+    const uint32_t act = R.SC.of(hit.wall, spc, hit.type == WALL_FRONT);
+    if (act == SC_ABSORB) {
+      (void)rng.dbl();  // test_intersect's two draws (rxn_utils.inl:593-626)
+      (void)rng.dbl();
+      out.pos = hit.pos;
+      out.absorbed = true;
+      return false;
+    }
     elapsed += t_steps * hit.t;
     const WallRec64& w = R.walls[hit.wall];
-    const double t_reflect = hit.t;
+    const double t_hit = hit.t;
     pos = hit.pos;
-    t_steps *= (1.0 - t_reflect);
+    t_steps *= (1.0 - t_hit);
     last_hit = hit.wall;
+    const double s = 1.0 - t_hit;
+    if (act == SC_TRANSPARENT) {
+      // cross_transparent_wall (diffuse_react_event.cpp:3071-3094)
+      if (t_steps < 1e-12) t_steps = 1e-12;
+      disp = {disp.x * s, disp.y * s, disp.z * s};
+      continue;
+    }
+    // reflect_from_wall (collision_utils.inl:1711-1747)
     const double rf = -2.0 * dot3(disp, d3(w.nx, w.ny, w.nz));
-    const double s = 1.0 - t_reflect;
     disp = {(disp.x + rf * w.nx) * s, (disp.y + rf * w.ny) * s, (disp.z + rf * w.nz) * s};
   }
   if (seg == kMaxSegments) err |= ERR_SEGMENTS;

@@ -64,7 +64,35 @@ __device__ __host__ __forceinline__ uint32_t tag_replica(uint32_t t) { return t 
 __device__ __host__ __forceinline__ uint32_t make_tag(uint32_t s, uint32_t f, uint32_t r) {
   return (s & 0xFFu) | ((f & 0xFFu) << 8) | (r << 16);
 }
+// Species parameters, one table for all replicas (stride 0) or one table of
+// `stride` species per replica.
+struct SpeciesTable {
+  const SpeciesRec* rec;
+  uint32_t stride;
+  __device__ __forceinline__ const SpeciesRec& of(uint32_t tag) const {
+    return rec[tag_replica(tag) * stride + tag_species(tag)];
+  }
+};
+
 enum : uint32_t { MF_DEFUNCT = 1u << 0 };
+
+// Surface classes: what a wall does to a species that hits it
+// (collide_and_react_with_walls, diffuse_react_event.cpp:991-1067).
+enum : uint32_t { SC_REFLECT = 0, SC_TRANSPARENT = 1, SC_ABSORB = 2 };
+struct SurfaceClasses {
+  const uint32_t* __restrict__ wall_class;  // per wall: 0 = none, else 1-based class
+  const uint8_t* __restrict__ action;       // [(class - 1) * num_species + species]: front | back << 2
+  uint32_t num_species;
+  uint32_t on;                              // any class with a non-reflect action
+  // hit_front: the hit was WALL_FRONT (the move ran against the normal)
+  __device__ __forceinline__ uint32_t of(uint32_t wall, uint32_t species, bool hit_front) const {
+    if (!on) return SC_REFLECT;
+    const uint32_t c = wall_class[wall];
+    if (c == 0) return SC_REFLECT;
+    const uint32_t a = action[(c - 1) * num_species + species];
+    return hit_front ? (a & 3u) : ((a >> 2) & 3u);
+  }
+};
 
 // error bits reported through a device word
 enum : uint32_t {

@@ -105,7 +105,6 @@ int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint
   for (uint32_t i = 0; i < nw; ++i) {
     if (walls[3 * i] >= nv || walls[3 * i + 1] >= nv || walls[3 * i + 2] >= nv)
       return fail(MCG_E_INVALID, "wall vertex index out of range");
-    if (wall_class && wall_class[i] != 0) return fail(MCG_E_UNSUPPORTED, "only reflective walls on this path");
   }
   const PartitionParams& P = c->P;
   for (uint32_t i = 0; i < nv; ++i) {
@@ -114,6 +113,13 @@ int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint
       return fail(MCG_E_INVALID, "vertex outside the partition");
   }
   c->vert_host.assign(vertices, vertices + 3 * (size_t)nv);
+  c->wall_class_host.assign(nw, 0u);
+  if (wall_class) c->wall_class_host.assign(wall_class, wall_class + nw);
+  for (uint32_t v : c->wall_class_host)
+    if (c->sc_on && v > c->num_surface_classes) return fail(MCG_E_INVALID, "wall class without a surface class");
+  if (c->wall_class_d.alloc(std::max<size_t>(nw, 1))) return fail(MCG_E_NOMEM, "wall classes");
+  if (nw)
+    HIP_TRY(hipMemcpy(c->wall_class_d.p, c->wall_class_host.data(), nw * 4, hipMemcpyHostToDevice));
   c->rx.grid_dirty = true;
   c->walls_host.resize(nw);
   for (uint32_t i = 0; i < nw; ++i)
@@ -160,7 +166,8 @@ int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
     h[i].time_step = sp[i].time_step;
     h[i].flags = sp[i].flags;
   }
-  if (c->rx.on) return fail(MCG_E_STATE, "species must be set before the reaction classes");
+  if (c->rx.on || c->per_replica)
+    return fail(MCG_E_STATE, "species must be set before reaction classes and per-replica parameters");
   if (c->species.alloc(ns)) return fail(MCG_E_NOMEM, "species buffer");
   HIP_TRY(hipMemcpy(c->species.p, h.data(), ns * sizeof(SpeciesRec), hipMemcpyHostToDevice));
   c->species_host = h;
@@ -245,10 +252,10 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   for (uint32_t k = 0; k < num_iterations; ++k) {
     if (c->fp32())
-      HIP_TRY(launch_diffuse_reflect32(c->stream, mf, n, c->species.p, c->P, W, c->walls32.p, c->max_abs_coord,
+      HIP_TRY(launch_diffuse_reflect32(c->stream, mf, n, c->species_table(), c->surface_classes(), c->P, W, c->walls32.p, c->max_abs_coord,
                                        c->cfg.seed, first_iteration + k, Q, c->err.p, tr));
     else
-      HIP_TRY(launch_diffuse_reflect64(c->stream, m, n, c->species.p, c->P, W, c->cfg.seed, first_iteration + k,
+      HIP_TRY(launch_diffuse_reflect64(c->stream, m, n, c->species_table(), c->surface_classes(), c->P, W, c->cfg.seed, first_iteration + k,
                                        Q, c->err.p, tr));
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -281,6 +288,34 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
   return MCG_OK;
 }
 
+int mcg_set_surface_classes(mcg_ctx* c, uint32_t num_classes, const uint8_t* action) {
+  if (!c || (num_classes && !action)) return fail(MCG_E_INVALID, "null argument");
+  if (c->num_species == 0 || !c->have_geometry) return fail(MCG_E_STATE, "geometry and species must be set first");
+  if (c->per_replica) return fail(MCG_E_STATE, "surface classes must be set before per-replica parameters");
+  const uint32_t ns = c->num_species;
+  for (uint32_t v : c->wall_class_host)
+    if (v > num_classes) return fail(MCG_E_INVALID, "a wall names a surface class beyond num_classes");
+  bool on = false;
+  std::vector<SpeciesRec> sp = c->species_host;
+  for (uint32_t k = 0; k < num_classes; ++k)
+    for (uint32_t s = 0; s < ns; ++s) {
+      const uint8_t a = action[(size_t)k * ns + s];
+      if ((a & 3u) > MCG_WALL_ABSORB || ((a >> 2) & 3u) > MCG_WALL_ABSORB || (a >> 4))
+        return fail(MCG_E_INVALID, "bad surface-class action");
+      if (a) {
+        on = true;
+        sp[s].flags |= SPF_CAN_VOLWALL;
+      }
+    }
+  if (c->sc_action_d.alloc(std::max<size_t>((size_t)num_classes * ns, 1))) return fail(MCG_E_NOMEM, "class table");
+  if (num_classes) HIP_TRY(hipMemcpy(c->sc_action_d.p, action, (size_t)num_classes * ns, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->species.p, sp.data(), ns * sizeof(SpeciesRec), hipMemcpyHostToDevice));
+  c->species_host = sp;
+  c->num_surface_classes = num_classes;
+  c->sc_on = on;
+  return MCG_OK;
+}
+
 int mcg_sync(mcg_ctx* c) {
   if (!c) return fail(MCG_E_INVALID, "null context");
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -289,7 +324,7 @@ int mcg_sync(mcg_ctx* c) {
 
 int mcg_num_molecules(mcg_ctx* c, uint64_t* count) {
   if (!c || !count) return fail(MCG_E_INVALID, "null argument");
-  if (!c->rx.on || c->num_species == 0) {
+  if ((!c->rx.on && !c->sc_on) || c->num_species == 0) {
     *count = c->n;
     return MCG_OK;
   }

@@ -46,7 +46,8 @@ struct DevBuf {
 // Buffers of the reactive path (allocated by mcg_set_bimol_rxns).
 struct ReactState {
   bool on = false;
-  std::vector<RxnRec> rxns;
+  std::vector<RxnRec> rxns;           // base table
+  std::vector<RxnRec> rxns_replicas;  // num_replicas * rxns.size() when per replica
   std::vector<int32_t> pair;  // species x species -> rxn index or -1
   DevBuf<RxnRec> rxns_d;
   DevBuf<int32_t> pair_d;
@@ -93,11 +94,23 @@ struct mcg_ctx {
   mcg::DevBuf<mcg::WallRec64> ent64;  // wall records in list (entry) order
   mcg::DevBuf<mcg::WallRec32> ent32;
   bool have_geometry = false;
+  // surface classes
+  std::vector<uint32_t> wall_class_host;
+  mcg::DevBuf<uint32_t> wall_class_d;
+  mcg::DevBuf<uint8_t> sc_action_d;
+  uint32_t num_surface_classes = 0;
+  bool sc_on = false;
+  mcg::SurfaceClasses surface_classes() const {
+    return {wall_class_d.p, sc_action_d.p, num_species, sc_on ? 1u : 0u};
+  }
 
-  // species
+  // species: one table, or after mcg_set_replica_params one table per replica
   mcg::DevBuf<mcg::SpeciesRec> species;
-  std::vector<mcg::SpeciesRec> species_host;
+  std::vector<mcg::SpeciesRec> species_host;      // base table (num_species)
+  std::vector<mcg::SpeciesRec> species_replicas;  // num_replicas * num_species when per replica
   uint32_t num_species = 0;
+  bool per_replica = false;
+  mcg::SpeciesTable species_table() const { return {species.p, per_replica ? num_species : 0u}; }
 
   // molecules (structure of arrays): FP64 or FP32 positions by precision.
   // n counts array slots; with reactions on, consumed molecules stay as

@@ -53,12 +53,14 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.R.G = r.grid;
   p.R.G.start = r.cell_start.p;
   p.R.rxns = r.rxns_d.p;
+  p.R.rxn_stride = c->per_replica ? (uint32_t)r.rxns.size() : 0u;
   p.R.pair_rxn = r.pair_d.p;
   p.R.num_species = c->num_species;
   p.R.seed = c->cfg.seed;
   p.R.F = make_filter_params(c->max_abs_coord);
   p.R.band = 2.0f * p.R.F.e_dd + 8.0f * p.R.F.eps * c->max_abs_coord;
-  p.species = c->species.p;
+  p.R.SC = c->surface_classes();
+  p.species = c->species_table();
   p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p};
   p.Sm = RMolView<PosT>{sx, sy, sz, r.sid.p, r.stag.p, r.stoff.p};
   p.key = r.key.p;
@@ -267,6 +269,7 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   if (c->rx.on) return fail(MCG_E_STATE, "reaction classes are already set");
+  if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
   if (n == 0) return MCG_OK;
   const uint32_t ns = c->num_species;
   std::vector<RxnRec> h(n);
@@ -331,6 +334,61 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   r.next_id_dirty = true;
   r.grid_dirty = true;
   r.on = true;
+  return MCG_OK;
+}
+
+int mcg_set_replica_params(mcg_ctx* c, uint32_t replica, const mcg_species* species, const mcg_bimol_rxn* rxns) {
+  if (!c || !species) return fail(MCG_E_INVALID, "null argument");
+  if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  if (replica >= c->cfg.num_replicas) return fail(MCG_E_INVALID, "replica out of range");
+  const uint32_t ns = c->num_species, R = c->cfg.num_replicas;
+  ReactState& r = c->rx;
+  const size_t nrx = r.rxns.size();
+  if (nrx && !rxns) return fail(MCG_E_INVALID, "the replica's reaction classes are required");
+  for (uint32_t s = 0; s < ns; ++s) {
+    if (!(species[s].space_step >= 0)) return fail(MCG_E_INVALID, "negative space step");
+    if (!(species[s].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
+  }
+  std::vector<RxnRec> rep_rx(nrx);
+  for (size_t i = 0; i < nrx; ++i) {
+    const mcg_bimol_rxn& x = rxns[i];
+    RxnRec q = r.rxns[i];
+    if (x.reactant_a != q.reactant_a || x.reactant_b != q.reactant_b || x.num_pathways != q.num_pathways)
+      return fail(MCG_E_INVALID, "per-replica classes must keep the base table's reactants and pathways");
+    double last = 0;
+    for (uint32_t k = 0; k < q.num_pathways; ++k) {
+      if (x.num_products[k] != q.num_products[k]) return fail(MCG_E_INVALID, "per-replica products differ");
+      for (uint32_t m = 0; m < q.num_products[k]; ++m)
+        if (x.products[k][m] != q.products[k][m]) return fail(MCG_E_INVALID, "per-replica products differ");
+      if (!(x.cum_probs[k] >= last)) return fail(MCG_E_INVALID, "cumulative probabilities must not decrease");
+      last = x.cum_probs[k];
+      q.cum_probs[k] = x.cum_probs[k];
+    }
+    q.max_fixed_p = q.cum_probs[q.num_pathways - 1];
+    q.max_p_f = (float)q.max_fixed_p;
+    rep_rx[i] = q;
+  }
+  if (!c->per_replica) {
+    c->species_replicas.clear();
+    for (uint32_t k = 0; k < R; ++k)
+      c->species_replicas.insert(c->species_replicas.end(), c->species_host.begin(), c->species_host.end());
+    r.rxns_replicas.clear();
+    for (uint32_t k = 0; k < R; ++k) r.rxns_replicas.insert(r.rxns_replicas.end(), r.rxns.begin(), r.rxns.end());
+    if (c->species.alloc((size_t)R * ns)) return fail(MCG_E_NOMEM, "species tables");
+    if (nrx && r.rxns_d.alloc((size_t)R * nrx)) return fail(MCG_E_NOMEM, "reaction tables");
+    c->per_replica = true;
+  }
+  for (uint32_t s = 0; s < ns; ++s) {
+    SpeciesRec& d = c->species_replicas[(size_t)replica * ns + s];
+    d.space_step = species[s].space_step;
+    d.time_step = species[s].time_step;
+  }
+  for (size_t i = 0; i < nrx; ++i) r.rxns_replicas[(size_t)replica * nrx + i] = rep_rx[i];
+  HIP_TRY(hipMemcpy(c->species.p, c->species_replicas.data(), c->species_replicas.size() * sizeof(SpeciesRec),
+                    hipMemcpyHostToDevice));
+  if (nrx)
+    HIP_TRY(hipMemcpy(r.rxns_d.p, r.rxns_replicas.data(), r.rxns_replicas.size() * sizeof(RxnRec),
+                      hipMemcpyHostToDevice));
   return MCG_OK;
 }
 

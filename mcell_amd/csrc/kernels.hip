@@ -38,7 +38,7 @@ __global__ void k_reset_counters(uint32_t* c, uint32_t n) {
 }
 
 // ------------------------------------------------------------------- FP64 --
-__global__ __launch_bounds__(256) void k_light64(MolView64 m, uint32_t n, const SpeciesRec* __restrict__ species,
+__global__ __launch_bounds__(256) void k_light64(MolView64 m, uint32_t n, SpeciesTable species,
                                                  PartitionParams P, WallView64 W, uint64_t seed, uint32_t iteration,
                                                  SlowQueue Q, HitTrace trace) {
   __shared__ ZigguratTables T;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_light64(MolView64 m, uint32_t n, const 
   bool hard = false;
   if (i < n) {
     const uint32_t tag = m.tag[i];
-    const SpeciesRec sp = species[tag_species(tag)];
+    const SpeciesRec sp = species.of(tag);
     if (!(tag_flags(tag) & MF_DEFUNCT) && sp.space_step > 0) {
       PhiloxStream rng;
       rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_light64(MolView64 m, uint32_t n, const 
   wave_append(hard, i, Q.slots + (size_t)shard * Q.shard_cap, Q.count + shard);
 }
 
-__global__ __launch_bounds__(256) void k_hard64(MolView64 m, const SpeciesRec* __restrict__ species,
+__global__ __launch_bounds__(256) void k_hard64(MolView64 m, SpeciesTable species, SurfaceClasses SC,
                                                 PartitionParams P, WallView64 W, uint64_t seed, uint32_t iteration,
                                                 SlowQueue Q, uint32_t* __restrict__ err_out, HitTrace trace) {
   __shared__ ZigguratTables T;
@@ -80,24 +80,25 @@ __global__ __launch_bounds__(256) void k_hard64(MolView64 m, const SpeciesRec* _
   for (uint32_t q = threadIdx.x; q < count; q += blockDim.x) {
     const uint32_t i = slots[q];
     const uint32_t tag = m.tag[i];
-    const SpeciesRec sp = species[tag_species(tag)];
+    const SpeciesRec sp = species.of(tag);
     PhiloxStream rng;
     rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
     D3 pos = {m.x[i], m.y[i], m.z[i]};
     const D3 disp = draw_displacement(rng, T, sp.space_step);
     uint32_t err = 0, tn = 0;
     uint32_t* tw = trace.enabled ? trace.wall + (size_t)i * kTraceHits : nullptr;
-    diffuse_reflect64(P, W.walls, W.off, W.idx, pos, disp, rng, err, tw, tn);
+    const bool alive = diffuse_reflect64(P, W.walls, W.off, W.idx, SC, tag_species(tag), pos, disp, rng, err, tw, tn);
     m.x[i] = pos.x;
     m.y[i] = pos.y;
     m.z[i] = pos.z;
+    if (!alive) m.tag[i] = tag | (MF_DEFUNCT << 8);
     if (trace.enabled) trace.count[i] = tn;
     if (err) atomicOr(err_out, err);
   }
 }
 
 // ------------------------------------------------------------------- FP32 --
-__global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, const SpeciesRec* __restrict__ species,
+__global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, SpeciesTable species,
                                                  PartitionParams P, Walls32View W, FilterParams F, float band,
                                                  uint64_t seed, uint32_t iteration, SlowQueue Q, HitTrace trace) {
   __shared__ ZigguratTables T;
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, const 
   bool hard = false;
   if (i < n) {
     const uint32_t tag = m.tag[i];
-    const SpeciesRec sp = species[tag_species(tag)];
+    const SpeciesRec sp = species.of(tag);
     if (!(tag_flags(tag) & MF_DEFUNCT) && sp.space_step > 0) {
       PhiloxStream rng;
       rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, const 
   wave_append(hard, i, Q.slots + (size_t)shard * Q.shard_cap, Q.count + shard);
 }
 
-__global__ __launch_bounds__(256) void k_hard32(MolView32 m, const SpeciesRec* __restrict__ species,
+__global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable species, SurfaceClasses SC,
                                                 PartitionParams P, Walls32View W, FilterParams F, float band,
                                                 uint64_t seed, uint32_t iteration, SlowQueue Q,
                                                 uint32_t* __restrict__ err_out, HitTrace trace) {
@@ -142,14 +143,16 @@ __global__ __launch_bounds__(256) void k_hard32(MolView32 m, const SpeciesRec* _
   for (uint32_t q = threadIdx.x; q < count; q += blockDim.x) {
     const uint32_t i = slots[q];
     const uint32_t tag = m.tag[i];
-    const SpeciesRec sp = species[tag_species(tag)];
+    const SpeciesRec sp = species.of(tag);
     PhiloxStream rng;
     rng.init(seed, m.id[i], iteration, tag_replica(tag), 0);
     D3 pos = {(double)m.x[i], (double)m.y[i], (double)m.z[i]};
     const D3 disp = draw_displacement(rng, T, sp.space_step);
     uint32_t err = 0, tn = 0;
     uint32_t* tw = trace.enabled ? trace.wall + (size_t)i * kTraceHits : nullptr;
-    bool near_wall = diffuse_reflect32(P, W, F, band, pos, disp, rng, err, tw, tn);
+    bool alive = true;
+    bool near_wall = diffuse_reflect32(P, W, F, band, SC, tag_species(tag), pos, disp, rng, err, tw, tn, alive);
+    if (!alive) m.tag[i] = tag | (MF_DEFUNCT << 8);
     float fx, fy, fz;
     if (near_wall) {
       round_to_fp32_same_side(P, W.w64, W.off, W.idx, pos.x, pos.y, pos.z, &fx, &fy, &fz);
@@ -183,7 +186,8 @@ uint32_t shard_capacity(uint32_t n) {
 }
 
 
-hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, const SpeciesRec* species,
+hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, SpeciesTable species,
+                                    const SurfaceClasses& SC,
                                     const PartitionParams& P, const WallView64& W, uint64_t seed, uint32_t iteration,
                                     SlowQueue Q, uint32_t* err, const HitTrace& trace) {
   if (n == 0) return hipSuccess;
@@ -191,11 +195,12 @@ hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t 
   hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
   hipLaunchKernelGGL(k_light64, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W, seed, iteration, Q,
                      trace);
-  hipLaunchKernelGGL(k_hard64, dim3(kQueueShards), dim3(256), 0, s, m, species, P, W, seed, iteration, Q, err, trace);
+  hipLaunchKernelGGL(k_hard64, dim3(kQueueShards), dim3(256), 0, s, m, species, SC, P, W, seed, iteration, Q, err, trace);
   return hipGetLastError();
 }
 
-hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t n, const SpeciesRec* species,
+hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t n, SpeciesTable species,
+                                    const SurfaceClasses& SC,
                                     const PartitionParams& P, const WallView64& W, const WallRec32* w32,
                                     float max_abs_coord, uint64_t seed, uint32_t iteration, SlowQueue Q,
                                     uint32_t* err, const HitTrace& trace) {
@@ -208,7 +213,7 @@ hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t 
   hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
   hipLaunchKernelGGL(k_light32, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W32, F, band, seed,
                      iteration, Q, trace);
-  hipLaunchKernelGGL(k_hard32, dim3(kQueueShards), dim3(256), 0, s, m, species, P, W32, F, band, seed, iteration, Q,
+  hipLaunchKernelGGL(k_hard32, dim3(kQueueShards), dim3(256), 0, s, m, species, SC, P, W32, F, band, seed, iteration, Q,
                      err, trace);
   return hipGetLastError();
 }

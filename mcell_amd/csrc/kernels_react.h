@@ -49,7 +49,7 @@ enum : uint32_t {
 template <typename PosT>
 struct ReactPipeline {
   ReactParams R;
-  const SpeciesRec* species;
+  SpeciesTable species;
   RMolView<PosT> M;   // molecules: input of the sort, output of the moves
   RMolView<PosT> Sm;  // cell-sorted snapshot
   uint32_t *key, *rank, *cell_count, *cell_start;

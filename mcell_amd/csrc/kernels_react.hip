@@ -126,7 +126,7 @@ template <typename PosT>
 __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, RMolView<PosT> src,
                                               RMolView<PosT> out, const uint32_t* __restrict__ list,
                                               const uint32_t* __restrict__ count,
-                                              const SpeciesRec* __restrict__ species, double barrier,
+                                              SpeciesTable species, double barrier,
                                               uint32_t propose_only, Proposal* __restrict__ prop,
                                               uint32_t* __restrict__ plist, uint32_t* __restrict__ pcount,
                                               uint32_t* __restrict__ owner, uint32_t* __restrict__ err_out,
@@ -147,13 +147,14 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
     const uint32_t tag = src.tag[j];
     const uint32_t idv = src.id[j];
     const double toff = src.toff[j];
-    const SpeciesRec sp = species[tag_species(tag)];
+    const SpeciesRec sp = species.of(tag);
     D3 pos = {(double)src.x[j], (double)src.y[j], (double)src.z[j]};
     uint32_t err = 0;
     MoverEnd e;
     e.pos = pos;
     e.t_steps = 0;
     e.near_plane = false;
+    e.absorbed = false;
     Proposal p;
     bool moves = sp.space_step > 0;
     if (moves) proposed = react_mover(R, S, sp, T, pos, idv, tag, toff, barrier, p, e, err);
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
     if (!proposed || propose_only) {
       store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
       out.id[j] = idv;
-      out.tag[j] = tag;
+      out.tag[j] = e.absorbed ? (tag | (MF_DEFUNCT << 8)) : tag;
       out.toff[j] = moves ? next_toff(R.iteration, toff, e.t_steps) : toff;
     }
     if (err) atomicOr(err_out, err);
@@ -267,7 +268,8 @@ __device__ __forceinline__ uint32_t new_products(const RxnRec& r, uint32_t pathw
 
 __global__ __launch_bounds__(256) void k_prod_counts(const uint32_t* __restrict__ sorted_slots, uint32_t n,
                                                      const Proposal* __restrict__ prop,
-                                                     const RxnRec* __restrict__ rxns, uint32_t* __restrict__ cnt) {
+                                                     const RxnRec* __restrict__ rxns, uint32_t rxn_stride,
+                                                     uint32_t* __restrict__ cnt) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > n) return;
   if (k == n) {
@@ -275,7 +277,7 @@ __global__ __launch_bounds__(256) void k_prod_counts(const uint32_t* __restrict_
     return;
   }
   const Proposal& p = prop[sorted_slots[k]];
-  cnt[k] = new_products(rxns[p.rxn], p.pathway);
+  cnt[k] = new_products(rxns[p.replica * rxn_stride + p.rxn], p.pathway);
 }
 
 __global__ __launch_bounds__(256) void k_seg_heads(const uint64_t* __restrict__ keys, uint32_t n,
@@ -303,7 +305,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const Proposal& p = prop[sorted_slots[k]];
-  const RxnRec& rx = R.rxns[p.rxn];
+  const RxnRec& rx = R.rxns[p.replica * R.rxn_stride + p.rxn];
   const uint32_t rep = (uint32_t)(keys[k] >> 32);
   const uint32_t base = hdr[kHdrNumOut];
   const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
@@ -447,7 +449,7 @@ hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t
   hipError_t e =
       hipcub::DeviceRadixSort::SortPairs(scan_tmp, tmp, keys_a, keys_b, alist, alist_sorted, n_acc, 0, 64, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_prod_counts, blocks_for(n_acc + 1), dim3(256), 0, s, alist_sorted, n_acc, prop, R.rxns, cnt);
+  hipLaunchKernelGGL(k_prod_counts, blocks_for(n_acc + 1), dim3(256), 0, s, alist_sorted, n_acc, prop, R.rxns, R.rxn_stride, cnt);
   tmp = scan_tmp_bytes;
   e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt, pre, n_acc + 1, s);
   if (e != hipSuccess) return e;

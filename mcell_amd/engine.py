@@ -53,15 +53,20 @@ class DiffuseReactEngine:
                       propose_only: bool = False, log_capacity: int = 0, trace: bool = False):
         """An engine holding the workload's geometry, species, reaction
         classes and molecules (every replica)."""
-        from .workloads import rxn_table
+        from .workloads import rxn_table, species_steps
         cap = capacity if capacity is not None else max(2 * wl.num_molecules, 1024)
         eng = cls(wl.part, precision=precision, seed=wl.config.seed if seed is None else seed,
                   num_replicas=wl.num_replicas, capacity=cap, trace=trace, barrier_interval=wl.barrier_interval,
                   propose_only=propose_only)
         eng.set_geometry(wl.vertices, wl.walls, wl.wall_class)
         eng.set_species(wl.species_steps)
+        if wl.surface_classes is not None:
+            eng.set_surface_classes(wl.surface_classes)
         if wl.rxns:
             eng.set_bimol_rxns(rxn_table(wl))
+        if wl.replica_D is not None:
+            for r in range(wl.num_replicas):
+                eng.set_replica_params(r, species_steps(wl, r), rxn_table(wl, r) if wl.rxns else None)
         if log_capacity:
             eng.set_rxn_log(log_capacity)
         for r in range(wl.num_replicas):
@@ -99,6 +104,15 @@ class DiffuseReactEngine:
 
     def set_bimol_rxns(self, rxns):
         """rxns: (reactant_a, reactant_b, cum_probs, products per pathway)."""
+        _lib.check(self.lib.mcg_set_bimol_rxns(self.ctx, self._rxn_array(rxns), len(rxns)))
+
+    def set_surface_classes(self, action: np.ndarray):
+        """action: uint8 [num_classes, num_species], front | back << 2."""
+        a = np.ascontiguousarray(action, np.uint8)
+        _lib.check(self.lib.mcg_set_surface_classes(self.ctx, a.shape[0], _ptr(a)))
+
+    @staticmethod
+    def _rxn_array(rxns):
         arr = (_lib.McgBimolRxn * max(len(rxns), 1))()
         for i, (a, b, cum, prods) in enumerate(rxns):
             arr[i].reactant_a = a
@@ -109,7 +123,16 @@ class DiffuseReactEngine:
                 arr[i].num_products[k] = len(prods[k])
                 for q, ps in enumerate(prods[k]):
                     arr[i].products[k][q] = ps
-        _lib.check(self.lib.mcg_set_bimol_rxns(self.ctx, arr, len(rxns)))
+        return arr
+
+    def set_replica_params(self, replica: int, species: Sequence[tuple], rxns=None):
+        sp = (_lib.McgSpecies * len(species))()
+        for i, (ss, ts, fl) in enumerate(species):
+            sp[i].space_step = ss
+            sp[i].time_step = ts
+            sp[i].flags = fl
+        arr = None if rxns is None else self._rxn_array(rxns)
+        _lib.check(self.lib.mcg_set_replica_params(self.ctx, replica, sp, arr))
 
     def set_rxn_log(self, capacity: int):
         _lib.check(self.lib.mcg_set_rxn_log(self.ctx, capacity))

@@ -16,7 +16,7 @@ Sizes follow the five workloads of the MCell4 diffuse-react benchmark:
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List
+from typing import List, Optional
 
 import numpy as np
 
@@ -42,6 +42,13 @@ class Workload:
     config: M.Config = None
     objects: List[M.GeometryObject] = field(default_factory=list)
     barrier_interval: int = 1     # iterations between count barriers
+    replica_D: Optional[np.ndarray] = None  # (num_replicas, num_species) cm^2/s overrides, or None
+    surface_classes: Optional[np.ndarray] = None  # uint8 (num_classes, num_species): front | back << 2
+
+    def species_for(self, replica: int) -> List[M.Species]:
+        if self.replica_D is None:
+            return self.species
+        return [M.Species(s.name, float(d)) for s, d in zip(self.species, self.replica_D[replica])]
 
     @property
     def num_molecules(self):
@@ -60,14 +67,21 @@ def _species_table(species, part, rxns):
     return out
 
 
-def rxn_table(wl: Workload):
+def rxn_table(wl: Workload, replica: Optional[int] = None):
     """The reaction classes as the engine takes them: (a, b, cum_probs,
-    products per pathway), with P = rate * pb_factor (one pathway)."""
+    products per pathway), with P = rate * pb_factor (one pathway), for the
+    base species or for one replica's species."""
+    species = wl.species if replica is None else wl.species_for(replica)
     out = []
     for r in wl.rxns:
-        pb = r.pb_factor(wl.part, wl.species[r.reactant_a], wl.species[r.reactant_b])
+        pb = r.pb_factor(wl.part, species[r.reactant_a], species[r.reactant_b])
         out.append((r.reactant_a, r.reactant_b, [r.rate * pb], [list(r.products)]))
     return out
+
+
+def species_steps(wl: Workload, replica: int):
+    """(space_step, time_step, flags) of one replica's species."""
+    return _species_table(wl.species_for(replica), wl.part, wl.rxns)
 
 
 def free_diffusion_box(scale: float = 1.0, iterations: int = 1000, seed: int = 1) -> Workload:
@@ -121,3 +135,92 @@ def bimol_box(n_a: int = 50000, n_b: int = 50000, iterations: int = 2000, seed: 
                     np.tile(np.arange(n, dtype=np.uint32), num_replicas), np.tile(sp1, num_replicas), pos,
                     np.repeat(np.arange(num_replicas, dtype=np.uint32), n), iterations, rxns=rxns,
                     num_replicas=num_replicas, config=cfg, objects=[box], barrier_interval=barrier_interval)
+
+
+def place_outside_sphere(n: int, half_edge_um: float, radius_um: float, seed: int, part: M.PartitionConfig):
+    """Uniform positions in the cube [-h, h]^3 outside the centred sphere of
+    the given radius (rejection sampling, numpy PCG64), internal units."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.empty((0, 3))
+    r2 = radius_um * radius_um
+    while len(out) < n:
+        p = rng.uniform(-half_edge_um, half_edge_um, size=(2 * (n - len(out)) + 16, 3))
+        p = p[(p * p).sum(axis=1) > r2]
+        out = np.concatenate([out, p])
+    return out[:n] * part.rcp_length_unit
+
+
+def replicas_sphere_in_cube(num_replicas: int = 256, iterations: int = 500, seed: int = 4,
+                            subpartition_dimension: float = 0.05, n_min: int = 1000, n_max: int = 200000,
+                            d_min: float = 1e-7, d_max: float = 5e-6, rate: float = 1e8,
+                            sphere_subdivisions: int = 5) -> Workload:
+    """Batched replicas of A + B -> C over one mesh: a reflective 2 um cube
+    holding a reflective icosphere of radius 0.5 um (API subdivisions 5: 5,120
+    triangles).  Instance i draws N_A ~ U[n_min, n_max] (N_B = N_A / 2) and one
+    D ~ U[d_min, d_max] cm^2/s for A and B from PCG64(seed); C diffuses with
+    D = 5e-7.  Instance i places its molecules outside the sphere with seed
+    seed + i; its ids are 0..N_A-1 (A) then B."""
+    cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
+    box = M.create_box("box", edge_dimension=2.0)
+    ball = M.create_icosphere("sphere", 0.5, sphere_subdivisions)
+    objects = [box, ball]
+    part = M.derive_partition(cfg, M.bounding_box(objects), has_bimol_vol_rxns=True)
+    verts, walls, classes = M.vertices_internal(objects, part)
+    species = [M.Species("A", 1e-6), M.Species("B", 1e-6), M.Species("C", 5e-7)]
+    rxns = [M.BimolReaction(0, 1, rate, [2])]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_a = rng.integers(n_min, n_max + 1, size=num_replicas)
+    d = rng.uniform(d_min, d_max, size=num_replicas)
+    replica_D = np.stack([d, d, np.full(num_replicas, 5e-7)], axis=1)
+    pos, ids, sp, rep = [], [], [], []
+    for i in range(num_replicas):
+        na, nb = int(n_a[i]), int(n_a[i]) // 2
+        pos.append(place_outside_sphere(na + nb, 1.0, 0.5, seed + i, part))
+        ids.append(np.arange(na + nb, dtype=np.uint32))
+        sp.append(np.concatenate([np.zeros(na, np.uint32), np.ones(nb, np.uint32)]))
+        rep.append(np.full(na + nb, i, np.uint32))
+    wl = Workload("replicas_sphere_in_cube", part, verts, walls, classes, species,
+                  _species_table(species, part, rxns), np.concatenate(ids), np.concatenate(sp),
+                  np.concatenate(pos), np.concatenate(rep), iterations, rxns=rxns, num_replicas=num_replicas,
+                  config=cfg, objects=objects, barrier_interval=10, replica_D=replica_D)
+    return wl
+
+
+def with_subpartition(wl: Workload, subpartition_dimension: float) -> Workload:
+    """The same workload on a partition with another subpartition edge (um):
+    molecules and mesh are unchanged in internal units, only the grid the
+    engine bins walls and molecules into differs."""
+    import dataclasses
+    cfg = dataclasses.replace(wl.config, subpartition_dimension=subpartition_dimension)
+    part = M.derive_partition(cfg, M.bounding_box(wl.objects), has_bimol_vol_rxns=bool(wl.rxns))
+    return dataclasses.replace(wl, part=part, config=cfg, species_steps=_species_table(wl.species, part, wl.rxns))
+
+
+def sphere_in_cube(n_l: int = 100000, n_t: int = 10000, iterations: int = 1000, seed: int = 3,
+                   subpartition_dimension: float = 0.5, sphere_subdivisions: int = 5,
+                   t_action: int = 1) -> Workload:
+    """The volume part of the sphere-in-cube workload: a reflective 2 um cube
+    holding an icosphere of radius 0.5 um (API subdivisions 5: 5,120
+    triangles) with surface class 1 on the sphere.  L (D = 1e-6 cm^2/s) starts
+    outside the sphere and is reflected by it; T (D = 1e-6) starts anywhere in
+    the cube and the class acts on it with `t_action` from both sides
+    (1 = transparent, 2 = absorptive).  Placement seed `seed` (L), seed + 1 (T);
+    ids are 0..n_l-1 (L) then T."""
+    cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
+    box = M.create_box("box", edge_dimension=2.0)
+    ball = M.create_icosphere("sphere", 0.5, sphere_subdivisions)
+    ball.surface_class = 1
+    objects = [box, ball]
+    part = M.derive_partition(cfg, M.bounding_box(objects), has_bimol_vol_rxns=False)
+    verts, walls, classes = M.vertices_internal(objects, part)
+    species = [M.Species("L", 1e-6), M.Species("T", 1e-6)]
+    sc = np.array([[0, t_action | (t_action << 2)]], np.uint8)
+    pos = np.concatenate([place_outside_sphere(n_l, 1.0, 0.5, seed, part),
+                          M.place_uniform_in_box(n_t, 1.0, seed + 1, part)])
+    sp = np.concatenate([np.zeros(n_l, np.uint32), np.ones(n_t, np.uint32)])
+    n = n_l + n_t
+    steps = _species_table(species, part, [])
+    steps = [(ss, ts, f | (2 if sc[0, k] else 0)) for k, (ss, ts, f) in enumerate(steps)]
+    return Workload("sphere_in_cube", part, verts, walls, classes, species, steps, np.arange(n, dtype=np.uint32),
+                    sp, pos, np.zeros(n, np.uint32), iterations, config=cfg, objects=objects,
+                    surface_classes=sc)

@@ -59,6 +59,32 @@ void orc_add_walls(void* wp, const uint32_t* tri, uint32_t n, uint32_t vert_offs
 
 void orc_finalize_walls(void* wp) { static_cast<World*>(wp)->finalize_walls(); }
 
+// Surface classes 1..n: action[(c - 1) * num_species + s] = front | back << 2
+// (0 reflect, 1 transparent, 2 absorptive).  Species with any non-reflect
+// action get SPF_CAN_VOLWALL.  Call after the species are added.
+void orc_set_surface_classes(void* wp, uint32_t n, const uint8_t* action) {
+  World* w = static_cast<World*>(wp);
+  size_t ns = w->species.size();
+  w->num_surface_classes = n;
+  w->surf_action.assign(action, action + n * ns);
+  for (size_t s = 0; s < ns; ++s)
+    for (uint32_t c = 0; c < n; ++c)
+      if (action[c * ns + s] != 0) w->species[s].flags |= SPF_CAN_VOLWALL;
+}
+
+// Sets the surface class of walls [first, first + n).
+void orc_set_wall_classes(void* wp, uint32_t first, uint32_t n, uint32_t cls) {
+  World* w = static_cast<World*>(wp);
+  for (uint32_t i = first; i < first + n && i < w->walls.size(); ++i) w->walls[i].surf_class = cls;
+}
+
+// transparent crossings, absorbed molecules
+void orc_get_wall_stats(void* wp, uint64_t* out) {
+  World* w = static_cast<World*>(wp);
+  out[0] = w->transparent_crossings;
+  out[1] = w->absorbed;
+}
+
 uint32_t orc_num_walls(void* wp) { return (uint32_t)static_cast<World*>(wp)->walls.size(); }
 
 // 16 doubles per wall: normal xyz, unit_u xyz, unit_v xyz, d, uv_vert1_u,

@@ -192,6 +192,9 @@ struct Wall {
 };
 
 // ------------------------------------------------------------------ species
+// surface-class actions (reflect is the default of every wall)
+enum : int { SC_REFLECT = 0, SC_TRANSPARENT = 1, SC_ABSORB = 2 };
+
 enum : uint32_t {
   SPF_CAN_VOLVOL = 1u << 0,
   SPF_CAN_VOLWALL = 1u << 1,
@@ -276,6 +279,11 @@ class World {
   std::vector<Species> species;
   std::vector<BimolRxn> rxns;
   std::vector<int32_t> pair_rxn;        // species x species -> rxn index or -1
+  // surface classes: action of class c (1-based on walls) for species s hit
+  // from the front (bits 0-1) and from the back (bits 2-3)
+  uint32_t num_surface_classes = 0;
+  std::vector<uint8_t> surf_action;     // [(c - 1) * species.size() + s]
+  uint64_t transparent_crossings = 0, absorbed = 0;
   std::vector<Mol> mols;                // indexed by id
   uint64_t iteration = 0;
 
@@ -377,6 +385,9 @@ class World {
   void jump_away_line(V3 p, double k, V3 A, V3 B, V3 n, V3& v);
   bool collide_mol(const Mol& vm, V3 displacement, uint32_t partner, double& t, V3& pos) const;
   void reflect_from_wall(const Collision& c, Mol& vm, V3& disp, double& t_steps, uint32_t& last_hit);
+  int wall_action(const Wall& w, uint32_t species, CollType type) const;
+  void cross_transparent_wall(const Collision& c, Mol& vm, V3& disp, double& t_steps, double& elapsed,
+                              uint32_t& last_hit);
   bool collide_and_react_with_vol_mol(const Collision& c, V3& disp, double t_steps, double r_rate_factor,
                                       double elapsed);
   int test_bimolecular(const BimolRxn& rx, double scaling);

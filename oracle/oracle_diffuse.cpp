@@ -217,8 +217,30 @@ void World::diffuse_vol_molecule(uint32_t id, double& max_time, double diffusion
           break;
         }
       } else {
-        // walls are reflective (no surface classes on this path yet)
         Mol& vm = mols[id];
+        // collide_and_react_with_walls (diffuse_react_event.cpp:529-549, 991-1067)
+        if (species[vm.species].flags & SPF_CAN_VOLWALL) {
+          int act = wall_action(walls[c.wall], vm.species, c.type);
+          if (act == SC_TRANSPARENT) {
+            cross_transparent_wall(c, vm, disp, t_steps, elapsed, last_hit);
+            ++transparent_crossings;
+            if (trace) hits.push_back({vm.id, (uint32_t)iteration, c.wall, (uint32_t)c.type, vm.subpart, c.time});
+            break;
+          }
+          if (act == SC_ABSORB) {
+            // test_intersect (rxn_utils.inl:593-626) with the absorptive class's
+            // unbounded rate: two draws, then outcome_intersect destroys the molecule
+            DrawSource& r = rng();
+            (void)r.dbl();
+            (void)r.dbl();
+            vm.flags |= MF_DEFUNCT;
+            register_reactant(vm, false);
+            ++absorbed;
+            if (trace) hits.push_back({vm.id, (uint32_t)iteration, c.wall, (uint32_t)c.type, vm.subpart, c.time});
+            was_defunct = true;
+            break;
+          }
+        }
         elapsed += t_steps * c.time;
         reflect_from_wall(c, vm, disp, t_steps, last_hit);
         ++mol_wall_reflections;
@@ -546,6 +568,31 @@ bool World::collide_mol(const Mol& vm, V3 displacement, uint32_t pid, double& t,
   t = d / movelen2;
   cpos = vm.pos + t * displacement;
   return true;
+}
+
+// The surface class of a wall decides what a hit does to a species:
+// trigger_intersect / find_mol_reactions_with_surf_classes
+// (rxn_utils.inl:120-287) look the class up by the wall's regions and the hit
+// orientation (front = WALL_FRONT).
+int World::wall_action(const Wall& w, uint32_t sp, CollType type) const {
+  if (w.surf_class == 0 || w.surf_class > num_surface_classes) return SC_REFLECT;
+  uint8_t a = surf_action[(size_t)(w.surf_class - 1) * species.size() + sp];
+  return type == CollType::WALL_FRONT ? (a & 3) : ((a >> 2) & 3);
+}
+
+// cross_transparent_wall (diffuse_react_event.cpp:3007-3099), the branch for
+// walls that bound no compartment: move to the hit point, keep the direction
+// and scale the remaining displacement and time.
+void World::cross_transparent_wall(const Collision& c, Mol& vm, V3& disp, double& t_steps, double& elapsed,
+                                   uint32_t& last_hit) {
+  vm.pos = c.pos;
+  vm.subpart = subpart_index(vm.pos);
+  double t_smash = c.time;
+  disp = disp * v3(1.0 - t_smash);
+  elapsed += t_steps * t_smash;
+  t_steps *= (1.0 - t_smash);
+  if (t_steps < EPS) t_steps = EPS;
+  last_hit = c.wall;
 }
 
 // collision_utils.inl:1711-1747 (FP64 branch)

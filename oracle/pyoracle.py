@@ -62,6 +62,9 @@ def load():
         "orc_get_rxn_log": (u64, [P, P, P, P]),
         "orc_clear_rxn_log": (None, [P]),
         "orc_exact_disk": (d, [P, P, P, u32]),
+        "orc_set_surface_classes": (None, [P, u32, P]),
+        "orc_set_wall_classes": (None, [P, u32, u32, u32]),
+        "orc_get_wall_stats": (None, [P, P]),
         "orc_run": (ctypes.c_int, [P, u32, P]),
         "orc_iteration": (u64, [P]),
         "orc_num_molecules": (u32, [P]),
@@ -138,6 +141,16 @@ class OracleWorld:
         t = np.ascontiguousarray(obj.walls(), np.uint32)
         first = self.lib.orc_add_vertices_um(self.w, _p(v), len(v))
         self.lib.orc_add_walls(self.w, _p(t), len(t), first, object_id, obj.surface_class)
+
+    def set_surface_classes(self, action):
+        """action: uint8 array [num_classes, num_species], front | back << 2."""
+        a = np.ascontiguousarray(action, np.uint8)
+        self.lib.orc_set_surface_classes(self.w, a.shape[0], _p(a))
+
+    def wall_stats(self):
+        out = np.zeros(2, np.uint64)
+        self.lib.orc_get_wall_stats(self.w, _p(out))
+        return {"transparent_crossings": int(out[0]), "absorbed": int(out[1])}
 
     def finalize_walls(self):
         self.lib.orc_finalize_walls(self.w)
@@ -264,10 +277,13 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
     for k, obj in enumerate(wl.objects):
         w.add_object(obj, k)
     w.finalize_walls()
-    for s in wl.species:
+    species = wl.species_for(replica) if hasattr(wl, "species_for") else wl.species
+    for s in species:
         w.add_species(s.diffusion_constant_3d)
     for r in wl.rxns:
         w.add_bimol_rxn(r.reactant_a, r.reactant_b, r.rate, r.products)
+    if getattr(wl, "surface_classes", None) is not None:
+        w.set_surface_classes(wl.surface_classes)
     sel = wl.replica == replica
     ids = wl.ids[sel]
     assert np.array_equal(ids, np.arange(len(ids))), "oracle ids are insertion order"

@@ -175,3 +175,45 @@ def test_fp32_snapshot_proposals_agree_with_fp64(gpu_available):
     agree = len(a & b) / max(len(a | b), 1)
     print("fp32 vs fp64 proposals: %d / %d agree (%.4f)" % (len(a & b), len(a | b), agree), flush=True)
     assert len(a) > 100 and agree >= 0.98
+
+
+def test_replica_params_sphere_mesh_snapshot_parity(gpu_available, oracle_lib):
+    """Replicas with their own diffusion constants and reaction probabilities
+    over the shared sphere-in-cube mesh: each replica's proposals and
+    positions equal the oracle's for that instance."""
+    from oracle import pyoracle as po
+    seed = 13
+    wl = W.replicas_sphere_in_cube(num_replicas=3, iterations=6, n_min=12000, n_max=20000, rate=1.36e9,
+                                   subpartition_dimension=0.1)
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed, propose_only=True, log_capacity=1 << 16)
+    worlds = []
+    for r in range(wl.num_replicas):
+        w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed, replica=r)
+        w.set_snapshot(True)
+        w.set_barrier_interval(wl.barrier_interval)
+        worlds.append(w)
+    total = 0
+    for it in range(wl.iterations):
+        eng.step(it, 1)
+        g = eng.rxn_log()
+        m = eng.molecules()
+        for r, w in enumerate(worlds):
+            w.run(1)
+            u, rx, d = w.rxn_log()
+            w.lib.orc_clear_rxn_log(w.w)
+            gr = g[g["replica"] == r]
+            gr = gr[np.argsort(gr["mover_id"], kind="stable")]
+            o = np.argsort(u[:, 1], kind="stable")
+            assert np.array_equal(gr["mover_id"], u[o, 1]) and np.array_equal(gr["partner_id"], u[o, 2])
+            np.testing.assert_allclose(gr["time"], d[o, 0], rtol=1e-12, atol=0)
+            total += len(gr)
+            sel = m["replica"] == r
+            ids = m["id"][sel]
+            pos = m["pos"][sel][np.argsort(ids)]
+            om = w.molecules()
+            assert np.abs(pos - om["pos"]).max() <= 1e-12 * 100, (it, r)
+        print("iteration %d: %d proposals so far" % (it, total), flush=True)
+    assert total > 50
+    for w in worlds:
+        w.close()
+    eng.close()
