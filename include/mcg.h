@@ -105,13 +105,30 @@ typedef struct mcg_bimol_rxn {
   uint32_t products[4][3];         /* product species per pathway */
 } mcg_bimol_rxn;
 
+/* A unimolecular reaction class of one volume species (BNG::RxnClass with
+ * one reactant as pick_unimol_rxn_classes, time_of_unimol and
+ * which_unimolecular use it, rxn_utils.inl:696-783).  Rates are cumulative
+ * over pathways, per internal time unit (rate [1/s] * time_unit; the last
+ * entry is k_tot).  A molecule draws its lifetime -ln(u)/k_tot when it is
+ * created; at that time it is replaced by the pathway's products, which are
+ * new volume molecules at its position (outcome_unimolecular,
+ * diffuse_react_event.cpp:2939-3003). */
+typedef struct mcg_unimol_rxn {
+  uint32_t reactant;
+  uint32_t num_pathways;           /* 1..4 */
+  double cum_rates[4];
+  uint32_t num_products[4];        /* per pathway, 0..3 */
+  uint32_t products[4][3];
+} mcg_unimol_rxn;
+
 /* One reaction that happened (or, with MCG_FLAG_PROPOSE_ONLY, was proposed). */
 typedef struct mcg_rxn_event {
   double time;                /* absolute time in iterations */
   double pos[3];              /* collision point */
   uint32_t iteration, replica;
   uint32_t mover_id, partner_id;
-  uint32_t rxn, pathway;      /* index into the mcg_set_bimol_rxns table, pathway */
+  uint32_t rxn, pathway;      /* index into the mcg_set_bimol_rxns table, or for a unimolecular
+                                 reaction 0x80000000 | reactant species (partner_id 0xFFFFFFFF) */
   uint32_t first_product_id;  /* products get ids first_product_id + k */
   uint32_t num_products;
 } mcg_rxn_event;
@@ -178,6 +195,9 @@ int mcg_set_bimol_rxns(mcg_ctx* ctx, const mcg_bimol_rxn* rxns, uint32_t num_rxn
  * mcg_set_bimol_rxns; replicas never set keep the base tables.  No
  * counterpart in MCell4, where each instance is its own process. */
 int mcg_set_replica_params(mcg_ctx* ctx, uint32_t replica, const mcg_species* species, const mcg_bimol_rxn* rxns);
+
+/* Unimolecular reaction classes (at most one per species). */
+int mcg_set_unimol_rxns(mcg_ctx* ctx, const mcg_unimol_rxn* rxns, uint32_t num_rxns);
 
 /* Keeps up to `capacity` reaction events of each mcg_step call (0 = off). */
 int mcg_set_rxn_log(mcg_ctx* ctx, uint64_t capacity);

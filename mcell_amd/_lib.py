@@ -35,6 +35,7 @@ EXPORTED = [
     "mcg_add_volume_molecules", "mcg_step", "mcg_sync", "mcg_num_molecules", "mcg_get_volume_molecules",
     "mcg_get_walls", "mcg_get_hit_trace", "mcg_last_kernel_time", "mcg_set_bimol_rxns", "mcg_set_rxn_log",
     "mcg_get_rxn_log", "mcg_count_species", "mcg_set_replica_params", "mcg_set_surface_classes",
+    "mcg_set_unimol_rxns",
 ]
 
 
@@ -74,6 +75,16 @@ class McgBimolRxn(ctypes.Structure):
         ("num_pathways", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
         ("cum_probs", ctypes.c_double * 4),
+        ("num_products", ctypes.c_uint32 * 4),
+        ("products", (ctypes.c_uint32 * 3) * 4),
+    ]
+
+
+class McgUnimolRxn(ctypes.Structure):
+    _fields_ = [
+        ("reactant", ctypes.c_uint32),
+        ("num_pathways", ctypes.c_uint32),
+        ("cum_rates", ctypes.c_double * 4),
         ("num_products", ctypes.c_uint32 * 4),
         ("products", (ctypes.c_uint32 * 3) * 4),
     ]
@@ -150,6 +161,7 @@ def load():
     lib.mcg_get_rxn_log.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
     lib.mcg_count_species.argtypes = [P, P]
     lib.mcg_set_surface_classes.argtypes = [P, ctypes.c_uint32, P]
+    lib.mcg_set_unimol_rxns.argtypes = [P, ctypes.POINTER(McgUnimolRxn), ctypes.c_uint32]
     lib.mcg_set_replica_params.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(McgSpecies), P]
     for name in EXPORTED:
         if not name.startswith("mcg_last_error") and name != "mcg_abi_version":

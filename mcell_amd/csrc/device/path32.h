@@ -59,8 +59,14 @@ __device__ __forceinline__ bool closest_wall_in_subpart32(const PartitionParams&
     const float mx = (float)move.x, my = (float)move.y, mz = (float)move.z;
     for (uint32_t k = k0; k < k1; ++k) {
       uint32_t wi = W.idx[k];
-      if (wi == last_hit) continue;
       float de;
+      if (wi == last_hit) {
+        // not hit again, but the end point may lie next to its plane (a
+        // short remainder after the reflection): FP32 rounding must see it
+        (void)certain_miss32(W.ent32[k], F, px, py, pz, mx, my, mz, &de);
+        if (fabs_f(de) < band) end_band = true;
+        continue;
+      }
       if (certain_miss32(W.ent32[k], F, px, py, pz, mx, my, mz, &de)) {
         if (fabs_f(de) < band) end_band = true;
         continue;
@@ -111,7 +117,9 @@ __device__ __forceinline__ void round_to_fp32_same_side(const PartitionParams& P
         const WallRec64& w = walls[idx[k]];
         double sd = dot3(D3{w.nx, w.ny, w.nz}, D3{px, py, pz}) - w.d;
         double sf = dot3(D3{w.nx, w.ny, w.nz}, D3{(double)fx, (double)fy, (double)fz}) - w.d;
-        if (sd == 0 || (sd > 0) == (sf > 0)) continue;
+        // strictly on the FP64 point's side; a rounded point exactly on the
+        // plane counts as crossed (collide_wall lets a move leave from there)
+        if (sd == 0 || (sd > 0 && sf > 0) || (sd < 0 && sf < 0)) continue;
         // step the FP32 point one ulp per axis towards the FP64 side
         float dir = sd > 0 ? 1.0f : -1.0f;
         if (w.nx * dir > 0) fx = nextafterf(fx, 3.4e38f);

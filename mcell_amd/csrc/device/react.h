@@ -23,6 +23,16 @@ struct RxnRec {
   uint32_t products[kMaxPathways][kMaxProducts];
 };
 
+// The unimolecular class of one species: cumulative pathway rates per
+// internal time unit (k_tot = cum[num_pathways - 1]) and products.
+struct UnimolRec {
+  double cum[kMaxPathways];
+  uint32_t num_pathways;  // 0: the species has no unimolecular reaction
+  uint32_t num_products[kMaxPathways];
+  uint32_t products[kMaxPathways][kMaxProducts];
+};
+constexpr uint32_t kUnimolBit = 0x80000000u;  // Proposal::rxn of a unimolecular event: bit | species
+
 // Cell grid over the region molecules can occupy: subpartition index box
 // [lo, lo + dim) per replica.  start[] is the CSR over the cell-sorted
 // snapshot, per_replica = dim.x * dim.y * dim.z.
@@ -50,6 +60,8 @@ struct Snapshot {
   const uint32_t* __restrict__ id;
   const uint32_t* __restrict__ tag;
   const uint32_t* __restrict__ consumed;  // nonzero: taken by an accepted reaction this iteration
+  const double* __restrict__ toff;        // diffusion-time offsets of the snapshot's molecules
+  const double* __restrict__ utime;       // unimolecular reaction times (absolute) or kTimeInvalid
 };
 
 // A small set of subpartition indices (the reference's SubpartIndicesSet).

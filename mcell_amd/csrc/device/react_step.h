@@ -10,6 +10,7 @@
 #include "exact_disk.h"
 #include "filter32.h"
 #include "path32.h"
+#include "light.h"
 #include "react.h"
 
 namespace mcg {
@@ -20,6 +21,7 @@ struct ReactParams {
   const uint32_t* __restrict__ off;
   const uint32_t* __restrict__ idx;
   const FilterWall* __restrict__ ent32;  // FP32 records in list order (FP32 form)
+  const WallRec64* __restrict__ ent64;   // FP64 records in list order
   CellGrid G;
   const RxnRec* __restrict__ rxns;
   uint32_t rxn_stride;                   // reaction classes per replica table (0: one shared table)
@@ -30,6 +32,7 @@ struct ReactParams {
   FilterParams F;
   float band;
   SurfaceClasses SC;
+  const UnimolRec* __restrict__ unimol;  // per species (num_pathways 0: none)
 };
 
 struct Proposal {
@@ -99,9 +102,26 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
 struct MoverEnd {
   D3 pos;          // end position; with a proposal, the start of the segment it was found in
   double t_steps;  // time the step covered (units of the time step), before reflections
+  double utime;    // unimolecular reaction time after the step (kTimeInvalid: none)
   bool near_plane; // FP32 form: the end point lies near a wall plane
   bool absorbed;   // destroyed by an absorptive surface class
+  uint32_t due;    // 0, or 1 + the stream pass that draws the unimolecular pathway
 };
+
+// The molecule's unimolecular time at the start of its step
+// (diffuse_single_molecule, diffuse_react_event.cpp:213-241): a newly created
+// molecule draws its lifetime first (time_of_unimol, rxn_utils.inl:720-735).
+__device__ __forceinline__ double start_unimol_time(const UnimolRec* __restrict__ unimol, uint32_t species,
+                                                    uint32_t tag, double utime, double start_time,
+                                                    PhiloxStream& rng) {
+  const UnimolRec& u = unimol[species];
+  if (u.num_pathways == 0) return kTimeInvalid;
+  if (!(tag_flags(tag) & MF_SCHEDULE_UNIMOL)) return utime;
+  const double k_tot = u.cum[u.num_pathways - 1];
+  const double p = rng.dbl();
+  if (k_tot <= 0 || !distinguishable(p, 0.0, 1e-12)) return start_time + kTimeForever;
+  return start_time + (-log(p) / k_tot);
+}
 
 // The step of one mover (diffuse_vol_molecule, diffuse_react_event.cpp:367-
 // 618) against the snapshot: displacement clipped to max_time = barrier -
@@ -112,19 +132,31 @@ struct MoverEnd {
 template <typename PosT>
 __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot<PosT>& S, const SpeciesRec& sp,
                                             const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
-                                            double barrier, Proposal& prop, MoverEnd& out, uint32_t& err) {
+                                            double utime, double barrier, Proposal& prop, MoverEnd& out,
+                                            uint32_t& err) {
   constexpr bool kF32 = sizeof(PosT) == 4;  // FP32 positions: FP32 wall filter
   const uint32_t rep = tag_replica(tag), spc = tag_species(tag);
   PhiloxStream rng;
   rng.init(R.seed, idv, R.iteration, rep, 0);
-  double t_steps;
-  D3 disp;
-  const double r_rate_factor = clipped_displacement(rng, T, sp.space_step, sp.time_step, barrier - toff, t_steps, disp);
-  out.t_steps = t_steps;
+  double elapsed = (double)R.iteration + toff;
   out.near_plane = false;
   out.absorbed = false;
+  out.due = 0;
+  out.t_steps = 0;
+  out.pos = pos;
+  out.utime = start_unimol_time(R.unimol, spc, tag, utime, elapsed, rng);
+  if (out.utime != kTimeInvalid && !(tag_flags(tag) & MF_SCHEDULE_UNIMOL) && out.utime <= elapsed) {
+    out.due = 1;  // due at the start: react_unimol_single_molecule before any move
+    return false;
+  }
+  // get_max_time (diffuse_react_event.cpp:164-198)
+  double max_time = barrier - toff;
+  if (out.utime != kTimeInvalid && out.utime < elapsed + max_time) max_time = out.utime - elapsed;
+  double t_steps;
+  D3 disp;
+  const double r_rate_factor = clipped_displacement(rng, T, sp.space_step, sp.time_step, max_time, t_steps, disp);
+  out.t_steps = t_steps;
   uint32_t last_hit = kInvalid;
-  double elapsed = (double)R.iteration + toff;
   const bool can_react = (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY);
   SubpartSet set;
   int seg;
@@ -239,6 +271,9 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   if (seg == kMaxSegments) err |= ERR_SEGMENTS;
   if (!in_partition(R.P, pos)) err |= ERR_ESCAPED;
   out.pos = pos;
+  // the reaction is due within this iteration: the reference diffuses the
+  // molecule again right away and reacts (diffuse_react_event.cpp:298-306)
+  if (out.utime != kTimeInvalid && out.utime < (double)R.iteration + 1.0) out.due = 2;
   return false;
 }
 

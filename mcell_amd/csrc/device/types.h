@@ -74,7 +74,12 @@ struct SpeciesTable {
   }
 };
 
-enum : uint32_t { MF_DEFUNCT = 1u << 0 };
+enum : uint32_t {
+  MF_DEFUNCT = 1u << 0,
+  MF_SCHEDULE_UNIMOL = 1u << 1,  // lifetime not drawn yet (MOLECULE_FLAG_SCHEDULE_UNIMOL_RXN)
+};
+constexpr double kTimeInvalid = -256.0;   // TIME_INVALID (defines.h:144)
+constexpr double kTimeForever = 1e20;     // TIME_FOREVER (defines.h:145)
 
 // Surface classes: what a wall does to a species that hits it
 // (collide_and_react_with_walls, diffuse_react_event.cpp:991-1067).

@@ -189,7 +189,7 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
       return fail(MCG_E_INVALID, "molecule outside the partition (partition.h:559-563)");
     if (species[i] >= c->num_species) return fail(MCG_E_INVALID, "species id out of range");
     hx[i] = p[0]; hy[i] = p[1]; hz[i] = p[2];
-    ht[i] = make_tag(species[i], 0, replica);
+    ht[i] = make_tag(species[i], MF_SCHEDULE_UNIMOL, replica);
   }
   if (c->rx.on && count) {
     // the device holds the id counters once steps have run
@@ -197,6 +197,10 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
       HIP_TRY(hipMemcpy(c->rx.next_id_host.data(), c->rx.next_id.p, c->rx.next_id_host.size() * 4,
                         hipMemcpyDeviceToHost));
     HIP_TRY(hipMemsetAsync(c->rx.toff.p + c->n, 0, count * sizeof(double), c->stream));
+    std::vector<double> inv(count, kTimeInvalid);
+    HIP_TRY(hipMemcpyAsync(c->rx.utime.p + c->n, inv.data(), count * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
   }
   if (c->rx.next_id_host.size() != c->cfg.num_replicas) c->rx.next_id_host.assign(c->cfg.num_replicas, 0);
   for (uint64_t i = 0; i < count; ++i) {

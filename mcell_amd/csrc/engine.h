@@ -52,13 +52,16 @@ struct ReactState {
   DevBuf<RxnRec> rxns_d;
   DevBuf<int32_t> pair_d;
   DevBuf<double> toff;                       // molecules' diffusion-time offsets
+  DevBuf<double> utime, sutime;              // unimolecular reaction times (molecules, snapshot)
+  std::vector<UnimolRec> unimol;             // per species
+  DevBuf<UnimolRec> unimol_d;
   DevBuf<double> sx, sy, sz;                 // snapshot positions (FP64 form)
   DevBuf<float> sxf, syf, szf;               // snapshot positions (FP32 form)
   DevBuf<uint32_t> sid, stag;
   DevBuf<double> stoff;
   DevBuf<uint32_t> key, rank, consumed, owner;
   DevBuf<Proposal> prop;
-  DevBuf<uint32_t> plist, alist, alist_sorted, rlist0, rlist1, mlist;
+  DevBuf<uint32_t> plist, alist, alist_sorted, rlist0, rlist1, mlist, ulist;
   DevBuf<uint64_t> keys_a, keys_b;
   DevBuf<uint32_t> cnt, pre, head, next_id;
   DevBuf<uint32_t> hdr;
@@ -138,6 +141,8 @@ struct mcg_ctx {
 namespace mcg {
 // The reactive iteration loop (engine_react.cpp).
 int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats);
+// Allocates the reactive path's buffers once (reaction classes of any kind).
+int ensure_react_buffers(mcg_ctx* c);
 // Live molecules per (replica, species) into counts (size num_replicas * num_species).
 int count_species(mcg_ctx* c, uint64_t* counts);
 }  // namespace mcg

@@ -50,6 +50,8 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.R.off = c->off.p;
   p.R.idx = c->idx.p;
   p.R.ent32 = reinterpret_cast<const FilterWall*>(c->ent32.p);
+  p.R.ent64 = c->ent64.p;
+  p.queue = SlowQueue{c->queue.p, c->qcount.p, 0};
   p.R.G = r.grid;
   p.R.G.start = r.cell_start.p;
   p.R.rxns = r.rxns_d.p;
@@ -61,8 +63,9 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.R.band = 2.0f * p.R.F.e_dd + 8.0f * p.R.F.eps * c->max_abs_coord;
   p.R.SC = c->surface_classes();
   p.species = c->species_table();
-  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p};
-  p.Sm = RMolView<PosT>{sx, sy, sz, r.sid.p, r.stag.p, r.stoff.p};
+  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p, r.utime.p};
+  p.Sm = RMolView<PosT>{sx, sy, sz, r.sid.p, r.stag.p, r.stoff.p, r.sutime.p};
+  p.R.unimol = r.unimol_d.p;
   p.key = r.key.p;
   p.rank = r.rank.p;
   p.cell_count = r.cell_count.p;
@@ -76,6 +79,7 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.rlist0 = r.rlist0.p;
   p.rlist1 = r.rlist1.p;
   p.mlist = r.mlist.p;
+  p.ulist = r.ulist.p;
   p.keys_a = r.keys_a.p;
   p.keys_b = r.keys_b.p;
   p.cnt = r.cnt.p;
@@ -114,7 +118,7 @@ int settle_rounds(mcg_ctx* c, ReactPipeline<PosT>& p, bool snapshot_src, double 
     const uint32_t nr = h[(r & 1) ? kHdrRerun1 : kHdrRerun0];
     ++rounds;
     if (nr == 0) return MCG_OK;
-    HIP_TRY(p.move(c->stream, snapshot_src, rl, rc, nr, barrier, false, err, nullptr));
+    HIP_TRY(p.move(c->stream, snapshot_src, rl, rc, nr, barrier, false, err));
   }
 }
 
@@ -140,10 +144,10 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
     const uint32_t n_in = (uint32_t)c->n;
     HIP_TRY(p.sort(c->stream, n_in, r.num_cells, c->err.p));
     HIP_TRY(launch_copy_u32(c->stream, p.cell_start + r.num_cells, p.hdr + kHdrNumOut));
+    HIP_TRY(launch_copy_u32(c->stream, p.cell_start + r.num_cells, p.hdr + kHdrSnapshot));
     HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
     HIP_TRY(hipEventRecord(c->evm0, c->stream));
-    HIP_TRY(p.move(c->stream, true, nullptr, p.cell_start + r.num_cells, n_in, barrier, propose_only, c->err.p,
-                   p.hdr + kHdrMoved));
+    HIP_TRY(p.move(c->stream, true, nullptr, p.cell_start + r.num_cells, n_in, barrier, propose_only, c->err.p));
     HIP_TRY(hipEventRecord(c->evm1, c->stream));
     if (propose_only) {
       HIP_TRY(p.log_proposals(c->stream, n_in, it, r.log.p, p.hdr + kHdrLog, (uint32_t)r.log_cap));
@@ -152,6 +156,12 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
       for (;;) {
         int st = settle_rounds(c, p, snapshot_src, barrier, c->err.p, rounds);
         if (st) return st;
+        if (h[kHdrUnimolDue]) {
+          HIP_TRY(p.fire_unimol(c->stream, h[kHdrUnimolDue]));
+          HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrUnimolDue, 0u, 1u));
+          st = read_hdr(c);
+          if (st) return st;
+        }
         const uint32_t na = h[kHdrAccepted];
         if (na == 0) break;
         reactions += na;
@@ -165,8 +175,8 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
         const uint32_t nm = h[kHdrProductMovers];
         if (nm == 0) break;
         snapshot_src = false;
-        HIP_TRY(p.move(c->stream, false, p.mlist, p.hdr + kHdrProductMovers, nm, barrier, false, c->err.p,
-                       p.hdr + kHdrMoved));
+        steps += nm;
+        HIP_TRY(p.move(c->stream, false, p.mlist, p.hdr + kHdrProductMovers, nm, barrier, false, c->err.p));
       }
     }
     int st = read_hdr(c);
@@ -175,8 +185,7 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
     HIP_TRY(hipEventElapsedTime(&ms, c->evm0, c->evm1));
     move_ms += ms;
     c->n = h[kHdrNumOut];
-    steps += h[kHdrMoved];
-    HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrMoved, 0u, 1u));
+    steps += h[kHdrSnapshot];
     uint32_t e = 0;
     HIP_TRY(hipMemcpy(&e, c->err.p, 4, hipMemcpyDeviceToHost));
     if (e) {
@@ -197,6 +206,40 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
 }
 
 }  // namespace
+
+// Buffers of the reactive path, allocated once (by the first reaction-class call).
+int ensure_react_buffers(mcg_ctx* c) {
+  ReactState& r = c->rx;
+  if (r.on) return MCG_OK;
+  const size_t cap = c->cfg.molecule_capacity;
+  const uint32_t ns = c->num_species;
+  bool bad = r.pair_d.alloc((size_t)ns * ns) || r.toff.alloc(cap) || r.utime.alloc(cap) || r.sutime.alloc(cap) ||
+             r.sid.alloc(cap) || r.stag.alloc(cap) || r.stoff.alloc(cap) || r.key.alloc(cap) ||
+             r.rank.alloc(cap) || r.consumed.alloc(cap) || r.owner.alloc(cap) || r.prop.alloc(cap) ||
+             r.plist.alloc(cap) || r.alist.alloc(cap) || r.alist_sorted.alloc(cap) || r.rlist0.alloc(cap) ||
+             r.rlist1.alloc(cap) || r.mlist.alloc(cap) || r.ulist.alloc(cap) || r.keys_a.alloc(cap) ||
+             r.keys_b.alloc(cap) || r.cnt.alloc(cap + 1) || r.pre.alloc(cap + 1) ||
+             r.head.alloc(c->cfg.num_replicas) || r.next_id.alloc(c->cfg.num_replicas) || r.hdr.alloc(kHdrWords) ||
+             r.unimol_d.alloc(ns) || r.rxns_d.alloc(1);
+  if (!bad)
+    bad = c->fp32() ? (r.sxf.alloc(cap) || r.syf.alloc(cap) || r.szf.alloc(cap))
+                    : (r.sx.alloc(cap) || r.sy.alloc(cap) || r.sz.alloc(cap));
+  if (bad) return fail(MCG_E_NOMEM, "reaction buffers");
+  if (!r.hdr_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.hdr_host), kHdrWords * 4, hipHostMallocDefault));
+  std::vector<int32_t> none((size_t)ns * ns, -1);
+  if (r.pair.empty()) r.pair = none;
+  HIP_TRY(hipMemcpy(r.pair_d.p, r.pair.data(), r.pair.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(r.unimol_d.p, 0, ns * sizeof(UnimolRec)));
+  HIP_TRY(hipMemset(r.toff.p, 0, cap * sizeof(double)));
+  // molecules added so far have no unimolecular time yet
+  std::vector<double> inv(std::max<size_t>(c->n, 1), kTimeInvalid);
+  if (c->n) HIP_TRY(hipMemcpy(r.utime.p, inv.data(), c->n * sizeof(double), hipMemcpyHostToDevice));
+  if (r.next_id_host.size() != c->cfg.num_replicas) r.next_id_host.assign(c->cfg.num_replicas, 0);
+  r.next_id_dirty = true;
+  r.grid_dirty = true;
+  r.on = true;
+  return MCG_OK;
+}
 
 int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats) {
   if (c->cfg.flags & MCG_FLAG_TRACE_HITS) return fail(MCG_E_UNSUPPORTED, "hit traces need a context without reactions");
@@ -248,11 +291,11 @@ int count_species(mcg_ctx* c, uint64_t* counts) {
   if (c->n) {
     if (c->fp32()) {
       ReactPipeline<float> p{};
-      p.M = RMolView<float>{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p, nullptr};
+      p.M = RMolView<float>{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p, nullptr, nullptr};
       HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, d.p));
     } else {
       ReactPipeline<double> p{};
-      p.M = RMolView<double>{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p, nullptr};
+      p.M = RMolView<double>{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p, nullptr, nullptr};
       HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, d.p));
     }
   }
@@ -268,7 +311,7 @@ extern "C" {
 int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
-  if (c->rx.on) return fail(MCG_E_STATE, "reaction classes are already set");
+  if (!c->rx.rxns.empty()) return fail(MCG_E_STATE, "reaction classes are already set");
   if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
   if (n == 0) return MCG_OK;
   const uint32_t ns = c->num_species;
@@ -310,30 +353,53 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   for (uint32_t s = 0; s < ns; ++s)
     if (!(sp[s].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
   ReactState& r = c->rx;
-  const size_t cap = c->cfg.molecule_capacity;
-  bool bad = r.rxns_d.alloc(n) || r.pair_d.alloc(pair.size()) || r.toff.alloc(cap) || r.sid.alloc(cap) ||
-             r.stag.alloc(cap) || r.stoff.alloc(cap) || r.key.alloc(cap) || r.rank.alloc(cap) ||
-             r.consumed.alloc(cap) || r.owner.alloc(cap) || r.prop.alloc(cap) || r.plist.alloc(cap) ||
-             r.alist.alloc(cap) || r.alist_sorted.alloc(cap) || r.rlist0.alloc(cap) || r.rlist1.alloc(cap) ||
-             r.mlist.alloc(cap) || r.keys_a.alloc(cap) || r.keys_b.alloc(cap) || r.cnt.alloc(cap + 1) ||
-             r.pre.alloc(cap + 1) || r.head.alloc(c->cfg.num_replicas) || r.next_id.alloc(c->cfg.num_replicas) ||
-             r.hdr.alloc(kHdrWords);
-  if (!bad)
-    bad = c->fp32() ? (r.sxf.alloc(cap) || r.syf.alloc(cap) || r.szf.alloc(cap))
-                    : (r.sx.alloc(cap) || r.sy.alloc(cap) || r.sz.alloc(cap));
-  if (bad) return fail(MCG_E_NOMEM, "reaction buffers");
-  if (!r.hdr_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r.hdr_host), kHdrWords * 4, hipHostMallocDefault));
+  int st = ensure_react_buffers(c);
+  if (st) return st;
+  if (r.rxns_d.alloc(n)) return fail(MCG_E_NOMEM, "reaction buffers");
   HIP_TRY(hipMemcpy(r.rxns_d.p, h.data(), n * sizeof(RxnRec), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(r.pair_d.p, pair.data(), pair.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->species.p, sp.data(), ns * sizeof(SpeciesRec), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(r.toff.p, 0, cap * sizeof(double)));
   c->species_host = sp;
   r.rxns = h;
   r.pair = pair;
-  if (r.next_id_host.size() != c->cfg.num_replicas) r.next_id_host.assign(c->cfg.num_replicas, 0);
-  r.next_id_dirty = true;
-  r.grid_dirty = true;
-  r.on = true;
+  return MCG_OK;
+}
+
+int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
+  if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
+  if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
+  const uint32_t ns = c->num_species;
+  std::vector<UnimolRec> u(ns);
+  std::memset(u.data(), 0, ns * sizeof(UnimolRec));
+  for (uint32_t i = 0; i < n; ++i) {
+    const mcg_unimol_rxn& x = rxns[i];
+    if (x.reactant >= ns) return fail(MCG_E_INVALID, "reactant species out of range");
+    if (u[x.reactant].num_pathways) return fail(MCG_E_INVALID, "one unimolecular class per species");
+    if (x.num_pathways < 1 || x.num_pathways > (uint32_t)kMaxPathways) return fail(MCG_E_INVALID, "1..4 pathways");
+    double last = 0;
+    for (uint32_t k = 0; k < x.num_pathways; ++k) {
+      if (!(x.cum_rates[k] >= last)) return fail(MCG_E_INVALID, "cumulative rates must not decrease");
+      last = x.cum_rates[k];
+      if (x.num_products[k] > (uint32_t)kMaxProducts) return fail(MCG_E_INVALID, "at most 3 products per pathway");
+      for (uint32_t q = 0; q < x.num_products[k]; ++q) {
+        if (x.products[k][q] >= ns) return fail(MCG_E_INVALID, "product species out of range");
+        if (x.products[k][q] == x.reactant)
+          return fail(MCG_E_UNSUPPORTED, "unimolecular classes that keep the reactant are not on this path yet");
+        u[x.reactant].products[k][q] = x.products[k][q];
+      }
+      u[x.reactant].cum[k] = x.cum_rates[k];
+      u[x.reactant].num_products[k] = x.num_products[k];
+    }
+    u[x.reactant].num_pathways = x.num_pathways;
+  }
+  for (uint32_t s = 0; s < ns; ++s)
+    if (u[s].num_pathways && !(c->species_host[s].time_step == 1.0))
+      return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
+  int st = ensure_react_buffers(c);
+  if (st) return st;
+  c->rx.unimol = u;
+  HIP_TRY(hipMemcpy(c->rx.unimol_d.p, u.data(), ns * sizeof(UnimolRec), hipMemcpyHostToDevice));
   return MCG_OK;
 }
 

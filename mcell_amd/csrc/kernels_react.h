@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "device/react_step.h"
+#include "kernels.h"
 
 namespace mcg {
 
@@ -19,6 +20,7 @@ struct RMolView {
   uint32_t* id;
   uint32_t* tag;
   double* toff;
+  double* utime;  // unimolecular reaction time (absolute) or kTimeInvalid
 };
 
 // One carried-out (or, in the snapshot mode, proposed) reaction: 64 bytes,
@@ -41,9 +43,10 @@ enum : uint32_t {
   kHdrProductMovers = 3,  // products that move in this iteration
   kHdrRerun0 = 4,         // movers queued for another round (two lists)
   kHdrRerun1 = 5,
-  kHdrMoved = 6,          // molecule steps taken
+  kHdrSnapshot = 6,       // molecules in the snapshot (the iteration's movers)
   kHdrLog = 7,            // events logged (snapshot mode)
-  kHdrWords = 8,
+  kHdrUnimolDue = 8,      // molecules whose unimolecular reaction falls in this phase
+  kHdrWords = 12,
 };
 
 template <typename PosT>
@@ -55,18 +58,20 @@ struct ReactPipeline {
   uint32_t *key, *rank, *cell_count, *cell_start;
   uint32_t *consumed, *owner;
   Proposal* prop;
-  uint32_t *plist, *alist, *alist_sorted, *rlist0, *rlist1, *mlist;
+  uint32_t *plist, *alist, *alist_sorted, *rlist0, *rlist1, *mlist, *ulist;
   uint64_t *keys_a, *keys_b;
   uint32_t *cnt, *pre, *head, *next_id;
   uint32_t* hdr;
+  SlowQueue queue;  // light-pass queue of movers that need the full step
   void* scan_tmp;
   size_t scan_tmp_bytes;
 
   static size_t scan_bytes(uint32_t num_cells, uint32_t capacity);
   hipError_t sort(hipStream_t s, uint32_t n_in, uint32_t num_cells, uint32_t* err);
   hipError_t move(hipStream_t s, bool snapshot_src, const uint32_t* list, const uint32_t* count, uint32_t max_movers,
-                  double barrier, bool propose_only, uint32_t* err, uint32_t* moved);
+                  double barrier, bool propose_only, uint32_t* err);
   hipError_t settle(hipStream_t s, uint32_t max_props, uint32_t* rlist, uint32_t* rcount);
+  hipError_t fire_unimol(hipStream_t s, uint32_t max_due);
   hipError_t products(hipStream_t s, uint32_t n_acc, uint32_t capacity, RxnEvent* log, uint32_t log_base,
                       uint32_t log_cap, uint32_t* err);
   hipError_t log_proposals(hipStream_t s, uint32_t max_props, uint32_t iteration, RxnEvent* log, uint32_t* log_count,

@@ -29,6 +29,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "device/displace.h"
+#include "device/light.h"
 #include "device/react_step.h"
 #include "kernels_react.h"
 
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, c
   s.id[j] = m.id[i];
   s.tag[j] = m.tag[i];
   s.toff[j] = m.toff[i];
+  s.utime[j] = m.utime[i];
   consumed[j] = 0;
   owner[j] = kInvalid;
 }
@@ -119,6 +121,90 @@ __device__ __forceinline__ double next_toff(uint32_t iteration, double toff, dou
   return t - (double)(iteration + 1);
 }
 
+__device__ __forceinline__ void stage_zig(ZigguratTables& T) {
+  const uint32_t* zs = reinterpret_cast<const uint32_t*>(&c_zig_r);
+  uint32_t* zd = reinterpret_cast<uint32_t*>(&T);
+  for (int k = threadIdx.x; k < (int)(sizeof(ZigguratTables) / 4); k += blockDim.x) zd[k] = zs[k];
+  __syncthreads();
+}
+
+// Wave-aggregated append of slot j for the lanes with `want` set.
+__device__ __forceinline__ void append_slot(bool want, uint32_t j, uint32_t* __restrict__ list,
+                                            uint32_t* __restrict__ count) {
+  const uint64_t mask = __ballot(want);
+  if (!mask) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  if (want) list[base + __popcll(mask & ((1ull << lane) - 1))] = j;
+}
+
+// The full step of the mover in slot j (react_step.h): writes its end state,
+// or records its proposal and claims.  Returns whether it proposed.
+template <typename PosT>
+__device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<PosT>& S, const RMolView<PosT>& src,
+                                         const RMolView<PosT>& out, uint32_t j, SpeciesTable species,
+                                         const ZigguratTables& T, double barrier, uint32_t propose_only,
+                                         Proposal* __restrict__ prop, uint32_t* __restrict__ owner,
+                                         uint32_t* __restrict__ err_out, bool& due) {
+  const uint32_t tag = src.tag[j];
+  const uint32_t idv = src.id[j];
+  const double toff = src.toff[j];
+  const double utime = src.utime[j];
+  const SpeciesRec sp = species.of(tag);
+  D3 pos = {(double)src.x[j], (double)src.y[j], (double)src.z[j]};
+  uint32_t err = 0;
+  MoverEnd e;
+  e.pos = pos;
+  e.t_steps = 0;
+  e.utime = utime;
+  e.near_plane = false;
+  e.absorbed = false;
+  e.due = 0;
+  Proposal p;
+  bool proposed = false;
+  const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
+  if (moves) proposed = react_mover(R, S, sp, T, pos, idv, tag, toff, utime, barrier, p, e, err);
+  due = e.due != 0;
+  if (due) {
+    // the unimolecular event, settled after the phase's bimolecular claims
+    p.time = e.utime;
+    p.x = e.pos.x;
+    p.y = e.pos.y;
+    p.z = e.pos.z;
+    p.mover_slot = j;
+    p.partner_slot = kInvalid;
+    p.mover_id = idv;
+    p.partner_id = kInvalid;
+    p.rxn = kUnimolBit | tag_species(tag);
+    p.pathway = e.due - 1;  // the stream pass of the pathway draw
+    p.replica = tag_replica(tag);
+    p.pad = 0;
+    prop[j] = p;
+  }
+  if (proposed) {
+    p.mover_slot = j;
+    p.pad = 0;
+    prop[j] = p;
+    if (!propose_only) {
+      atomicMin(owner + j, idv);
+      atomicMin(owner + p.partner_slot, idv);
+    }
+  }
+  if (!proposed || propose_only) {
+    store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
+    out.id[j] = idv;
+    const uint32_t t2 = tag & ~(MF_SCHEDULE_UNIMOL << 8);
+    out.tag[j] = e.absorbed ? (t2 | (MF_DEFUNCT << 8)) : t2;
+    out.toff[j] = sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff;
+    out.utime[j] = e.utime;
+  }
+  if (err) atomicOr(err_out, err);
+  return proposed;
+}
+
 // One mover per thread.  Movers are snapshot slots [0, *count) when list is
 // null, else slots list[0, *count).  src holds the movers' start state (the
 // snapshot, or the output arrays for products); out receives end states.
@@ -130,65 +216,143 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
                                               uint32_t propose_only, Proposal* __restrict__ prop,
                                               uint32_t* __restrict__ plist, uint32_t* __restrict__ pcount,
                                               uint32_t* __restrict__ owner, uint32_t* __restrict__ err_out,
-                                              uint32_t* __restrict__ moved) {
+                                              uint32_t* __restrict__ ulist, uint32_t* __restrict__ ucount) {
   __shared__ ZigguratTables T;
-  {
-    const uint32_t* zs = reinterpret_cast<const uint32_t*>(&c_zig_r);
-    uint32_t* zd = reinterpret_cast<uint32_t*>(&T);
-    for (int k = threadIdx.x; k < (int)(sizeof(ZigguratTables) / 4); k += blockDim.x) zd[k] = zs[k];
-    __syncthreads();
-  }
+  stage_zig(T);
   const uint32_t n = *count;
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  bool proposed = false;
+  bool proposed = false, due = false;
   uint32_t j = 0;
   if (q < n) {
     j = list ? list[q] : q;
-    const uint32_t tag = src.tag[j];
-    const uint32_t idv = src.id[j];
-    const double toff = src.toff[j];
+    proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, err_out, due);
+  }
+  append_slot(proposed, j, plist, pcount);
+  append_slot(due, j, ulist, ucount);
+}
+
+// Workgroup k drains queue shard k with the full step.
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT> S, RMolView<PosT> src,
+                                                    RMolView<PosT> out, SlowQueue Q, SpeciesTable species, double barrier,
+                                                    uint32_t propose_only, Proposal* __restrict__ prop,
+                                                    uint32_t* __restrict__ plist, uint32_t* __restrict__ pcount,
+                                                    uint32_t* __restrict__ owner, uint32_t* __restrict__ err_out,
+                                                    uint32_t* __restrict__ ulist, uint32_t* __restrict__ ucount) {
+  __shared__ ZigguratTables T;
+  stage_zig(T);
+  const uint32_t count = Q.count[blockIdx.x];
+  const uint32_t* slots = Q.slots + (size_t)blockIdx.x * Q.shard_cap;
+  for (uint32_t base = 0; base < count; base += blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    bool proposed = false, due = false;
+    uint32_t j = 0;
+    if (q < count) {
+      j = slots[q];
+      proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, err_out, due);
+    }
+    append_slot(proposed, j, plist, pcount);
+    append_slot(due, j, ulist, ucount);
+  }
+}
+
+// The common step, register-light: movers whose straight move certainly
+// meets no wall and passes no reaction partner within the reaction radius
+// commit pos + disp here (the reference's ray trace would find no collision
+// and commit the same point, diffuse_react_event.cpp:773-777); the others go
+// to the queue for the full step, which re-draws the same displacement from
+// the same stream.  The partner test scans the cells of the move's
+// subpartition box widened by rxn_radius * sqrt(2), a superset of the cells
+// the reference collects, so a committed mover had no collision there.
+template <typename PosT>
+__global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT> S, RMolView<PosT> out,
+                                                    const uint32_t* __restrict__ count, SpeciesTable species,
+                                                    double barrier, SlowQueue Q) {
+  constexpr bool kF32 = sizeof(PosT) == 4;
+  __shared__ ZigguratTables T;
+  stage_zig(T);
+  const uint32_t n = *count;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  bool hard = false;
+  if (j < n) {
+    const uint32_t tag = S.tag[j];
+    const uint32_t idv = S.id[j];
+    const double toff = S.toff[j];
+    const double utime = S.utime[j];
     const SpeciesRec sp = species.of(tag);
-    D3 pos = {(double)src.x[j], (double)src.y[j], (double)src.z[j]};
-    uint32_t err = 0;
-    MoverEnd e;
-    e.pos = pos;
-    e.t_steps = 0;
-    e.near_plane = false;
-    e.absorbed = false;
-    Proposal p;
-    bool moves = sp.space_step > 0;
-    if (moves) proposed = react_mover(R, S, sp, T, pos, idv, tag, toff, barrier, p, e, err);
-    if (proposed) {
-      p.mover_slot = j;
-      p.pad = 0;
-      prop[j] = p;
-      if (!propose_only) {
-        atomicMin(owner + j, idv);
-        atomicMin(owner + p.partner_slot, idv);
+    const D3 pos = {(double)S.x[j], (double)S.y[j], (double)S.z[j]};
+    // molecules with a unimolecular class take the full step when their
+    // lifetime is still to be drawn or ends within the step's time budget
+    const bool unimol_now = R.unimol[tag_species(tag)].num_pathways > 0 &&
+                            ((tag_flags(tag) & MF_SCHEDULE_UNIMOL) || utime < (double)R.iteration + barrier);
+    if (unimol_now) {
+      hard = true;
+    } else if (sp.space_step > 0) {
+      PhiloxStream rng;
+      rng.init(R.seed, idv, R.iteration, tag_replica(tag), 0);
+      double t_steps;
+      D3 disp;
+      (void)clipped_displacement(rng, T, sp.space_step, sp.time_step, barrier - toff, t_steps, disp);
+      const D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
+      int lo[3], hi[3];
+      bool clear = move_index_box(R.P, pos, end, lo, hi);
+      bool near_plane = false;
+      if (clear) {
+        if (kF32) clear = all_walls_miss32(R.P, R.ent32, R.off, R.F, R.band, pos, disp, lo, hi, near_plane);
+        else clear = all_walls_miss64(R.P, R.ent64, R.off, pos, disp, lo, hi);
       }
-    }
-    if (!proposed || propose_only) {
-      store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
+      if (clear && !near_plane && (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY)) {
+        // partners within the reaction radius of the segment
+        const double r_nb = R.P.rxn_radius * 1.41421356238;
+        const double sigma2 = R.P.rxn_radius * R.P.rxn_radius;
+        const double o[3] = {R.P.ox, R.P.oy, R.P.oz};
+        const double a[3] = {pos.x, pos.y, pos.z}, b[3] = {end.x, end.y, end.z};
+        int clo[3], chi[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double mn = (a[k] < b[k] ? a[k] : b[k]) - r_nb, mx = (a[k] < b[k] ? b[k] : a[k]) + r_nb;
+          int l = (int)floor((mn - o[k]) * R.P.sp_rcp), h = (int)floor((mx - o[k]) * R.P.sp_rcp);
+          clo[k] = l < R.G.lo[k] ? R.G.lo[k] : l;
+          chi[k] = h > R.G.lo[k] + R.G.dim[k] - 1 ? R.G.lo[k] + R.G.dim[k] - 1 : h;
+        }
+        const uint32_t my_sp = tag_species(tag), rep = tag_replica(tag);
+        for (int z = clo[2]; z <= chi[2] && clear; ++z)
+          for (int y = clo[1]; y <= chi[1] && clear; ++y)
+            for (int x = clo[0]; x <= chi[0] && clear; ++x) {
+              uint32_t c;
+              if (!cell_of(R.G, x, y, z, rep, c)) continue;
+              const uint32_t k1 = R.G.start[c + 1];
+              for (uint32_t k = R.G.start[c]; k < k1; ++k) {
+                if (R.pair_rxn[my_sp * R.num_species + tag_species(S.tag[k])] < 0) continue;
+                double t;
+                D3 cp;
+                if (!collide_mol64(pos, disp, D3{(double)S.x[k], (double)S.y[k], (double)S.z[k]}, sigma2, t, cp))
+                  continue;
+                if (k == j) continue;
+                clear = false;
+                break;
+              }
+            }
+      }
+      if (clear && !near_plane) {
+        store_pos(R, out.x, out.y, out.z, j, end, false);
+        out.id[j] = idv;
+        out.tag[j] = tag & ~(MF_SCHEDULE_UNIMOL << 8);
+        out.toff[j] = next_toff(R.iteration, toff, t_steps);
+        out.utime[j] = utime;
+      } else {
+        hard = true;
+      }
+    } else {
+      store_pos(R, out.x, out.y, out.z, j, pos, false);
       out.id[j] = idv;
-      out.tag[j] = e.absorbed ? (tag | (MF_DEFUNCT << 8)) : tag;
-      out.toff[j] = moves ? next_toff(R.iteration, toff, e.t_steps) : toff;
+      out.tag[j] = tag & ~(MF_SCHEDULE_UNIMOL << 8);
+      out.toff[j] = toff;
+      out.utime[j] = utime;
     }
-    if (err) atomicOr(err_out, err);
   }
-  // proposals are rare: one atomic per wave that has any
-  const uint64_t mask = __ballot(proposed);
-  if (mask) {
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(pcount, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (proposed) plist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
-  }
-  if (moved && q < n) {
-    const uint64_t all = __ballot(true);
-    if (__lane_id() == __ffsll((long long)all) - 1) atomicAdd(moved, (uint32_t)__popcll(all));
-  }
+  const uint32_t shard = blockIdx.x % kQueueShards;
+  wave_append(hard, j, Q.slots + (size_t)shard * Q.shard_cap, Q.count + shard);
 }
 
 // A proposal whose mover and partner both carry its claim is accepted.
@@ -224,6 +388,40 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
     base = __shfl(base, leader);
     if (acc) alist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
   }
+}
+
+// Unimolecular events of the phase (react_unimol_single_molecule +
+// outcome_unimolecular, diffuse_react_event.cpp:1760-1826, 2939-3003): every
+// due molecule that no accepted bimolecular reaction consumed reacts at its
+// scheduled time; the pathway is drawn from the molecule's stream when the
+// class has several (which_unimolecular, rxn_utils.inl:772-783).
+__global__ __launch_bounds__(256) void k_unimol_fire(ReactParams R, const uint32_t* __restrict__ ulist,
+                                                     const uint32_t* __restrict__ ucount,
+                                                     Proposal* __restrict__ prop, uint32_t* __restrict__ consumed,
+                                                     uint32_t* __restrict__ out_tag, uint32_t* __restrict__ alist,
+                                                     uint32_t* __restrict__ acount) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  bool fire = false;
+  uint32_t j = 0;
+  if (q < *ucount) {
+    j = ulist[q];
+    if (!consumed[j]) {
+      Proposal& p = prop[j];
+      const UnimolRec& u = R.unimol[p.rxn & ~kUnimolBit];
+      uint32_t pw = 0;
+      if (u.num_pathways > 1) {
+        PhiloxStream rng;
+        rng.init(R.seed, p.mover_id, R.iteration, p.replica, p.pathway);
+        const double match = rng.dbl() * u.cum[u.num_pathways - 1];
+        pw = (uint32_t)binary_search_double(u.cum, match, (int)u.num_pathways - 1, 1);
+      }
+      p.pathway = pw;
+      consumed[j] = 1;
+      out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
+      fire = true;
+    }
+  }
+  append_slot(fire, j, alist, acount);
 }
 
 // Releases the claims of every proposal of the round and queues the movers
@@ -269,6 +467,7 @@ __device__ __forceinline__ uint32_t new_products(const RxnRec& r, uint32_t pathw
 __global__ __launch_bounds__(256) void k_prod_counts(const uint32_t* __restrict__ sorted_slots, uint32_t n,
                                                      const Proposal* __restrict__ prop,
                                                      const RxnRec* __restrict__ rxns, uint32_t rxn_stride,
+                                                     const UnimolRec* __restrict__ unimol,
                                                      uint32_t* __restrict__ cnt) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > n) return;
@@ -277,7 +476,8 @@ __global__ __launch_bounds__(256) void k_prod_counts(const uint32_t* __restrict_
     return;
   }
   const Proposal& p = prop[sorted_slots[k]];
-  cnt[k] = new_products(rxns[p.replica * rxn_stride + p.rxn], p.pathway);
+  cnt[k] = (p.rxn & kUnimolBit) ? unimol[p.rxn & ~kUnimolBit].num_products[p.pathway]
+                                 : new_products(rxns[p.replica * rxn_stride + p.rxn], p.pathway);
 }
 
 __global__ __launch_bounds__(256) void k_seg_heads(const uint64_t* __restrict__ keys, uint32_t n,
@@ -305,11 +505,14 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const Proposal& p = prop[sorted_slots[k]];
-  const RxnRec& rx = R.rxns[p.replica * R.rxn_stride + p.rxn];
+  const bool uni = (p.rxn & kUnimolBit) != 0;
+  const uint32_t* prods = uni ? R.unimol[p.rxn & ~kUnimolBit].products[p.pathway]
+                              : R.rxns[p.replica * R.rxn_stride + p.rxn].products[p.pathway];
+  const uint32_t np = uni ? R.unimol[p.rxn & ~kUnimolBit].num_products[p.pathway]
+                          : R.rxns[p.replica * R.rxn_stride + p.rxn].num_products[p.pathway];
   const uint32_t rep = (uint32_t)(keys[k] >> 32);
   const uint32_t base = hdr[kHdrNumOut];
   const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
-  const uint32_t np = rx.num_products[p.pathway];
   const bool moves_now = cmp_lt_d(p.time, (double)R.iteration + 1.0, 1e-12);
   for (uint32_t i = 0; i < np; ++i) {
     const uint32_t slot = base + pre[k] + i;
@@ -319,8 +522,9 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
     }
     store_pos(R, out.x, out.y, out.z, slot, D3{p.x, p.y, p.z}, true);
     out.id[slot] = first_id + i;
-    out.tag[slot] = make_tag(rx.products[p.pathway][i], 0, rep);
+    out.tag[slot] = make_tag(prods[i], MF_SCHEDULE_UNIMOL, rep);
     out.toff[slot] = p.time - (double)(R.iteration + (moves_now ? 0u : 1u));
+    out.utime[slot] = kTimeInvalid;
     consumed[slot] = 0;
     owner[slot] = kInvalid;
     if (moves_now) mlist[atomicAdd(mcount, 1u)] = slot;
@@ -419,14 +623,25 @@ size_t ReactPipeline<PosT>::scan_bytes(uint32_t num_cells, uint32_t capacity) {
 
 template <typename PosT>
 hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uint32_t* list, const uint32_t* count,
-                                     uint32_t max_movers, double barrier, bool propose_only, uint32_t* err,
-                                     uint32_t* moved) {
+                                     uint32_t max_movers, double barrier, bool propose_only, uint32_t* err) {
   if (max_movers == 0) return hipSuccess;
-  Snapshot<PosT> S{Sm.x, Sm.y, Sm.z, Sm.id, Sm.tag, consumed};
+  Snapshot<PosT> S{Sm.x, Sm.y, Sm.z, Sm.id, Sm.tag, consumed, Sm.toff, Sm.utime};
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrProposals, 0u, 1u);
+  if (list == nullptr && snapshot_src) {
+    // every snapshot slot: light pass, then the queued movers in full
+    SlowQueue Q = queue;
+    Q.shard_cap = shard_capacity(max_movers);
+    hipLaunchKernelGGL(k_fill_u32, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, 0u, kQueueShards);
+    hipLaunchKernelGGL(k_move_light<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, M, count, species, barrier,
+                       Q);
+    hipLaunchKernelGGL(k_move_drain<PosT>, dim3(kQueueShards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
+                       propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, err, ulist,
+                       hdr + kHdrUnimolDue);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_move<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, snapshot_src ? Sm : M, M, list,
                      count, species, barrier, propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, err,
-                     moved);
+                     ulist, hdr + kHdrUnimolDue);
   return hipGetLastError();
 }
 
@@ -441,6 +656,14 @@ hipError_t ReactPipeline<PosT>::settle(hipStream_t s, uint32_t max_props, uint32
 }
 
 template <typename PosT>
+hipError_t ReactPipeline<PosT>::fire_unimol(hipStream_t s, uint32_t max_due) {
+  if (max_due == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_unimol_fire, blocks_for(max_due), dim3(256), 0, s, R, ulist, hdr + kHdrUnimolDue, prop,
+                     consumed, M.tag, alist, hdr + kHdrAccepted);
+  return hipGetLastError();
+}
+
+template <typename PosT>
 hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t capacity, RxnEvent* log,
                                          uint32_t log_base, uint32_t log_cap, uint32_t* err) {
   if (n_acc == 0) return hipSuccess;
@@ -449,7 +672,7 @@ hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t
   hipError_t e =
       hipcub::DeviceRadixSort::SortPairs(scan_tmp, tmp, keys_a, keys_b, alist, alist_sorted, n_acc, 0, 64, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_prod_counts, blocks_for(n_acc + 1), dim3(256), 0, s, alist_sorted, n_acc, prop, R.rxns, R.rxn_stride, cnt);
+  hipLaunchKernelGGL(k_prod_counts, blocks_for(n_acc + 1), dim3(256), 0, s, alist_sorted, n_acc, prop, R.rxns, R.rxn_stride, R.unimol, cnt);
   tmp = scan_tmp_bytes;
   e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt, pre, n_acc + 1, s);
   if (e != hipSuccess) return e;

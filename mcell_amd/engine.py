@@ -53,7 +53,7 @@ class DiffuseReactEngine:
                       propose_only: bool = False, log_capacity: int = 0, trace: bool = False):
         """An engine holding the workload's geometry, species, reaction
         classes and molecules (every replica)."""
-        from .workloads import rxn_table, species_steps
+        from .workloads import rxn_table, species_steps, unimol_table
         cap = capacity if capacity is not None else max(2 * wl.num_molecules, 1024)
         eng = cls(wl.part, precision=precision, seed=wl.config.seed if seed is None else seed,
                   num_replicas=wl.num_replicas, capacity=cap, trace=trace, barrier_interval=wl.barrier_interval,
@@ -64,6 +64,8 @@ class DiffuseReactEngine:
             eng.set_surface_classes(wl.surface_classes)
         if wl.rxns:
             eng.set_bimol_rxns(rxn_table(wl))
+        if wl.unimol_rxns:
+            eng.set_unimol_rxns(unimol_table(wl))
         if wl.replica_D is not None:
             for r in range(wl.num_replicas):
                 eng.set_replica_params(r, species_steps(wl, r), rxn_table(wl, r) if wl.rxns else None)
@@ -133,6 +135,19 @@ class DiffuseReactEngine:
             sp[i].flags = fl
         arr = None if rxns is None else self._rxn_array(rxns)
         _lib.check(self.lib.mcg_set_replica_params(self.ctx, replica, sp, arr))
+
+    def set_unimol_rxns(self, rxns):
+        """rxns: (species, cumulative rates per time unit, products per pathway)."""
+        arr = (_lib.McgUnimolRxn * max(len(rxns), 1))()
+        for i, (sp, cum, prods) in enumerate(rxns):
+            arr[i].reactant = sp
+            arr[i].num_pathways = len(cum)
+            for k, r in enumerate(cum):
+                arr[i].cum_rates[k] = r
+                arr[i].num_products[k] = len(prods[k])
+                for q, ps in enumerate(prods[k]):
+                    arr[i].products[k][q] = ps
+        _lib.check(self.lib.mcg_set_unimol_rxns(self.ctx, arr, len(rxns)))
 
     def set_rxn_log(self, capacity: int):
         _lib.check(self.lib.mcg_set_rxn_log(self.ctx, capacity))

@@ -266,6 +266,17 @@ class BimolReaction:
         return pb
 
 
+@dataclass
+class UnimolReaction:
+    """reactant -> products for a volume species (rate in s^-1); the
+    per-step probability rate is rate * time_unit (pb = time_unit, MCell3
+    src/react_util.c:74-78)."""
+
+    reactant: int
+    rate: float
+    products: List[int] = field(default_factory=list)
+
+
 # -------------------------------------------------------------- placement ---
 def place_uniform_in_box(n: int, half_edge_um: float, seed: int, part: PartitionConfig) -> np.ndarray:
     """Uniform positions inside the axis-aligned cube [-h, h]^3 (um), returned

@@ -44,6 +44,7 @@ class Workload:
     barrier_interval: int = 1     # iterations between count barriers
     replica_D: Optional[np.ndarray] = None  # (num_replicas, num_species) cm^2/s overrides, or None
     surface_classes: Optional[np.ndarray] = None  # uint8 (num_classes, num_species): front | back << 2
+    unimol_rxns: List[M.UnimolReaction] = field(default_factory=list)
 
     def species_for(self, replica: int) -> List[M.Species]:
         if self.replica_D is None:
@@ -224,3 +225,37 @@ def sphere_in_cube(n_l: int = 100000, n_t: int = 10000, iterations: int = 1000, 
     return Workload("sphere_in_cube", part, verts, walls, classes, species, steps, np.arange(n, dtype=np.uint32),
                     sp, pos, np.zeros(n, np.uint32), iterations, config=cfg, objects=objects,
                     surface_classes=sc)
+
+
+def unimol_table(wl: Workload):
+    """Unimolecular classes as the engine takes them: (species, cumulative
+    rates per internal time unit, products per pathway), one class per species."""
+    by = {}
+    for u in wl.unimol_rxns:
+        by.setdefault(u.reactant, []).append(u)
+    out = []
+    for sp, us in sorted(by.items()):
+        cum, acc = [], 0.0
+        for u in us:
+            acc += u.rate * wl.part.time_unit
+            cum.append(acc)
+        out.append((sp, cum, [list(u.products) for u in us]))
+    return out
+
+
+def decay_box(n_e: int = 20000, iterations: int = 2000, seed: int = 6, rate: float = 1e3,
+              d_e: float = 2e-7, back_rate: float = 0.0) -> Workload:
+    """E -> A + B at `rate` /s in the reflective 1 um cube (E with D = d_e
+    cm^2/s, A and B with 1e-6), optionally A + B -> E at `back_rate` /M/s;
+    E uniform (seed), counts every 10 iterations."""
+    cfg = M.Config(time_step=1e-6, subpartition_dimension=0.05, seed=seed)
+    box = M.create_box("box", edge_dimension=1.0)
+    rxns = [M.BimolReaction(0, 1, back_rate, [2])] if back_rate > 0 else []
+    part = M.derive_partition(cfg, M.bounding_box([box]), has_bimol_vol_rxns=bool(rxns))
+    verts, walls, classes = M.vertices_internal([box], part)
+    species = [M.Species("A", 1e-6), M.Species("B", 1e-6), M.Species("E", d_e)]
+    pos = M.place_uniform_in_box(n_e, 0.5, seed, part)
+    return Workload("decay_box", part, verts, walls, classes, species, _species_table(species, part, rxns),
+                    np.arange(n_e, dtype=np.uint32), np.full(n_e, 2, np.uint32), pos, np.zeros(n_e, np.uint32),
+                    iterations, rxns=rxns, config=cfg, objects=[box], barrier_interval=10,
+                    unimol_rxns=[M.UnimolReaction(2, rate, [0, 1])])

@@ -133,6 +133,13 @@ int orc_add_bimol_rxn(void* wp, uint32_t a, uint32_t b, double rate, const uint3
 }
 
 // out: pb_factor, max_fixed_p
+void orc_add_unimol_rxn(void* wp, uint32_t species, double rate, const uint32_t* products, uint32_t nprod) {
+  std::vector<uint32_t> p(products, products + nprod);
+  static_cast<World*>(wp)->add_unimol_rxn(species, rate, p);
+}
+
+uint64_t orc_unimol_reactions(void* wp) { return static_cast<World*>(wp)->unimol_reactions; }
+
 void orc_get_rxn(void* wp, int r, double* out) {
   const BimolRxn& x = static_cast<World*>(wp)->rxns[r];
   out[0] = x.pb_factor; out[1] = x.max_fixed_p;

@@ -209,6 +209,13 @@ struct Species {
   bool is_vol;
 };
 
+// A unimolecular reaction class of one species: cumulative pathway rates
+// per internal time unit (k_tot = cum.back()), products per pathway.
+struct UnimolRxn {
+  std::vector<double> cum;
+  std::vector<std::vector<uint32_t>> products;
+};
+
 struct BimolRxn {
   uint32_t a, b;              // reactant species (a = rule reactant 0)
   double rate;                // M^-1 s^-1
@@ -222,6 +229,7 @@ struct BimolRxn {
 enum : uint32_t {
   MF_DEFUNCT = 1u << 0,
   MF_MATURE = 1u << 1,
+  MF_SCHEDULE_UNIMOL = 1u << 2,  // MOLECULE_FLAG_SCHEDULE_UNIMOL_RXN: lifetime not drawn yet
 };
 struct Mol {
   uint32_t id;
@@ -279,6 +287,9 @@ class World {
   std::vector<Species> species;
   std::vector<BimolRxn> rxns;
   std::vector<int32_t> pair_rxn;        // species x species -> rxn index or -1
+  std::vector<UnimolRxn> unimol;        // per species (empty cum: none)
+  std::vector<uint32_t> calls_this_iteration;  // per molecule id: diffuse calls (stream pass)
+  uint64_t unimol_reactions = 0;
   // surface classes: action of class c (1-based on walls) for species s hit
   // from the front (bits 0-1) and from the back (bits 2-3)
   uint32_t num_surface_classes = 0;
@@ -320,6 +331,8 @@ class World {
   void finalize_walls();
   uint32_t add_species(double D_cm2s, bool is_vol, uint32_t extra_flags);
   int add_bimol_rxn(uint32_t a, uint32_t b, double rate, const std::vector<uint32_t>& products);
+  // rate in s^-1 per pathway (pb = time_unit, MCell3 src/react_util.c:74-78)
+  void add_unimol_rxn(uint32_t species, double rate, const std::vector<uint32_t>& products);
   uint32_t add_volume_molecule(uint32_t species, V3 pos, double time);
 
   void run_iteration();
@@ -378,6 +391,8 @@ class World {
   void start_stream(const Mol& m, uint32_t pass);
 
   void diffuse_single_molecule(uint32_t id);
+  void pick_unimol_time(Mol& m, double current_time);
+  bool react_unimol(uint32_t id);
   double get_max_time(Mol& m);
   void diffuse_vol_molecule(uint32_t id, double& max_time, double diffusion_start_time);
   int ray_trace_vol(uint32_t id, bool can_vol_react, uint32_t last_hit, V3& remaining_disp,

@@ -58,6 +58,7 @@ uint32_t World::add_volume_molecule(uint32_t sp, V3 pos, double time) {
   m.diffusion_time = time;
   m.birthday = time;
   m.unimol_rxn_time = TIME_INVALID;
+  m.flags |= MF_SCHEDULE_UNIMOL;  // set for every new molecule (release_event.cpp, outcome_products_random)
   mols.push_back(m);
   schedulable.push_back(m.id);
   register_reactant(mols.back(), true);
@@ -66,6 +67,51 @@ uint32_t World::add_volume_molecule(uint32_t sp, V3 pos, double time) {
 
 void World::start_stream(const Mol& m, uint32_t pass) {
   if (mode == DRAW_PHILOX) philox.reset(philox_seed, m.id, (uint32_t)iteration, replica, pass);
+}
+
+// diffusion_utils / rxn_utils.inl:720-735 (time_of_unimol) through
+// pick_unimol_rxn_class_and_set_rxn_time (diffuse_react_event.cpp:1731-1755):
+// one draw, lifetime -ln(p) / k_tot, none when p is indistinguishable from 0.
+void World::pick_unimol_time(Mol& m, double current_time) {
+  if (m.species >= unimol.size() || unimol[m.species].cum.empty()) {
+    m.unimol_rxn_time = TIME_INVALID;
+    return;
+  }
+  double k_tot = unimol[m.species].cum.back();
+  double p = rng().dbl();
+  double t;
+  if (k_tot <= 0 || !distinguishable(p, 0, EPS)) t = TIME_FOREVER;
+  else t = -std::log(p) / k_tot;
+  m.unimol_rxn_time = current_time + t;
+}
+
+// react_unimol_single_molecule + outcome_unimolecular
+// (diffuse_react_event.cpp:1760-1826, 2939-3003): the pathway (a draw only with
+// several), products at the molecule's position, created at the scheduled
+// time, then the molecule is destroyed.  Returns whether it survived.
+bool World::react_unimol(uint32_t id) {
+  Mol& m = mols[id];
+  double t = m.unimol_rxn_time;
+  const UnimolRxn& u = unimol[m.species];
+  int pw = 0;
+  if (u.cum.size() > 1) {
+    double match = rng().dbl() * u.cum.back();
+    pw = 0;
+    while (pw + 1 < (int)u.cum.size() && match > u.cum[pw]) ++pw;
+  }
+  V3 pos = m.pos;
+  const uint32_t sp = m.species;  // `m` dangles once products are added
+  std::vector<uint32_t> prods = u.products[pw];
+  for (uint32_t ps : prods) {
+    uint32_t nid = add_volume_molecule(ps, pos, t);
+    if (cmp_lt(t, event_time + 1.0, EPS)) new_diffuse_actions.push_back({nid});
+  }
+  ++unimol_reactions;
+  rxn_log.push_back({(uint32_t)iteration, id, INVALID_U32, (int32_t)(rxns.size() + sp), pw, t, pos});
+  Mol& mm = mols[id];
+  mm.flags |= MF_DEFUNCT;
+  register_reactant(mm, false);
+  return false;
 }
 
 // ------------------------------------------------------------ one iteration
@@ -78,6 +124,7 @@ void World::run_iteration() {
   uint64_t b = barrier_interval ? barrier_interval : 1;
   uint64_t next_barrier = (iteration / b + 1) * b;
   time_up_to_next_barrier = std::min((double)(next_barrier - iteration), 100.0);
+  calls_this_iteration.assign(mols.size(), 0);
   if (snapshot) {
     snapshot_pos.resize(mols.size());
     for (size_t i = 0; i < mols.size(); ++i) snapshot_pos[i] = mols[i].pos;
@@ -129,12 +176,22 @@ double World::get_max_time(Mol& m) {
   return max_time;
 }
 
-// diffuse_react_event.cpp:201-337 (volume molecules, no unimolecular rxns)
+// diffuse_react_event.cpp:201-337 (volume molecules).  In the per-molecule
+// draw mode the n-th call for a molecule within an iteration reads stream
+// pass n.
 void World::diffuse_single_molecule(uint32_t id) {
   Mol& m0 = mols[id];
   if (m0.flags & MF_DEFUNCT) return;
-  start_stream(m0, 0);
+  if (calls_this_iteration.size() < mols.size()) calls_this_iteration.resize(mols.size(), 0);
+  start_stream(m0, calls_this_iteration[id]++);
   double diffusion_start_time = m0.diffusion_time;
+  if (m0.unimol_rxn_time != TIME_INVALID && m0.unimol_rxn_time <= diffusion_start_time) {
+    if (!react_unimol(id)) return;
+  }
+  if (mols[id].flags & MF_SCHEDULE_UNIMOL) {
+    mols[id].flags &= ~MF_SCHEDULE_UNIMOL;
+    pick_unimol_time(mols[id], diffusion_start_time);
+  }
   double max_time = get_max_time(mols[id]);
   if (species[m0.species].is_vol) diffuse_vol_molecule(id, max_time, diffusion_start_time);
   Mol& m = mols[id];
@@ -150,6 +207,10 @@ void World::diffuse_single_molecule(uint32_t id) {
       double rounded = std::round(m.diffusion_time);
       if (cmp_eq(m.diffusion_time, rounded, SQRT_EPS)) m.diffusion_time = rounded;
     }
+  } else if (m.unimol_rxn_time != TIME_INVALID) {
+    // cannot diffuse: scheduled for its unimolecular reaction
+    m.diffusion_time = m.unimol_rxn_time;
+    if (m.unimol_rxn_time < event_end_time) new_diffuse_actions.push_back({m.id});
   } else {
     m.diffusion_time = TIME_FOREVER;
   }

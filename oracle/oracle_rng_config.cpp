@@ -428,3 +428,15 @@ int World::add_bimol_rxn(uint32_t a, uint32_t b, double rate, const std::vector<
 }
 
 }  // namespace orc
+
+namespace orc {
+
+void World::add_unimol_rxn(uint32_t sp, double rate, const std::vector<uint32_t>& products) {
+  if (unimol.size() < species.size()) unimol.resize(species.size());
+  UnimolRxn& u = unimol[sp];
+  double prev = u.cum.empty() ? 0.0 : u.cum.back();
+  u.cum.push_back(prev + rate * cfg.time_unit);
+  u.products.push_back(products);
+}
+
+}  // namespace orc

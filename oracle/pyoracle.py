@@ -54,6 +54,8 @@ def load():
         "orc_get_species": (None, [P, u32, P]),
         "orc_add_bimol_rxn": (ctypes.c_int, [P, u32, u32, d, P, u32]),
         "orc_get_rxn": (None, [P, ctypes.c_int, P]),
+        "orc_add_unimol_rxn": (None, [P, u32, d, P, u32]),
+        "orc_unimol_reactions": (u64, [P]),
         "orc_add_molecules": (ctypes.c_int, [P, u32, u32, P]),
         "orc_set_rng": (None, [P, ctypes.c_int, u64, u32]),
         "orc_set_snapshot": (None, [P, ctypes.c_int]),
@@ -185,6 +187,13 @@ class OracleWorld:
         p = np.asarray(products, np.uint32)
         return self.lib.orc_add_bimol_rxn(self.w, a, b, rate, _p(p), len(p))
 
+    def add_unimol_rxn(self, species, rate, products):
+        p = np.ascontiguousarray(products, np.uint32)
+        self.lib.orc_add_unimol_rxn(self.w, species, rate, _p(p) if len(p) else None, len(p))
+
+    def unimol_reactions(self):
+        return int(self.lib.orc_unimol_reactions(self.w))
+
     def rxn(self, r):
         out = np.zeros(2)
         self.lib.orc_get_rxn(self.w, r, _p(out))
@@ -282,6 +291,8 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
         w.add_species(s.diffusion_constant_3d)
     for r in wl.rxns:
         w.add_bimol_rxn(r.reactant_a, r.reactant_b, r.rate, r.products)
+    for u in getattr(wl, "unimol_rxns", []) or []:
+        w.add_unimol_rxn(u.reactant, u.rate, u.products)
     if getattr(wl, "surface_classes", None) is not None:
         w.set_surface_classes(wl.surface_classes)
     sel = wl.replica == replica

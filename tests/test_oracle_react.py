@@ -150,3 +150,24 @@ def test_ensemble_fixture_is_reproduced_by_the_oracle(oracle_lib):
     ref = np.load(os.path.join(HERE, "golden", "bimol_ensemble.npz"))
     run0 = mk.run_one(0)
     assert np.array_equal(run0, ref["counts"][0])
+
+
+def test_unimolecular_decay_is_exponential(oracle_lib):
+    """E -> A + B at 1e3 /s: E(t) ~ Binomial(E0, exp(-k t)) with the
+    lifetimes -ln(u)/k_tot of rxn_utils.inl:720-735; every decay leaves one A
+    and one B at the E's position."""
+    from oracle import pyoracle as po
+    wl = W.decay_box(n_e=20000, iterations=600)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_ISAAC, seed=3)
+    w.set_barrier_interval(wl.barrier_interval)
+    n0 = 20000
+    for k in range(1, 4):
+        w.run(200)
+        m = w.molecules()
+        live = (m["flags"] & 1) == 0
+        c = np.bincount(m["species"][live], minlength=3)
+        p = math.exp(-1e-3 * 200 * k)
+        sd = math.sqrt(n0 * p * (1 - p))
+        assert abs(c[2] - n0 * p) < 4 * sd, (k, c, n0 * p)
+        assert c[0] == c[1] == n0 - c[2] == w.unimol_reactions()
+    w.close()
