@@ -28,10 +28,11 @@ B_ALG = {"fp64": 64.0, "fp32": 40.0}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--workload", default="free_diffusion_replicas")
+    ap.add_argument("--precision", default="both", choices=["fp64", "fp32", "both"],
+                    help="both: FP32+FP64-fallback is the reported value, the plain FP64 form runs beside it")
+    ap.add_argument("--workload", default="replicas_sphere")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--mols-per-replica", type=int, default=150000)
     ap.add_argument("--subpart", type=float, default=None,
@@ -98,6 +99,10 @@ def cpu_baseline(seconds, workload):
     if workload.startswith("free"):
         wl = W.free_diffusion_box()
         what = "free diffusion of 10,000 A in the reflective 1 um cube"
+    elif workload == "replicas_sphere":
+        wl = W.replicas_sphere_in_cube(num_replicas=1)
+        what = ("instance 0 of the replica batch (A + B -> C, %d molecules, sphere-in-cube mesh, 5,132 walls), "
+                "one single-threaded process" % wl.num_molecules)
     else:
         wl = W.bimol_box()
         what = "A + B -> C, 50,000 A + 50,000 B in the reflective 1 um cube"
@@ -115,10 +120,33 @@ def cpu_baseline(seconds, workload):
                       "%.1f s" % (what, done, secs)}
 
 
-def kernel_name(args, wl):
+def kernel_name(precision, wl):
     if wl.rxns:
-        return "k_move<double>" if args.precision == "fp64" else "k_move<float>"
-    return "k_diffuse_reflect64" if args.precision == "fp64" else "k_diffuse_reflect32"
+        return "k_move_light + k_move_drain <%s>" % ("double" if precision == "fp64" else "float")
+    return "k_light64 + k_hard64" if precision == "fp64" else "k_light32 + k_hard32"
+
+
+def run_gpu(args, wl, precision):
+    """Warm-up, then K timed iterations of one precision form; returns the
+    step statistics and timings."""
+    from mcell_amd.engine import DiffuseReactEngine
+    eng = DiffuseReactEngine.from_workload(wl, precision=precision,
+                                           capacity=wl.num_molecules * (2 if wl.rxns else 1))
+    print("bench: %s, %d molecules, precision %s, warmup %d, steps %d" % (args.workload, wl.num_molecules, precision,
+                                                                          args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+    if args.warmup:
+        eng.step(0, args.warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    st = eng.step(args.warmup, args.steps)
+    eng.sync()
+    secs = time.perf_counter() - t0
+    avg_ms, launches = eng.last_kernel_time()
+    eng.close()
+    steps_done = st.molecule_steps if wl.rxns else wl.num_molecules * args.steps
+    return {"secs": secs, "steps_done": steps_done, "value": steps_done / secs, "avg_ms": avg_ms,
+            "reactions": st.reactions, "rounds": st.wall_hits, "hard": st.slow_path_last}
 
 
 def main():
@@ -130,62 +158,51 @@ def main():
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-    from mcell_amd.engine import DiffuseReactEngine
     from mcell_amd import workloads as W
     wl, cfg = build_workload(args)
+    main_prec = "fp32" if args.precision in ("fp32", "both") else "fp64"
     if args.subpart is not None:
         wl = W.with_subpartition(wl, args.subpart)
     elif wl.num_replicas > 1 and not args.no_autotune:
         from mcell_amd import autotune
         best, rates = autotune.choose_subpartition(
-            wl, args.precision, log=lambda m: print(m, file=sys.stderr, flush=True))
+            wl, main_prec, log=lambda m: print(m, file=sys.stderr, flush=True))
         wl = W.with_subpartition(wl, best)
         cfg["autotune"] = {"subpartition_um": best, "rates": {str(k): v for k, v in rates.items()}}
     cfg["subpartition_um"] = wl.config.subpartition_dimension
-    eng = DiffuseReactEngine.from_workload(wl, precision=args.precision,
-                                           capacity=wl.num_molecules * (2 if wl.rxns else 1))
-    n = wl.num_molecules
-    print("bench: %s, %d molecules, precision %s, warmup %d, steps %d" % (args.workload, n, args.precision,
-                                                                          args.warmup, args.steps),
-          file=sys.stderr, flush=True)
-    for it in range(args.warmup):
-        eng.step(it, 1)
-    eng.sync()
-    t0 = time.perf_counter()
-    st = eng.step(args.warmup, args.steps)
-    eng.sync()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    steps_done = st.molecule_steps if wl.rxns else n * args.steps
-    value = steps_done / secs
-    avg_ms, launches = eng.last_kernel_time()
-    per_step = steps_done / args.steps
-    achieved = B_ALG[args.precision] * per_step / (avg_ms * 1e-3) / 1e9
-    eng.close()
+    r = run_gpu(args, wl, main_prec)
+    per_step = r["steps_done"] / args.steps
+    achieved = B_ALG[main_prec] * per_step / (r["avg_ms"] * 1e-3) / 1e9
     out = {
         "metric": "molecule-timesteps/s on 1 MI355X (% HBM peak); FP32+fallback vs FP64 speedup",
-        "value": value,
+        "value": r["value"],
         "unit": "molecule-timesteps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * secs / args.steps,
+        "ms_per_step": 1e3 * r["secs"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if main_prec == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": kernel_name(args, wl), "kernel_avg_ms": avg_ms,
-                     "bytes_per_molecule_step": B_ALG[args.precision]},
+                     "kernel": kernel_name(main_prec, wl), "kernel_avg_ms": r["avg_ms"],
+                     "bytes_per_molecule_step": B_ALG[main_prec]},
     }
     if wl.rxns:
-        out["reactions"] = st.reactions
-        out["settle_rounds"] = st.slow_path_last
+        out["reactions"] = r["reactions"]
+        out["settle_rounds"] = r["rounds"]
+        out["roofline"]["full_step_fraction"] = r["hard"] / max(per_step, 1)
     else:
-        out["roofline"]["slow_path_fraction"] = st.slow_path_last / max(n, 1)
+        out["roofline"]["slow_path_fraction"] = r["hard"] / max(wl.num_molecules, 1)
+    if args.precision == "both":
+        r64 = run_gpu(args, wl, "fp64")
+        out["fp64"] = {"value": r64["value"], "ms_per_step": 1e3 * r64["secs"] / args.steps,
+                       "kernel_avg_ms": r64["avg_ms"], "reactions": r64["reactions"]}
+        out["fp32_over_fp64"] = r["value"] / r64["value"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, args.workload)
     print(json.dumps(out), flush=True)

@@ -34,12 +34,15 @@ struct UnimolRec {
 constexpr uint32_t kUnimolBit = 0x80000000u;  // Proposal::rxn of a unimolecular event: bit | species
 
 // Cell grid over the region molecules can occupy: subpartition index box
-// [lo, lo + dim) per replica.  start[] is the CSR over the cell-sorted
-// snapshot, per_replica = dim.x * dim.y * dim.z.
+// [lo, lo + dim) per replica, per_replica = dim.x * dim.y * dim.z cells.  The
+// snapshot is sorted by bin = cell * ns + species, so a mover reads exactly
+// the partners of the species it reacts with: start[] is the CSR over the
+// bins (a cell's molecules are start[c * ns] .. start[(c + 1) * ns]).
 struct CellGrid {
   int lo[3];
   int dim[3];
   uint32_t per_replica;
+  uint32_t ns;
   const uint32_t* __restrict__ start;
 };
 

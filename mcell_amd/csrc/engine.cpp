@@ -32,6 +32,42 @@ mcg_ctx::~mcg_ctx() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+namespace {
+
+// The FP32 position of an FP64 point, kept strictly on the point's side of
+// every wall plane of its subpartition (a rounded point on a plane would let
+// the first move leave through the wall: collide_wall misses a move that
+// starts on the plane, collision_utils.inl:647-665).
+void round_same_side_host(const mcg_ctx* c, double px, double py, double pz, float& fx, float& fy, float& fz) {
+  const PartitionParams& P = c->P;
+  fx = (float)px;
+  fy = (float)py;
+  fz = (float)pz;
+  const int sx = (int)((px - P.ox) * P.sp_rcp), sy = (int)((py - P.oy) * P.sp_rcp), sz = (int)((pz - P.oz) * P.sp_rcp);
+  if (sx < 0 || sy < 0 || sz < 0 || sx >= (int)P.n || sy >= (int)P.n || sz >= (int)P.n) return;
+  const uint32_t s = (uint32_t)sx + (uint32_t)sy * P.n + (uint32_t)sz * P.n2;
+  for (int pass = 0; pass < 4; ++pass) {
+    bool moved = false;
+    for (uint32_t k = c->off_host[s]; k < c->off_host[s + 1]; ++k) {
+      const WallRec64& w = c->walls_host[c->idx_host[k]];
+      const double sd = ((w.nx * px + w.ny * py) + w.nz * pz) - w.d;
+      const double sf = ((w.nx * (double)fx + w.ny * (double)fy) + w.nz * (double)fz) - w.d;
+      if (sd == 0 || (sd > 0 && sf > 0) || (sd < 0 && sf < 0)) continue;
+      const float dir = sd > 0 ? 1.0f : -1.0f;
+      if (w.nx * dir > 0) fx = std::nextafter(fx, 3.4e38f);
+      else if (w.nx * dir < 0) fx = std::nextafter(fx, -3.4e38f);
+      if (w.ny * dir > 0) fy = std::nextafter(fy, 3.4e38f);
+      else if (w.ny * dir < 0) fy = std::nextafter(fy, -3.4e38f);
+      if (w.nz * dir > 0) fz = std::nextafter(fz, 3.4e38f);
+      else if (w.nz * dir < 0) fz = std::nextafter(fz, -3.4e38f);
+      moved = true;
+    }
+    if (!moved) break;
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* mcg_last_error(void) { return g_err.c_str(); }
@@ -216,9 +252,10 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
   }
   std::vector<float> fx, fy, fz;
   if (c->fp32()) {
+    if (!c->have_geometry) return fail(MCG_E_STATE, "the FP32 form needs the geometry before the molecules");
     fx.resize(count); fy.resize(count); fz.resize(count);
     for (uint64_t i = 0; i < count; ++i) {
-      fx[i] = (float)hx[i]; fy[i] = (float)hy[i]; fz[i] = (float)hz[i];
+      round_same_side_host(c, hx[i], hy[i], hz[i], fx[i], fy[i], fz[i]);
       if (!(fx[i] < (float)P.qx && fy[i] < (float)P.qy && fz[i] < (float)P.qz))
         return fail(MCG_E_INVALID, "molecule rounds onto the partition boundary in FP32");
     }

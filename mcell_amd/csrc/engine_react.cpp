@@ -38,7 +38,10 @@ void setup_grid(mcg_ctx* c) {
     G.dim[a] = h - l + 1;
   }
   G.per_replica = (uint32_t)G.dim[0] * (uint32_t)G.dim[1] * (uint32_t)G.dim[2];
-  c->rx.num_cells = G.per_replica * c->cfg.num_replicas;
+  G.ns = c->num_species;
+  // sort bins: (replica, cell, species)
+  const uint64_t bins = (uint64_t)G.per_replica * c->cfg.num_replicas * G.ns;
+  c->rx.num_cells = bins < 0xFFFFFFF0ull ? (uint32_t)bins : 0u;
 }
 
 template <typename PosT>
@@ -200,7 +203,14 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
   if (stats) {
     stats->molecule_steps = steps;
     stats->reactions = reactions;
-    stats->slow_path_last = rounds;
+    stats->wall_hits = rounds;  // settling rounds (claims + reruns) over the call
+    // movers of the last iteration that took the full step
+    std::vector<uint32_t> q(kNumQueueShards, 0);
+    if (num && c->n)
+      HIP_TRY(hipMemcpy(q.data(), c->qcount.p, kNumQueueShards * 4, hipMemcpyDeviceToHost));
+    uint64_t hard = 0;
+    for (uint32_t v : q) hard += v;
+    stats->slow_path_last = (uint32_t)hard;
   }
   return MCG_OK;
 }
@@ -247,6 +257,7 @@ int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mc
   auto t0 = std::chrono::steady_clock::now();
   if (r.grid_dirty) {
     setup_grid(c);
+    if (r.num_cells == 0) return fail(MCG_E_LIMIT, "reaction grid too large (cells x replicas x species)");
     if (r.cell_count.alloc((size_t)r.num_cells + 1) || r.cell_start.alloc((size_t)r.num_cells + 1))
       return fail(MCG_E_NOMEM, "cell grid");
     size_t tmp = ReactPipeline<double>::scan_bytes(r.num_cells, (uint32_t)c->cfg.molecule_capacity);

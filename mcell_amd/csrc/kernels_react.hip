@@ -60,6 +60,8 @@ __global__ __launch_bounds__(256) void k_cell_keys(RMolView<PosT> m, uint32_t n,
     if (!cell_of(G, sx, sy, sz, tag_replica(tag), c)) {
       atomicOr(err, ERR_GRID);
       c = kInvalid;
+    } else {
+      c = c * G.ns + tag_species(tag);
     }
   }
   key[i] = c;
@@ -316,21 +318,29 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
           chi[k] = h > R.G.lo[k] + R.G.dim[k] - 1 ? R.G.lo[k] + R.G.dim[k] - 1 : h;
         }
         const uint32_t my_sp = tag_species(tag), rep = tag_replica(tag);
+        // the segment's box widened by the radius: partners outside it cannot collide
+        const double R1 = R.P.rxn_radius;
+        const double bx0 = (a[0] < b[0] ? a[0] : b[0]) - R1, bx1 = (a[0] < b[0] ? b[0] : a[0]) + R1;
+        const double by0 = (a[1] < b[1] ? a[1] : b[1]) - R1, by1 = (a[1] < b[1] ? b[1] : a[1]) + R1;
+        const double bz0 = (a[2] < b[2] ? a[2] : b[2]) - R1, bz1 = (a[2] < b[2] ? b[2] : a[2]) + R1;
         for (int z = clo[2]; z <= chi[2] && clear; ++z)
           for (int y = clo[1]; y <= chi[1] && clear; ++y)
             for (int x = clo[0]; x <= chi[0] && clear; ++x) {
               uint32_t c;
               if (!cell_of(R.G, x, y, z, rep, c)) continue;
-              const uint32_t k1 = R.G.start[c + 1];
-              for (uint32_t k = R.G.start[c]; k < k1; ++k) {
-                if (R.pair_rxn[my_sp * R.num_species + tag_species(S.tag[k])] < 0) continue;
-                double t;
-                D3 cp;
-                if (!collide_mol64(pos, disp, D3{(double)S.x[k], (double)S.y[k], (double)S.z[k]}, sigma2, t, cp))
-                  continue;
-                if (k == j) continue;
-                clear = false;
-                break;
+              for (uint32_t t = 0; t < R.G.ns && clear; ++t) {
+                if (R.pair_rxn[my_sp * R.num_species + t] < 0) continue;
+                const uint32_t k1 = R.G.start[c * R.G.ns + t + 1];
+                for (uint32_t k = R.G.start[c * R.G.ns + t]; k < k1; ++k) {
+                  const double qx = (double)S.x[k], qy = (double)S.y[k], qz = (double)S.z[k];
+                  if (qx < bx0 || qx > bx1 || qy < by0 || qy > by1 || qz < bz0 || qz > bz1) continue;
+                  double tc;
+                  D3 cp;
+                  if (!collide_mol64(pos, disp, D3{qx, qy, qz}, sigma2, tc, cp)) continue;
+                  if (k == j) continue;
+                  clear = false;
+                  break;
+                }
               }
             }
       }
