@@ -67,20 +67,25 @@ struct Snapshot {
   const double* __restrict__ utime;       // unimolecular reaction times (absolute) or kTimeInvalid
 };
 
-// A small set of subpartition indices (the reference's SubpartIndicesSet).
+// A small set of subpartition indices (the reference's SubpartIndicesSet),
+// stored in the calling workgroup's LDS: v[k * stride] is entry k of this
+// lane's set (stride = the workgroup size, so that a wave's lanes touch
+// consecutive words).
 struct SubpartSet {
   static constexpr int kCap = 48;
-  uint32_t v[kCap];
+  uint32_t* v;
+  uint32_t stride;
   int n;
   bool overflow;
+  __device__ __forceinline__ uint32_t at(int k) const { return v[(uint32_t)k * stride]; }
   __device__ __forceinline__ void clear() {
     n = 0;
     overflow = false;
   }
   __device__ __forceinline__ void insert(uint32_t s) {
     for (int k = 0; k < n; ++k)
-      if (v[k] == s) return;
-    if (n < kCap) v[n++] = s;
+      if (at(k) == s) return;
+    if (n < kCap) v[(uint32_t)(n++) * stride] = s;
     else overflow = true;
   }
 };

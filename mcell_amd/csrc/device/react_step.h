@@ -64,12 +64,13 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
                                                uint32_t replica, uint32_t my_species, uint32_t my_id, D3 pos,
                                                D3 disp, double t_limit, double t_last, uint32_t id_last,
                                                bool first, double& t_best, uint32_t& slot_best, D3& pos_best,
-                                               int32_t& rxn_best) {
+                                               int32_t& rxn_best, uint32_t& candidates) {
   const double sigma2 = R.P.rxn_radius * R.P.rxn_radius;
   bool found = false;
   uint32_t id_best = 0;
+  candidates = 0;
   for (int k = 0; k < set.n; ++k) {
-    const uint32_t s = set.v[k];
+    const uint32_t s = set.at(k);
     const int gx = (int)(s % R.P.n), gy = (int)((s / R.P.n) % R.P.n), gz = (int)((s / R.P.n2) % R.P.n);
     uint32_t c;
     if (!cell_of(R.G, gx, gy, gz, replica, c)) continue;  // no molecules outside the grid
@@ -86,6 +87,7 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
       if (pid == my_id || S.consumed[j]) continue;
       // strictly after the last processed key
       if (!first && !(t > t_last || (t == t_last && pid < id_last))) continue;
+      ++candidates;
       if (!found || t < t_best || (t == t_best && pid > id_best)) {
         found = true;
         t_best = t;
@@ -135,7 +137,7 @@ template <typename PosT>
 __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot<PosT>& S, const SpeciesRec& sp,
                                             const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
                                             double utime, double barrier, Proposal& prop, MoverEnd& out,
-                                            uint32_t& err) {
+                                            uint32_t& err, uint32_t* set_lds) {
   constexpr bool kF32 = sizeof(PosT) == 4;  // FP32 positions: FP32 wall filter
   const uint32_t rep = tag_replica(tag), spc = tag_species(tag);
   PhiloxStream rng;
@@ -161,6 +163,9 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   uint32_t last_hit = kInvalid;
   const bool can_react = (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY);
   SubpartSet set;
+  set.v = set_lds;
+  set.stride = blockDim.x;
+  set.clear();
   int seg;
   for (seg = 0; seg < kMaxSegments; ++seg) {
     bool near_plane = false;
@@ -208,17 +213,25 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
         uint32_t slot;
         D3 cp;
         int32_t rx;
-        if (!next_collision(R, S, set, rep, spc, idv, pos, disp, t_limit, t_last, id_last, first, tc, slot, cp, rx))
+        uint32_t candidates;
+        if (!next_collision(R, S, set, rep, spc, idv, pos, disp, t_limit, t_last, id_last, first, tc, slot, cp, rx,
+                            candidates))
           break;
         first = false;
         t_last = tc;
         id_last = S.id[slot];
-        if (tc < STIME_EPS) continue;  // is_immediate_collision (collision_utils.inl:814-816)
+        if (tc < STIME_EPS) {  // is_immediate_collision (collision_utils.inl:814-816)
+          if (candidates == 1) break;
+          continue;
+        }
         bool full = false;
         const D3 target = {(double)S.x[slot], (double)S.y[slot], (double)S.z[slot]};
         const double factor = exact_disk_dev(R.P, R.walls, R.off, R.idx, cp, disp, R.P.rxn_radius, target, &full);
         if (full) err |= ERR_EXD;
-        if (factor < 0) continue;  // the disk is occluded
+        if (factor < 0) {  // the disk is occluded
+          if (candidates == 1) break;
+          continue;
+        }
         const int pw = test_bimolecular(R.rxns[rep * R.rxn_stride + rx], factor * r_rate_factor, rng);
         if (pw >= 0) {
           prop.time = elapsed + t_steps * tc;
@@ -236,6 +249,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
           out.pos = found ? pos : D3{pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
           return true;
         }
+        if (candidates == 1) break;  // that was the last collision of the segment
       }
     }
     if (!found) {

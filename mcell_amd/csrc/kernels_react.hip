@@ -150,7 +150,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
                                          const RMolView<PosT>& out, uint32_t j, SpeciesTable species,
                                          const ZigguratTables& T, double barrier, uint32_t propose_only,
                                          Proposal* __restrict__ prop, uint32_t* __restrict__ owner,
-                                         uint32_t* __restrict__ err_out, bool& due) {
+                                         uint32_t* __restrict__ err_out, bool& due, uint32_t* set_lds) {
   const uint32_t tag = src.tag[j];
   const uint32_t idv = src.id[j];
   const double toff = src.toff[j];
@@ -168,7 +168,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   Proposal p;
   bool proposed = false;
   const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
-  if (moves) proposed = react_mover(R, S, sp, T, pos, idv, tag, toff, utime, barrier, p, e, err);
+  if (moves) proposed = react_mover(R, S, sp, T, pos, idv, tag, toff, utime, barrier, p, e, err, set_lds);
   due = e.due != 0;
   if (due) {
     // the unimolecular event, settled after the phase's bimolecular claims
@@ -220,6 +220,7 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
                                               uint32_t* __restrict__ owner, uint32_t* __restrict__ err_out,
                                               uint32_t* __restrict__ ulist, uint32_t* __restrict__ ucount) {
   __shared__ ZigguratTables T;
+  __shared__ uint32_t sets[SubpartSet::kCap * 256];
   stage_zig(T);
   const uint32_t n = *count;
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,7 +228,8 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
   uint32_t j = 0;
   if (q < n) {
     j = list ? list[q] : q;
-    proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, err_out, due);
+    proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, err_out, due,
+                        sets + threadIdx.x);
   }
   append_slot(proposed, j, plist, pcount);
   append_slot(due, j, ulist, ucount);
@@ -242,6 +244,7 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
                                                     uint32_t* __restrict__ owner, uint32_t* __restrict__ err_out,
                                                     uint32_t* __restrict__ ulist, uint32_t* __restrict__ ucount) {
   __shared__ ZigguratTables T;
+  __shared__ uint32_t sets[SubpartSet::kCap * 256];
   stage_zig(T);
   const uint32_t count = Q.count[blockIdx.x];
   const uint32_t* slots = Q.slots + (size_t)blockIdx.x * Q.shard_cap;
@@ -251,7 +254,8 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
     uint32_t j = 0;
     if (q < count) {
       j = slots[q];
-      proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, err_out, due);
+      proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, err_out, due,
+                          sets + threadIdx.x);
     }
     append_slot(proposed, j, plist, pcount);
     append_slot(due, j, ulist, ucount);
