@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 2
+#define MCG_ABI_VERSION 3
 
 typedef struct mcg_ctx mcg_ctx;
 
@@ -127,8 +127,9 @@ typedef struct mcg_rxn_event {
   double pos[3];              /* collision point */
   uint32_t iteration, replica;
   uint32_t mover_id, partner_id;
-  uint32_t rxn, pathway;      /* index into the mcg_set_bimol_rxns table, or for a unimolecular
-                                 reaction 0x80000000 | reactant species (partner_id 0xFFFFFFFF) */
+  uint32_t rxn, pathway;      /* index into the mcg_set_bimol_rxns table; for a unimolecular
+                                 reaction 0x80000000 | reactant species (partner_id 0xFFFFFFFF);
+                                 for a volume-surface one 0x40000000 | mcg_set_volsurf_rxns index */
   uint32_t first_product_id;  /* products get ids first_product_id + k */
   uint32_t num_products;
 } mcg_rxn_event;
@@ -199,6 +200,42 @@ int mcg_set_replica_params(mcg_ctx* ctx, uint32_t replica, const mcg_species* sp
 /* Unimolecular reaction classes (at most one per species). */
 int mcg_set_unimol_rxns(mcg_ctx* ctx, const mcg_unimol_rxn* rxns, uint32_t num_rxns);
 
+/* A volume + surface reaction class: the volume reactant hits a wall tile
+ * holding the surface reactant (collide_and_react_with_surf_mol,
+ * diffuse_react_event.cpp:845-975).  Pathway k replaces the surface reactant
+ * on its tile by product[k] with orientation product_orient[k] and destroys
+ * the volume reactant.  cum_probs are cumulative reaction probabilities per
+ * hit (rate x pb_factor, MCell3 src/react_util.c:132-157) before the tile
+ * binding factor; geom_vol and geom_surf are the orientations in the rule
+ * (0 none, +1, -1), matched as trigger_bimolecular does (rxn_utils.inl:58-84).
+ * Volume reactants get MCG_SPF_CAN_VOLSURF. */
+typedef struct mcg_volsurf_rxn {
+  uint32_t vol_species, surf_species;
+  int32_t geom_vol, geom_surf;
+  uint32_t num_pathways;           /* 1..4 */
+  uint32_t reserved;
+  double cum_probs[4];
+  uint32_t product[4];             /* surface product species per pathway */
+  int32_t product_orient[4];       /* +1 or -1 */
+} mcg_volsurf_rxn;
+
+int mcg_set_volsurf_rxns(mcg_ctx* ctx, const mcg_volsurf_rxn* rxns, uint32_t num_rxns);
+
+/* Surface molecules of one replica (Molecule's surface part: wall, tile,
+ * orientation, molecule.h:52-167).  A wall gets its tile grid (Grid::initialize,
+ * wall.cpp:38-74: ceil(sqrt(area))^2 tiles) with its first molecule; a tile
+ * holds at most one molecule.  Surface molecules do not diffuse on this path;
+ * volume molecules react with them through mcg_set_volsurf_rxns.  ids share
+ * the replica's id space with the volume molecules.  Call after
+ * mcg_set_geometry and mcg_set_species. */
+int mcg_add_surface_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids, const uint32_t* species,
+                              const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, uint64_t count);
+
+/* The surface molecules back, in (replica, wall, tile) order.  Any output
+ * pointer may be NULL; *count gets the number. */
+int mcg_get_surface_molecules(mcg_ctx* ctx, uint32_t* ids, uint32_t* species, uint32_t* replica, uint32_t* walls,
+                              uint32_t* tiles, int32_t* orients, uint64_t capacity, uint64_t* count);
+
 /* Keeps up to `capacity` reaction events of each mcg_step call (0 = off). */
 int mcg_set_rxn_log(mcg_ctx* ctx, uint64_t capacity);
 
@@ -207,8 +244,9 @@ int mcg_set_rxn_log(mcg_ctx* ctx, uint64_t capacity);
  * exceed what the log kept). */
 int mcg_get_rxn_log(mcg_ctx* ctx, mcg_rxn_event* events, uint64_t capacity, uint64_t* count);
 
-/* Live volume molecules per (replica, species): counts[r * num_species + s]
- * (MolOrRxnCountEvent's molecule counts over the whole partition). */
+/* Live molecules (volume and surface) per (replica, species):
+ * counts[r * num_species + s] (MolOrRxnCountEvent's molecule counts over the
+ * whole partition). */
 int mcg_count_species(mcg_ctx* ctx, uint64_t* counts);
 
 /* Volume molecules of one replica (Partition::add_volume_molecule semantics:

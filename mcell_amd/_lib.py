@@ -35,7 +35,7 @@ EXPORTED = [
     "mcg_add_volume_molecules", "mcg_step", "mcg_sync", "mcg_num_molecules", "mcg_get_volume_molecules",
     "mcg_get_walls", "mcg_get_hit_trace", "mcg_last_kernel_time", "mcg_set_bimol_rxns", "mcg_set_rxn_log",
     "mcg_get_rxn_log", "mcg_count_species", "mcg_set_replica_params", "mcg_set_surface_classes",
-    "mcg_set_unimol_rxns",
+    "mcg_set_unimol_rxns", "mcg_set_volsurf_rxns", "mcg_add_surface_molecules", "mcg_get_surface_molecules",
 ]
 
 
@@ -87,6 +87,20 @@ class McgUnimolRxn(ctypes.Structure):
         ("cum_rates", ctypes.c_double * 4),
         ("num_products", ctypes.c_uint32 * 4),
         ("products", (ctypes.c_uint32 * 3) * 4),
+    ]
+
+
+class McgVolSurfRxn(ctypes.Structure):
+    _fields_ = [
+        ("vol_species", ctypes.c_uint32),
+        ("surf_species", ctypes.c_uint32),
+        ("geom_vol", ctypes.c_int32),
+        ("geom_surf", ctypes.c_int32),
+        ("num_pathways", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("cum_probs", ctypes.c_double * 4),
+        ("product", ctypes.c_uint32 * 4),
+        ("product_orient", ctypes.c_int32 * 4),
     ]
 
 
@@ -163,6 +177,10 @@ def load():
     lib.mcg_set_surface_classes.argtypes = [P, ctypes.c_uint32, P]
     lib.mcg_set_unimol_rxns.argtypes = [P, ctypes.POINTER(McgUnimolRxn), ctypes.c_uint32]
     lib.mcg_set_replica_params.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(McgSpecies), P]
+    lib.mcg_set_volsurf_rxns.argtypes = [P, ctypes.POINTER(McgVolSurfRxn), ctypes.c_uint32]
+    lib.mcg_add_surface_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, ctypes.c_uint64]
+    lib.mcg_get_surface_molecules.argtypes = [P, P, P, P, P, P, P, ctypes.c_uint64,
+                                              ctypes.POINTER(ctypes.c_uint64)]
     for name in EXPORTED:
         if not name.startswith("mcg_last_error") and name != "mcg_abi_version":
             getattr(lib, name).restype = ctypes.c_int

@@ -33,6 +33,37 @@ struct UnimolRec {
 };
 constexpr uint32_t kUnimolBit = 0x80000000u;  // Proposal::rxn of a unimolecular event: bit | species
 
+// Surface grid of a wall (Grid::initialize, src4/wall.cpp:38-74); num_tiles
+// 0 = the wall has no grid.  Tiles of all grids are numbered from tile_base.
+struct WallGrid {
+  double strip_width_rcp, vert2_slope, fullslope, binding_factor;
+  double vert0_u, vert0_v;
+  uint32_t n_axis, num_tiles, tile_base, pad;
+};
+static_assert(sizeof(WallGrid) == 64, "grid record");
+
+// A volume + surface reaction class: the volume reactant hitting the tile of
+// the surface reactant; the surface product takes the tile, the volume
+// reactant is destroyed.
+struct VolSurfRec {
+  double max_fixed_p;
+  double cum_probs[kMaxPathways];
+  float max_p_f;
+  uint32_t num_pathways;
+  int32_t geom_vol, geom_surf;  // orientations in the rule (0 none, +1, -1)
+  uint32_t product[kMaxPathways];
+  int32_t product_orient[kMaxPathways];
+};
+constexpr uint32_t kVolSurfBit = 0x40000000u;  // Proposal::rxn of a volume-surface reaction: bit | class
+constexpr uint32_t kTileBit = 0x80000000u;     // Proposal::partner_slot naming a tile: bit | tile slot
+constexpr uint32_t kTileEmpty = 0xFFFFFFFFu;
+// tile word: species (8 bits) | orientation + 1 (2 bits) << 8
+__device__ __host__ __forceinline__ uint32_t tile_species(uint32_t w) { return w & 0xFFu; }
+__device__ __host__ __forceinline__ int tile_orient(uint32_t w) { return (int)((w >> 8) & 3u) - 1; }
+__device__ __host__ __forceinline__ uint32_t make_tile(uint32_t species, int orient) {
+  return (species & 0xFFu) | ((uint32_t)(orient + 1) << 8);
+}
+
 // Cell grid over the region molecules can occupy: subpartition index box
 // [lo, lo + dim) per replica, per_replica = dim.x * dim.y * dim.z cells.  The
 // snapshot is sorted by bin = cell * ns + species, so a mover reads exactly

@@ -33,7 +33,40 @@ struct ReactParams {
   float band;
   SurfaceClasses SC;
   const UnimolRec* __restrict__ unimol;  // per species (num_pathways 0: none)
+  // surface molecules on wall grids
+  const WallGrid* __restrict__ grids;    // per wall, or null without surface molecules
+  const uint32_t* __restrict__ tiles;    // [replica * total_tiles + tile]: tile word or kTileEmpty
+  const uint32_t* __restrict__ tile_id;
+  const uint32_t* __restrict__ tile_consumed;
+  uint32_t total_tiles;
+  const VolSurfRec* __restrict__ vs_rxns;
+  const int32_t* __restrict__ pair_vs;   // [vol species * ns + surf species], -1 = none
 };
+
+// xyz2grid_tile_index (grid_utils.inl:48-118) for a point on the wall.
+__device__ __forceinline__ uint32_t xyz2grid(const WallRec64& w, const WallGrid& g, D3 v) {
+  if (g.num_tiles == 1) return 0;
+  const uint32_t tile_idx_mid = g.num_tiles - 2 * g.n_axis + 1;
+  if (!exd::distinguishable_vec3(v, d3(w.p0x, w.p0y, w.p0z), POS_EPS)) return tile_idx_mid;
+  if (!exd::distinguishable_vec3(v, d3(w.p1x, w.p1y, w.p1z), POS_EPS)) return g.num_tiles - 1;
+  if (!exd::distinguishable_vec3(v, d3(w.p2x, w.p2y, w.p2z), POS_EPS)) return 0;
+  const double i = dot3(v, d3(w.ux, w.uy, w.uz)) - g.vert0_u;
+  const double j = dot3(v, d3(w.vx, w.vy, w.vz)) - g.vert0_v;
+  const double striploc = j * g.strip_width_rcp;
+  int strip = (int)striploc;
+  const double striprem = striploc - strip;
+  strip = (int)g.n_axis - strip - 1;
+  const double u0 = j * g.vert2_slope;
+  const double u1_u0 = w.uv1u - j * g.fullslope;
+  const double stripeloc = ((i - u0) / u1_u0) * (strip + (1 - striprem));
+  const int stripe = (int)stripeloc;
+  const double striperem = stripeloc - stripe;
+  const int flip = (striperem < 1 - striprem) ? 0 : 1;
+  int idx = strip * strip + 2 * stripe + flip;
+  if (idx < 0) idx = 0;
+  if ((uint32_t)idx >= g.num_tiles) idx = (int)g.num_tiles - 1;  // the reference asserts this cannot happen
+  return (uint32_t)idx;
+}
 
 struct Proposal {
   double time;    // absolute time of the collision
@@ -257,6 +290,44 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
       pos = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
       out.near_plane = near_plane;
       break;
+    }
+    // collide_and_react_with_surf_mol (diffuse_react_event.cpp:505-527, 845-975):
+    // the surface molecule on the tile the hit point falls in
+    if ((sp.flags & SPF_CAN_VOLSURF) && R.grids && R.grids[hit.wall].num_tiles) {
+      const WallGrid& g = R.grids[hit.wall];
+      const uint32_t slot = rep * R.total_tiles + g.tile_base + xyz2grid(R.walls[hit.wall], g, hit.pos);
+      const uint32_t tw = R.tiles[slot];
+      if (tw != kTileEmpty && !R.tile_consumed[slot]) {
+        const int32_t ri = R.pair_vs[spc * R.num_species + tile_species(tw)];
+        if (ri >= 0) {
+          const VolSurfRec& vs = R.vs_rxns[ri];
+          // trigger_bimolecular's orientation rule (rxn_utils.inl:58-84)
+          const int ga = vs.geom_vol, gb = vs.geom_surf;
+          const int orient_vol = hit.type == WALL_FRONT ? 1 : -1;
+          if (ga == 0 || gb == 0 || (ga + gb) * (ga - gb) != 0 || orient_vol * tile_orient(tw) * ga * gb > 0) {
+            RxnRec rr;
+            rr.max_fixed_p = vs.max_fixed_p;
+            rr.max_p_f = vs.max_p_f;
+            rr.num_pathways = vs.num_pathways;
+            for (int k = 0; k < kMaxPathways; ++k) rr.cum_probs[k] = vs.cum_probs[k];
+            const int pw = test_bimolecular(rr, r_rate_factor / g.binding_factor, rng);
+            if (pw >= 0) {
+              prop.time = elapsed + t_steps * hit.t;
+              prop.x = hit.pos.x;
+              prop.y = hit.pos.y;
+              prop.z = hit.pos.z;
+              prop.partner_slot = kTileBit | slot;
+              prop.mover_id = idv;
+              prop.partner_id = R.tile_id[slot];
+              prop.rxn = kVolSurfBit | (uint32_t)ri;
+              prop.pathway = (uint32_t)pw;
+              prop.replica = rep;
+              out.pos = pos;
+              return true;
+            }
+          }
+        }
+      }
     }
     // collide_and_react_with_walls (diffuse_react_event.cpp:529-561, 991-1067)
     const uint32_t act = R.SC.of(hit.wall, spc, hit.type == WALL_FRONT);

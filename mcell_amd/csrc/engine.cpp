@@ -194,8 +194,6 @@ int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
   if (!c || !sp || ns == 0 || ns > 256) return fail(MCG_E_INVALID, "bad species table");
   std::vector<SpeciesRec> h(ns);
   for (uint32_t i = 0; i < ns; ++i) {
-    if (sp[i].flags & MCG_SPF_CAN_VOLSURF)
-      return fail(MCG_E_UNSUPPORTED, "volume-surface reactions are not on this path yet");
     if (!(sp[i].space_step >= 0)) return fail(MCG_E_INVALID, "negative space step");
     if (!(sp[i].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
     h[i].space_step = sp[i].space_step;
@@ -370,7 +368,7 @@ int mcg_num_molecules(mcg_ctx* c, uint64_t* count) {
     return MCG_OK;
   }
   std::vector<uint64_t> counts((size_t)c->cfg.num_replicas * c->num_species);
-  int st = count_species(c, counts.data());
+  int st = count_species(c, counts.data(), false);
   if (st != MCG_OK) return st;
   uint64_t t = 0;
   for (uint64_t v : counts) t += v;

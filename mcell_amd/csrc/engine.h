@@ -73,6 +73,20 @@ struct ReactState {
   std::vector<uint32_t> next_id_host;  // per replica: 1 + largest id seen
   bool next_id_dirty = true;
   uint32_t* hdr_host = nullptr;        // pinned mirror of hdr
+  // volume-surface reaction classes
+  std::vector<VolSurfRec> vs;
+  std::vector<int32_t> pair_vs;  // vol species x surf species -> class or -1
+  DevBuf<VolSurfRec> vs_d;
+  DevBuf<int32_t> pair_vs_d;
+  // surface molecules: per-wall tile grids, tiles replica-major
+  // [replica * total_tiles + grid.tile_base + tile]
+  std::vector<WallGrid> grids_host;  // per wall once a wall has a grid
+  std::vector<uint32_t> tiles_host, tile_id_host;
+  uint32_t total_tiles = 0;
+  bool tiles_dirty = false;      // the host copy is newer than the device's
+  bool tiles_on_device = false;  // steps may have changed the device copy
+  DevBuf<WallGrid> grids_d;
+  DevBuf<uint32_t> tiles, tile_id, tile_owner, tile_consumed;
   // reaction log of the last mcg_step call
   DevBuf<RxnEvent> log;
   uint64_t log_cap = 0;
@@ -143,6 +157,9 @@ namespace mcg {
 int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats);
 // Allocates the reactive path's buffers once (reaction classes of any kind).
 int ensure_react_buffers(mcg_ctx* c);
-// Live molecules per (replica, species) into counts (size num_replicas * num_species).
-int count_species(mcg_ctx* c, uint64_t* counts);
+// Live molecules per (replica, species) into counts (size num_replicas *
+// num_species); surface molecules too when with_surface.
+int count_species(mcg_ctx* c, uint64_t* counts, bool with_surface = true);
+// Uploads the host copy of the tiles when it is newer (before steps and counts).
+int sync_tiles_to_device(mcg_ctx* c);
 }  // namespace mcg

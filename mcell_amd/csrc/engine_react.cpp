@@ -69,6 +69,18 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p, r.utime.p};
   p.Sm = RMolView<PosT>{sx, sy, sz, r.sid.p, r.stag.p, r.stoff.p, r.sutime.p};
   p.R.unimol = r.unimol_d.p;
+  const bool surf = r.total_tiles && !r.vs.empty();
+  p.R.grids = surf ? r.grids_d.p : nullptr;
+  p.R.tiles = r.tiles.p;
+  p.R.tile_id = r.tile_id.p;
+  p.R.tile_consumed = r.tile_consumed.p;
+  p.R.total_tiles = r.total_tiles;
+  p.R.vs_rxns = r.vs_d.p;
+  p.R.pair_vs = r.pair_vs_d.p;
+  p.tiles = r.tiles.p;
+  p.tile_id = r.tile_id.p;
+  p.tile_owner = r.tile_owner.p;
+  p.tile_consumed = r.tile_consumed.p;
   p.key = r.key.p;
   p.rank = r.rank.p;
   p.cell_count = r.cell_count.p;
@@ -268,6 +280,7 @@ int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mc
     HIP_TRY(hipMemcpy(r.next_id.p, r.next_id_host.data(), r.next_id_host.size() * 4, hipMemcpyHostToDevice));
     r.next_id_dirty = false;
   }
+  if (int st = sync_tiles_to_device(c)) return st;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   double move_ms = 0;
@@ -294,19 +307,27 @@ int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mc
   return MCG_OK;
 }
 
-int count_species(mcg_ctx* c, uint64_t* counts) {
+int count_species(mcg_ctx* c, uint64_t* counts, bool with_surface) {
   const size_t nc = (size_t)c->cfg.num_replicas * c->num_species;
+  ReactState& r = c->rx;
+  if (with_surface)
+    if (int st = sync_tiles_to_device(c)) return st;
+  const uint32_t tiles = with_surface ? r.total_tiles : 0u;
   DevBuf<uint64_t> d;
   if (d.alloc(std::max<size_t>(nc, 1))) return fail(MCG_E_NOMEM, "count buffer");
   HIP_TRY(hipMemsetAsync(d.p, 0, nc * 8, c->stream));
-  if (c->n) {
+  if (c->n || tiles) {
     if (c->fp32()) {
       ReactPipeline<float> p{};
       p.M = RMolView<float>{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p, nullptr, nullptr};
+      p.R.total_tiles = tiles;
+      p.tiles = r.tiles.p;
       HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, d.p));
     } else {
       ReactPipeline<double> p{};
       p.M = RMolView<double>{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p, nullptr, nullptr};
+      p.R.total_tiles = tiles;
+      p.tiles = r.tiles.p;
       HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, d.p));
     }
   }
@@ -494,6 +515,201 @@ int mcg_count_species(mcg_ctx* c, uint64_t* counts) {
   if (!c || !counts) return fail(MCG_E_INVALID, "null argument");
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   return count_species(c, counts);
+}
+
+}  // extern "C"
+
+namespace mcg {
+namespace {
+
+// Grid::initialize (src4/wall.cpp:38-74) from the wall's constants.
+WallGrid make_wall_grid(const WallRec64& w, uint32_t tile_base) {
+  WallGrid g{};
+  uint32_t n_axis = (uint32_t)std::ceil(std::sqrt(w.area));
+  if (n_axis < 1) n_axis = 1;
+  g.n_axis = n_axis;
+  g.num_tiles = n_axis * n_axis;
+  g.tile_base = tile_base;
+  g.strip_width_rcp = 1 / (w.uv2v / ((double)n_axis));
+  g.vert2_slope = w.uv2u / w.uv2v;
+  g.fullslope = w.uv1u / w.uv2v;
+  g.binding_factor = ((double)g.num_tiles) / w.area;
+  g.vert0_u = w.p0x * w.ux + w.p0y * w.uy + w.p0z * w.uz;
+  g.vert0_v = w.p0x * w.vx + w.p0y * w.vy + w.p0z * w.vz;
+  return g;
+}
+
+// Brings the host copy of the tiles up to date with the device's.
+int download_tiles(mcg_ctx* c) {
+  ReactState& r = c->rx;
+  if (!r.tiles_on_device || r.tiles_dirty) return MCG_OK;
+  const size_t nt = (size_t)r.total_tiles * c->cfg.num_replicas;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(r.tiles_host.data(), r.tiles.p, nt * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(r.tile_id_host.data(), r.tile_id.p, nt * 4, hipMemcpyDeviceToHost));
+  return MCG_OK;
+}
+
+}  // namespace
+
+int sync_tiles_to_device(mcg_ctx* c) {
+  ReactState& r = c->rx;
+  if (!r.tiles_dirty) return MCG_OK;
+  const size_t nt = (size_t)r.total_tiles * c->cfg.num_replicas;
+  if (nt > 0xFFFFFFFFull) return fail(MCG_E_LIMIT, "too many tiles (tiles x replicas >= 2^32)");
+  if (r.tiles.n != nt) {
+    if (r.tiles.alloc(nt) || r.tile_id.alloc(nt) || r.tile_owner.alloc(nt) || r.tile_consumed.alloc(nt))
+      return fail(MCG_E_NOMEM, "tile buffers");
+  }
+  if (r.grids_d.n != r.grids_host.size() && r.grids_d.alloc(r.grids_host.size()))
+    return fail(MCG_E_NOMEM, "grid buffer");
+  HIP_TRY(hipMemcpy(r.grids_d.p, r.grids_host.data(), r.grids_host.size() * sizeof(WallGrid), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(r.tiles.p, r.tiles_host.data(), nt * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(r.tile_id.p, r.tile_id_host.data(), nt * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(r.tile_owner.p, 0xFF, nt * 4));
+  HIP_TRY(hipMemset(r.tile_consumed.p, 0, nt * 4));
+  r.tiles_dirty = false;
+  r.tiles_on_device = true;
+  return MCG_OK;
+}
+
+}  // namespace mcg
+
+extern "C" {
+
+int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
+  if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
+  if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
+  ReactState& r = c->rx;
+  if (!r.vs.empty()) return fail(MCG_E_STATE, "volume-surface reaction classes are already set");
+  if (n == 0) return MCG_OK;
+  const uint32_t ns = c->num_species;
+  std::vector<VolSurfRec> h(n);
+  std::vector<int32_t> pair((size_t)ns * ns, -1);
+  std::vector<SpeciesRec> sp = c->species_host;
+  auto orient_ok = [](int32_t o) { return o >= -1 && o <= 1; };
+  for (uint32_t i = 0; i < n; ++i) {
+    const mcg_volsurf_rxn& x = rxns[i];
+    VolSurfRec& v = h[i];
+    std::memset(&v, 0, sizeof(v));
+    if (x.vol_species >= ns || x.surf_species >= ns) return fail(MCG_E_INVALID, "reactant species out of range");
+    if (pair[x.vol_species * ns + x.surf_species] >= 0)
+      return fail(MCG_E_INVALID, "one class per (volume, surface) species pair");
+    if (!orient_ok(x.geom_vol) || !orient_ok(x.geom_surf)) return fail(MCG_E_INVALID, "orientations are 0, +1 or -1");
+    if (x.num_pathways < 1 || x.num_pathways > (uint32_t)kMaxPathways) return fail(MCG_E_INVALID, "1..4 pathways");
+    double last = 0;
+    for (uint32_t k = 0; k < x.num_pathways; ++k) {
+      if (!(x.cum_probs[k] >= last)) return fail(MCG_E_INVALID, "cumulative probabilities must not decrease");
+      last = x.cum_probs[k];
+      if (x.product[k] >= ns || x.product[k] > 0xFFu) return fail(MCG_E_INVALID, "product species out of range");
+      if (x.product_orient[k] != 1 && x.product_orient[k] != -1)
+        return fail(MCG_E_INVALID, "surface products are oriented +1 or -1");
+      v.cum_probs[k] = x.cum_probs[k];
+      v.product[k] = x.product[k];
+      v.product_orient[k] = x.product_orient[k];
+    }
+    v.num_pathways = x.num_pathways;
+    v.max_fixed_p = v.cum_probs[x.num_pathways - 1];
+    v.max_p_f = (float)v.max_fixed_p;
+    v.geom_vol = x.geom_vol;
+    v.geom_surf = x.geom_surf;
+    pair[x.vol_species * ns + x.surf_species] = (int32_t)i;
+    sp[x.vol_species].flags |= SPF_CAN_VOLSURF;
+  }
+  int st = ensure_react_buffers(c);
+  if (st) return st;
+  if (r.vs_d.alloc(n) || r.pair_vs_d.alloc(pair.size())) return fail(MCG_E_NOMEM, "reaction buffers");
+  HIP_TRY(hipMemcpy(r.vs_d.p, h.data(), n * sizeof(VolSurfRec), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(r.pair_vs_d.p, pair.data(), pair.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->species.p, sp.data(), ns * sizeof(SpeciesRec), hipMemcpyHostToDevice));
+  c->species_host = sp;
+  r.vs = h;
+  r.pair_vs = pair;
+  return MCG_OK;
+}
+
+int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, const uint32_t* species,
+                              const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, uint64_t count) {
+  if (!c || (count && (!ids || !species || !walls || !tiles || !orients))) return fail(MCG_E_INVALID, "null argument");
+  if (!c->have_geometry) return fail(MCG_E_STATE, "surface molecules need the geometry first");
+  if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  if (replica >= c->cfg.num_replicas) return fail(MCG_E_INVALID, "replica out of range");
+  if (count == 0) return MCG_OK;
+  ReactState& r = c->rx;
+  const uint32_t R = c->cfg.num_replicas;
+  const uint32_t nw = (uint32_t)c->walls_host.size();
+  for (uint64_t i = 0; i < count; ++i) {
+    if (walls[i] >= nw) return fail(MCG_E_INVALID, "wall index out of range");
+    if (species[i] >= c->num_species) return fail(MCG_E_INVALID, "species id out of range");
+    if (orients[i] != 1 && orients[i] != -1) return fail(MCG_E_INVALID, "surface molecules are oriented +1 or -1");
+  }
+  if (int st = download_tiles(c)) return st;
+  // grids for walls without one, appended after the existing tiles
+  if (r.grids_host.size() != nw) r.grids_host.assign(nw, WallGrid{});
+  uint64_t total = r.total_tiles;
+  std::vector<WallGrid> g = r.grids_host;
+  for (uint64_t i = 0; i < count; ++i) {
+    WallGrid& w = g[walls[i]];
+    if (w.num_tiles) continue;
+    w = make_wall_grid(c->walls_host[walls[i]], (uint32_t)total);
+    total += w.num_tiles;
+    if (total * R > 0xFFFFFFFFull) return fail(MCG_E_LIMIT, "too many tiles (tiles x replicas >= 2^32)");
+  }
+  for (uint64_t i = 0; i < count; ++i)
+    if (tiles[i] >= g[walls[i]].num_tiles) return fail(MCG_E_INVALID, "tile index out of range");
+  std::vector<uint32_t> nt(total * R, kTileEmpty), nid(total * R, kInvalid);
+  for (uint32_t rep = 0; rep < R; ++rep)
+    for (uint32_t t = 0; t < r.total_tiles; ++t) {
+      nt[rep * total + t] = r.tiles_host[(size_t)rep * r.total_tiles + t];
+      nid[rep * total + t] = r.tile_id_host[(size_t)rep * r.total_tiles + t];
+    }
+  for (uint64_t i = 0; i < count; ++i) {
+    const size_t slot = replica * total + g[walls[i]].tile_base + tiles[i];
+    if (nt[slot] != kTileEmpty) return fail(MCG_E_INVALID, "tile already occupied");
+    nt[slot] = make_tile(species[i], orients[i]);
+    nid[slot] = ids[i];
+  }
+  // the id counters (the device holds them once reactive steps have run)
+  if (r.on && !r.next_id_dirty && r.next_id_host.size() == R)
+    HIP_TRY(hipMemcpy(r.next_id_host.data(), r.next_id.p, R * 4, hipMemcpyDeviceToHost));
+  if (r.next_id_host.size() != R) r.next_id_host.assign(R, 0);
+  for (uint64_t i = 0; i < count; ++i) r.next_id_host[replica] = std::max(r.next_id_host[replica], ids[i] + 1);
+  r.next_id_dirty = true;
+  r.grids_host = g;
+  r.tiles_host.swap(nt);
+  r.tile_id_host.swap(nid);
+  r.total_tiles = (uint32_t)total;
+  r.tiles_dirty = true;
+  return MCG_OK;
+}
+
+int mcg_get_surface_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, uint32_t* walls,
+                              uint32_t* tiles, int32_t* orients, uint64_t capacity, uint64_t* count) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  ReactState& r = c->rx;
+  if (int st = download_tiles(c)) return st;
+  uint64_t k = 0;
+  for (uint32_t rep = 0; rep < c->cfg.num_replicas; ++rep)
+    for (uint32_t w = 0; w < (uint32_t)r.grids_host.size(); ++w) {
+      const WallGrid& g = r.grids_host[w];
+      for (uint32_t t = 0; t < g.num_tiles; ++t) {
+        const size_t slot = (size_t)rep * r.total_tiles + g.tile_base + t;
+        const uint32_t tw = r.tiles_host[slot];
+        if (tw == kTileEmpty) continue;
+        if (k < capacity) {
+          if (ids) ids[k] = r.tile_id_host[slot];
+          if (species) species[k] = tile_species(tw);
+          if (replica) replica[k] = rep;
+          if (walls) walls[k] = w;
+          if (tiles) tiles[k] = t;
+          if (orients) orients[k] = tile_orient(tw);
+        }
+        ++k;
+      }
+    }
+  if (count) *count = k;
+  return MCG_OK;
 }
 
 }  // extern "C"

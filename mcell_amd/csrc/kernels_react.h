@@ -57,6 +57,7 @@ struct ReactPipeline {
   RMolView<PosT> Sm;  // cell-sorted snapshot
   uint32_t *key, *rank, *cell_count, *cell_start;
   uint32_t *consumed, *owner;
+  uint32_t *tiles, *tile_id, *tile_owner, *tile_consumed;  // surface molecules per (replica, tile)
   Proposal* prop;
   uint32_t *plist, *alist, *alist_sorted, *rlist0, *rlist1, *mlist, *ulist;
   uint64_t *keys_a, *keys_b;

@@ -150,7 +150,8 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
                                          const RMolView<PosT>& out, uint32_t j, SpeciesTable species,
                                          const ZigguratTables& T, double barrier, uint32_t propose_only,
                                          Proposal* __restrict__ prop, uint32_t* __restrict__ owner,
-                                         uint32_t* __restrict__ err_out, bool& due, uint32_t* set_lds) {
+                                         uint32_t* __restrict__ tile_owner, uint32_t* __restrict__ err_out,
+                                         bool& due, uint32_t* set_lds) {
   const uint32_t tag = src.tag[j];
   const uint32_t idv = src.id[j];
   const double toff = src.toff[j];
@@ -192,7 +193,8 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
     prop[j] = p;
     if (!propose_only) {
       atomicMin(owner + j, idv);
-      atomicMin(owner + p.partner_slot, idv);
+      if (p.partner_slot & kTileBit) atomicMin(tile_owner + (p.partner_slot & ~kTileBit), idv);
+      else atomicMin(owner + p.partner_slot, idv);
     }
   }
   if (!proposed || propose_only) {
@@ -217,8 +219,9 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
                                               SpeciesTable species, double barrier,
                                               uint32_t propose_only, Proposal* __restrict__ prop,
                                               uint32_t* __restrict__ plist, uint32_t* __restrict__ pcount,
-                                              uint32_t* __restrict__ owner, uint32_t* __restrict__ err_out,
-                                              uint32_t* __restrict__ ulist, uint32_t* __restrict__ ucount) {
+                                              uint32_t* __restrict__ owner, uint32_t* __restrict__ tile_owner,
+                                              uint32_t* __restrict__ err_out, uint32_t* __restrict__ ulist,
+                                              uint32_t* __restrict__ ucount) {
   __shared__ ZigguratTables T;
   __shared__ uint32_t sets[SubpartSet::kCap * 256];
   stage_zig(T);
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
   uint32_t j = 0;
   if (q < n) {
     j = list ? list[q] : q;
-    proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, err_out, due,
+    proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner, err_out, due,
                         sets + threadIdx.x);
   }
   append_slot(proposed, j, plist, pcount);
@@ -241,8 +244,9 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
                                                     RMolView<PosT> out, SlowQueue Q, SpeciesTable species, double barrier,
                                                     uint32_t propose_only, Proposal* __restrict__ prop,
                                                     uint32_t* __restrict__ plist, uint32_t* __restrict__ pcount,
-                                                    uint32_t* __restrict__ owner, uint32_t* __restrict__ err_out,
-                                                    uint32_t* __restrict__ ulist, uint32_t* __restrict__ ucount) {
+                                                    uint32_t* __restrict__ owner, uint32_t* __restrict__ tile_owner,
+                                                    uint32_t* __restrict__ err_out, uint32_t* __restrict__ ulist,
+                                                    uint32_t* __restrict__ ucount) {
   __shared__ ZigguratTables T;
   __shared__ uint32_t sets[SubpartSet::kCap * 256];
   stage_zig(T);
@@ -254,8 +258,8 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
     uint32_t j = 0;
     if (q < count) {
       j = slots[q];
-      proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, err_out, due,
-                          sets + threadIdx.x);
+      proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner, err_out,
+                          due, sets + threadIdx.x);
     }
     append_slot(proposed, j, plist, pcount);
     append_slot(due, j, ulist, ucount);
@@ -373,7 +377,9 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
 template <typename PosT>
 __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ plist, const uint32_t* __restrict__ pcount,
                                                 Proposal* __restrict__ prop, const uint32_t* __restrict__ owner,
-                                                uint32_t* __restrict__ consumed, uint32_t* __restrict__ out_tag,
+                                                const uint32_t* __restrict__ tile_owner,
+                                                uint32_t* __restrict__ consumed, uint32_t* __restrict__ tile_consumed,
+                                                uint32_t* __restrict__ out_tag,
                                                 const RxnRec* __restrict__ rxns, uint32_t* __restrict__ alist,
                                                 uint32_t* __restrict__ acount) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,15 +388,21 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
   if (q < *pcount) {
     j = plist[q];
     const Proposal& p = prop[j];
-    if (owner[j] == p.mover_id && owner[p.partner_slot] == p.mover_id) {
+    const bool tile = (p.partner_slot & kTileBit) != 0;
+    const uint32_t ps = p.partner_slot & ~kTileBit;
+    if (owner[j] == p.mover_id && (tile ? tile_owner[ps] : owner[ps]) == p.mover_id) {
       acc = true;
       // outcome_bimolecular (diffuse_react_event.cpp:1833-1895): both
       // reactants are consumed (reactions keeping a reactant are refused at
-      // set-up)
+      // set-up); a surface reactant's tile goes to its product
       consumed[j] = 1;
-      consumed[p.partner_slot] = 1;
       out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
-      out_tag[p.partner_slot] = out_tag[p.partner_slot] | (MF_DEFUNCT << 8);
+      if (tile) {
+        tile_consumed[ps] = 1;
+      } else {
+        consumed[ps] = 1;
+        out_tag[ps] = out_tag[ps] | (MF_DEFUNCT << 8);
+      }
     }
   }
   const uint64_t mask = __ballot(acc);
@@ -443,6 +455,7 @@ __global__ __launch_bounds__(256) void k_unimol_fire(ReactParams R, const uint32
 __global__ __launch_bounds__(256) void k_requeue(const uint32_t* __restrict__ plist,
                                                  const uint32_t* __restrict__ pcount,
                                                  const Proposal* __restrict__ prop, uint32_t* __restrict__ owner,
+                                                 uint32_t* __restrict__ tile_owner,
                                                  const uint32_t* __restrict__ consumed, uint32_t* __restrict__ rlist,
                                                  uint32_t* __restrict__ rcount) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -451,7 +464,9 @@ __global__ __launch_bounds__(256) void k_requeue(const uint32_t* __restrict__ pl
   if (q < *pcount) {
     j = plist[q];
     owner[j] = kInvalid;
-    owner[prop[j].partner_slot] = kInvalid;
+    const uint32_t ps = prop[j].partner_slot;
+    if (ps & kTileBit) tile_owner[ps & ~kTileBit] = kInvalid;
+    else owner[ps] = kInvalid;
     again = consumed[j] == 0;
   }
   const uint64_t mask = __ballot(again);
@@ -490,8 +505,9 @@ __global__ __launch_bounds__(256) void k_prod_counts(const uint32_t* __restrict_
     return;
   }
   const Proposal& p = prop[sorted_slots[k]];
-  cnt[k] = (p.rxn & kUnimolBit) ? unimol[p.rxn & ~kUnimolBit].num_products[p.pathway]
-                                 : new_products(rxns[p.replica * rxn_stride + p.rxn], p.pathway);
+  cnt[k] = (p.rxn & kUnimolBit)    ? unimol[p.rxn & ~kUnimolBit].num_products[p.pathway]
+           : (p.rxn & kVolSurfBit) ? 1u
+                                   : new_products(rxns[p.replica * rxn_stride + p.rxn], p.pathway);
 }
 
 __global__ __launch_bounds__(256) void k_seg_heads(const uint64_t* __restrict__ keys, uint32_t n,
@@ -515,10 +531,47 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
                                                   uint32_t capacity, RMolView<PosT> out, uint32_t* __restrict__ consumed,
                                                   uint32_t* __restrict__ owner, uint32_t* __restrict__ mlist,
                                                   uint32_t* __restrict__ mcount, RxnEvent* __restrict__ log,
-                                                  uint32_t log_base, uint32_t log_cap, uint32_t* __restrict__ err) {
+                                                  uint32_t log_base, uint32_t log_cap, uint32_t* __restrict__ err,
+                                                  uint32_t* __restrict__ tiles, uint32_t* __restrict__ tile_id,
+                                                  uint32_t* __restrict__ tile_consumed) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const Proposal& p = prop[sorted_slots[k]];
+  if (p.rxn & kVolSurfBit) {
+    // the surface product takes the reactant's tile (outcome_products_random
+    // recycles it, diffuse_react_event.cpp:2145-2155); its output slot stays a hole
+    const VolSurfRec& vs = R.vs_rxns[p.rxn & ~kVolSurfBit];
+    const uint32_t rep = (uint32_t)(keys[k] >> 32);
+    const uint32_t id = next_id[rep] + pre[k] - pre[head[rep]];
+    const uint32_t ts = p.partner_slot & ~kTileBit;
+    tiles[ts] = make_tile(vs.product[p.pathway], vs.product_orient[p.pathway]);
+    tile_id[ts] = id;
+    tile_consumed[ts] = 0;  // the product reacts with later movers
+    const uint32_t slot = hdr[kHdrNumOut] + pre[k];
+    if (slot < capacity) {
+      out.tag[slot] = make_tag(0, MF_DEFUNCT, rep);
+      consumed[slot] = 1;
+      owner[slot] = kInvalid;
+    } else {
+      atomicOr(err, ERR_QUEUE);
+    }
+    if (log && log_base + k < log_cap) {
+      RxnEvent& e = log[log_base + k];
+      e.time = p.time;
+      e.x = p.x;
+      e.y = p.y;
+      e.z = p.z;
+      e.iteration = R.iteration;
+      e.replica = rep;
+      e.mover_id = p.mover_id;
+      e.partner_id = p.partner_id;
+      e.rxn = p.rxn;
+      e.pathway = p.pathway;
+      e.first_product_id = id;
+      e.num_products = 1;
+    }
+    return;
+  }
   const bool uni = (p.rxn & kUnimolBit) != 0;
   const uint32_t* prods = uni ? R.unimol[p.rxn & ~kUnimolBit].products[p.pathway]
                               : R.rxns[p.replica * R.rxn_stride + p.rxn].products[p.pathway];
@@ -607,6 +660,17 @@ __global__ __launch_bounds__(256) void k_count_species(RMolView<PosT> m, uint32_
   if (s < num_species && r < num_replicas) atomicAdd((unsigned long long*)(counts + (size_t)r * num_species + s), 1ull);
 }
 
+__global__ __launch_bounds__(256) void k_count_tiles(const uint32_t* __restrict__ tiles, uint32_t total_tiles,
+                                                     uint32_t num_replicas, uint32_t num_species,
+                                                     uint64_t* __restrict__ counts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)total_tiles * num_replicas) return;
+  const uint32_t w = tiles[i];
+  if (w == kTileEmpty) return;
+  const uint32_t r = (uint32_t)(i / total_tiles), s = tile_species(w);
+  if (s < num_species) atomicAdd((unsigned long long*)(counts + (size_t)r * num_species + s), 1ull);
+}
+
 // --------------------------------------------------------------- launchers --
 static inline dim3 blocks_for(uint32_t n) { return dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
 
@@ -649,23 +713,23 @@ hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uin
     hipLaunchKernelGGL(k_move_light<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, M, count, species, barrier,
                        Q);
     hipLaunchKernelGGL(k_move_drain<PosT>, dim3(kQueueShards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
-                       propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, err, ulist,
+                       propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist,
                        hdr + kHdrUnimolDue);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_move<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, snapshot_src ? Sm : M, M, list,
-                     count, species, barrier, propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, err,
-                     ulist, hdr + kHdrUnimolDue);
+                     count, species, barrier, propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner,
+                     tile_owner, err, ulist, hdr + kHdrUnimolDue);
   return hipGetLastError();
 }
 
 template <typename PosT>
 hipError_t ReactPipeline<PosT>::settle(hipStream_t s, uint32_t max_props, uint32_t* rlist, uint32_t* rcount) {
   hipLaunchKernelGGL(k_accept<PosT>, blocks_for(max_props), dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
-                     consumed, M.tag, R.rxns, alist, hdr + kHdrAccepted);
+                     tile_owner, consumed, tile_consumed, M.tag, R.rxns, alist, hdr + kHdrAccepted);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, rcount, 0u, 1u);
   hipLaunchKernelGGL(k_requeue, blocks_for(max_props), dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
-                     consumed, rlist, rcount);
+                     tile_owner, consumed, rlist, rcount);
   return hipGetLastError();
 }
 
@@ -694,7 +758,7 @@ hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrProductMovers, 0u, 1u);
   hipLaunchKernelGGL(k_products<PosT>, blocks_for(n_acc), dim3(256), 0, s, R, alist_sorted, keys_b, n_acc, prop, pre,
                      head, next_id, hdr, capacity, M, consumed, owner, mlist, hdr + kHdrProductMovers, log, log_base,
-                     log_cap, err);
+                     log_cap, err, tiles, tile_id, tile_consumed);
   hipLaunchKernelGGL(k_bump_ids, blocks_for(n_acc), dim3(256), 0, s, keys_b, n_acc, pre, head, next_id, hdr);
   return hipGetLastError();
 }
@@ -711,8 +775,13 @@ hipError_t ReactPipeline<PosT>::log_proposals(hipStream_t s, uint32_t max_props,
 template <typename PosT>
 hipError_t ReactPipeline<PosT>::count_species(hipStream_t s, uint32_t n, uint32_t num_species, uint32_t num_replicas,
                                               uint64_t* counts) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_count_species<PosT>, blocks_for(n), dim3(256), 0, s, M, n, num_species, num_replicas, counts);
+  if (n)
+    hipLaunchKernelGGL(k_count_species<PosT>, blocks_for(n), dim3(256), 0, s, M, n, num_species, num_replicas,
+                       counts);
+  const uint64_t nt = (uint64_t)R.total_tiles * num_replicas;
+  if (nt)
+    hipLaunchKernelGGL(k_count_tiles, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, s, tiles, R.total_tiles,
+                       num_replicas, num_species, counts);
   return hipGetLastError();
 }
 

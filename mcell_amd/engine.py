@@ -53,7 +53,7 @@ class DiffuseReactEngine:
                       propose_only: bool = False, log_capacity: int = 0, trace: bool = False):
         """An engine holding the workload's geometry, species, reaction
         classes and molecules (every replica)."""
-        from .workloads import rxn_table, species_steps, unimol_table
+        from .workloads import rxn_table, species_steps, unimol_table, volsurf_table
         cap = capacity if capacity is not None else max(2 * wl.num_molecules, 1024)
         eng = cls(wl.part, precision=precision, seed=wl.config.seed if seed is None else seed,
                   num_replicas=wl.num_replicas, capacity=cap, trace=trace, barrier_interval=wl.barrier_interval,
@@ -66,6 +66,8 @@ class DiffuseReactEngine:
             eng.set_bimol_rxns(rxn_table(wl))
         if wl.unimol_rxns:
             eng.set_unimol_rxns(unimol_table(wl))
+        if wl.volsurf_rxns:
+            eng.set_volsurf_rxns(volsurf_table(wl))
         if wl.replica_D is not None:
             for r in range(wl.num_replicas):
                 eng.set_replica_params(r, species_steps(wl, r), rxn_table(wl, r) if wl.rxns else None)
@@ -75,6 +77,13 @@ class DiffuseReactEngine:
             sel = wl.replica == r
             if sel.any():
                 eng.add_volume_molecules(r, wl.ids[sel], wl.mol_species[sel], wl.positions[sel])
+        sm = wl.surface_mols
+        if sm is not None:
+            for r in range(wl.num_replicas):
+                sel = sm["replica"] == r
+                if sel.any():
+                    eng.add_surface_molecules(r, sm["ids"][sel], sm["species"][sel], sm["walls"][sel],
+                                              sm["tiles"][sel], sm["orients"][sel])
         return eng
 
     def close(self):
@@ -148,6 +157,42 @@ class DiffuseReactEngine:
                 for q, ps in enumerate(prods[k]):
                     arr[i].products[k][q] = ps
         _lib.check(self.lib.mcg_set_unimol_rxns(self.ctx, arr, len(rxns)))
+
+    def set_volsurf_rxns(self, rxns):
+        """rxns: (vol, surf, geom_vol, geom_surf, cum_probs, products,
+        product orientations) per class."""
+        arr = (_lib.McgVolSurfRxn * max(len(rxns), 1))()
+        for i, (v, sf, gv, gs, cum, prods, orients) in enumerate(rxns):
+            arr[i].vol_species = v
+            arr[i].surf_species = sf
+            arr[i].geom_vol = gv
+            arr[i].geom_surf = gs
+            arr[i].num_pathways = len(cum)
+            for k, p in enumerate(cum):
+                arr[i].cum_probs[k] = p
+                arr[i].product[k] = prods[k]
+                arr[i].product_orient[k] = orients[k]
+        _lib.check(self.lib.mcg_set_volsurf_rxns(self.ctx, arr, len(rxns)))
+
+    def add_surface_molecules(self, replica: int, ids, species, walls, tiles, orients):
+        a = [np.ascontiguousarray(x, dtype=np.uint32) for x in (ids, species, walls, tiles)]
+        o = np.ascontiguousarray(orients, dtype=np.int32)
+        assert all(len(x) == len(o) for x in a)
+        _lib.check(self.lib.mcg_add_surface_molecules(self.ctx, replica, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
+                                                      _ptr(a[3]), _ptr(o), len(o)))
+
+    def surface_molecules(self):
+        """The surface molecules in (replica, wall, tile) order."""
+        cnt = ctypes.c_uint64()
+        _lib.check(self.lib.mcg_get_surface_molecules(self.ctx, None, None, None, None, None, None, 0,
+                                                      ctypes.byref(cnt)))
+        n = cnt.value
+        out = {k: np.zeros(n, np.uint32) for k in ("id", "species", "replica", "wall", "tile")}
+        out["orient"] = np.zeros(n, np.int32)
+        _lib.check(self.lib.mcg_get_surface_molecules(self.ctx, _ptr(out["id"]), _ptr(out["species"]),
+                                                      _ptr(out["replica"]), _ptr(out["wall"]), _ptr(out["tile"]),
+                                                      _ptr(out["orient"]), n, ctypes.byref(cnt)))
+        return out
 
     def set_rxn_log(self, capacity: int):
         _lib.check(self.lib.mcg_set_rxn_log(self.ctx, capacity))

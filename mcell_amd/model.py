@@ -277,6 +277,48 @@ class UnimolReaction:
     products: List[int] = field(default_factory=list)
 
 
+@dataclass
+class VolSurfReaction:
+    """vol + surf -> surface product (rate in M^-1 s^-1): the volume reactant
+    hitting the tile of the surface reactant; the product takes the tile with
+    orientation product_orient and the volume reactant is destroyed.
+    geom_vol / geom_surf are the orientations written in the rule (0 none,
+    +1 up ('), -1 down (,))."""
+
+    vol: int
+    surf: int
+    rate: float
+    product: int
+    product_orient: int = 1
+    geom_vol: int = 0
+    geom_surf: int = 0
+
+    def pb_factor(self, part: PartitionConfig, sp_vol: Species) -> float:
+        """Vol-surf pb_factor (MCell3 src/react_util.c:132-157): the volume
+        reactant's D and time step, doubled for a rule whose two orientations
+        are both set and equal or opposite."""
+        D = sp_vol.diffusion_constant_3d
+        t_step = 1.0 * part.time_unit
+        if D <= 0.0:
+            return 0.0
+        pb = 1.0e11 * part.grid_density / (2.0 * N_AV) * math.sqrt(MY_PI * t_step / D)
+        gv, gs = self.geom_vol, self.geom_surf
+        if (gv + gs) * (gv - gs) == 0 and gv * gs != 0:
+            pb *= 2.0
+        return pb
+
+
+def wall_num_tiles(vertices: np.ndarray, walls: np.ndarray) -> np.ndarray:
+    """Tiles per wall of a surface grid (Grid::initialize, src4/wall.cpp:51-56):
+    ceil(sqrt(area))^2 with the area in internal units."""
+    v0 = vertices[walls[:, 0]]
+    e1 = vertices[walls[:, 1]] - v0
+    e2 = vertices[walls[:, 2]] - v0
+    area = 0.5 * np.linalg.norm(np.cross(e1, e2), axis=1)
+    n_axis = np.maximum(np.ceil(np.sqrt(area)), 1).astype(np.int64)
+    return n_axis * n_axis
+
+
 # -------------------------------------------------------------- placement ---
 def place_uniform_in_box(n: int, half_edge_um: float, seed: int, part: PartitionConfig) -> np.ndarray:
     """Uniform positions inside the axis-aligned cube [-h, h]^3 (um), returned

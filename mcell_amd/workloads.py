@@ -45,6 +45,10 @@ class Workload:
     replica_D: Optional[np.ndarray] = None  # (num_replicas, num_species) cm^2/s overrides, or None
     surface_classes: Optional[np.ndarray] = None  # uint8 (num_classes, num_species): front | back << 2
     unimol_rxns: List[M.UnimolReaction] = field(default_factory=list)
+    volsurf_rxns: List[M.VolSurfReaction] = field(default_factory=list)
+    # surface molecules: dict of equal-length arrays ids, species, walls,
+    # tiles, orients, replica (or None)
+    surface_mols: Optional[dict] = None
 
     def species_for(self, replica: int) -> List[M.Species]:
         if self.replica_D is None:
@@ -241,6 +245,62 @@ def unimol_table(wl: Workload):
             cum.append(acc)
         out.append((sp, cum, [list(u.products) for u in us]))
     return out
+
+
+def volsurf_table(wl: Workload):
+    """Volume-surface classes as the engine takes them: (vol, surf, geom_vol,
+    geom_surf, cum_probs, products, product orientations), P = rate * pb_factor."""
+    out = []
+    for v in wl.volsurf_rxns:
+        pb = v.pb_factor(wl.part, wl.species[v.vol])
+        out.append((v.vol, v.surf, v.geom_vol, v.geom_surf, [v.rate * pb], [v.product], [v.product_orient]))
+    return out
+
+
+def place_on_walls(n: int, wall_ids: np.ndarray, vertices: np.ndarray, walls: np.ndarray, seed: int):
+    """n distinct tiles drawn uniformly over all tiles of the given walls:
+    (walls, tiles)."""
+    nt = M.wall_num_tiles(vertices, walls[wall_ids])
+    base = np.concatenate([[0], np.cumsum(nt)])
+    if n > base[-1]:
+        raise ValueError("more surface molecules than tiles")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    flat = np.sort(rng.choice(base[-1], size=n, replace=False))
+    k = np.searchsorted(base, flat, side="right") - 1
+    return wall_ids[k].astype(np.uint32), (flat - base[k]).astype(np.uint32)
+
+
+def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, seed: int = 3,
+                    subpartition_dimension: float = 0.1, sphere_subdivisions: int = 5, rate: float = 1e8,
+                    down_fraction: float = 0.5) -> Workload:
+    """Ligand-receptor binding on a sphere: the reflective 2 um cube holding
+    the icosphere of radius 0.5 um (reflective); L (D = 1e-6 cm^2/s) starts
+    outside the sphere, the surface receptors R sit on distinct tiles of the
+    sphere, a fraction `down_fraction` of them oriented down (-1), the rest up
+    (+1).  L' + R' -> LR' at `rate` /M/s: only up receptors hit from the
+    outside (front) react, and the product LR takes the receptor's tile.
+    Placement seeds: seed (L), seed + 2 (receptor tiles), seed + 3
+    (orientations); ids 0..n_l-1 (L) then the receptors."""
+    cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
+    box = M.create_box("box", edge_dimension=2.0)
+    ball = M.create_icosphere("sphere", 0.5, sphere_subdivisions)
+    objects = [box, ball]
+    part = M.derive_partition(cfg, M.bounding_box(objects), has_bimol_vol_rxns=False)
+    verts, walls, classes = M.vertices_internal(objects, part)
+    species = [M.Species("L", 1e-6), M.Species("R", 0.0), M.Species("LR", 0.0)]
+    rx = [M.VolSurfReaction(0, 1, rate, 2, product_orient=1, geom_vol=1, geom_surf=1)]
+    sphere_walls = np.arange(len(box.wall_list), len(walls), dtype=np.uint32)
+    sw, st = place_on_walls(n_r, sphere_walls, verts, walls, seed + 2)
+    rng = np.random.Generator(np.random.PCG64(seed + 3))
+    orients = np.where(rng.uniform(size=n_r) < down_fraction, -1, 1).astype(np.int32)
+    pos = place_outside_sphere(n_l, 1.0, 0.5, seed, part)
+    surf = {"ids": np.arange(n_l, n_l + n_r, dtype=np.uint32), "species": np.ones(n_r, np.uint32),
+            "walls": sw, "tiles": st, "orients": orients, "replica": np.zeros(n_r, np.uint32)}
+    steps = _species_table(species, part, [])
+    steps[0] = (steps[0][0], steps[0][1], steps[0][2] | 4)
+    return Workload("receptor_sphere", part, verts, walls, classes, species, steps,
+                    np.arange(n_l, dtype=np.uint32), np.zeros(n_l, np.uint32), pos, np.zeros(n_l, np.uint32),
+                    iterations, config=cfg, objects=objects, volsurf_rxns=rx, surface_mols=surf)
 
 
 def decay_box(n_e: int = 20000, iterations: int = 2000, seed: int = 6, rate: float = 1e3,

@@ -1,6 +1,7 @@
 // TEST INFRASTRUCTURE ONLY (CPU oracle; see oracle_core.h).
 // A C interface over the oracle so that tests (ctypes) and the CPU-baseline
 // leg of bench.py can drive it.
+#include <cstring>
 #include <chrono>
 #include <stdexcept>
 #include <string>
@@ -102,6 +103,13 @@ void orc_get_walls(void* wp, double* out) {
   }
 }
 
+// vertex indices of every wall (3 per wall)
+void orc_get_wall_vertices(void* wp, uint32_t* out) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->walls.size(); ++i)
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = w->walls[i].v[k];
+}
+
 void orc_get_vertices(void* wp, double* out) {
   World* w = static_cast<World*>(wp);
   for (size_t i = 0; i < w->verts.size(); ++i) {
@@ -139,6 +147,62 @@ void orc_add_unimol_rxn(void* wp, uint32_t species, double rate, const uint32_t*
 }
 
 uint64_t orc_unimol_reactions(void* wp) { return static_cast<World*>(wp)->unimol_reactions; }
+
+int orc_add_volsurf_rxn(void* wp, uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate,
+                        uint32_t product, int product_orient) {
+  return static_cast<World*>(wp)->add_volsurf_rxn(vol, surf, geom_vol, geom_surf, rate, product, product_orient);
+}
+
+// out: pb_factor, max_fixed_p
+void orc_get_volsurf_rxn(void* wp, int r, double* out) {
+  const VolSurfRxn& x = static_cast<World*>(wp)->vs_rxns[r];
+  out[0] = x.pb_factor;
+  out[1] = x.max_fixed_p;
+}
+
+int orc_add_surface_molecules(void* wp, uint32_t species, uint32_t n, const uint32_t* walls, const uint32_t* tiles,
+                              const int32_t* orients) {
+  World* w = static_cast<World*>(wp);
+  try {
+    for (uint32_t i = 0; i < n; ++i) w->add_surface_molecule(species, walls[i], tiles[i], orients[i]);
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+// Grid of a wall (initialized on demand): out = n_axis, num_tiles,
+// strip_width_rcp, vert2_slope, fullslope, binding_factor, vert0_u, vert0_v
+void orc_grid_info(void* wp, uint32_t wall, double* out) {
+  World* w = static_cast<World*>(wp);
+  w->init_grid(wall);
+  const Grid& g = w->grids[wall];
+  double v[8] = {(double)g.n_axis, (double)g.num_tiles, g.strip_width_rcp, g.vert2_slope, g.fullslope,
+                 g.binding_factor, g.vert0_u, g.vert0_v};
+  std::memcpy(out, v, sizeof(v));
+}
+
+int64_t orc_xyz2grid(void* wp, uint32_t wall, const double* pos) {
+  World* w = static_cast<World*>(wp);
+  w->init_grid(wall);
+  try {
+    return (int64_t)w->xyz2grid({pos[0], pos[1], pos[2]}, wall);
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// surface molecules per tile of a wall: out[k] = molecule id or 0xFFFFFFFF
+void orc_get_tiles(void* wp, uint32_t wall, uint32_t* out) {
+  World* w = static_cast<World*>(wp);
+  if (wall >= w->grids.size()) return;
+  const Grid& g = w->grids[wall];
+  for (uint32_t k = 0; k < g.num_tiles; ++k) out[k] = g.tile_mol[k];
+}
+
+uint64_t orc_volsurf_reactions(void* wp) { return static_cast<World*>(wp)->volsurf_reactions; }
 
 void orc_get_rxn(void* wp, int r, double* out) {
   const BimolRxn& x = static_cast<World*>(wp)->rxns[r];

@@ -209,6 +209,27 @@ struct Species {
   bool is_vol;
 };
 
+// Surface grid of a wall (Grid::initialize, src4/wall.cpp:38-74).
+struct Grid {
+  uint32_t n_axis = 0, num_tiles = 0;  // num_tiles 0: not initialized
+  double strip_width_rcp = 0, vert2_slope = 0, fullslope = 0, binding_factor = 0;
+  double vert0_u = 0, vert0_v = 0;
+  std::vector<uint32_t> tile_mol;      // molecule id per tile, or INVALID
+};
+
+// Volume + surface reaction class: the volume reactant hitting a tile
+// occupied by the surface reactant; the surface product replaces the
+// surface reactant on its tile and the volume reactant is destroyed.
+struct VolSurfRxn {
+  uint32_t vol, surf;            // species
+  int geom_vol, geom_surf;       // orientations in the rule (0 none, +1, -1)
+  double rate, pb_factor;
+  double max_fixed_p;
+  std::vector<double> cum_probs;
+  std::vector<uint32_t> product;       // surface product species per pathway
+  std::vector<int> product_orient;     // its orientation per pathway
+};
+
 // A unimolecular reaction class of one species: cumulative pathway rates
 // per internal time unit (k_tot = cum.back()), products per pathway.
 struct UnimolRxn {
@@ -230,11 +251,15 @@ enum : uint32_t {
   MF_DEFUNCT = 1u << 0,
   MF_MATURE = 1u << 1,
   MF_SCHEDULE_UNIMOL = 1u << 2,  // MOLECULE_FLAG_SCHEDULE_UNIMOL_RXN: lifetime not drawn yet
+  MF_SURFACE = 1u << 3,          // a surface molecule (on a wall tile)
 };
 struct Mol {
   uint32_t id;
   uint32_t species;
   uint32_t flags;
+  uint32_t wall = 0xFFFFFFFFu;  // surface molecules: wall, tile, orientation (+1 up, -1 down)
+  uint32_t tile = 0;
+  int orient = 0;
   V3 pos;
   uint32_t subpart;
   uint32_t reactant_subpart;  // subpart whose reactant sets hold this molecule
@@ -288,6 +313,10 @@ class World {
   std::vector<BimolRxn> rxns;
   std::vector<int32_t> pair_rxn;        // species x species -> rxn index or -1
   std::vector<UnimolRxn> unimol;        // per species (empty cum: none)
+  std::vector<Grid> grids;              // per wall
+  std::vector<VolSurfRxn> vs_rxns;
+  std::vector<int32_t> pair_vs;         // vol species x surf species -> vs rxn index or -1
+  uint64_t volsurf_reactions = 0;
   std::vector<uint32_t> calls_this_iteration;  // per molecule id: diffuse calls (stream pass)
   uint64_t unimol_reactions = 0;
   // surface classes: action of class c (1-based on walls) for species s hit
@@ -331,6 +360,12 @@ class World {
   void finalize_walls();
   uint32_t add_species(double D_cm2s, bool is_vol, uint32_t extra_flags);
   int add_bimol_rxn(uint32_t a, uint32_t b, double rate, const std::vector<uint32_t>& products);
+  int add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate, uint32_t product,
+                      int product_orient);
+  uint32_t add_surface_molecule(uint32_t species, uint32_t wall, uint32_t tile, int orient);
+  void init_grid(uint32_t wall);
+  uint32_t xyz2grid(V3 v, uint32_t wall) const;
+  int collide_and_react_with_surf_mol(const Collision& c, double t_steps, double r_rate_factor, double elapsed);
   // rate in s^-1 per pathway (pb = time_unit, MCell3 src/react_util.c:74-78)
   void add_unimol_rxn(uint32_t species, double rate, const std::vector<uint32_t>& products);
   uint32_t add_volume_molecule(uint32_t species, V3 pos, double time);
@@ -406,6 +441,7 @@ class World {
   bool collide_and_react_with_vol_mol(const Collision& c, V3& disp, double t_steps, double r_rate_factor,
                                       double elapsed);
   int test_bimolecular(const BimolRxn& rx, double scaling);
+  int test_bimolecular_p(double max_fixed_p, const std::vector<double>& cum_probs, double scaling);
   void outcome_bimolecular(const Collision& c, int pathway, double time);
  public:
   double exact_disk(V3 loc, V3 mv, double R, const Mol& moving, const Mol& target) const;

@@ -198,7 +198,7 @@ void World::diffuse_single_molecule(uint32_t id) {
   if (m.flags & MF_DEFUNCT) return;
   const Species& sp = species[m.species];
   double event_end_time = (double)iteration + 1;
-  if (sp.space_step > 0) {  // can_diffuse
+  if ((m.flags & MF_SURFACE) == 0 && sp.space_step > 0) {  // can_diffuse
     m.diffusion_time += max_time;
     if ((m.unimol_rxn_time != TIME_INVALID && m.unimol_rxn_time < event_end_time) ||
         cmp_lt(m.diffusion_time, event_time + 1.0, EPS)) {  // before_this_iterations_end
@@ -278,6 +278,14 @@ void World::diffuse_vol_molecule(uint32_t id, double& max_time, double diffusion
           break;
         }
       } else {
+        // collide_and_react_with_surf_mol first (diffuse_react_event.cpp:505-527)
+        if ((species[mols[id].species].flags & SPF_CAN_VOLSURF) && c.wall < grids.size() &&
+            grids[c.wall].num_tiles) {
+          if (collide_and_react_with_surf_mol(c, t_steps, r_rate_factor, elapsed) == 1) {
+            was_defunct = true;
+            break;
+          }
+        }
         Mol& vm = mols[id];
         // collide_and_react_with_walls (diffuse_react_event.cpp:529-549, 991-1067)
         if (species[vm.species].flags & SPF_CAN_VOLWALL) {

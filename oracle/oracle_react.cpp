@@ -521,14 +521,17 @@ static int binary_search_double(const std::vector<double>& A, double match, int 
 // RxnClass::get_pathway_index_for_probability: a binary search over the
 // cumulative pathway probabilities (MCell3 src/react_cond.c:120-173).
 int World::test_bimolecular(const BimolRxn& rx, double scaling) {
-  double max_fixed_p = rx.max_fixed_p;
+  return test_bimolecular_p(rx.max_fixed_p, rx.cum_probs, scaling);
+}
+
+int World::test_bimolecular_p(double max_fixed_p, const std::vector<double>& cum_probs, double scaling) {
   double prob;
   DrawSource& r = rng();
   if (max_fixed_p < scaling) {
     prob = r.dbl() * scaling;
     if (prob >= max_fixed_p) return -1;
   } else {
-    float max_p = (float)rx.max_fixed_p;  // rxn_utils.inl:367 keeps max_p as float
+    float max_p = (float)max_fixed_p;  // rxn_utils.inl:367 keeps max_p as float
     if (max_p >= scaling) {
       prob = r.dbl() * max_p;
     } else {
@@ -536,7 +539,7 @@ int World::test_bimolecular(const BimolRxn& rx, double scaling) {
       if (prob >= max_p) return -1;
     }
   }
-  return binary_search_double(rx.cum_probs, prob, (int)rx.cum_probs.size() - 1, 1);
+  return binary_search_double(cum_probs, prob, (int)cum_probs.size() - 1, 1);
 }
 
 // diffuse_react_event.cpp:786-829
@@ -588,6 +591,82 @@ void World::outcome_bimolecular(const Collision& c, int pathway, double time) {
     mols[reacB].flags |= MF_DEFUNCT;
     register_reactant(mols[reacB], false);
   }
+}
+
+// xyz2grid_tile_index (grid_utils.inl:48-118): the hit point to the
+// strip / stripe / flip tile index; points on a vertex map to that vertex's
+// corner tile.
+uint32_t World::xyz2grid(V3 v, uint32_t wi) const {
+  const Wall& w = walls[wi];
+  const Grid& g = grids[wi];
+  if (g.num_tiles == 1) return 0;
+  uint32_t tile_idx_0 = 0;
+  uint32_t tile_idx_mid = g.num_tiles - 2 * g.n_axis + 1;
+  uint32_t tile_idx_last = g.num_tiles - 1;
+  V3 v0 = verts[w.v[0]], v1 = verts[w.v[1]], v2 = verts[w.v[2]];
+  if (!distinguishable_vec3(v, v0, POS_EPS)) return tile_idx_mid;
+  if (!distinguishable_vec3(v, v1, POS_EPS)) return tile_idx_last;
+  if (!distinguishable_vec3(v, v2, POS_EPS)) return tile_idx_0;
+  double i = dot(v, w.unit_u) - g.vert0_u;
+  double j = dot(v, w.unit_v) - g.vert0_v;
+  double striploc = j * g.strip_width_rcp;
+  int strip = (int)striploc;
+  double striprem = striploc - strip;
+  strip = (int)g.n_axis - strip - 1;
+  double u0 = j * g.vert2_slope;
+  double u1_u0 = w.uv_vert1_u - j * g.fullslope;
+  double stripeloc = ((i - u0) / u1_u0) * (strip + (1 - striprem));
+  int stripe = (int)stripeloc;
+  double striperem = stripeloc - stripe;
+  int flip = (striperem < 1 - striprem) ? 0 : 1;
+  int idx = strip * strip + 2 * stripe + flip;
+  if ((uint32_t)idx >= g.num_tiles) throw std::runtime_error("xyz2grid: tile index out of range");
+  return (uint32_t)idx;
+}
+
+// collide_and_react_with_surf_mol (diffuse_react_event.cpp:845-975) for the
+// classes of VolSurfRxn; 1 = the volume molecule was destroyed (or, in the
+// snapshot mode, proposed a reaction), -1 = nothing happened.
+int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, double r_rate_factor,
+                                           double elapsed) {
+  const Grid& g = grids[c.wall];
+  uint32_t j = xyz2grid(c.pos, c.wall);
+  uint32_t occ = g.tile_mol[j];
+  if (occ == INVALID_U32) return -1;
+  const Mol& sm = mols[occ];
+  const Mol& vm = mols[c.mover];
+  size_t n = species.size();
+  if (pair_vs.size() != n * n) return -1;
+  int32_t ri = pair_vs[vm.species * n + sm.species];
+  if (ri < 0) return -1;
+  const VolSurfRxn& rx = vs_rxns[ri];
+  // trigger_bimolecular's orientation rule (rxn_utils.inl:58-84)
+  int orient_vol = (c.type == CollType::WALL_FRONT) ? 1 : -1;
+  int ga = rx.geom_vol, gb = rx.geom_surf;
+  bool match = ga == 0 || gb == 0 || (ga + gb) * (ga - gb) != 0 || orient_vol * sm.orient * ga * gb > 0;
+  if (!match) return -1;
+  double scaling = r_rate_factor / g.binding_factor;
+  double time = elapsed + t_steps * c.time;
+  int pw = test_bimolecular_p(rx.max_fixed_p, rx.cum_probs, scaling);
+  if (pw < 0) return -1;
+  uint32_t code = 0x40000000u | (uint32_t)ri;
+  if (snapshot) {
+    rxn_log.push_back({(uint32_t)iteration, c.mover, occ, (int32_t)code, pw, time, c.pos});
+    return 1;
+  }
+  // outcome: the surface product takes the reactant's tile (outcome_products_random
+  // recycles it, diffuse_react_event.cpp:2145-2155); both reactants are destroyed
+  uint32_t wall = sm.wall, tile = sm.tile;
+  mols[occ].flags |= MF_DEFUNCT;
+  grids[wall].tile_mol[tile] = INVALID_U32;
+  add_surface_molecule(rx.product[pw], wall, tile, rx.product_orient[pw]);
+  mols.back().birthday = time;
+  Mol& mv = mols[c.mover];
+  mv.flags |= MF_DEFUNCT;
+  register_reactant(mv, false);
+  ++volsurf_reactions;
+  rxn_log.push_back({(uint32_t)iteration, c.mover, occ, (int32_t)code, pw, time, c.pos});
+  return 1;
 }
 
 }  // namespace orc

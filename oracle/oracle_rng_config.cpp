@@ -440,3 +440,73 @@ void World::add_unimol_rxn(uint32_t sp, double rate, const std::vector<uint32_t>
 }
 
 }  // namespace orc
+
+namespace orc {
+
+// Grid::initialize (src4/wall.cpp:38-74)
+void World::init_grid(uint32_t wi) {
+  if (grids.size() < walls.size()) grids.resize(walls.size());
+  Grid& g = grids[wi];
+  if (g.num_tiles) return;
+  const Wall& w = walls[wi];
+  g.n_axis = (uint32_t)std::ceil(std::sqrt(w.area));
+  if (g.n_axis < 1) g.n_axis = 1;
+  g.num_tiles = g.n_axis * g.n_axis;
+  g.tile_mol.assign(g.num_tiles, INVALID_U32);
+  g.strip_width_rcp = 1 / (w.uv2_v / ((double)g.n_axis));
+  g.vert2_slope = w.uv2_u / w.uv2_v;
+  g.fullslope = w.uv_vert1_u / w.uv2_v;
+  g.binding_factor = ((double)g.num_tiles) / w.area;
+  const V3 v0 = verts[w.v[0]];
+  g.vert0_u = dot(v0, w.unit_u);
+  g.vert0_v = dot(v0, w.unit_v);
+}
+
+// vol-surf pb_factor (MCell3 src/react_util.c:132-157): the volume reactant's
+// D and time step; doubled when both reactants are oriented the same way.
+int World::add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate, uint32_t product,
+                           int product_orient) {
+  VolSurfRxn r;
+  r.vol = vol;
+  r.surf = surf;
+  r.geom_vol = geom_vol;
+  r.geom_surf = geom_surf;
+  r.rate = rate;
+  const Species& sv = species[vol];
+  double t_step = sv.time_step * cfg.time_unit;
+  double pb = (sv.D <= 0) ? 0.0 : 1.0e11 * cfg.grid_density / (2.0 * N_AV) * std::sqrt(MY_PI * t_step / sv.D);
+  if ((geom_vol + geom_surf) * (geom_vol - geom_surf) == 0 && geom_vol * geom_surf != 0) pb *= 2.0;
+  r.pb_factor = pb;
+  r.cum_probs.push_back(rate * pb);
+  r.max_fixed_p = r.cum_probs.back();
+  r.product.push_back(product);
+  r.product_orient.push_back(product_orient);
+  vs_rxns.push_back(r);
+  size_t n = species.size();
+  if (pair_vs.size() != n * n) pair_vs.assign(n * n, -1);
+  pair_vs[vol * n + surf] = (int32_t)(vs_rxns.size() - 1);
+  species[vol].flags |= SPF_CAN_VOLSURF;
+  return (int)vs_rxns.size() - 1;
+}
+
+uint32_t World::add_surface_molecule(uint32_t sp, uint32_t wi, uint32_t tile, int orient) {
+  init_grid(wi);
+  Grid& g = grids[wi];
+  if (tile >= g.num_tiles) throw std::runtime_error("tile index out of range");
+  if (g.tile_mol[tile] != INVALID_U32) throw std::runtime_error("tile already occupied");
+  Mol m{};
+  m.id = (uint32_t)mols.size();
+  m.species = sp;
+  m.flags = MF_SURFACE;
+  m.wall = wi;
+  m.tile = tile;
+  m.orient = orient;
+  m.diffusion_time = TIME_FOREVER;  // surface molecules do not move on this path
+  m.birthday = (double)iteration;
+  m.unimol_rxn_time = TIME_INVALID;
+  mols.push_back(m);
+  g.tile_mol[tile] = m.id;
+  return m.id;
+}
+
+}  // namespace orc

@@ -47,6 +47,7 @@ def load():
         "orc_add_walls": (None, [P, P, u32, u32, u32, u32]),
         "orc_finalize_walls": (None, [P]),
         "orc_num_walls": (u32, [P]),
+        "orc_get_wall_vertices": (None, [P, P]),
         "orc_get_walls": (None, [P, P]),
         "orc_get_vertices": (None, [P, P]),
         "orc_get_subpart_walls": (u64, [P, P, P]),
@@ -56,6 +57,13 @@ def load():
         "orc_get_rxn": (None, [P, ctypes.c_int, P]),
         "orc_add_unimol_rxn": (None, [P, u32, d, P, u32]),
         "orc_unimol_reactions": (u64, [P]),
+        "orc_add_volsurf_rxn": (ctypes.c_int, [P, u32, u32, ctypes.c_int, ctypes.c_int, d, u32, ctypes.c_int]),
+        "orc_get_volsurf_rxn": (None, [P, ctypes.c_int, P]),
+        "orc_add_surface_molecules": (ctypes.c_int, [P, u32, u32, P, P, P]),
+        "orc_grid_info": (None, [P, u32, P]),
+        "orc_xyz2grid": (ctypes.c_int64, [P, u32, P]),
+        "orc_get_tiles": (None, [P, u32, P]),
+        "orc_volsurf_reactions": (u64, [P]),
         "orc_add_molecules": (ctypes.c_int, [P, u32, u32, P]),
         "orc_set_rng": (None, [P, ctypes.c_int, u64, u32]),
         "orc_set_snapshot": (None, [P, ctypes.c_int]),
@@ -191,6 +199,38 @@ class OracleWorld:
         p = np.ascontiguousarray(products, np.uint32)
         self.lib.orc_add_unimol_rxn(self.w, species, rate, _p(p) if len(p) else None, len(p))
 
+    def add_volsurf_rxn(self, vol, surf, geom_vol, geom_surf, rate, product, product_orient):
+        return self.lib.orc_add_volsurf_rxn(self.w, vol, surf, geom_vol, geom_surf, rate, product, product_orient)
+
+    def volsurf_rxn(self, r):
+        out = np.zeros(2)
+        self.lib.orc_get_volsurf_rxn(self.w, r, _p(out))
+        return out
+
+    def add_surface_molecules(self, species, walls, tiles, orients):
+        wa = np.ascontiguousarray(walls, np.uint32)
+        ta = np.ascontiguousarray(tiles, np.uint32)
+        oa = np.ascontiguousarray(orients, np.int32)
+        if self.lib.orc_add_surface_molecules(self.w, species, len(wa), _p(wa), _p(ta), _p(oa)) != 0:
+            raise RuntimeError(self.lib.orc_error().decode())
+
+    def grid_info(self, wall):
+        out = np.zeros(8)
+        self.lib.orc_grid_info(self.w, wall, _p(out))
+        return out
+
+    def xyz2grid(self, wall, pos):
+        return int(self.lib.orc_xyz2grid(self.w, wall, _p(np.asarray(pos, np.float64))))
+
+    def tiles(self, wall):
+        n = int(self.grid_info(wall)[1])
+        out = np.zeros(n, np.uint32)
+        self.lib.orc_get_tiles(self.w, wall, _p(out))
+        return out
+
+    def volsurf_reactions(self):
+        return int(self.lib.orc_volsurf_reactions(self.w))
+
     def unimol_reactions(self):
         return int(self.lib.orc_unimol_reactions(self.w))
 
@@ -293,6 +333,8 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
         w.add_bimol_rxn(r.reactant_a, r.reactant_b, r.rate, r.products)
     for u in getattr(wl, "unimol_rxns", []) or []:
         w.add_unimol_rxn(u.reactant, u.rate, u.products)
+    for v in getattr(wl, "volsurf_rxns", []) or []:
+        w.add_volsurf_rxn(v.vol, v.surf, v.geom_vol, v.geom_surf, v.rate, v.product, v.product_orient)
     if getattr(wl, "surface_classes", None) is not None:
         w.set_surface_classes(wl.surface_classes)
     sel = wl.replica == replica
@@ -308,5 +350,17 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
             end += 1
         w.add_molecules(int(sp[start]), pos[start:end])
         start = end
+    sm = getattr(wl, "surface_mols", None)
+    if sm is not None:
+        ssel = np.nonzero(sm["replica"] == replica)[0]
+        assert np.array_equal(sm["ids"][ssel], len(ids) + np.arange(len(ssel))), "surface ids follow the volume ids"
+        start = 0
+        while start < len(ssel):
+            end = start
+            while end < len(ssel) and sm["species"][ssel[end]] == sm["species"][ssel[start]]:
+                end += 1
+            k = ssel[start:end]
+            w.add_surface_molecules(int(sm["species"][k[0]]), sm["walls"][k], sm["tiles"][k], sm["orients"][k])
+            start = end
     w.set_rng(draw_mode, wl.config.seed if seed is None else seed, replica)
     return w

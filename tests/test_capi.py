@@ -31,18 +31,19 @@ def test_abi_version_and_struct_sizes(tmp_path):
     import shutil
     import subprocess
     lib = _lib.load()
-    assert lib.mcg_abi_version() == 2
+    assert lib.mcg_abi_version() == 3
     sizes = (ctypes.sizeof(_lib.McgConfig), ctypes.sizeof(_lib.McgSpecies), ctypes.sizeof(_lib.McgStepStats),
-             ctypes.sizeof(_lib.McgBimolRxn), ctypes.sizeof(_lib.McgRxnEvent), ctypes.sizeof(_lib.McgUnimolRxn))
+             ctypes.sizeof(_lib.McgBimolRxn), ctypes.sizeof(_lib.McgRxnEvent), ctypes.sizeof(_lib.McgUnimolRxn),
+             ctypes.sizeof(_lib.McgVolSurfRxn))
     import numpy as np
     assert np.dtype(_lib.RXN_EVENT_DTYPE).itemsize == ctypes.sizeof(_lib.McgRxnEvent)
     if shutil.which("gcc") is None:
-        assert sizes == (112, 24, 48, 112, 64, 104)
+        assert sizes == (112, 24, 48, 112, 64, 104, 88)
         return
     src = tmp_path / "sz.c"
-    src.write_text('#include <stdio.h>\n#include "mcg.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu", '
+    src.write_text('#include <stdio.h>\n#include "mcg.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu", '
                    'sizeof(mcg_config), sizeof(mcg_species), sizeof(mcg_step_stats), sizeof(mcg_bimol_rxn), '
-                   'sizeof(mcg_rxn_event), sizeof(mcg_unimol_rxn));return 0;}\n')
+                   'sizeof(mcg_rxn_event), sizeof(mcg_unimol_rxn), sizeof(mcg_volsurf_rxn));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     c_sizes = tuple(int(v) for v in subprocess.check_output([str(exe)]).split())

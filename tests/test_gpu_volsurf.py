@@ -1,0 +1,118 @@
+"""GPU parity of volume-surface reactions (L' + R' -> LR' on the receptor
+sphere) against the CPU oracle with the same per-molecule Philox draws.
+
+A ligand reacts at most once and a receptor at most once per iteration; two
+ligands on one tile settle as the oracle's molecule order does (the smaller
+id first, the other finds the tile taken and reflects), so the whole run is
+compared exactly: the surviving ligands and their positions, and which
+receptors became LR with which ids.
+"""
+import numpy as np
+import pytest
+
+from mcell_amd import workloads as W
+from mcell_amd.engine import DiffuseReactEngine
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_surface(w, wl):
+    """(id, species, wall, tile) of the oracle's live surface molecules."""
+    om = w.molecules()
+    walls = np.unique(wl.surface_mols["walls"])
+    rows = []
+    for wall in walls:
+        t = w.tiles(int(wall))
+        occ = np.nonzero(t != 0xFFFFFFFF)[0]
+        for k in occ:
+            rows.append((int(t[k]), int(om["species"][t[k]]), int(wall), int(k)))
+    rows.sort()
+    return np.array(rows, np.int64).reshape(-1, 4)
+
+
+def _engine_surface(eng):
+    s = eng.surface_molecules()
+    rows = np.stack([s["id"], s["species"], s["wall"], s["tile"]], axis=1).astype(np.int64)
+    return rows[np.argsort(rows[:, 0], kind="stable")]
+
+
+@pytest.mark.parametrize("rate", [2e8, 4e8])
+def test_volsurf_snapshot_proposals_match_oracle(gpu_available, oracle_lib, rate):
+    """Proposals (mover, receptor, time, collision point) without carrying
+    them out; rate 2e8 keeps max_fixed_p below the per-tile scaling, 4e8
+    above it (the two branches of test_bimolecular)."""
+    from oracle import pyoracle as po
+    seed = 29
+    wl = W.receptor_sphere(n_l=150000, n_r=30000, iterations=4, rate=rate)
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed, propose_only=True, log_capacity=1 << 16)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
+    w.set_snapshot(True)
+    w.set_barrier_interval(wl.barrier_interval)
+    total = 0
+    for it in range(wl.iterations):
+        eng.step(it, 1)
+        w.run(1)
+        g = eng.rxn_log()
+        u, rx, d = w.rxn_log()
+        w.lib.orc_clear_rxn_log(w.w)
+        g = g[np.argsort(g["mover_id"], kind="stable")]
+        o = np.argsort(u[:, 1], kind="stable")
+        assert np.array_equal(g["mover_id"], u[o, 1]), "iteration %d: movers" % it
+        assert np.array_equal(g["partner_id"], u[o, 2]), "iteration %d: receptors" % it
+        assert np.all(g["rxn"] == 0x40000000) and np.array_equal(g["rxn"], rx[o].astype(np.uint32))
+        assert np.abs(g["time"] - d[o, 0]).max(initial=0) <= 1e-12
+        assert np.abs(g["pos"] - d[o, 1:4]).max(initial=0) <= 1e-9
+        total += len(g)
+    print("rate %.0e: %d proposals" % (rate, total), flush=True)
+    assert total > 50
+    w.close()
+    eng.close()
+
+
+def test_volsurf_full_run_matches_oracle(gpu_available, oracle_lib):
+    from oracle import pyoracle as po
+    seed = 31
+    wl = W.receptor_sphere(n_l=60000, n_r=30000, iterations=12, rate=4e8)
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
+    w.set_barrier_interval(wl.barrier_interval)
+    for it in range(wl.iterations):
+        eng.step(it, 1)
+        w.run(1)
+        g = eng.molecules()
+        o = np.argsort(g["id"], kind="stable")
+        om = w.molecules()
+        live = np.nonzero(((om["flags"] & 1) == 0) & (om["species"] == 0))[0]
+        assert np.array_equal(g["id"][o], live), "iteration %d: the same ligands" % it
+        d = np.abs(g["pos"][o] - om["pos"][live]).max(initial=0)
+        assert d <= 1e-12 * 100, "iteration %d: max |dpos| %g" % (it, d)
+        assert np.array_equal(_engine_surface(eng), _oracle_surface(w, wl)), "iteration %d: surface" % it
+    n_rx = w.volsurf_reactions()
+    c = eng.count_species(1, 3)[0].astype(np.int64)
+    print("%d reactions; counts L %d R %d LR %d" % (n_rx, c[0], c[1], c[2]), flush=True)
+    assert n_rx > 100 and c[2] == n_rx and c[0] == wl.num_molecules - n_rx
+    w.close()
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_volsurf_conservation_and_orientation(gpu_available, precision):
+    wl = W.receptor_sphere(n_l=200000, n_r=40000, iterations=40, rate=4e8)
+    sm = wl.surface_mols
+    eng = DiffuseReactEngine.from_workload(wl, precision=precision, seed=3)
+    eng.step(0, wl.iterations)
+    c = eng.count_species(1, 3)[0].astype(np.int64)
+    s = eng.surface_molecules()
+    print("%s: L %d R %d LR %d" % (precision, c[0], c[1], c[2]), flush=True)
+    assert c[2] > 1000
+    assert c[0] + c[2] == wl.num_molecules and c[1] + c[2] == len(sm["ids"])
+    assert eng.num_molecules() == c[0]
+    # products sit on tiles of up receptors, oriented up; down receptors never react
+    start = {(int(a), int(b)): int(o) for a, b, o in zip(sm["walls"], sm["tiles"], sm["orients"])}
+    for sp, wall, tile, orient in zip(s["species"], s["wall"], s["tile"], s["orient"]):
+        if sp == 2:
+            assert start[(int(wall), int(tile))] == 1 and orient == 1
+        else:
+            assert orient == start[(int(wall), int(tile))]
+    assert len(s["id"]) == len(sm["ids"]) and len(np.unique(s["id"])) == len(s["id"])
+    eng.close()
