@@ -76,6 +76,21 @@ def build_workload(args):
                            "D ~ U[1e-7, 5e-6] cm^2/s (seed 4 + i), C with D 5e-7, counts every 10 iterations, "
                            "dt 1e-6 s" % args.replicas,
                "replicas": args.replicas, "molecules": wl.num_molecules, "walls": int(len(wl.walls))}
+    elif args.workload == "volume_surface":
+        wl = W.volume_surface(iterations=args.steps)
+        cfg = {"workload": "L' + R' -> LR' (1e8 /M/s) on a sphere: reflective 2 um cube + icosphere r 0.5 um "
+                           "(5,120 triangles), 20,000 receptors R' uniform by area, 100,000 L (D 1e-6 cm^2/s) "
+                           "outside the sphere, 10,000 T crossing it through a transparent surface class, "
+                           "0.1 um subpartitions, dt 1e-6 s", "molecules": wl.num_molecules,
+               "surface_molecules": int(len(wl.surface_mols["ids"])), "walls": int(len(wl.walls)), "replicas": 1}
+    elif args.workload == "crowded_spheres":
+        wl = W.crowded_spheres(iterations=args.steps)
+        cfg = {"workload": "64 perturbed icospheres (20,480 triangles each, radii U[0.1, 0.3] um, +-2% radial "
+                           "vertex noise) in a reflective 4 um cube, 10M volume molecules A:B:E = 5:4:1 (D 1e-6, "
+                           "1e-6, 2e-7 cm^2/s), A + B -> E (1e8 /M/s), E -> A + B (1e3 /s), 200,000 S' on the "
+                           "spheres, A + S' -> AS' (5e7 /M/s), 0.05 um subpartitions, dt 1e-6 s",
+               "molecules": wl.num_molecules, "surface_molecules": int(len(wl.surface_mols["ids"])),
+               "walls": int(len(wl.walls)), "replicas": 1}
     elif args.workload == "bimol_box":
         wl = W.bimol_box(iterations=args.steps)
         cfg = {"workload": "A + B -> C (1e8 /M/s) in a reflective 1 um cube (12 walls), 50,000 A + 50,000 B "
@@ -99,6 +114,13 @@ def cpu_baseline(seconds, workload):
     if workload.startswith("free"):
         wl = W.free_diffusion_box()
         what = "free diffusion of 10,000 A in the reflective 1 um cube"
+    elif workload == "volume_surface":
+        wl = W.volume_surface()
+        what = "the whole volume-surface case (110,000 volume molecules, 20,000 receptors, 5,132 walls)"
+    elif workload == "crowded_spheres":
+        wl = W.crowded_spheres(n_molecules=200_000)
+        what = ("the crowded geometry (1,310,732 walls, 200,000 S') with 200,000 volume molecules A:B:E = 5:4:1 "
+                "(1/50 of the GPU run's)")
     elif workload == "replicas_sphere":
         wl = W.replicas_sphere_in_cube(num_replicas=1)
         what = ("instance 0 of the replica batch (A + B -> C, %d molecules, sphere-in-cube mesh, 5,132 walls), "
@@ -120,8 +142,12 @@ def cpu_baseline(seconds, workload):
                       "%.1f s" % (what, done, secs)}
 
 
+def reactive(wl):
+    return bool(wl.rxns or wl.unimol_rxns or wl.volsurf_rxns)
+
+
 def kernel_name(precision, wl):
-    if wl.rxns:
+    if reactive(wl):
         return "k_move_light + k_move_drain <%s>" % ("double" if precision == "fp64" else "float")
     return "k_light64 + k_hard64" if precision == "fp64" else "k_light32 + k_hard32"
 
@@ -131,7 +157,7 @@ def run_gpu(args, wl, precision):
     step statistics and timings."""
     from mcell_amd.engine import DiffuseReactEngine
     eng = DiffuseReactEngine.from_workload(wl, precision=precision,
-                                           capacity=wl.num_molecules * (2 if wl.rxns else 1))
+                                           capacity=wl.num_molecules * (2 if reactive(wl) else 1))
     print("bench: %s, %d molecules, precision %s, warmup %d, steps %d" % (args.workload, wl.num_molecules, precision,
                                                                           args.warmup, args.steps),
           file=sys.stderr, flush=True)
@@ -144,7 +170,7 @@ def run_gpu(args, wl, precision):
     secs = time.perf_counter() - t0
     avg_ms, launches = eng.last_kernel_time()
     eng.close()
-    steps_done = st.molecule_steps if wl.rxns else wl.num_molecules * args.steps
+    steps_done = st.molecule_steps if reactive(wl) else wl.num_molecules * args.steps
     return {"secs": secs, "steps_done": steps_done, "value": steps_done / secs, "avg_ms": avg_ms,
             "reactions": st.reactions, "rounds": st.wall_hits, "hard": st.slow_path_last}
 
@@ -192,7 +218,7 @@ def main():
                      "kernel": kernel_name(main_prec, wl), "kernel_avg_ms": r["avg_ms"],
                      "bytes_per_molecule_step": B_ALG[main_prec]},
     }
-    if wl.rxns:
+    if reactive(wl):
         out["reactions"] = r["reactions"]
         out["settle_rounds"] = r["rounds"]
         out["roofline"]["full_step_fraction"] = r["hard"] / max(per_step, 1)

@@ -24,9 +24,11 @@ struct V {
 };
 static_assert(sizeof(V) == 40, "vertex record");
 
-// 2 vertices per wall edge plus crossings and spans: enough for the handful
-// of walls that can cut one disk; running out is reported as an error.
-constexpr int kPool = 48;
+// 2 vertices per wall edge plus crossings and spans.  A disk near a finely
+// meshed surface (triangle edges below the reaction radius) is cut by tens
+// of walls; the pool lives in private memory, so its size costs nothing until
+// used.  Running out is reported as an error.
+constexpr int kPool = 256;
 
 struct Pool {
   V p[kPool];

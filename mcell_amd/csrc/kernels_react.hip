@@ -26,6 +26,8 @@
 // consumed molecules stay as defunct holes until the next sort drops them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "device/displace.h"
@@ -238,7 +240,12 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
   append_slot(due, j, ulist, ucount);
 }
 
-// Workgroup k drains queue shard k with the full step.
+// The full step for every queued mover.  The shards are read as one list
+// (their counts prefix-summed in LDS by every workgroup) and dealt out
+// grid-stride, so that movers crowded into a few shards (the queue is
+// cell-ordered, hard movers cluster near surfaces) still spread over the
+// whole GPU; consecutive threads take consecutive slots of a shard, which
+// came from neighbouring cells.
 template <typename PosT>
 __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT> S, RMolView<PosT> src,
                                                     RMolView<PosT> out, SlowQueue Q, SpeciesTable species, double barrier,
@@ -249,15 +256,48 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
                                                     uint32_t* __restrict__ ucount) {
   __shared__ ZigguratTables T;
   __shared__ uint32_t sets[SubpartSet::kCap * 256];
+  __shared__ uint32_t pre[kQueueShards + 1];
+  __shared__ uint32_t part[256];
+  static_assert(kQueueShards % 256 == 0, "shards per thread");
+  constexpr uint32_t kPer = kQueueShards / 256;
+  // exclusive prefix of the shard counts: per-thread sums, then a scan of 256
+  uint32_t local[kPer];
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    local[k] = Q.count[threadIdx.x * kPer + k];
+    sum += local[k];
+  }
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - sum;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    pre[threadIdx.x * kPer + k] = run;
+    run += local[k];
+  }
+  if (threadIdx.x == 255) pre[kQueueShards] = part[255];
+  __syncthreads();
+  const uint32_t total = pre[kQueueShards];
+  if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_zig(T);
-  const uint32_t count = Q.count[blockIdx.x];
-  const uint32_t* slots = Q.slots + (size_t)blockIdx.x * Q.shard_cap;
-  for (uint32_t base = 0; base < count; base += blockDim.x) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     bool proposed = false, due = false;
     uint32_t j = 0;
-    if (q < count) {
-      j = slots[q];
+    if (q < total) {
+      // the shard holding list position q: the last k with pre[k] <= q
+      uint32_t lo = 0, hi = kQueueShards;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= q) lo = mid;
+        else hi = mid;
+      }
+      j = Q.slots[(size_t)lo * Q.shard_cap + (q - pre[lo])];
       proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner, err_out,
                           due, sets + threadIdx.x);
     }
@@ -649,26 +689,57 @@ __global__ __launch_bounds__(256) void k_log_proposals(const uint32_t* __restric
   e.num_products = 0;
 }
 
+// Per-(replica, species) counts: a workgroup histogram in LDS when the bins
+// fit, flushed with one atomic per non-empty bin; global atomics otherwise.
+constexpr uint32_t kCountLdsBins = 4096;
+
+__device__ __forceinline__ void count_add(uint32_t* lds, uint64_t* counts, uint32_t bins, uint32_t b) {
+  if (bins <= kCountLdsBins) atomicAdd(lds + b, 1u);
+  else atomicAdd((unsigned long long*)(counts + b), 1ull);
+}
+
+__device__ __forceinline__ void count_flush(const uint32_t* lds, uint64_t* counts, uint32_t bins) {
+  if (bins > kCountLdsBins) return;
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x)
+    if (lds[b]) atomicAdd((unsigned long long*)(counts + b), (unsigned long long)lds[b]);
+}
+
+__device__ __forceinline__ void count_clear(uint32_t* lds, uint32_t bins) {
+  if (bins > kCountLdsBins) return;
+  for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x) lds[b] = 0;
+  __syncthreads();
+}
+
 template <typename PosT>
 __global__ __launch_bounds__(256) void k_count_species(RMolView<PosT> m, uint32_t n, uint32_t num_species,
                                                        uint32_t num_replicas, uint64_t* __restrict__ counts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t tag = m.tag[i];
-  if (tag_flags(tag) & MF_DEFUNCT) return;
-  const uint32_t s = tag_species(tag), r = tag_replica(tag);
-  if (s < num_species && r < num_replicas) atomicAdd((unsigned long long*)(counts + (size_t)r * num_species + s), 1ull);
+  __shared__ uint32_t h[kCountLdsBins];
+  const uint32_t bins = num_species * num_replicas;
+  count_clear(h, bins);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t tag = m.tag[i];
+    if (tag_flags(tag) & MF_DEFUNCT) continue;
+    const uint32_t s = tag_species(tag), r = tag_replica(tag);
+    if (s < num_species && r < num_replicas) count_add(h, counts, bins, r * num_species + s);
+  }
+  count_flush(h, counts, bins);
 }
 
 __global__ __launch_bounds__(256) void k_count_tiles(const uint32_t* __restrict__ tiles, uint32_t total_tiles,
                                                      uint32_t num_replicas, uint32_t num_species,
                                                      uint64_t* __restrict__ counts) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (uint64_t)total_tiles * num_replicas) return;
-  const uint32_t w = tiles[i];
-  if (w == kTileEmpty) return;
-  const uint32_t r = (uint32_t)(i / total_tiles), s = tile_species(w);
-  if (s < num_species) atomicAdd((unsigned long long*)(counts + (size_t)r * num_species + s), 1ull);
+  __shared__ uint32_t h[kCountLdsBins];
+  const uint32_t bins = num_species * num_replicas;
+  count_clear(h, bins);
+  const uint64_t nt = (uint64_t)total_tiles * num_replicas;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t w = tiles[i];
+    if (w == kTileEmpty) continue;
+    const uint32_t r = (uint32_t)(i / total_tiles), sp = tile_species(w);
+    if (sp < num_species) count_add(h, counts, bins, r * num_species + sp);
+  }
+  count_flush(h, counts, bins);
 }
 
 // --------------------------------------------------------------- launchers --
@@ -776,12 +847,12 @@ template <typename PosT>
 hipError_t ReactPipeline<PosT>::count_species(hipStream_t s, uint32_t n, uint32_t num_species, uint32_t num_replicas,
                                               uint64_t* counts) {
   if (n)
-    hipLaunchKernelGGL(k_count_species<PosT>, blocks_for(n), dim3(256), 0, s, M, n, num_species, num_replicas,
-                       counts);
+    hipLaunchKernelGGL(k_count_species<PosT>, dim3(std::min<uint32_t>(blocks_for(n).x, 2048u)), dim3(256), 0, s, M,
+                       n, num_species, num_replicas, counts);
   const uint64_t nt = (uint64_t)R.total_tiles * num_replicas;
   if (nt)
-    hipLaunchKernelGGL(k_count_tiles, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, s, tiles, R.total_tiles,
-                       num_replicas, num_species, counts);
+    hipLaunchKernelGGL(k_count_tiles, dim3((uint32_t)std::min<uint64_t>((nt + 255) / 256, 2048u)), dim3(256), 0, s,
+                       tiles, R.total_tiles, num_replicas, num_species, counts);
   return hipGetLastError();
 }
 

@@ -258,49 +258,153 @@ def volsurf_table(wl: Workload):
 
 
 def place_on_walls(n: int, wall_ids: np.ndarray, vertices: np.ndarray, walls: np.ndarray, seed: int):
-    """n distinct tiles drawn uniformly over all tiles of the given walls:
-    (walls, tiles)."""
+    """n distinct tiles of the given walls drawn uniformly by area (each tile
+    of a wall covers area / tiles of it; weighted sampling without replacement
+    by exponential keys, numpy PCG64): (walls, tiles)."""
+    v = vertices[walls[wall_ids]]
+    area = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
     nt = M.wall_num_tiles(vertices, walls[wall_ids])
     base = np.concatenate([[0], np.cumsum(nt)])
     if n > base[-1]:
         raise ValueError("more surface molecules than tiles")
+    weight = np.repeat(area / nt, nt)
     rng = np.random.Generator(np.random.PCG64(seed))
-    flat = np.sort(rng.choice(base[-1], size=n, replace=False))
+    key = rng.exponential(size=len(weight)) / weight
+    flat = np.sort(np.argpartition(key, n - 1)[:n]) if n < len(key) else np.arange(n)
     k = np.searchsorted(base, flat, side="right") - 1
     return wall_ids[k].astype(np.uint32), (flat - base[k]).astype(np.uint32)
 
 
 def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, seed: int = 3,
                     subpartition_dimension: float = 0.1, sphere_subdivisions: int = 5, rate: float = 1e8,
-                    down_fraction: float = 0.5) -> Workload:
+                    down_fraction: float = 0.5, n_t: int = 0, t_action: int = 1) -> Workload:
     """Ligand-receptor binding on a sphere: the reflective 2 um cube holding
-    the icosphere of radius 0.5 um (reflective); L (D = 1e-6 cm^2/s) starts
-    outside the sphere, the surface receptors R sit on distinct tiles of the
-    sphere, a fraction `down_fraction` of them oriented down (-1), the rest up
-    (+1).  L' + R' -> LR' at `rate` /M/s: only up receptors hit from the
-    outside (front) react, and the product LR takes the receptor's tile.
-    Placement seeds: seed (L), seed + 2 (receptor tiles), seed + 3
-    (orientations); ids 0..n_l-1 (L) then the receptors."""
+    the icosphere of radius 0.5 um; L (D = 1e-6 cm^2/s) starts outside the
+    sphere and is reflected by it, the surface receptors R sit on distinct
+    tiles of the sphere drawn uniformly by area, a fraction `down_fraction`
+    of them oriented down (-1), the rest up (+1).  L' + R' -> LR' at `rate`
+    /M/s: only up receptors hit from the outside (front) react, and the
+    product LR takes the receptor's tile.  With n_t > 0 a fourth species T
+    (D = 1e-6) starts anywhere in the cube and surface class 1 on the sphere
+    acts on it with `t_action` from both sides (1 = transparent, 2 =
+    absorptive).  Species L, R, LR, T; placement seeds: seed (L), seed + 1
+    (T), seed + 2 (receptor tiles), seed + 3 (orientations); ids 0..n_l-1 (L),
+    then T, then the receptors."""
     cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
     box = M.create_box("box", edge_dimension=2.0)
     ball = M.create_icosphere("sphere", 0.5, sphere_subdivisions)
+    if n_t:
+        ball.surface_class = 1
     objects = [box, ball]
     part = M.derive_partition(cfg, M.bounding_box(objects), has_bimol_vol_rxns=False)
     verts, walls, classes = M.vertices_internal(objects, part)
     species = [M.Species("L", 1e-6), M.Species("R", 0.0), M.Species("LR", 0.0)]
+    if n_t:
+        species.append(M.Species("T", 1e-6))
     rx = [M.VolSurfReaction(0, 1, rate, 2, product_orient=1, geom_vol=1, geom_surf=1)]
     sphere_walls = np.arange(len(box.wall_list), len(walls), dtype=np.uint32)
     sw, st = place_on_walls(n_r, sphere_walls, verts, walls, seed + 2)
     rng = np.random.Generator(np.random.PCG64(seed + 3))
     orients = np.where(rng.uniform(size=n_r) < down_fraction, -1, 1).astype(np.int32)
     pos = place_outside_sphere(n_l, 1.0, 0.5, seed, part)
-    surf = {"ids": np.arange(n_l, n_l + n_r, dtype=np.uint32), "species": np.ones(n_r, np.uint32),
+    sp = np.zeros(n_l, np.uint32)
+    sc = None
+    if n_t:
+        pos = np.concatenate([pos, M.place_uniform_in_box(n_t, 1.0, seed + 1, part)])
+        sp = np.concatenate([sp, np.full(n_t, 3, np.uint32)])
+        sc = np.array([[0, 0, 0, t_action | (t_action << 2)]], np.uint8)
+    n_vol = len(sp)
+    surf = {"ids": np.arange(n_vol, n_vol + n_r, dtype=np.uint32), "species": np.ones(n_r, np.uint32),
             "walls": sw, "tiles": st, "orients": orients, "replica": np.zeros(n_r, np.uint32)}
     steps = _species_table(species, part, [])
     steps[0] = (steps[0][0], steps[0][1], steps[0][2] | 4)
+    if n_t:
+        steps[3] = (steps[3][0], steps[3][1], steps[3][2] | 2)
     return Workload("receptor_sphere", part, verts, walls, classes, species, steps,
-                    np.arange(n_l, dtype=np.uint32), np.zeros(n_l, np.uint32), pos, np.zeros(n_l, np.uint32),
-                    iterations, config=cfg, objects=objects, volsurf_rxns=rx, surface_mols=surf)
+                    np.arange(n_vol, dtype=np.uint32), sp, pos, np.zeros(n_vol, np.uint32),
+                    iterations, config=cfg, objects=objects, volsurf_rxns=rx, surface_mols=surf,
+                    surface_classes=sc)
+
+
+def volume_surface(iterations: int = 1000, seed: int = 3, subpartition_dimension: float = 0.1) -> Workload:
+    """The volume-surface case: the reflective 2 um cube with the 0.5 um
+    icosphere (5,120 triangles); 20,000 receptors R' on the sphere drawn
+    uniformly by area (seed 3), 100,000 L (D = 1e-6 cm^2/s) outside the
+    sphere, L' + R' -> LR' at 1e8 /M/s, and 10,000 T that cross the sphere
+    through a transparent surface class; dt 1e-6 s."""
+    return receptor_sphere(n_l=100000, n_r=20000, iterations=iterations, seed=seed,
+                           subpartition_dimension=subpartition_dimension, rate=1e8, down_fraction=0.0,
+                           n_t=10000, t_action=1)
+
+
+def crowded_spheres(n_molecules: int = 10_000_000, n_surface: int = 200_000, iterations: int = 200, seed: int = 5,
+                    num_spheres: int = 64, sphere_subdivisions: int = 6, r_min: float = 0.1, r_max: float = 0.3,
+                    perturb: float = 0.02, cube_um: float = 4.0, subpartition_dimension: float = 0.05) -> Workload:
+    """Large crowded geometry: `num_spheres` icospheres (API subdivisions 6:
+    20,480 triangles each) with radii ~ U[r_min, r_max] um, centres
+    rejection-sampled so that no two overlap and all lie inside the
+    reflective `cube_um` cube, every vertex moved radially by a factor
+    1 + U[-perturb, perturb] (non-convex surfaces; one PCG64(seed) stream for
+    centres, radii and perturbations).  n_molecules volume molecules A:B:E =
+    5:4:1 (D 1e-6, 1e-6, 2e-7 cm^2/s) uniform in the cube outside the spheres
+    (seed + 1); A + B -> E at 1e8 /M/s, E -> A + B at 1e3 /s; n_surface
+    surface molecules S' on the spheres drawn uniformly by area (seed + 2) and
+    A + S' -> AS' at 5e7 /M/s.  Species A, B, E, S, AS; ids: volume molecules
+    in species order, then S."""
+    cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
+    h = cube_um / 2
+    box = M.create_box("box", edge_dimension=cube_um)
+    unit = M.create_icosphere("unit", 1.0, sphere_subdivisions)
+    uv, uw = unit.vertices(), unit.walls()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    centres, radii = [], []
+    margin = 1.0 + perturb
+    while len(radii) < num_spheres:
+        r = rng.uniform(r_min, r_max)
+        c = rng.uniform(-h + r * margin, h - r * margin, size=3)
+        if all(np.linalg.norm(c - c2) > (r + r2) * margin for c2, r2 in zip(centres, radii)):
+            centres.append(c)
+            radii.append(r)
+    objects = [box]
+    for i, (c, r) in enumerate(zip(centres, radii)):
+        scale = r * (1.0 + rng.uniform(-perturb, perturb, size=len(uv)))
+        objects.append(M.GeometryObject("sphere%d" % i, c + uv * scale[:, None], uw))
+    part = M.derive_partition(cfg, M.bounding_box(objects), has_bimol_vol_rxns=True)
+    verts, walls, classes = M.vertices_internal(objects, part)
+    species = [M.Species("A", 1e-6), M.Species("B", 1e-6), M.Species("E", 2e-7), M.Species("S", 0.0),
+               M.Species("AS", 0.0)]
+    rxns = [M.BimolReaction(0, 1, 1e8, [2])]
+    unimol = [M.UnimolReaction(2, 1e3, [0, 1])]
+    vs = [M.VolSurfReaction(0, 3, 5e7, 4, product_orient=1, geom_vol=0, geom_surf=1)]
+    # molecules: uniform in the cube, outside every sphere's perturbed extent
+    prng = np.random.Generator(np.random.PCG64(seed + 1))
+    cs, rs = np.array(centres), np.array(radii) * margin
+    pos = np.empty((0, 3))
+    while len(pos) < n_molecules:
+        p = prng.uniform(-h, h, size=(min(4_000_000, 2 * (n_molecules - len(pos)) + 16), 3))
+        inside = np.zeros(len(p), bool)
+        order = np.argsort(p[:, 0], kind="stable")
+        xs = p[order, 0]
+        for c, r in zip(cs, rs):
+            lo, hi = np.searchsorted(xs, [c[0] - r, c[0] + r])
+            idx = order[lo:hi]
+            inside[idx] |= ((p[idx] - c) ** 2).sum(axis=1) <= r * r
+        pos = np.concatenate([pos, p[~inside]])
+    pos = pos[:n_molecules] * part.rcp_length_unit
+    n_a, n_b = n_molecules * 5 // 10, n_molecules * 4 // 10
+    n_e = n_molecules - n_a - n_b
+    sp = np.concatenate([np.zeros(n_a, np.uint32), np.ones(n_b, np.uint32), np.full(n_e, 2, np.uint32)])
+    sphere_walls = np.arange(len(box.wall_list), len(walls), dtype=np.uint32)
+    sw, st = place_on_walls(n_surface, sphere_walls, verts, walls, seed + 2)
+    surf = {"ids": np.arange(n_molecules, n_molecules + n_surface, dtype=np.uint32),
+            "species": np.full(n_surface, 3, np.uint32), "walls": sw, "tiles": st,
+            "orients": np.ones(n_surface, np.int32), "replica": np.zeros(n_surface, np.uint32)}
+    steps = _species_table(species, part, rxns)
+    steps[0] = (steps[0][0], steps[0][1], steps[0][2] | 4)
+    return Workload("crowded_spheres", part, verts, walls, classes, species, steps,
+                    np.arange(n_molecules, dtype=np.uint32), sp, pos, np.zeros(n_molecules, np.uint32), iterations,
+                    rxns=rxns, config=cfg, objects=objects, barrier_interval=10, unimol_rxns=unimol,
+                    volsurf_rxns=vs, surface_mols=surf)
 
 
 def decay_box(n_e: int = 20000, iterations: int = 2000, seed: int = 6, rate: float = 1e3,
