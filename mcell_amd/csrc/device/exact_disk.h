@@ -323,7 +323,8 @@ __device__ double area_multiple(Pool& P, int head, double R2) {
 // exact_disk_utils.inl:840-1145; `full` set when the pool ran out.
 __device__ __noinline__ double exact_disk_dev(const PartitionParams& PP, const WallRec64* __restrict__ walls,
                                               const uint32_t* __restrict__ off, const uint32_t* __restrict__ idx,
-                                              D3 loc, D3 mv, double R, D3 target, bool* full) {
+                                              const uint32_t* __restrict__ box, D3 loc, D3 mv, double R, D3 target,
+                                              bool* full) {
   using namespace exd;
   const double PI = 3.14159265358979323846;
   Pool P;
@@ -348,7 +349,11 @@ __device__ __noinline__ double exact_disk_dev(const PartitionParams& PP, const W
     sm_zeta = zetize(sm_v, sm_u);
   }
   const uint32_t csp = subpart_index(PP, loc);
+  // the walls the reference keeps lie within R of loc on every axis (its
+  // bounding-box test below), so the cube loc +- R bounds their entry boxes
+  const uint32_t db = region_box(PP, csp, loc.x - R, loc.y - R, loc.z - R, loc.x + R, loc.y + R, loc.z + R);
   for (uint32_t k = off[csp]; k < off[csp + 1]; ++k) {
+    if (!boxes_overlap(box[k], db)) continue;
     const WallRec64& w = walls[idx[k]];
     D3 wv[3] = {{w.p0x, w.p0y, w.p0z}, {w.p1x, w.p1y, w.p1z}, {w.p2x, w.p2y, w.p2z}};
     D3 n = {w.nx, w.ny, w.nz};

@@ -59,15 +59,16 @@ __device__ __forceinline__ bool move_index_box(const PartitionParams& P, D3 pos,
 // (no hit, no re-draw).  Exact in the reference's arithmetic.
 // `ent` holds the wall records in list order (ent[k] = walls[idx[k]]).
 __device__ __forceinline__ bool all_walls_miss64(const PartitionParams& P, const WallRec64* __restrict__ ent,
-                                                 const uint32_t* __restrict__ off, D3 pos, D3 disp,
-                                                 const int lo[3], const int hi[3]) {
+                                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ box,
+                                                 D3 pos, D3 disp, const int lo[3], const int hi[3]) {
   for (int z = lo[2]; z <= hi[2]; ++z)
     for (int y = lo[1]; y <= hi[1]; ++y)
       for (int x = lo[0]; x <= hi[0]; ++x) {
         const uint32_t s = subpart_of(P, x, y, z);
         const uint32_t k1 = off[s + 1];
+        const uint32_t sb = segment_box(P, s, pos.x, pos.y, pos.z, disp.x, disp.y, disp.z);
         for (uint32_t k = off[s]; k < k1; ++k)
-          if (!wall_certain_miss64(ent[k], pos, disp)) return false;
+          if (boxes_overlap(box[k], sb) && !wall_certain_miss64(ent[k], pos, disp)) return false;
       }
   return true;
 }
@@ -75,8 +76,8 @@ __device__ __forceinline__ bool all_walls_miss64(const PartitionParams& P, const
 // FP32 filter over the index box; *near_plane set when the FP32 end point is
 // within `band` of some wall plane.
 __device__ __forceinline__ bool all_walls_miss32(const PartitionParams& P, const FilterWall* __restrict__ ent,
-                                                 const uint32_t* __restrict__ off, const FilterParams& F,
-                                                 float band, D3 pos, D3 disp,
+                                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ box,
+                                                 const FilterParams& F, float band, D3 pos, D3 disp,
                                                  const int lo[3], const int hi[3], bool& near_plane) {
   const float px = (float)pos.x, py = (float)pos.y, pz = (float)pos.z;
   const float mx = (float)disp.x, my = (float)disp.y, mz = (float)disp.z;
@@ -85,7 +86,9 @@ __device__ __forceinline__ bool all_walls_miss32(const PartitionParams& P, const
       for (int x = lo[0]; x <= hi[0]; ++x) {
         const uint32_t s = subpart_of(P, x, y, z);
         const uint32_t k1 = off[s + 1];
+        const uint32_t sb = segment_box(P, s, pos.x, pos.y, pos.z, disp.x, disp.y, disp.z);
         for (uint32_t k = off[s]; k < k1; ++k) {
+          if (!boxes_overlap(box[k], sb)) continue;
           float de;
           if (!certain_miss32(ent[k], F, px, py, pz, mx, my, mz, &de)) return false;
           if (fabs_f(de) < band) near_plane = true;

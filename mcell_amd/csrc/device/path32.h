@@ -19,6 +19,7 @@ struct Walls32View {
   const uint32_t* __restrict__ off;
   const uint32_t* __restrict__ idx;
   const FilterWall* __restrict__ ent32;  // w32[idx[k]] for every list entry k
+  const uint32_t* __restrict__ box;      // sub-cell box of every list entry k
 };
 
 // The FP64 test of one candidate wall (inlined: an out-of-line call forces
@@ -57,7 +58,11 @@ __device__ __forceinline__ bool closest_wall_in_subpart32(const PartitionParams&
     bool restarted = false;
     const float px = (float)pos.x, py = (float)pos.y, pz = (float)pos.z;
     const float mx = (float)move.x, my = (float)move.y, mz = (float)move.z;
+    const uint32_t sb = segment_box(P, subpart, pos.x, pos.y, pos.z, move.x, move.y, move.z);
     for (uint32_t k = k0; k < k1; ++k) {
+      // a wall the move cannot come near: a certain miss whose end point
+      // is not next to the wall itself
+      if (!boxes_overlap(W.box[k], sb)) continue;
       uint32_t wi = W.idx[k];
       float de;
       if (wi == last_hit) {

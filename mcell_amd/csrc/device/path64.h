@@ -6,6 +6,7 @@
 #pragma once
 #include "rng.h"
 #include "types.h"
+#include "wallbox.h"
 
 namespace mcg {
 
@@ -280,7 +281,8 @@ struct WallHit {
 // re-drawn displacement.
 __device__ __forceinline__ bool closest_wall_in_subpart64(const PartitionParams& P, const WallRec64* __restrict__ walls,
                                                           const uint32_t* __restrict__ off,
-                                                          const uint32_t* __restrict__ idx, uint32_t subpart,
+                                                          const uint32_t* __restrict__ idx,
+                                                          const uint32_t* __restrict__ box, uint32_t subpart,
                                                           uint32_t last_hit, D3 pos, D3& move, PhiloxStream& rng,
                                                           WallHit& best, uint32_t& err) {
   const uint32_t k0 = off[subpart], k1 = off[subpart + 1];
@@ -290,7 +292,9 @@ __device__ __forceinline__ bool closest_wall_in_subpart64(const PartitionParams&
     found = false;
     double closest = 1e20;  // TIME_FOREVER
     bool restarted = false;
+    const uint32_t sb = segment_box(P, subpart, pos.x, pos.y, pos.z, move.x, move.y, move.z);
     for (uint32_t k = k0; k < k1; ++k) {
+      if (!boxes_overlap(box[k], sb)) continue;
       uint32_t wi = idx[k];
       if (wi == last_hit) continue;
       double t;
@@ -329,7 +333,8 @@ __device__ __forceinline__ bool closest_wall_in_subpart64(const PartitionParams&
 // absorb; returns false when the molecule was absorbed.
 __device__ __forceinline__ bool diffuse_reflect64(const PartitionParams& P, const WallRec64* __restrict__ walls,
                                                   const uint32_t* __restrict__ off,
-                                                  const uint32_t* __restrict__ idx, const SurfaceClasses& SC,
+                                                  const uint32_t* __restrict__ idx,
+                                                  const uint32_t* __restrict__ box, const SurfaceClasses& SC,
                                                   uint32_t species, D3& pos, D3 disp, PhiloxStream& rng,
                                                   uint32_t& err, uint32_t* trace_walls, uint32_t& trace_n) {
   uint32_t last_hit = kInvalid;
@@ -345,7 +350,7 @@ __device__ __forceinline__ bool diffuse_reflect64(const PartitionParams& P, cons
     }
     uint32_t start = subpart_of(P, sx, sy, sz);
     WallHit hit;
-    bool found = closest_wall_in_subpart64(P, walls, off, idx, start, last_hit, pos, disp, rng, hit, err);
+    bool found = closest_wall_in_subpart64(P, walls, off, idx, box, start, last_hit, pos, disp, rng, hit, err);
     if (!found) {
       // the walk only runs when the move leaves the start subpartition
       D3 dst = {pos.x + part_disp.x, pos.y + part_disp.y, pos.z + part_disp.z};
@@ -355,7 +360,7 @@ __device__ __forceinline__ bool diffuse_reflect64(const PartitionParams& P, cons
         SubpartWalk64 walk;
         walk.init(P, pos, start, part_disp);
         while (walk.next(P, err)) {
-          if (closest_wall_in_subpart64(P, walls, off, idx, walk.cur, last_hit, pos, disp, rng, hit, err)) {
+          if (closest_wall_in_subpart64(P, walls, off, idx, box, walk.cur, last_hit, pos, disp, rng, hit, err)) {
             found = true;
             break;
           }

@@ -22,6 +22,7 @@ struct ReactParams {
   const uint32_t* __restrict__ idx;
   const FilterWall* __restrict__ ent32;  // FP32 records in list order (FP32 form)
   const WallRec64* __restrict__ ent64;   // FP64 records in list order
+  const uint32_t* __restrict__ ent_box;  // sub-cell box of every list entry
   CellGrid G;
   const RxnRec* __restrict__ rxns;
   uint32_t rxn_stride;                   // reaction classes per replica table (0: one shared table)
@@ -83,10 +84,10 @@ __device__ __forceinline__ bool closest_wall_any(const ReactParams& R, uint32_t 
                                                  D3& disp, PhiloxStream& rng, WallHit& hit, uint32_t& err,
                                                  bool& end_band) {
   if (kFilter32) {
-    Walls32View W{R.walls, nullptr, R.off, R.idx, R.ent32};
+    Walls32View W{R.walls, nullptr, R.off, R.idx, R.ent32, R.ent_box};
     return closest_wall_in_subpart32(R.P, W, R.F, s, last_hit, pos, disp, rng, hit, err, R.band, end_band);
   }
-  return closest_wall_in_subpart64(R.P, R.walls, R.off, R.idx, s, last_hit, pos, disp, rng, hit, err);
+  return closest_wall_in_subpart64(R.P, R.walls, R.off, R.idx, R.ent_box, s, last_hit, pos, disp, rng, hit, err);
 }
 
 // The next molecule collision after key (t_last, id_last) in time order
@@ -259,7 +260,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
         }
         bool full = false;
         const D3 target = {(double)S.x[slot], (double)S.y[slot], (double)S.z[slot]};
-        const double factor = exact_disk_dev(R.P, R.walls, R.off, R.idx, cp, disp, R.P.rxn_radius, target, &full);
+        const double factor = exact_disk_dev(R.P, R.walls, R.off, R.idx, R.ent_box, cp, disp, R.P.rxn_radius, target, &full);
         if (full) err |= ERR_EXD;
         if (factor < 0) {  // the disk is occluded
           if (candidates == 1) break;

@@ -183,9 +183,21 @@ int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint
     e64[k] = c->walls_host[c->idx_host[k]];
     e32[k] = w32[c->idx_host[k]];
   }
-  if (c->ent64.alloc(e64.size()) || c->ent32.alloc(e32.size())) return fail(MCG_E_NOMEM, "wall entry buffers");
+  // sub-cell box of every entry: the wall's bounding box within the entry's subpartition
+  std::vector<uint32_t> eb(std::max<size_t>(ne, 1), kBoxAll);
+  const size_t n3 = c->off_host.size() - 1;
+  for (size_t s = 0; s < n3; ++s)
+    for (uint32_t k = c->off_host[s]; k < c->off_host[s + 1]; ++k) {
+      const WallRec64& w = c->walls_host[c->idx_host[k]];
+      eb[k] = region_box(P, (uint32_t)s, std::min({w.p0x, w.p1x, w.p2x}), std::min({w.p0y, w.p1y, w.p2y}),
+                         std::min({w.p0z, w.p1z, w.p2z}), std::max({w.p0x, w.p1x, w.p2x}),
+                         std::max({w.p0y, w.p1y, w.p2y}), std::max({w.p0z, w.p1z, w.p2z}));
+    }
+  if (c->ent64.alloc(e64.size()) || c->ent32.alloc(e32.size()) || c->ent_box.alloc(eb.size()))
+    return fail(MCG_E_NOMEM, "wall entry buffers");
   HIP_TRY(hipMemcpy(c->ent64.p, e64.data(), e64.size() * sizeof(WallRec64), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->ent32.p, e32.data(), e32.size() * sizeof(WallRec32), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->ent_box.p, eb.data(), eb.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   c->have_geometry = true;
   return MCG_OK;
 }
@@ -284,7 +296,7 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
   auto t0 = std::chrono::steady_clock::now();
   MolView64 m{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p};
   MolView32 mf{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p};
-  WallView64 W{c->walls64.p, c->off.p, c->idx.p, c->ent64.p, c->ent32.p};
+  WallView64 W{c->walls64.p, c->off.p, c->idx.p, c->ent64.p, c->ent32.p, c->ent_box.p};
   HitTrace tr{c->trace_wall.p, c->trace_count.p, (c->cfg.flags & MCG_FLAG_TRACE_HITS) ? 1u : 0u};
   uint32_t n = (uint32_t)c->n;
   SlowQueue Q{c->queue.p, c->qcount.p, 0};

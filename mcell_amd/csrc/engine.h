@@ -110,6 +110,7 @@ struct mcg_ctx {
   mcg::DevBuf<uint32_t> off, idx;
   mcg::DevBuf<mcg::WallRec64> ent64;  // wall records in list (entry) order
   mcg::DevBuf<mcg::WallRec32> ent32;
+  mcg::DevBuf<uint32_t> ent_box;  // sub-cell box of every list entry
   bool have_geometry = false;
   // surface classes
   std::vector<uint32_t> wall_class_host;

@@ -54,6 +54,7 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.R.idx = c->idx.p;
   p.R.ent32 = reinterpret_cast<const FilterWall*>(c->ent32.p);
   p.R.ent64 = c->ent64.p;
+  p.R.ent_box = c->ent_box.p;
   p.queue = SlowQueue{c->queue.p, c->qcount.p, 0};
   p.R.G = r.grid;
   p.R.G.start = r.cell_start.p;

@@ -32,6 +32,7 @@ struct WallView64 {
   const uint32_t* idx;    // wall indices, ascending within a subpartition
   const WallRec64* ent;   // walls[idx[k]] for every list entry k
   const WallRec32* ent32; // its FP32 form
+  const uint32_t* box;    // sub-cell box of every list entry
 };
 
 // Slow-path queue: kQueueShards counters and shards of shard_cap slots.

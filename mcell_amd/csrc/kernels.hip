@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_light64(MolView64 m, uint32_t n, Specie
       const D3 disp = draw_displacement(rng, T, sp.space_step);
       const D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
       int lo[3], hi[3];
-      if (move_index_box(P, pos, end, lo, hi) && all_walls_miss64(P, W.ent, W.off, pos, disp, lo, hi)) {
+      if (move_index_box(P, pos, end, lo, hi) && all_walls_miss64(P, W.ent, W.off, W.box, pos, disp, lo, hi)) {
         m.x[i] = end.x;
         m.y[i] = end.y;
         m.z[i] = end.z;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_hard64(MolView64 m, SpeciesTable specie
     const D3 disp = draw_displacement(rng, T, sp.space_step);
     uint32_t err = 0, tn = 0;
     uint32_t* tw = trace.enabled ? trace.wall + (size_t)i * kTraceHits : nullptr;
-    const bool alive = diffuse_reflect64(P, W.walls, W.off, W.idx, SC, tag_species(tag), pos, disp, rng, err, tw, tn);
+    const bool alive = diffuse_reflect64(P, W.walls, W.off, W.idx, W.box, SC, tag_species(tag), pos, disp, rng, err, tw, tn);
     m.x[i] = pos.x;
     m.y[i] = pos.y;
     m.z[i] = pos.z;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, Specie
       int lo[3], hi[3];
       bool near_plane = false;
       if (move_index_box(P, pos, end, lo, hi) &&
-          all_walls_miss32(P, W.ent32, W.off, F, band, pos, disp, lo, hi, near_plane) && !near_plane) {
+          all_walls_miss32(P, W.ent32, W.off, W.box, F, band, pos, disp, lo, hi, near_plane) && !near_plane) {
         m.x[i] = (float)end.x;
         m.y[i] = (float)end.y;
         m.z[i] = (float)end.z;
@@ -208,7 +208,7 @@ hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t 
   FilterParams F = make_filter_params(max_abs_coord);
   float band = 2.0f * F.e_dd + 8.0f * F.eps * max_abs_coord;
   Walls32View W32{W.walls, reinterpret_cast<const FilterWall*>(w32), W.off, W.idx,
-                  reinterpret_cast<const FilterWall*>(W.ent32)};
+                  reinterpret_cast<const FilterWall*>(W.ent32), W.box};
   Q.shard_cap = shard_capacity(n);
   hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
   hipLaunchKernelGGL(k_light32, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W32, F, band, seed,

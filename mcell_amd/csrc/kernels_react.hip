@@ -348,8 +348,8 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
       bool clear = move_index_box(R.P, pos, end, lo, hi);
       bool near_plane = false;
       if (clear) {
-        if (kF32) clear = all_walls_miss32(R.P, R.ent32, R.off, R.F, R.band, pos, disp, lo, hi, near_plane);
-        else clear = all_walls_miss64(R.P, R.ent64, R.off, pos, disp, lo, hi);
+        if (kF32) clear = all_walls_miss32(R.P, R.ent32, R.off, R.ent_box, R.F, R.band, pos, disp, lo, hi, near_plane);
+        else clear = all_walls_miss64(R.P, R.ent64, R.off, R.ent_box, pos, disp, lo, hi);
       }
       if (clear && !near_plane && (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY)) {
         // partners within the reaction radius of the segment
