@@ -65,9 +65,10 @@ __device__ __forceinline__ bool all_walls_miss64(const PartitionParams& P, const
     for (int y = lo[1]; y <= hi[1]; ++y)
       for (int x = lo[0]; x <= hi[0]; ++x) {
         const uint32_t s = subpart_of(P, x, y, z);
-        const uint32_t k1 = off[s + 1];
+        const uint32_t k0 = off[s], k1 = off[s + 1];
+        if (k0 == k1) continue;
         const uint32_t sb = segment_box(P, s, pos.x, pos.y, pos.z, disp.x, disp.y, disp.z);
-        for (uint32_t k = off[s]; k < k1; ++k)
+        for (uint32_t k = k0; k < k1; ++k)
           if (boxes_overlap(box[k], sb) && !wall_certain_miss64(ent[k], pos, disp)) return false;
       }
   return true;
@@ -85,9 +86,10 @@ __device__ __forceinline__ bool all_walls_miss32(const PartitionParams& P, const
     for (int y = lo[1]; y <= hi[1]; ++y)
       for (int x = lo[0]; x <= hi[0]; ++x) {
         const uint32_t s = subpart_of(P, x, y, z);
-        const uint32_t k1 = off[s + 1];
+        const uint32_t k0 = off[s], k1 = off[s + 1];
+        if (k0 == k1) continue;
         const uint32_t sb = segment_box(P, s, pos.x, pos.y, pos.z, disp.x, disp.y, disp.z);
-        for (uint32_t k = off[s]; k < k1; ++k) {
+        for (uint32_t k = k0; k < k1; ++k) {
           if (!boxes_overlap(box[k], sb)) continue;
           float de;
           if (!certain_miss32(ent[k], F, px, py, pz, mx, my, mz, &de)) return false;

@@ -50,6 +50,7 @@ __device__ __forceinline__ bool closest_wall_in_subpart32(const PartitionParams&
                                                           D3 pos, D3& move, PhiloxStream& rng, WallHit& best,
                                                           uint32_t& err, float band, bool& end_band) {
   const uint32_t k0 = W.off[subpart], k1 = W.off[subpart + 1];
+  if (k0 == k1) return false;
   int redo = 0;
   bool found;
   for (;;) {
@@ -105,20 +106,26 @@ __device__ __forceinline__ bool closest_wall_in_subpart32(const PartitionParams&
   return found;
 }
 
-// Rounds the FP64 end point to FP32 keeping it on the same side of every wall
-// plane of its subpartition that it is within a few ulps of.
+// Rounds the FP64 end point to FP32 keeping it on the same side of the plane
+// of every wall of its subpartition that passes next to it (entry box within
+// the box margin of the point).  The plane of a wall whose triangle is farther
+// away may be crossed: a later move from the rounded point meets that plane
+// within rounding distance of the point, outside the triangle, and misses.
 __device__ __forceinline__ void round_to_fp32_same_side(const PartitionParams& P, const WallRec64* __restrict__ walls,
                                                      const uint32_t* __restrict__ off,
-                                                     const uint32_t* __restrict__ idx, double px, double py,
+                                                     const uint32_t* __restrict__ idx,
+                                                     const uint32_t* __restrict__ box, double px, double py,
                                                      double pz, float* ox, float* oy, float* oz) {
   float fx = (float)px, fy = (float)py, fz = (float)pz;
   int sx, sy, sz;
   subpart_3d(P, D3{px, py, pz}, sx, sy, sz);
   if (!(sx < 0 || sy < 0 || sz < 0 || sx >= (int)P.n || sy >= (int)P.n || sz >= (int)P.n)) {
     uint32_t s = subpart_of(P, sx, sy, sz);
+    const uint32_t pb = segment_box(P, s, px, py, pz, 0.0, 0.0, 0.0);
     for (int pass = 0; pass < 4; ++pass) {
       bool moved = false;
       for (uint32_t k = off[s]; k < off[s + 1]; ++k) {
+        if (!boxes_overlap(box[k], pb)) continue;
         const WallRec64& w = walls[idx[k]];
         double sd = dot3(D3{w.nx, w.ny, w.nz}, D3{px, py, pz}) - w.d;
         double sf = dot3(D3{w.nx, w.ny, w.nz}, D3{(double)fx, (double)fy, (double)fz}) - w.d;

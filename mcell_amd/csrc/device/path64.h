@@ -286,6 +286,7 @@ __device__ __forceinline__ bool closest_wall_in_subpart64(const PartitionParams&
                                                           uint32_t last_hit, D3 pos, D3& move, PhiloxStream& rng,
                                                           WallHit& best, uint32_t& err) {
   const uint32_t k0 = off[subpart], k1 = off[subpart + 1];
+  if (k0 == k1) return false;
   int redo = 0;
   bool found;
   for (;;) {

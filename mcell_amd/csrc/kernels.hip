@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable specie
     if (!alive) m.tag[i] = tag | (MF_DEFUNCT << 8);
     float fx, fy, fz;
     if (near_wall) {
-      round_to_fp32_same_side(P, W.w64, W.off, W.idx, pos.x, pos.y, pos.z, &fx, &fy, &fz);
+      round_to_fp32_same_side(P, W.w64, W.off, W.idx, W.box, pos.x, pos.y, pos.z, &fx, &fy, &fz);
     } else {
       fx = (float)pos.x;
       fy = (float)pos.y;

@@ -104,7 +104,7 @@ __device__ __forceinline__ void store_pos(const ReactParams& R, float* x, float*
                                           bool near_plane) {
   float fx, fy, fz;
   if (near_plane) {
-    round_to_fp32_same_side(R.P, R.walls, R.off, R.idx, p.x, p.y, p.z, &fx, &fy, &fz);
+    round_to_fp32_same_side(R.P, R.walls, R.off, R.idx, R.ent_box, p.x, p.y, p.z, &fx, &fy, &fz);
   } else {
     fx = (float)p.x;
     fy = (float)p.y;
