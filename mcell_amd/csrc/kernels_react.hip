@@ -342,7 +342,8 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
       rng.init(R.seed, idv, R.iteration, tag_replica(tag), 0);
       double t_steps;
       D3 disp;
-      (void)clipped_displacement(rng, T, sp.space_step, sp.time_step, barrier - toff, t_steps, disp);
+      const double r_rate_factor =
+          clipped_displacement(rng, T, sp.space_step, sp.time_step, barrier - toff, t_steps, disp);
       const D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
       int lo[3], hi[3];
       bool clear = move_index_box(R.P, pos, end, lo, hi);
@@ -371,13 +372,30 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         const double bx0 = (a[0] < b[0] ? a[0] : b[0]) - R1, bx1 = (a[0] < b[0] ? b[0] : a[0]) + R1;
         const double by0 = (a[1] < b[1] ? a[1] : b[1]) - R1, by1 = (a[1] < b[1] ? b[1] : a[1]) + R1;
         const double bz0 = (a[2] < b[2] ? a[2] : b[2]) - R1, bz1 = (a[2] < b[2] ? b[2] : a[2]) + R1;
+        // up to two collisions are resolved here when both partners sit in the
+        // start or end cell (cells every collection of the reference holds)
+        uint32_t c_start = kInvalid, c_end = kInvalid;
+        {
+          int sx, sy, sz;
+          subpart_3d(R.P, pos, sx, sy, sz);
+          (void)cell_of(R.G, sx, sy, sz, rep, c_start);
+          subpart_3d(R.P, end, sx, sy, sz);
+          (void)cell_of(R.G, sx, sy, sz, rep, c_end);
+        }
+        int ncand = 0;
+        double ct[2];
+        uint32_t cid[2];
+        int32_t crx[2];
+        D3 ccp[2];
+        bool cok[2];
         for (int z = clo[2]; z <= chi[2] && clear; ++z)
           for (int y = clo[1]; y <= chi[1] && clear; ++y)
             for (int x = clo[0]; x <= chi[0] && clear; ++x) {
               uint32_t c;
               if (!cell_of(R.G, x, y, z, rep, c)) continue;
               for (uint32_t t = 0; t < R.G.ns && clear; ++t) {
-                if (R.pair_rxn[my_sp * R.num_species + t] < 0) continue;
+                const int32_t rx = R.pair_rxn[my_sp * R.num_species + t];
+                if (rx < 0) continue;
                 const uint32_t k1 = R.G.start[c * R.G.ns + t + 1];
                 for (uint32_t k = R.G.start[c * R.G.ns + t]; k < k1; ++k) {
                   const double qx = (double)S.x[k], qy = (double)S.y[k], qz = (double)S.z[k];
@@ -386,11 +404,52 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
                   D3 cp;
                   if (!collide_mol64(pos, disp, D3{qx, qy, qz}, sigma2, tc, cp)) continue;
                   if (k == j) continue;
-                  clear = false;
-                  break;
+                  if (ncand == 2) {
+                    clear = false;
+                    break;
+                  }
+                  ct[ncand] = tc;
+                  cid[ncand] = S.id[k];
+                  crx[ncand] = rx;
+                  ccp[ncand] = cp;
+                  cok[ncand] = c == c_start || c == c_end;
+                  ++ncand;
                 }
               }
             }
+        if (clear && ncand > 0) {
+          // the reference's order: time, then the larger partner id first
+          if (ncand == 2 && (ct[1] < ct[0] || (ct[1] == ct[0] && cid[1] > cid[0]))) {
+            const double tt = ct[0]; ct[0] = ct[1]; ct[1] = tt;
+            const uint32_t ti = cid[0]; cid[0] = cid[1]; cid[1] = ti;
+            const int32_t tr = crx[0]; crx[0] = crx[1]; crx[1] = tr;
+            const D3 tp = ccp[0]; ccp[0] = ccp[1]; ccp[1] = tp;
+            const bool tb = cok[0]; cok[0] = cok[1]; cok[1] = tb;
+          }
+          for (int q = 0; q < ncand && clear; ++q) {
+            if (!cok[q]) {
+              clear = false;
+              break;
+            }
+            if (ct[q] < STIME_EPS) continue;  // is_immediate_collision (collision_utils.inl:814-816)
+            // exact_disk is exactly 1 when no wall of the collision point's
+            // subpartition comes within R of it (no wall is processed)
+            int qx, qy, qz;
+            subpart_3d(R.P, ccp[q], qx, qy, qz);
+            const uint32_t csp = subpart_of(R.P, qx, qy, qz);
+            const uint32_t db = region_box(R.P, csp, ccp[q].x - R1, ccp[q].y - R1, ccp[q].z - R1, ccp[q].x + R1,
+                                           ccp[q].y + R1, ccp[q].z + R1);
+            for (uint32_t k = R.off[csp]; k < R.off[csp + 1]; ++k)
+              if (boxes_overlap(R.ent_box[k], db)) {
+                clear = false;
+                break;
+              }
+            if (!clear) break;
+            // test_bimolecular with factor 1: a reaction goes to the full step,
+            // which re-draws the same numbers and proposes it
+            if (test_bimolecular(R.rxns[rep * R.rxn_stride + crx[q]], r_rate_factor, rng) >= 0) clear = false;
+          }
+        }
       }
       if (clear && !near_plane) {
         store_pos(R, out.x, out.y, out.z, j, end, false);
