@@ -65,10 +65,11 @@ __device__ __host__ __forceinline__ uint32_t make_tile(uint32_t species, int ori
 }
 
 // Cell grid over the region molecules can occupy: subpartition index box
-// [lo, lo + dim) per replica, per_replica = dim.x * dim.y * dim.z cells.  The
-// snapshot is sorted by bin = cell * ns + species, so a mover reads exactly
-// the partners of the species it reacts with: start[] is the CSR over the
-// bins (a cell's molecules are start[c * ns] .. start[(c + 1) * ns]).
+// [lo, lo + dim) per replica, per_replica = dim.x * dim.y * dim.z cells,
+// numbered x fastest.  The snapshot is sorted by bin = (replica, species,
+// cell), so a mover reads exactly the partners of the species it reacts with,
+// and one species' molecules in a row of cells along x are one contiguous
+// range: start[] is the CSR over the bins.
 struct CellGrid {
   int lo[3];
   int dim[3];
@@ -77,11 +78,17 @@ struct CellGrid {
   const uint32_t* __restrict__ start;
 };
 
-__device__ __forceinline__ bool cell_of(const CellGrid& G, int gx, int gy, int gz, uint32_t replica, uint32_t& c) {
+// The replica-local cell of subpartition (gx, gy, gz); false outside the grid.
+__device__ __forceinline__ bool cell_of(const CellGrid& G, int gx, int gy, int gz, uint32_t& c) {
   const int x = gx - G.lo[0], y = gy - G.lo[1], z = gz - G.lo[2];
   if (x < 0 || y < 0 || z < 0 || x >= G.dim[0] || y >= G.dim[1] || z >= G.dim[2]) return false;
-  c = replica * G.per_replica + (uint32_t)x + (uint32_t)G.dim[0] * ((uint32_t)y + (uint32_t)G.dim[1] * (uint32_t)z);
+  c = (uint32_t)x + (uint32_t)G.dim[0] * ((uint32_t)y + (uint32_t)G.dim[1] * (uint32_t)z);
   return true;
+}
+
+// The bin of species t in replica-local cell c.
+__device__ __forceinline__ uint32_t bin_of(const CellGrid& G, uint32_t replica, uint32_t t, uint32_t c) {
+  return (replica * G.ns + t) * G.per_replica + c;
 }
 
 // Snapshot of the start-of-iteration molecules, cell-sorted (FP64 positions

@@ -107,12 +107,13 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
     const uint32_t s = set.at(k);
     const int gx = (int)(s % R.P.n), gy = (int)((s / R.P.n) % R.P.n), gz = (int)((s / R.P.n2) % R.P.n);
     uint32_t c;
-    if (!cell_of(R.G, gx, gy, gz, replica, c)) continue;  // no molecules outside the grid
+    if (!cell_of(R.G, gx, gy, gz, c)) continue;  // no molecules outside the grid
     for (uint32_t t = 0; t < R.G.ns; ++t) {
     const int32_t rx = R.pair_rxn[my_species * R.num_species + t];
     if (rx < 0) continue;
-    const uint32_t j1 = R.G.start[c * R.G.ns + t + 1];
-    for (uint32_t j = R.G.start[c * R.G.ns + t]; j < j1; ++j) {
+    const uint32_t b = bin_of(R.G, replica, t, c);
+    const uint32_t j1 = R.G.start[b + 1];
+    for (uint32_t j = R.G.start[b]; j < j1; ++j) {
       double t;
       D3 cp;
       if (!collide_mol64(pos, disp, D3{(double)S.x[j], (double)S.y[j], (double)S.z[j]}, sigma2, t, cp)) continue;

@@ -59,11 +59,11 @@ __global__ __launch_bounds__(256) void k_cell_keys(RMolView<PosT> m, uint32_t n,
   if (!(tag_flags(tag) & MF_DEFUNCT)) {
     int sx, sy, sz;
     subpart_3d(P, D3{(double)m.x[i], (double)m.y[i], (double)m.z[i]}, sx, sy, sz);
-    if (!cell_of(G, sx, sy, sz, tag_replica(tag), c)) {
+    if (!cell_of(G, sx, sy, sz, c)) {
       atomicOr(err, ERR_GRID);
       c = kInvalid;
     } else {
-      c = c * G.ns + tag_species(tag);
+      c = bin_of(G, tag_replica(tag), tag_species(tag), c);
     }
   }
   key[i] = c;
@@ -307,13 +307,17 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
 }
 
 // The common step, register-light: movers whose straight move certainly
-// meets no wall and passes no reaction partner within the reaction radius
-// commit pos + disp here (the reference's ray trace would find no collision
-// and commit the same point, diffuse_react_event.cpp:773-777); the others go
-// to the queue for the full step, which re-draws the same displacement from
-// the same stream.  The partner test scans the cells of the move's
-// subpartition box widened by rxn_radius * sqrt(2), a superset of the cells
-// the reference collects, so a committed mover had no collision there.
+// meets no wall commit pos + disp here (the reference's ray trace would find
+// no wall and commit the same point, diffuse_react_event.cpp:773-777) unless
+// they pass a reaction partner within the reaction radius; the others go to
+// the queue for the full step, which re-draws the same displacement from the
+// same stream.  The partner scan covers every cell of the segment's box
+// widened by the radius, so it finds every partner collide_mol accepts (a
+// superset of the reference's candidates).  One or two collisions with
+// partners in the start or end subpartition, which the reference always
+// collects, are processed here in the reference's order when no wall is near
+// the collision point: exact_disk is then exactly 1, and a failed
+// test_bimolecular draw leaves the move as it was.
 template <typename PosT>
 __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT> S, RMolView<PosT> out,
                                                     const uint32_t* __restrict__ count, SpeciesTable species,
@@ -353,15 +357,16 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         else clear = all_walls_miss64(R.P, R.ent64, R.off, R.ent_box, pos, disp, lo, hi);
       }
       if (clear && !near_plane && (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY)) {
-        // partners within the reaction radius of the segment
-        const double r_nb = R.P.rxn_radius * 1.41421356238;
+        // partners within the reaction radius of the segment: every one lies
+        // in the cells of the segment's box widened by the radius
         const double sigma2 = R.P.rxn_radius * R.P.rxn_radius;
         const double o[3] = {R.P.ox, R.P.oy, R.P.oz};
         const double a[3] = {pos.x, pos.y, pos.z}, b[3] = {end.x, end.y, end.z};
         int clo[3], chi[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const double mn = (a[k] < b[k] ? a[k] : b[k]) - r_nb, mx = (a[k] < b[k] ? b[k] : a[k]) + r_nb;
+          const double mn = (a[k] < b[k] ? a[k] : b[k]) - R.P.rxn_radius;
+          const double mx = (a[k] < b[k] ? b[k] : a[k]) + R.P.rxn_radius;
           int l = (int)floor((mn - o[k]) * R.P.sp_rcp), h = (int)floor((mx - o[k]) * R.P.sp_rcp);
           clo[k] = l < R.G.lo[k] ? R.G.lo[k] : l;
           chi[k] = h > R.G.lo[k] + R.G.dim[k] - 1 ? R.G.lo[k] + R.G.dim[k] - 1 : h;
@@ -373,50 +378,50 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         const double by0 = (a[1] < b[1] ? a[1] : b[1]) - R1, by1 = (a[1] < b[1] ? b[1] : a[1]) + R1;
         const double bz0 = (a[2] < b[2] ? a[2] : b[2]) - R1, bz1 = (a[2] < b[2] ? b[2] : a[2]) + R1;
         // up to two collisions are resolved here when both partners sit in the
-        // start or end cell (cells every collection of the reference holds)
-        uint32_t c_start = kInvalid, c_end = kInvalid;
-        {
-          int sx, sy, sz;
-          subpart_3d(R.P, pos, sx, sy, sz);
-          (void)cell_of(R.G, sx, sy, sz, rep, c_start);
-          subpart_3d(R.P, end, sx, sy, sz);
-          (void)cell_of(R.G, sx, sy, sz, rep, c_end);
-        }
+        // start or end subpartition (cells every collection of the reference holds)
+        int ss[3], se[3];
+        subpart_3d(R.P, pos, ss[0], ss[1], ss[2]);
+        subpart_3d(R.P, end, se[0], se[1], se[2]);
         int ncand = 0;
         double ct[2];
         uint32_t cid[2];
         int32_t crx[2];
         D3 ccp[2];
         bool cok[2];
+        // one species' partners in a row of cells along x are one range of the snapshot
+        const int nx = chi[0] - clo[0];
         for (int z = clo[2]; z <= chi[2] && clear; ++z)
-          for (int y = clo[1]; y <= chi[1] && clear; ++y)
-            for (int x = clo[0]; x <= chi[0] && clear; ++x) {
-              uint32_t c;
-              if (!cell_of(R.G, x, y, z, rep, c)) continue;
-              for (uint32_t t = 0; t < R.G.ns && clear; ++t) {
-                const int32_t rx = R.pair_rxn[my_sp * R.num_species + t];
-                if (rx < 0) continue;
-                const uint32_t k1 = R.G.start[c * R.G.ns + t + 1];
-                for (uint32_t k = R.G.start[c * R.G.ns + t]; k < k1; ++k) {
-                  const double qx = (double)S.x[k], qy = (double)S.y[k], qz = (double)S.z[k];
-                  if (qx < bx0 || qx > bx1 || qy < by0 || qy > by1 || qz < bz0 || qz > bz1) continue;
-                  double tc;
-                  D3 cp;
-                  if (!collide_mol64(pos, disp, D3{qx, qy, qz}, sigma2, tc, cp)) continue;
-                  if (k == j) continue;
-                  if (ncand == 2) {
-                    clear = false;
-                    break;
-                  }
-                  ct[ncand] = tc;
-                  cid[ncand] = S.id[k];
-                  crx[ncand] = rx;
-                  ccp[ncand] = cp;
-                  cok[ncand] = c == c_start || c == c_end;
-                  ++ncand;
+          for (int y = clo[1]; y <= chi[1] && clear; ++y) {
+            uint32_t c0;
+            if (nx < 0 || !cell_of(R.G, clo[0], y, z, c0)) continue;
+            for (uint32_t t = 0; t < R.G.ns && clear; ++t) {
+              const int32_t rx = R.pair_rxn[my_sp * R.num_species + t];
+              if (rx < 0) continue;
+              const uint32_t b0 = bin_of(R.G, rep, t, c0);
+              const uint32_t k1 = R.G.start[b0 + (uint32_t)nx + 1];
+              for (uint32_t k = R.G.start[b0]; k < k1; ++k) {
+                const double qx = (double)S.x[k], qy = (double)S.y[k], qz = (double)S.z[k];
+                if (qx < bx0 || qx > bx1 || qy < by0 || qy > by1 || qz < bz0 || qz > bz1) continue;
+                double tc;
+                D3 cp;
+                if (!collide_mol64(pos, disp, D3{qx, qy, qz}, sigma2, tc, cp)) continue;
+                if (k == j) continue;
+                if (ncand == 2) {
+                  clear = false;
+                  break;
                 }
+                int q3[3];
+                subpart_3d(R.P, D3{qx, qy, qz}, q3[0], q3[1], q3[2]);
+                ct[ncand] = tc;
+                cid[ncand] = S.id[k];
+                crx[ncand] = rx;
+                ccp[ncand] = cp;
+                cok[ncand] = (q3[0] == ss[0] && q3[1] == ss[1] && q3[2] == ss[2]) ||
+                             (q3[0] == se[0] && q3[1] == se[1] && q3[2] == se[2]);
+                ++ncand;
               }
             }
+          }
         if (clear && ncand > 0) {
           // the reference's order: time, then the larger partner id first
           if (ncand == 2 && (ct[1] < ct[0] || (ct[1] == ct[0] && cid[1] > cid[0]))) {
