@@ -23,6 +23,52 @@ namespace mcg {
 // overflow: shard_cap = 256 * ceil(num_workgroups / kQueueShards).
 constexpr uint32_t kQueueShards = 2048;
 
+// The drains read the shards as one list: every workgroup prefix-sums the
+// shard counts in LDS (pre[0..kQueueShards], part[256] scratch; all 256
+// threads call this) and deals list positions out grid-stride, so that
+// movers crowded into a few shards (hard movers cluster near surfaces and the
+// light pass is cell-ordered) still spread over the whole GPU.  Returns the
+// number of queued movers.
+__device__ __forceinline__ uint32_t queue_prefix(const uint32_t* __restrict__ count, uint32_t* pre, uint32_t* part) {
+  static_assert(kQueueShards % 256 == 0, "shards per thread");
+  constexpr uint32_t kPer = kQueueShards / 256;
+  uint32_t local[kPer];
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    local[k] = count[threadIdx.x * kPer + k];
+    sum += local[k];
+  }
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - sum;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    pre[threadIdx.x * kPer + k] = run;
+    run += local[k];
+  }
+  if (threadIdx.x == 255) pre[kQueueShards] = part[255];
+  __syncthreads();
+  return pre[kQueueShards];
+}
+
+// The queued value at list position q < total: the last shard k with
+// pre[k] <= q holds it at offset q - pre[k].
+__device__ __forceinline__ uint32_t queue_at(const uint32_t* __restrict__ slots, uint32_t shard_cap,
+                                            const uint32_t* pre, uint32_t q) {
+  uint32_t lo = 0, hi = kQueueShards;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return slots[(size_t)lo * shard_cap + (q - pre[lo])];
+}
+
 // Wave-aggregated append of the `want` lanes' value to a shard: one atomic
 // per wave, slot order by lane.
 __device__ __forceinline__ void wave_append(bool want, uint32_t value, uint32_t* __restrict__ shard_slots,

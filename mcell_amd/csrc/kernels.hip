@@ -73,12 +73,14 @@ __global__ __launch_bounds__(256) void k_hard64(MolView64 m, SpeciesTable specie
                                                 PartitionParams P, WallView64 W, uint64_t seed, uint32_t iteration,
                                                 SlowQueue Q, uint32_t* __restrict__ err_out, HitTrace trace) {
   __shared__ ZigguratTables T;
+  __shared__ uint32_t pre[kQueueShards + 1];
+  __shared__ uint32_t part[256];
+  // the shards read as one list, grid-stride
+  const uint32_t total = queue_prefix(Q.count, pre, part);
+  if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_ziggurat(T);
-  // workgroup k drains shard k
-  const uint32_t count = Q.count[blockIdx.x];
-  const uint32_t* slots = Q.slots + (size_t)blockIdx.x * Q.shard_cap;
-  for (uint32_t q = threadIdx.x; q < count; q += blockDim.x) {
-    const uint32_t i = slots[q];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const uint32_t i = queue_at(Q.slots, Q.shard_cap, pre, q);
     const uint32_t tag = m.tag[i];
     const SpeciesRec sp = species.of(tag);
     PhiloxStream rng;
@@ -136,12 +138,14 @@ __global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable specie
                                                 uint64_t seed, uint32_t iteration, SlowQueue Q,
                                                 uint32_t* __restrict__ err_out, HitTrace trace) {
   __shared__ ZigguratTables T;
+  __shared__ uint32_t pre[kQueueShards + 1];
+  __shared__ uint32_t part[256];
+  // the shards read as one list, grid-stride
+  const uint32_t total = queue_prefix(Q.count, pre, part);
+  if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_ziggurat(T);
-  // workgroup k drains shard k
-  const uint32_t count = Q.count[blockIdx.x];
-  const uint32_t* slots = Q.slots + (size_t)blockIdx.x * Q.shard_cap;
-  for (uint32_t q = threadIdx.x; q < count; q += blockDim.x) {
-    const uint32_t i = slots[q];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const uint32_t i = queue_at(Q.slots, Q.shard_cap, pre, q);
     const uint32_t tag = m.tag[i];
     const SpeciesRec sp = species.of(tag);
     PhiloxStream rng;

@@ -240,12 +240,9 @@ __global__ __launch_bounds__(256) void k_move(ReactParams R, Snapshot<PosT> S, R
   append_slot(due, j, ulist, ucount);
 }
 
-// The full step for every queued mover.  The shards are read as one list
-// (their counts prefix-summed in LDS by every workgroup) and dealt out
-// grid-stride, so that movers crowded into a few shards (the queue is
-// cell-ordered, hard movers cluster near surfaces) still spread over the
-// whole GPU; consecutive threads take consecutive slots of a shard, which
-// came from neighbouring cells.
+// The full step for every queued mover (the shards read as one list,
+// grid-stride; consecutive threads take consecutive slots of a shard, which
+// came from neighbouring cells).
 template <typename PosT>
 __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT> S, RMolView<PosT> src,
                                                     RMolView<PosT> out, SlowQueue Q, SpeciesTable species, double barrier,
@@ -258,31 +255,7 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
   __shared__ uint32_t sets[SubpartSet::kCap * 256];
   __shared__ uint32_t pre[kQueueShards + 1];
   __shared__ uint32_t part[256];
-  static_assert(kQueueShards % 256 == 0, "shards per thread");
-  constexpr uint32_t kPer = kQueueShards / 256;
-  // exclusive prefix of the shard counts: per-thread sums, then a scan of 256
-  uint32_t local[kPer];
-  uint32_t sum = 0;
-  for (uint32_t k = 0; k < kPer; ++k) {
-    local[k] = Q.count[threadIdx.x * kPer + k];
-    sum += local[k];
-  }
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  for (uint32_t off = 1; off < 256; off <<= 1) {
-    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[threadIdx.x] - sum;
-  for (uint32_t k = 0; k < kPer; ++k) {
-    pre[threadIdx.x * kPer + k] = run;
-    run += local[k];
-  }
-  if (threadIdx.x == 255) pre[kQueueShards] = part[255];
-  __syncthreads();
-  const uint32_t total = pre[kQueueShards];
+  const uint32_t total = queue_prefix(Q.count, pre, part);
   if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_zig(T);
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
@@ -290,14 +263,7 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
     bool proposed = false, due = false;
     uint32_t j = 0;
     if (q < total) {
-      // the shard holding list position q: the last k with pre[k] <= q
-      uint32_t lo = 0, hi = kQueueShards;
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= q) lo = mid;
-        else hi = mid;
-      }
-      j = Q.slots[(size_t)lo * Q.shard_cap + (q - pre[lo])];
+      j = queue_at(Q.slots, Q.shard_cap, pre, q);
       proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner, err_out,
                           due, sets + threadIdx.x);
     }
