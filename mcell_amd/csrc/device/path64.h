@@ -95,9 +95,8 @@ struct SubpartWalk64 {
     urb[0] = nz.x > 0; urb[1] = nz.y > 0; urb[2] = nz.z > 0;
     add[0] = urb[0] ? 1 : -1; add[1] = urb[1] ? 1 : -1; add[2] = urb[2] ? 1 : -1;
     rcp = {1.0 / nz.x, 1.0 / nz.y, 1.0 / nz.z};
-    ci[0] = (int)(start % P.n);
-    ci[1] = (int)((start / P.n) % P.n);
-    ci[2] = (int)((start / P.n2) % P.n);
+    // start is the subpartition of pos in every caller
+    subpart_3d(P, pos, ci[0], ci[1], ci[2]);
     D3 dst = {pos.x + d.x, pos.y + d.y, pos.z + d.z};
     int dx, dy, dz;
     subpart_3d(P, dst, dx, dy, dz);

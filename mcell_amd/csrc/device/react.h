@@ -105,10 +105,14 @@ struct Snapshot {
   const double* __restrict__ utime;       // unimolecular reaction times (absolute) or kTimeInvalid
 };
 
-// A small set of subpartition indices (the reference's SubpartIndicesSet),
-// stored in the calling workgroup's LDS: v[k * stride] is entry k of this
+// A small set of subpartitions (the reference's SubpartIndicesSet), stored
+// in the calling workgroup's LDS as packed index triples x | y << 10 | z << 20
+// (at most 1024 subpartitions per edge): v[k * stride] is entry k of this
 // lane's set (stride = the workgroup size, so that a wave's lanes touch
 // consecutive words).
+__device__ __forceinline__ uint32_t pack_sp(int x, int y, int z) {
+  return (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
+}
 struct SubpartSet {
   static constexpr int kCap = 48;
   uint32_t* v;
@@ -137,16 +141,16 @@ __device__ __forceinline__ void collect_neighbors(const PartitionParams& P, D3 p
   D3 minus = {rel.x - r, rel.y - r, rel.z - r};
   D3 b = {(double)idx[0] * P.sp_len, (double)idx[1] * P.sp_len, (double)idx[2] * P.sp_len};
   int xd = 0, yd = 0, zd = 0;
-  if (minus.x < b.x && minus.x > 0.0) { out.insert(subpart_of(P, idx[0] - 1, idx[1], idx[2])); xd = -1; }
-  else if (plus.x > b.x + P.sp_len && plus.x < part_len) { out.insert(subpart_of(P, idx[0] + 1, idx[1], idx[2])); xd = 1; }
-  if (minus.y < b.y && minus.y > 0.0) { out.insert(subpart_of(P, idx[0], idx[1] - 1, idx[2])); yd = -1; }
-  else if (plus.y > b.y + P.sp_len && plus.y < part_len) { out.insert(subpart_of(P, idx[0], idx[1] + 1, idx[2])); yd = 1; }
-  if (minus.z < b.z && minus.z > 0.0) { out.insert(subpart_of(P, idx[0], idx[1], idx[2] - 1)); zd = -1; }
-  else if (plus.z > b.z + P.sp_len && plus.z < part_len) { out.insert(subpart_of(P, idx[0], idx[1], idx[2] + 1)); zd = 1; }
-  if (xd && yd) out.insert(subpart_of(P, idx[0] + xd, idx[1] + yd, idx[2]));
-  if (xd && zd) out.insert(subpart_of(P, idx[0] + xd, idx[1], idx[2] + zd));
-  if (yd && zd) out.insert(subpart_of(P, idx[0], idx[1] + yd, idx[2] + zd));
-  if (xd && yd && zd) out.insert(subpart_of(P, idx[0] + xd, idx[1] + yd, idx[2] + zd));
+  if (minus.x < b.x && minus.x > 0.0) { out.insert(pack_sp(idx[0] - 1, idx[1], idx[2])); xd = -1; }
+  else if (plus.x > b.x + P.sp_len && plus.x < part_len) { out.insert(pack_sp(idx[0] + 1, idx[1], idx[2])); xd = 1; }
+  if (minus.y < b.y && minus.y > 0.0) { out.insert(pack_sp(idx[0], idx[1] - 1, idx[2])); yd = -1; }
+  else if (plus.y > b.y + P.sp_len && plus.y < part_len) { out.insert(pack_sp(idx[0], idx[1] + 1, idx[2])); yd = 1; }
+  if (minus.z < b.z && minus.z > 0.0) { out.insert(pack_sp(idx[0], idx[1], idx[2] - 1)); zd = -1; }
+  else if (plus.z > b.z + P.sp_len && plus.z < part_len) { out.insert(pack_sp(idx[0], idx[1], idx[2] + 1)); zd = 1; }
+  if (xd && yd) out.insert(pack_sp(idx[0] + xd, idx[1] + yd, idx[2]));
+  if (xd && zd) out.insert(pack_sp(idx[0] + xd, idx[1], idx[2] + zd));
+  if (yd && zd) out.insert(pack_sp(idx[0], idx[1] + yd, idx[2] + zd));
+  if (xd && yd && zd) out.insert(pack_sp(idx[0] + xd, idx[1] + yd, idx[2] + zd));
 }
 
 // collect_crossed_subparts with collect_for_molecules (and not for walls):
@@ -156,14 +160,15 @@ __device__ __forceinline__ void collect_neighbors(const PartitionParams& P, D3 p
 __device__ __forceinline__ void collect_molecule_subparts(const PartitionParams& P, D3 pos, uint32_t start,
                                                           D3 disp, SubpartSet& out, uint32_t& err) {
   out.clear();
-  out.insert(start);
+  int src[3];
+  subpart_3d(P, pos, src[0], src[1], src[2]);  // the subpartition `start` names
+  out.insert(pack_sp(src[0], src[1], src[2]));
   const double r_nb = P.rxn_radius * 1.41421356238;  // POS_SQRT2, defines.h:148
-  int src[3] = {(int)(start % P.n), (int)((start / P.n) % P.n), (int)((start / P.n2) % P.n)};
   if (P.use_expanded_list) collect_neighbors(P, pos, src, r_nb, out);
   SubpartWalk64 walk;
   walk.init(P, pos, start, disp);
   while (walk.next(P, err)) {
-    out.insert(walk.cur);
+    out.insert(pack_sp(walk.ci[0], walk.ci[1], walk.ci[2]));
     if (P.use_expanded_list) collect_neighbors(P, walk.cp, walk.ci, r_nb, out);
   }
   if (P.use_expanded_list) {

@@ -105,7 +105,7 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
   candidates = 0;
   for (int k = 0; k < set.n; ++k) {
     const uint32_t s = set.at(k);
-    const int gx = (int)(s % R.P.n), gy = (int)((s / R.P.n) % R.P.n), gz = (int)((s / R.P.n2) % R.P.n);
+    const int gx = (int)(s & 1023u), gy = (int)((s >> 10) & 1023u), gz = (int)(s >> 20);
     uint32_t c;
     if (!cell_of(R.G, gx, gy, gz, c)) continue;  // no molecules outside the grid
     for (uint32_t t = 0; t < R.G.ns; ++t) {
