@@ -119,10 +119,20 @@ __global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, Specie
       int lo[3], hi[3];
       bool near_plane = false;
       if (move_index_box(P, pos, end, lo, hi) &&
-          all_walls_miss32(P, W.ent32, W.off, W.box, F, band, pos, disp, lo, hi, near_plane) && !near_plane) {
-        m.x[i] = (float)end.x;
-        m.y[i] = (float)end.y;
-        m.z[i] = (float)end.z;
+          all_walls_miss32(P, W.ent32, W.off, W.box, F, band, pos, disp, lo, hi, near_plane)) {
+        // an end point next to a wall plane keeps its side in FP32, as the
+        // full step's would
+        float fx, fy, fz;
+        if (near_plane) {
+          round_to_fp32_same_side(P, W.w64, W.off, W.idx, W.box, end.x, end.y, end.z, &fx, &fy, &fz);
+        } else {
+          fx = (float)end.x;
+          fy = (float)end.y;
+          fz = (float)end.z;
+        }
+        m.x[i] = fx;
+        m.y[i] = fy;
+        m.z[i] = fz;
         if (trace.enabled) trace.count[i] = 0;
       } else {
         hard = true;

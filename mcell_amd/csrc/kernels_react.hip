@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         if (kF32) clear = all_walls_miss32(R.P, R.ent32, R.off, R.ent_box, R.F, R.band, pos, disp, lo, hi, near_plane);
         else clear = all_walls_miss64(R.P, R.ent64, R.off, R.ent_box, pos, disp, lo, hi);
       }
-      if (clear && !near_plane && (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY)) {
+      if (clear && (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY)) {
         // partners within the reaction radius of the segment: every one lies
         // in the cells of the segment's box widened by the radius
         const double sigma2 = R.P.rxn_radius * R.P.rxn_radius;
@@ -422,8 +422,9 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
           }
         }
       }
-      if (clear && !near_plane) {
-        store_pos(R, out.x, out.y, out.z, j, end, false);
+      if (clear) {
+        // an end point next to a wall plane keeps its side in FP32
+        store_pos(R, out.x, out.y, out.z, j, end, near_plane);
         out.id[j] = idv;
         out.tag[j] = tag & ~(MF_SCHEDULE_UNIMOL << 8);
         out.toff[j] = next_toff(R.iteration, toff, t_steps);
