@@ -173,12 +173,16 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
                                             const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
                                             double utime, double barrier, Proposal& prop, MoverEnd& out,
                                             uint32_t& err, uint32_t* set_lds) {
-  constexpr bool kF32 = sizeof(PosT) == 4;  // FP32 positions: FP32 wall filter
+  // The full step tests its candidate walls in FP64 in both forms: after the
+  // entry-box pre-test a mover here meets few walls, and the FP32 filter in
+  // front of them cost more than it saved.  In the FP32 form the end point
+  // is then always rounded to FP32 on its FP64 side of nearby wall planes.
+  constexpr bool kF32 = sizeof(PosT) == 4;
   const uint32_t rep = tag_replica(tag), spc = tag_species(tag);
   PhiloxStream rng;
   rng.init(R.seed, idv, R.iteration, rep, 0);
   double elapsed = (double)R.iteration + toff;
-  out.near_plane = false;
+  out.near_plane = kF32;
   out.absorbed = false;
   out.due = 0;
   out.t_steps = 0;
@@ -220,12 +224,12 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
     const uint32_t dest = subpart_of(R.P, dx, dy, dz);
     WallHit hit;
     uint32_t hit_sp = start;
-    bool found = closest_wall_any<kF32>(R, start, last_hit, pos, disp, rng, hit, err, near_plane);
+    bool found = closest_wall_any<false>(R, start, last_hit, pos, disp, rng, hit, err, near_plane);
     if (!found && dest != start) {
       SubpartWalk64 walk;
       walk.init(R.P, pos, start, part_disp);
       while (walk.next(R.P, err)) {
-        if (closest_wall_any<kF32>(R, walk.cur, last_hit, pos, disp, rng, hit, err, near_plane)) {
+        if (closest_wall_any<false>(R, walk.cur, last_hit, pos, disp, rng, hit, err, near_plane)) {
           found = true;
           hit_sp = walk.cur;
           break;
@@ -290,7 +294,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
     if (!found) {
       // disp as the wall search left it (an edge re-draw may have moved it)
       pos = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
-      out.near_plane = near_plane;
+      out.near_plane = kF32 || near_plane;
       break;
     }
     // collide_and_react_with_surf_mol (diffuse_react_event.cpp:505-527, 845-975):

@@ -164,17 +164,15 @@ __global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable specie
     const D3 disp = draw_displacement(rng, T, sp.space_step);
     uint32_t err = 0, tn = 0;
     uint32_t* tw = trace.enabled ? trace.wall + (size_t)i * kTraceHits : nullptr;
-    bool alive = true;
-    bool near_wall = diffuse_reflect32(P, W, F, band, SC, tag_species(tag), pos, disp, rng, err, tw, tn, alive);
+    // the FP64 wall search (a queued molecule meets few walls after the
+    // entry-box pre-test; the FP32 filter in front of them cost more than
+    // it saved), then the end point rounded to FP32 on its side of nearby
+    // wall planes
+    const bool alive = diffuse_reflect64(P, W.w64, W.off, W.idx, W.box, SC, tag_species(tag), pos, disp, rng, err,
+                                         tw, tn);
     if (!alive) m.tag[i] = tag | (MF_DEFUNCT << 8);
     float fx, fy, fz;
-    if (near_wall) {
-      round_to_fp32_same_side(P, W.w64, W.off, W.idx, W.box, pos.x, pos.y, pos.z, &fx, &fy, &fz);
-    } else {
-      fx = (float)pos.x;
-      fy = (float)pos.y;
-      fz = (float)pos.z;
-    }
+    round_to_fp32_same_side(P, W.w64, W.off, W.idx, W.box, pos.x, pos.y, pos.z, &fx, &fy, &fz);
     m.x[i] = fx;
     m.y[i] = fy;
     m.z[i] = fz;
