@@ -78,15 +78,9 @@ struct Proposal {
   uint32_t replica, pad;
 };
 
-// Closest wall in one subpartition, exact FP64 or FP32-filtered.
-template <bool kFilter32>
-__device__ __forceinline__ bool closest_wall_any(const ReactParams& R, uint32_t s, uint32_t last_hit, D3 pos,
-                                                 D3& disp, PhiloxStream& rng, WallHit& hit, uint32_t& err,
-                                                 bool& end_band) {
-  if (kFilter32) {
-    Walls32View W{R.walls, nullptr, R.off, R.idx, R.ent32, R.ent_box};
-    return closest_wall_in_subpart32(R.P, W, R.F, s, last_hit, pos, disp, rng, hit, err, R.band, end_band);
-  }
+// Closest wall in one subpartition (FP64; the entry-box pre-test first).
+__device__ __forceinline__ bool closest_wall(const ReactParams& R, uint32_t s, uint32_t last_hit, D3 pos, D3& disp,
+                                             PhiloxStream& rng, WallHit& hit, uint32_t& err) {
   return closest_wall_in_subpart64(R.P, R.walls, R.off, R.idx, R.ent_box, s, last_hit, pos, disp, rng, hit, err);
 }
 
@@ -207,7 +201,6 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   set.clear();
   int seg;
   for (seg = 0; seg < kMaxSegments; ++seg) {
-    bool near_plane = false;
     const D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
     const D3 part_disp = in_partition(R.P, end) ? disp : displacement_up_to_partition_boundary(R.P, pos, disp);
     int sx, sy, sz;
@@ -224,12 +217,12 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
     const uint32_t dest = subpart_of(R.P, dx, dy, dz);
     WallHit hit;
     uint32_t hit_sp = start;
-    bool found = closest_wall_any<false>(R, start, last_hit, pos, disp, rng, hit, err, near_plane);
+    bool found = closest_wall(R, start, last_hit, pos, disp, rng, hit, err);
     if (!found && dest != start) {
       SubpartWalk64 walk;
       walk.init(R.P, pos, start, part_disp);
       while (walk.next(R.P, err)) {
-        if (closest_wall_any<false>(R, walk.cur, last_hit, pos, disp, rng, hit, err, near_plane)) {
+        if (closest_wall(R, walk.cur, last_hit, pos, disp, rng, hit, err)) {
           found = true;
           hit_sp = walk.cur;
           break;
@@ -294,7 +287,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
     if (!found) {
       // disp as the wall search left it (an edge re-draw may have moved it)
       pos = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
-      out.near_plane = kF32 || near_plane;
+      out.near_plane = kF32;
       break;
     }
     // collide_and_react_with_surf_mol (diffuse_react_event.cpp:505-527, 845-975):

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, Specie
 }
 
 __global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable species, SurfaceClasses SC,
-                                                PartitionParams P, Walls32View W, FilterParams F, float band,
+                                                PartitionParams P, Walls32View W,
                                                 uint64_t seed, uint32_t iteration, SlowQueue Q,
                                                 uint32_t* __restrict__ err_out, HitTrace trace) {
   __shared__ ZigguratTables T;
@@ -225,8 +225,8 @@ hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t 
   hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
   hipLaunchKernelGGL(k_light32, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W32, F, band, seed,
                      iteration, Q, trace);
-  hipLaunchKernelGGL(k_hard32, dim3(kQueueShards), dim3(256), 0, s, m, species, SC, P, W32, F, band, seed, iteration, Q,
-                     err, trace);
+  hipLaunchKernelGGL(k_hard32, dim3(kQueueShards), dim3(256), 0, s, m, species, SC, P, W32, seed, iteration, Q, err,
+                     trace);
   return hipGetLastError();
 }
 
