@@ -142,6 +142,43 @@ def cpu_baseline(seconds, workload):
                       "%.1f s" % (what, done, secs)}
 
 
+def fp32_hit_agreement(wl, iterations=20):
+    """FP32+fallback against FP64 on the workload's geometry and molecules
+    without its reaction classes (the wall path: reflections and surface
+    classes), per molecule-step: the walls hit in order (hit traces; the FP64
+    form equals the oracle bit for bit), and the position RMS at the end."""
+    import dataclasses
+    import numpy as np
+    from mcell_amd.engine import DiffuseReactEngine
+    free = dataclasses.replace(wl, rxns=[], unimol_rxns=[], volsurf_rxns=[], surface_mols=None,
+                               species_steps=[(ss, ts, f & ~5) for ss, ts, f in wl.species_steps])
+    eng = {p: DiffuseReactEngine.from_workload(free, precision=p, trace=True) for p in ("fp64", "fp32")}
+    steps = hits = differ = 0
+    for it in range(iterations):
+        tr = {}
+        for p, e in eng.items():
+            e.step(it, 1)
+            m = e.molecules()
+            w, c = e.hit_trace()
+            o = np.argsort(m["id"], kind="stable")
+            tr[p] = (w[o], c[o], m["pos"][o])
+        (w64, c64, p64), (w32, c32, p32) = tr["fp64"], tr["fp32"]
+        k = np.minimum(np.maximum(c64, c32), w64.shape[1])
+        hit = np.maximum(c64, c32) > 0
+        same = c64 == c32
+        for j in range(w64.shape[1]):
+            same &= (j >= k) | (w64[:, j] == w32[:, j])
+        steps += len(c64)
+        hits += int(hit.sum())
+        differ += int((hit & ~same).sum())
+    rms = float(np.sqrt(np.mean(np.sum((p64 - p32) ** 2, axis=1))))
+    for e in eng.values():
+        e.close()
+    return {"iterations": iterations, "molecule_steps": steps, "steps_with_hits": hits,
+            "hit_index_disagreements": differ, "disagreement_rate": differ / max(hits, 1),
+            "position_rms_lu": rms}
+
+
 def reactive(wl):
     return bool(wl.rxns or wl.unimol_rxns or wl.volsurf_rxns)
 
@@ -229,6 +266,8 @@ def main():
         out["fp64"] = {"value": r64["value"], "ms_per_step": 1e3 * r64["secs"] / args.steps,
                        "kernel_avg_ms": r64["avg_ms"], "reactions": r64["reactions"]}
         out["fp32_over_fp64"] = r["value"] / r64["value"]
+    if args.workload == "volume_surface":
+        out["fp32_vs_fp64"] = fp32_hit_agreement(wl)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, args.workload)
     print(json.dumps(out), flush=True)
