@@ -49,6 +49,7 @@ class Workload:
     # surface molecules: dict of equal-length arrays ids, species, walls,
     # tiles, orients, replica (or None)
     surface_mols: Optional[dict] = None
+    spheres: Optional[np.ndarray] = None  # (k, 4) nominal centres and radii (um) of generated spheres
 
     def species_for(self, replica: int) -> List[M.Species]:
         if self.replica_D is None:
@@ -404,7 +405,8 @@ def crowded_spheres(n_molecules: int = 10_000_000, n_surface: int = 200_000, ite
     return Workload("crowded_spheres", part, verts, walls, classes, species, steps,
                     np.arange(n_molecules, dtype=np.uint32), sp, pos, np.zeros(n_molecules, np.uint32), iterations,
                     rxns=rxns, config=cfg, objects=objects, barrier_interval=10, unimol_rxns=unimol,
-                    volsurf_rxns=vs, surface_mols=surf)
+                    volsurf_rxns=vs, surface_mols=surf,
+                    spheres=np.concatenate([np.array(centres), np.array(radii)[:, None]], axis=1))
 
 
 def decay_box(n_e: int = 20000, iterations: int = 2000, seed: int = 6, rate: float = 1e3,
