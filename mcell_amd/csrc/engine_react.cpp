@@ -115,26 +115,26 @@ int read_hdr(mcg_ctx* c) {
 }
 
 // Claims, acceptance and repeated moves until every proposal of the phase
-// is settled.
+// is settled.  max_props bounds the phase's proposals (its movers); the
+// settling kernels read the actual count on the device, so the header is
+// read once per round, after the claims are settled.
 template <typename PosT>
 int settle_rounds(mcg_ctx* c, ReactPipeline<PosT>& p, bool snapshot_src, double barrier, uint32_t* err,
-                  uint32_t& rounds) {
+                  uint32_t& rounds, uint32_t max_props) {
   uint32_t* h = c->rx.hdr_host;
   for (uint32_t r = 0;; ++r) {
-    int st = read_hdr(c);
-    if (st) return st;
-    const uint32_t np = h[kHdrProposals];
-    if (np == 0) return MCG_OK;
     if (r > 100000) return fail(MCG_E_LIMIT, "reaction settling did not converge");
     uint32_t* rl = (r & 1) ? p.rlist1 : p.rlist0;
     uint32_t* rc = p.hdr + ((r & 1) ? kHdrRerun1 : kHdrRerun0);
-    HIP_TRY(p.settle(c->stream, np, rl, rc));
-    st = read_hdr(c);
+    HIP_TRY(p.settle(c->stream, max_props, rl, rc));
+    int st = read_hdr(c);
     if (st) return st;
-    const uint32_t nr = h[(r & 1) ? kHdrRerun1 : kHdrRerun0];
+    if (h[kHdrProposals] == 0) return MCG_OK;
     ++rounds;
+    const uint32_t nr = h[(r & 1) ? kHdrRerun1 : kHdrRerun0];
     if (nr == 0) return MCG_OK;
     HIP_TRY(p.move(c->stream, snapshot_src, rl, rc, nr, barrier, false, err));
+    max_props = nr;
   }
 }
 
@@ -149,6 +149,8 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
   uint32_t rounds = 0;
   r.log_count = 0;
   HIP_TRY(launch_fill_u32(c->stream, p.hdr, 0u, kHdrWords));
+  // the kernels report errors into the header, which every iteration reads
+  uint32_t* const err = p.hdr + kHdrErr;
   move_ms = 0;
   for (uint32_t k = 0; k < num; ++k) {
     const uint32_t it = first + k;
@@ -158,19 +160,20 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
     const double barrier = std::min((double)(next_barrier - it), 100.0);
     p.R.iteration = it;
     const uint32_t n_in = (uint32_t)c->n;
-    HIP_TRY(p.sort(c->stream, n_in, r.num_cells, c->err.p));
+    HIP_TRY(p.sort(c->stream, n_in, r.num_cells, err));
     HIP_TRY(launch_copy_u32(c->stream, p.cell_start + r.num_cells, p.hdr + kHdrNumOut));
     HIP_TRY(launch_copy_u32(c->stream, p.cell_start + r.num_cells, p.hdr + kHdrSnapshot));
     HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
     HIP_TRY(hipEventRecord(c->evm0, c->stream));
-    HIP_TRY(p.move(c->stream, true, nullptr, p.cell_start + r.num_cells, n_in, barrier, propose_only, c->err.p));
+    HIP_TRY(p.move(c->stream, true, nullptr, p.cell_start + r.num_cells, n_in, barrier, propose_only, err));
     HIP_TRY(hipEventRecord(c->evm1, c->stream));
     if (propose_only) {
       HIP_TRY(p.log_proposals(c->stream, n_in, it, r.log.p, p.hdr + kHdrLog, (uint32_t)r.log_cap));
     } else {
       bool snapshot_src = true;
+      uint32_t phase_movers = n_in;
       for (;;) {
-        int st = settle_rounds(c, p, snapshot_src, barrier, c->err.p, rounds);
+        int st = settle_rounds(c, p, snapshot_src, barrier, err, rounds, phase_movers);
         if (st) return st;
         if (h[kHdrUnimolDue]) {
           HIP_TRY(p.fire_unimol(c->stream, h[kHdrUnimolDue]));
@@ -183,7 +186,7 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
         reactions += na;
         const uint64_t log_base = std::min<uint64_t>(r.log_count, 0xFFFFFFFFull);
         HIP_TRY(p.products(c->stream, na, (uint32_t)cap, r.log.p, (uint32_t)log_base, (uint32_t)r.log_cap,
-                           c->err.p));
+                           err));
         r.log_count += na;
         HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
         st = read_hdr(c);
@@ -191,21 +194,23 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
         const uint32_t nm = h[kHdrProductMovers];
         if (nm == 0) break;
         snapshot_src = false;
+        phase_movers = nm;
         steps += nm;
-        HIP_TRY(p.move(c->stream, false, p.mlist, p.hdr + kHdrProductMovers, nm, barrier, false, c->err.p));
+        HIP_TRY(p.move(c->stream, false, p.mlist, p.hdr + kHdrProductMovers, nm, barrier, false, err));
       }
     }
-    int st = read_hdr(c);
-    if (st) return st;
+    // the settling loop ends on a header read; the proposal log does not
+    if (propose_only) {
+      int st = read_hdr(c);
+      if (st) return st;
+    }
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->evm0, c->evm1));
     move_ms += ms;
     c->n = h[kHdrNumOut];
     steps += h[kHdrSnapshot];
-    uint32_t e = 0;
-    HIP_TRY(hipMemcpy(&e, c->err.p, 4, hipMemcpyDeviceToHost));
+    const uint32_t e = h[kHdrErr];
     if (e) {
-      HIP_TRY(hipMemset(c->err.p, 0, 4));
       if (e & ERR_ESCAPED) return fail(MCG_E_ESCAPED, "a molecule escaped the partition");
       if (e & ERR_GRID) return fail(MCG_E_LIMIT, "a molecule left the subpartition box of the reaction grid");
       if (e & ERR_QUEUE) return fail(MCG_E_CAPACITY, "molecule capacity exceeded by reaction products");

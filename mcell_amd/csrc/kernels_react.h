@@ -46,6 +46,7 @@ enum : uint32_t {
   kHdrSnapshot = 6,       // molecules in the snapshot (the iteration's movers)
   kHdrLog = 7,            // events logged (snapshot mode)
   kHdrUnimolDue = 8,      // molecules whose unimolecular reaction falls in this phase
+  kHdrErr = 9,            // error bits (ERR_*) of the call's kernels
   kHdrWords = 12,
 };
 

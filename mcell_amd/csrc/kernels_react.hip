@@ -453,10 +453,12 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
                                                 uint32_t* __restrict__ out_tag,
                                                 const RxnRec* __restrict__ rxns, uint32_t* __restrict__ alist,
                                                 uint32_t* __restrict__ acount) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = *pcount;
+  for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < n; q0 += gridDim.x * blockDim.x) {
+  const uint32_t q = q0 + threadIdx.x;
   bool acc = false;
   uint32_t j = 0;
-  if (q < *pcount) {
+  if (q < n) {
     j = plist[q];
     const Proposal& p = prop[j];
     const bool tile = (p.partner_slot & kTileBit) != 0;
@@ -484,6 +486,7 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
     if (lane == leader) base = atomicAdd(acount, (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
     if (acc) alist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
+  }
   }
 }
 
@@ -529,10 +532,12 @@ __global__ __launch_bounds__(256) void k_requeue(const uint32_t* __restrict__ pl
                                                  uint32_t* __restrict__ tile_owner,
                                                  const uint32_t* __restrict__ consumed, uint32_t* __restrict__ rlist,
                                                  uint32_t* __restrict__ rcount) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = *pcount;
+  for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < n; q0 += gridDim.x * blockDim.x) {
+  const uint32_t q = q0 + threadIdx.x;
   bool again = false;
   uint32_t j = 0;
-  if (q < *pcount) {
+  if (q < n) {
     j = plist[q];
     owner[j] = kInvalid;
     const uint32_t ps = prop[j].partner_slot;
@@ -548,6 +553,7 @@ __global__ __launch_bounds__(256) void k_requeue(const uint32_t* __restrict__ pl
     if (lane == leader) base = atomicAdd(rcount, (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
     if (again) rlist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
+  }
   }
 }
 
@@ -827,10 +833,12 @@ hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uin
 
 template <typename PosT>
 hipError_t ReactPipeline<PosT>::settle(hipStream_t s, uint32_t max_props, uint32_t* rlist, uint32_t* rcount) {
-  hipLaunchKernelGGL(k_accept<PosT>, blocks_for(max_props), dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
+  // grid-stride over the device's proposal count; max_props bounds it
+  const dim3 grid = dim3(std::min<uint32_t>(blocks_for(max_props).x, 1024u));
+  hipLaunchKernelGGL(k_accept<PosT>, grid, dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
                      tile_owner, consumed, tile_consumed, M.tag, R.rxns, alist, hdr + kHdrAccepted);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, rcount, 0u, 1u);
-  hipLaunchKernelGGL(k_requeue, blocks_for(max_props), dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
+  hipLaunchKernelGGL(k_requeue, grid, dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
                      tile_owner, consumed, rlist, rcount);
   return hipGetLastError();
 }
