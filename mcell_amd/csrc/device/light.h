@@ -29,13 +29,15 @@ constexpr uint32_t kQueueShards = 2048;
 // movers crowded into a few shards (hard movers cluster near surfaces and the
 // light pass is cell-ordered) still spread over the whole GPU.  Returns the
 // number of queued movers.
-__device__ __forceinline__ uint32_t queue_prefix(const uint32_t* __restrict__ count, uint32_t* pre, uint32_t* part) {
+__device__ __forceinline__ uint32_t queue_prefix(const uint32_t* __restrict__ count, uint32_t nshards, uint32_t* pre,
+                                                uint32_t* part) {
   static_assert(kQueueShards % 256 == 0, "shards per thread");
   constexpr uint32_t kPer = kQueueShards / 256;
   uint32_t local[kPer];
   uint32_t sum = 0;
   for (uint32_t k = 0; k < kPer; ++k) {
-    local[k] = count[threadIdx.x * kPer + k];
+    const uint32_t sh = threadIdx.x * kPer + k;
+    local[k] = sh < nshards ? count[sh] : 0u;
     sum += local[k];
   }
   part[threadIdx.x] = sum;

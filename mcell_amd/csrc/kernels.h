@@ -40,6 +40,7 @@ struct SlowQueue {
   uint32_t* slots;
   uint32_t* count;
   uint32_t shard_cap;
+  uint32_t nshards;  // shards the light pass can have filled: min(kQueueShards, its workgroups)
 };
 constexpr uint32_t kNumQueueShards = 2048;
 uint32_t shard_capacity(uint32_t n);

@@ -10,6 +10,8 @@
 // the register-heavy trace runs only for the few molecules near walls.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device/displace.h"
 #include "device/light.h"
 #include "device/path32.h"
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(256) void k_hard64(MolView64 m, SpeciesTable specie
   __shared__ uint32_t pre[kQueueShards + 1];
   __shared__ uint32_t part[256];
   // the shards read as one list, grid-stride
-  const uint32_t total = queue_prefix(Q.count, pre, part);
+  const uint32_t total = queue_prefix(Q.count, Q.nshards, pre, part);
   if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_ziggurat(T);
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable specie
   __shared__ uint32_t pre[kQueueShards + 1];
   __shared__ uint32_t part[256];
   // the shards read as one list, grid-stride
-  const uint32_t total = queue_prefix(Q.count, pre, part);
+  const uint32_t total = queue_prefix(Q.count, Q.nshards, pre, part);
   if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_ziggurat(T);
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
@@ -207,7 +209,8 @@ hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t 
   hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
   hipLaunchKernelGGL(k_light64, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W, seed, iteration, Q,
                      trace);
-  hipLaunchKernelGGL(k_hard64, dim3(kQueueShards), dim3(256), 0, s, m, species, SC, P, W, seed, iteration, Q, err, trace);
+  Q.nshards = std::min<uint32_t>(kQueueShards, (n + 255) / 256);
+  hipLaunchKernelGGL(k_hard64, dim3(Q.nshards), dim3(256), 0, s, m, species, SC, P, W, seed, iteration, Q, err, trace);
   return hipGetLastError();
 }
 
@@ -225,7 +228,8 @@ hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t 
   hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
   hipLaunchKernelGGL(k_light32, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W32, F, band, seed,
                      iteration, Q, trace);
-  hipLaunchKernelGGL(k_hard32, dim3(kQueueShards), dim3(256), 0, s, m, species, SC, P, W32, seed, iteration, Q, err,
+  Q.nshards = std::min<uint32_t>(kQueueShards, (n + 255) / 256);
+  hipLaunchKernelGGL(k_hard32, dim3(Q.nshards), dim3(256), 0, s, m, species, SC, P, W32, seed, iteration, Q, err,
                      trace);
   return hipGetLastError();
 }

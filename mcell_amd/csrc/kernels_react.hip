@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_move_drain(ReactParams R, Snapshot<PosT
   __shared__ uint32_t sets[SubpartSet::kCap * 256];
   __shared__ uint32_t pre[kQueueShards + 1];
   __shared__ uint32_t part[256];
-  const uint32_t total = queue_prefix(Q.count, pre, part);
+  const uint32_t total = queue_prefix(Q.count, Q.nshards, pre, part);
   if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_zig(T);
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
@@ -820,7 +820,8 @@ hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uin
     hipLaunchKernelGGL(k_fill_u32, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, 0u, kQueueShards);
     hipLaunchKernelGGL(k_move_light<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, M, count, species, barrier,
                        Q);
-    hipLaunchKernelGGL(k_move_drain<PosT>, dim3(kQueueShards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
+    Q.nshards = std::min<uint32_t>(kQueueShards, blocks_for(max_movers).x);
+    hipLaunchKernelGGL(k_move_drain<PosT>, dim3(Q.nshards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
                        propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist,
                        hdr + kHdrUnimolDue);
     return hipGetLastError();
