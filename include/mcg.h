@@ -197,7 +197,8 @@ int mcg_set_bimol_rxns(mcg_ctx* ctx, const mcg_bimol_rxn* rxns, uint32_t num_rxn
  * counterpart in MCell4, where each instance is its own process. */
 int mcg_set_replica_params(mcg_ctx* ctx, uint32_t replica, const mcg_species* species, const mcg_bimol_rxn* rxns);
 
-/* Unimolecular reaction classes (at most one per species). */
+/* Unimolecular reaction classes (at most one per species).  Reaction classes
+ * of every kind are set before the first mcg_step (MCG_E_STATE after it). */
 int mcg_set_unimol_rxns(mcg_ctx* ctx, const mcg_unimol_rxn* rxns, uint32_t num_rxns);
 
 /* A volume + surface reaction class: the volume reactant hits a wall tile

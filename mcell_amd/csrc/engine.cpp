@@ -292,6 +292,7 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
 int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats) {
   if (!c) return fail(MCG_E_INVALID, "null context");
   if (!c->have_geometry || c->num_species == 0) return fail(MCG_E_STATE, "geometry and species must be set first");
+  c->stepped = true;
   if (c->rx.on) return react_step(c, first_iteration, num_iterations, stats);
   auto t0 = std::chrono::steady_clock::now();
   MolView64 m{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p};

@@ -149,6 +149,7 @@ struct mcg_ctx {
   uint32_t last_launches = 0;
 
   mcg::ReactState rx;
+  bool stepped = false;  // reaction classes are fixed once a step has run
 
   ~mcg_ctx();
 };

@@ -67,8 +67,11 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.R.band = 2.0f * p.R.F.e_dd + 8.0f * p.R.F.eps * c->max_abs_coord;
   p.R.SC = c->surface_classes();
   p.species = c->species_table();
-  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p, r.utime.p};
-  p.Sm = RMolView<PosT>{sx, sy, sz, r.sid.p, r.stag.p, r.stoff.p, r.sutime.p};
+  // without unimolecular classes no molecule has a reaction time: the
+  // arrays are neither copied nor read
+  const bool unimol = std::any_of(r.unimol.begin(), r.unimol.end(), [](const UnimolRec& u) { return u.num_pathways > 0; });
+  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p, unimol ? r.utime.p : nullptr};
+  p.Sm = RMolView<PosT>{sx, sy, sz, r.sid.p, r.stag.p, r.stoff.p, unimol ? r.sutime.p : nullptr};
   p.R.unimol = r.unimol_d.p;
   const bool surf = r.total_tiles && !r.vs.empty();
   p.R.grids = surf ? r.grids_d.p : nullptr;
@@ -349,6 +352,7 @@ extern "C" {
 int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  if (c->stepped) return fail(MCG_E_STATE, "reaction classes must be set before the first step");
   if (!c->rx.rxns.empty()) return fail(MCG_E_STATE, "reaction classes are already set");
   if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
   if (n == 0) return MCG_OK;
@@ -406,6 +410,7 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
 int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  if (c->stepped) return fail(MCG_E_STATE, "reaction classes must be set before the first step");
   if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
   const uint32_t ns = c->num_species;
   std::vector<UnimolRec> u(ns);
@@ -586,6 +591,7 @@ extern "C" {
 int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  if (c->stepped) return fail(MCG_E_STATE, "reaction classes must be set before the first step");
   if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
   ReactState& r = c->rx;
   if (!r.vs.empty()) return fail(MCG_E_STATE, "volume-surface reaction classes are already set");

@@ -20,7 +20,7 @@ struct RMolView {
   uint32_t* id;
   uint32_t* tag;
   double* toff;
-  double* utime;  // unimolecular reaction time (absolute) or kTimeInvalid
+  double* utime;  // unimolecular reaction time (absolute) or kTimeInvalid; null without unimolecular classes
 };
 
 // One carried-out (or, in the snapshot mode, proposed) reaction: 64 bytes,

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, c
   s.id[j] = m.id[i];
   s.tag[j] = m.tag[i];
   s.toff[j] = m.toff[i];
-  s.utime[j] = m.utime[i];
+  if (s.utime) s.utime[j] = m.utime[i];
   consumed[j] = 0;
   owner[j] = kInvalid;
 }
@@ -157,7 +157,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   const uint32_t tag = src.tag[j];
   const uint32_t idv = src.id[j];
   const double toff = src.toff[j];
-  const double utime = src.utime[j];
+  const double utime = src.utime ? src.utime[j] : kTimeInvalid;
   const SpeciesRec sp = species.of(tag);
   D3 pos = {(double)src.x[j], (double)src.y[j], (double)src.z[j]};
   uint32_t err = 0;
@@ -205,7 +205,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
     const uint32_t t2 = tag & ~(MF_SCHEDULE_UNIMOL << 8);
     out.tag[j] = e.absorbed ? (t2 | (MF_DEFUNCT << 8)) : t2;
     out.toff[j] = sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff;
-    out.utime[j] = e.utime;
+    if (out.utime) out.utime[j] = e.utime;
   }
   if (err) atomicOr(err_out, err);
   return proposed;
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
     const uint32_t tag = S.tag[j];
     const uint32_t idv = S.id[j];
     const double toff = S.toff[j];
-    const double utime = S.utime[j];
+    const double utime = S.utime ? S.utime[j] : kTimeInvalid;
     const SpeciesRec sp = species.of(tag);
     const D3 pos = {(double)S.x[j], (double)S.y[j], (double)S.z[j]};
     // molecules with a unimolecular class take the full step when their
@@ -428,7 +428,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         out.id[j] = idv;
         out.tag[j] = tag & ~(MF_SCHEDULE_UNIMOL << 8);
         out.toff[j] = next_toff(R.iteration, toff, t_steps);
-        out.utime[j] = utime;
+        if (out.utime) out.utime[j] = utime;
       } else {
         hard = true;
       }
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
       out.id[j] = idv;
       out.tag[j] = tag & ~(MF_SCHEDULE_UNIMOL << 8);
       out.toff[j] = toff;
-      out.utime[j] = utime;
+      if (out.utime) out.utime[j] = utime;
     }
   }
   const uint32_t shard = blockIdx.x % kQueueShards;
@@ -668,7 +668,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
     out.id[slot] = first_id + i;
     out.tag[slot] = make_tag(prods[i], MF_SCHEDULE_UNIMOL, rep);
     out.toff[slot] = p.time - (double)(R.iteration + (moves_now ? 0u : 1u));
-    out.utime[slot] = kTimeInvalid;
+    if (out.utime) out.utime[slot] = kTimeInvalid;
     consumed[slot] = 0;
     owner[slot] = kInvalid;
     if (moves_now) mlist[atomicAdd(mcount, 1u)] = slot;
