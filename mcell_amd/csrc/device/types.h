@@ -77,6 +77,7 @@ struct SpeciesTable {
 enum : uint32_t {
   MF_DEFUNCT = 1u << 0,
   MF_SCHEDULE_UNIMOL = 1u << 1,  // lifetime not drawn yet (MOLECULE_FLAG_SCHEDULE_UNIMOL_RXN)
+  MF_TOFF = 1u << 2,             // reactive path: the diffusion-time offset array holds a non-zero value
 };
 constexpr double kTimeInvalid = -256.0;   // TIME_INVALID (defines.h:144)
 constexpr double kTimeForever = 1e20;     // TIME_FOREVER (defines.h:145)

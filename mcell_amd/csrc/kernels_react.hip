@@ -84,8 +84,9 @@ __global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, c
   s.y[j] = m.y[i];
   s.z[j] = m.z[i];
   s.id[j] = m.id[i];
-  s.tag[j] = m.tag[i];
-  s.toff[j] = m.toff[i];
+  const uint32_t tag = m.tag[i];
+  s.tag[j] = tag;
+  if (tag_flags(tag) & MF_TOFF) s.toff[j] = m.toff[i];
   if (s.utime) s.utime[j] = m.utime[i];
   consumed[j] = 0;
   owner[j] = kInvalid;
@@ -125,6 +126,20 @@ __device__ __forceinline__ double next_toff(uint32_t iteration, double toff, dou
   return t - (double)(iteration + 1);
 }
 
+// Diffusion-time offsets are zero for every molecule except products before
+// their next barrier; the offset array is only read and written for tags
+// that carry MF_TOFF.
+__device__ __forceinline__ double load_toff(const double* __restrict__ toff, uint32_t tag, uint32_t j) {
+  return (tag_flags(tag) & MF_TOFF) ? toff[j] : 0.0;
+}
+__device__ __forceinline__ uint32_t store_toff(double* __restrict__ toff, uint32_t j, double v, uint32_t tag) {
+  if (v != 0.0) {
+    toff[j] = v;
+    return tag | (MF_TOFF << 8);
+  }
+  return tag & ~(MF_TOFF << 8);
+}
+
 __device__ __forceinline__ void stage_zig(ZigguratTables& T) {
   const uint32_t* zs = reinterpret_cast<const uint32_t*>(&c_zig_r);
   uint32_t* zd = reinterpret_cast<uint32_t*>(&T);
@@ -156,7 +171,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
                                          bool& due, uint32_t* set_lds) {
   const uint32_t tag = src.tag[j];
   const uint32_t idv = src.id[j];
-  const double toff = src.toff[j];
+  const double toff = load_toff(src.toff, tag, j);
   const double utime = src.utime ? src.utime[j] : kTimeInvalid;
   const SpeciesRec sp = species.of(tag);
   D3 pos = {(double)src.x[j], (double)src.y[j], (double)src.z[j]};
@@ -202,9 +217,9 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   if (!proposed || propose_only) {
     store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
     out.id[j] = idv;
-    const uint32_t t2 = tag & ~(MF_SCHEDULE_UNIMOL << 8);
+    uint32_t t2 = tag & ~(MF_SCHEDULE_UNIMOL << 8);
+    t2 = store_toff(out.toff, j, sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff, t2);
     out.tag[j] = e.absorbed ? (t2 | (MF_DEFUNCT << 8)) : t2;
-    out.toff[j] = sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff;
     if (out.utime) out.utime[j] = e.utime;
   }
   if (err) atomicOr(err_out, err);
@@ -297,7 +312,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
   if (j < n) {
     const uint32_t tag = S.tag[j];
     const uint32_t idv = S.id[j];
-    const double toff = S.toff[j];
+    const double toff = load_toff(S.toff, tag, j);
     const double utime = S.utime ? S.utime[j] : kTimeInvalid;
     const SpeciesRec sp = species.of(tag);
     const D3 pos = {(double)S.x[j], (double)S.y[j], (double)S.z[j]};
@@ -426,8 +441,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         // an end point next to a wall plane keeps its side in FP32
         store_pos(R, out.x, out.y, out.z, j, end, near_plane);
         out.id[j] = idv;
-        out.tag[j] = tag & ~(MF_SCHEDULE_UNIMOL << 8);
-        out.toff[j] = next_toff(R.iteration, toff, t_steps);
+        out.tag[j] = store_toff(out.toff, j, next_toff(R.iteration, toff, t_steps), tag & ~(MF_SCHEDULE_UNIMOL << 8));
         if (out.utime) out.utime[j] = utime;
       } else {
         hard = true;
@@ -435,8 +449,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
     } else {
       store_pos(R, out.x, out.y, out.z, j, pos, false);
       out.id[j] = idv;
-      out.tag[j] = tag & ~(MF_SCHEDULE_UNIMOL << 8);
-      out.toff[j] = toff;
+      out.tag[j] = store_toff(out.toff, j, toff, tag & ~(MF_SCHEDULE_UNIMOL << 8));
       if (out.utime) out.utime[j] = utime;
     }
   }
@@ -666,8 +679,8 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
     }
     store_pos(R, out.x, out.y, out.z, slot, D3{p.x, p.y, p.z}, true);
     out.id[slot] = first_id + i;
-    out.tag[slot] = make_tag(prods[i], MF_SCHEDULE_UNIMOL, rep);
-    out.toff[slot] = p.time - (double)(R.iteration + (moves_now ? 0u : 1u));
+    out.tag[slot] = store_toff(out.toff, slot, p.time - (double)(R.iteration + (moves_now ? 0u : 1u)),
+                               make_tag(prods[i], MF_SCHEDULE_UNIMOL, rep));
     if (out.utime) out.utime[slot] = kTimeInvalid;
     consumed[slot] = 0;
     owner[slot] = kInvalid;
