@@ -73,8 +73,7 @@ __global__ __launch_bounds__(256) void k_cell_keys(RMolView<PosT> m, uint32_t n,
 template <typename PosT>
 __global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, const uint32_t* __restrict__ key,
                                                  const uint32_t* __restrict__ rank,
-                                                 const uint32_t* __restrict__ cell_start, RMolView<PosT> s,
-                                                 uint32_t* __restrict__ consumed, uint32_t* __restrict__ owner) {
+                                                 const uint32_t* __restrict__ cell_start, RMolView<PosT> s) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t c = key[i];
@@ -88,8 +87,6 @@ __global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, c
   s.tag[j] = tag;
   if (tag_flags(tag) & MF_TOFF) s.toff[j] = m.toff[i];
   if (s.utime) s.utime[j] = m.utime[i];
-  consumed[j] = 0;
-  owner[j] = kInvalid;
 }
 
 // ----------------------------------------------------------------- moves --
@@ -804,9 +801,12 @@ hipError_t ReactPipeline<PosT>::sort(hipStream_t s, uint32_t n_in, uint32_t num_
   size_t tmp = scan_tmp_bytes;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cell_count, cell_start, num_cells + 1, s);
   if (e != hipSuccess) return e;
-  if (n_in)
-    hipLaunchKernelGGL(k_scatter<PosT>, blocks_for(n_in), dim3(256), 0, s, M, n_in, key, rank, cell_start, Sm, consumed,
-                       owner);
+  if (n_in) {
+    hipLaunchKernelGGL(k_scatter<PosT>, blocks_for(n_in), dim3(256), 0, s, M, n_in, key, rank, cell_start, Sm);
+    // claim words of the snapshot's slots (n_in bounds it), written in order
+    hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, consumed, 0u, n_in);
+    hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, owner, kInvalid, n_in);
+  }
   return hipGetLastError();
 }
 
