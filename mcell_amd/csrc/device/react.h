@@ -180,6 +180,32 @@ __device__ __forceinline__ void collect_molecule_subparts(const PartitionParams&
   if (out.overflow) err |= ERR_CELLS;
 }
 
+// The box [lo, hi] partners of a segment must lie in (its bounding box
+// widened by the reaction radius), compared in the snapshot's own precision:
+// for FP32 positions the bounds are rounded outwards, so a partner the FP32
+// compare rejects lies outside the FP64 box too.
+template <typename PosT>
+struct PartnerBox;
+template <>
+struct PartnerBox<double> {
+  double x0, x1, y0, y1, z0, z1;
+  __device__ __forceinline__ PartnerBox(double ax0, double ax1, double ay0, double ay1, double az0, double az1)
+      : x0(ax0), x1(ax1), y0(ay0), y1(ay1), z0(az0), z1(az1) {}
+  __device__ __forceinline__ bool outside(double x, double y, double z) const {
+    return x < x0 || x > x1 || y < y0 || y > y1 || z < z0 || z > z1;
+  }
+};
+template <>
+struct PartnerBox<float> {
+  float x0, x1, y0, y1, z0, z1;
+  __device__ __forceinline__ PartnerBox(double ax0, double ax1, double ay0, double ay1, double az0, double az1)
+      : x0(__double2float_rd(ax0)), x1(__double2float_ru(ax1)), y0(__double2float_rd(ay0)),
+        y1(__double2float_ru(ay1)), z0(__double2float_rd(az0)), z1(__double2float_ru(az1)) {}
+  __device__ __forceinline__ bool outside(float x, float y, float z) const {
+    return x < x0 || x > x1 || y < y0 || y > y1 || z < z0 || z > z1;
+  }
+};
+
 // collide_mol (collision_utils.inl:464-515) without the defunct/self checks
 __device__ __forceinline__ bool collide_mol64(D3 pos, D3 disp, D3 other, double sigma2, double& t, D3& cpos) {
   D3 dir = {other.x - pos.x, other.y - pos.y, other.z - pos.z};

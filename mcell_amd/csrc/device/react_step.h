@@ -97,6 +97,12 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
   bool found = false;
   uint32_t id_best = 0;
   candidates = 0;
+  // partners farther than the radius from the segment's box cannot collide
+  const double R1 = R.P.rxn_radius;
+  const double ex = pos.x + disp.x, ey = pos.y + disp.y, ez = pos.z + disp.z;
+  const PartnerBox<PosT> box((pos.x < ex ? pos.x : ex) - R1, (pos.x < ex ? ex : pos.x) + R1,
+                             (pos.y < ey ? pos.y : ey) - R1, (pos.y < ey ? ey : pos.y) + R1,
+                             (pos.z < ez ? pos.z : ez) - R1, (pos.z < ez ? ez : pos.z) + R1);
   for (int k = 0; k < set.n; ++k) {
     const uint32_t s = set.at(k);
     const int gx = (int)(s & 1023u), gy = (int)((s >> 10) & 1023u), gz = (int)(s >> 20);
@@ -108,9 +114,11 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
     const uint32_t b = bin_of(R.G, replica, t, c);
     const uint32_t j1 = R.G.start[b + 1];
     for (uint32_t j = R.G.start[b]; j < j1; ++j) {
+      const PosT px = S.x[j], py = S.y[j], pz = S.z[j];
+      if (box.outside(px, py, pz)) continue;
       double t;
       D3 cp;
-      if (!collide_mol64(pos, disp, D3{(double)S.x[j], (double)S.y[j], (double)S.z[j]}, sigma2, t, cp)) continue;
+      if (!collide_mol64(pos, disp, D3{(double)px, (double)py, (double)pz}, sigma2, t, cp)) continue;
       if (!(t < t_limit)) continue;
       const uint32_t pid = S.id[j];
       if (pid == my_id || S.consumed[j]) continue;

@@ -352,9 +352,9 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         const uint32_t my_sp = tag_species(tag), rep = tag_replica(tag);
         // the segment's box widened by the radius: partners outside it cannot collide
         const double R1 = R.P.rxn_radius;
-        const double bx0 = (a[0] < b[0] ? a[0] : b[0]) - R1, bx1 = (a[0] < b[0] ? b[0] : a[0]) + R1;
-        const double by0 = (a[1] < b[1] ? a[1] : b[1]) - R1, by1 = (a[1] < b[1] ? b[1] : a[1]) + R1;
-        const double bz0 = (a[2] < b[2] ? a[2] : b[2]) - R1, bz1 = (a[2] < b[2] ? b[2] : a[2]) + R1;
+        const PartnerBox<PosT> box((a[0] < b[0] ? a[0] : b[0]) - R1, (a[0] < b[0] ? b[0] : a[0]) + R1,
+                                   (a[1] < b[1] ? a[1] : b[1]) - R1, (a[1] < b[1] ? b[1] : a[1]) + R1,
+                                   (a[2] < b[2] ? a[2] : b[2]) - R1, (a[2] < b[2] ? b[2] : a[2]) + R1);
         // up to two collisions are resolved here when both partners sit in the
         // start or end subpartition (cells every collection of the reference holds)
         int ss[3], se[3];
@@ -378,8 +378,9 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
               const uint32_t b0 = bin_of(R.G, rep, t, c0);
               const uint32_t k1 = R.G.start[b0 + (uint32_t)nx + 1];
               for (uint32_t k = R.G.start[b0]; k < k1; ++k) {
-                const double qx = (double)S.x[k], qy = (double)S.y[k], qz = (double)S.z[k];
-                if (qx < bx0 || qx > bx1 || qy < by0 || qy > by1 || qz < bz0 || qz > bz1) continue;
+                const PosT px = S.x[k], py = S.y[k], pz = S.z[k];
+                if (box.outside(px, py, pz)) continue;
+                const double qx = (double)px, qy = (double)py, qz = (double)pz;
                 double tc;
                 D3 cp;
                 if (!collide_mol64(pos, disp, D3{qx, qy, qz}, sigma2, tc, cp)) continue;
