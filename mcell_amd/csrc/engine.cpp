@@ -113,7 +113,8 @@ int mcg_create(const mcg_config* cfg, mcg_ctx** out) {
   bool pos_fail = c->fp32() ? (c->xf.alloc(cap) || c->yf.alloc(cap) || c->zf.alloc(cap))
                             : (c->x.alloc(cap) || c->y.alloc(cap) || c->z.alloc(cap));
   if (pos_fail || c->id.alloc(cap) || c->tag.alloc(cap) || c->err.alloc(1) ||
-      c->queue.alloc((size_t)kNumQueueShards * shard_capacity((uint32_t)cap)) || c->qcount.alloc(kNumQueueShards))
+      c->queue.alloc((size_t)kNumQueueShards * shard_capacity((uint32_t)cap, kNumQueueShards / 2)) ||
+      c->qcount.alloc(kNumQueueShards))
     return cleanup_fail(fail(MCG_E_NOMEM, "molecule buffers"));
   double m = 0;
   for (double v : {P.ox, P.oy, P.oz, P.qx, P.qy, P.qz}) m = std::max(m, std::fabs(v));

@@ -43,7 +43,9 @@ struct SlowQueue {
   uint32_t nshards;  // shards the light pass can have filled: min(kQueueShards, its workgroups)
 };
 constexpr uint32_t kNumQueueShards = 2048;
-uint32_t shard_capacity(uint32_t n);
+// Slots per shard when n movers' workgroups append round-robin to
+// `shards` shards (all of them, or one half per kind of mover).
+uint32_t shard_capacity(uint32_t n, uint32_t shards = kNumQueueShards);
 
 hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, SpeciesTable species,
                                     const SurfaceClasses& SC,

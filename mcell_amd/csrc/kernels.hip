@@ -194,9 +194,9 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 static_assert(kNumQueueShards == kQueueShards, "host and device shard counts differ");
 static_assert(kQueueShards % 256 == 0, "counter reset grid");
 
-uint32_t shard_capacity(uint32_t n) {
+uint32_t shard_capacity(uint32_t n, uint32_t shards) {
   uint32_t blocks = (n + 255) / 256;
-  return 256u * ((blocks + kQueueShards - 1) / kQueueShards);
+  return 256u * ((blocks + shards - 1) / shards);
 }
 
 

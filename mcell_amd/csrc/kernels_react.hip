@@ -305,7 +305,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
   stage_zig(T);
   const uint32_t n = *count;
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  bool hard = false;
+  bool hard = false, partner_hard = false;
   if (j < n) {
     const uint32_t tag = S.tag[j];
     const uint32_t idv = S.id[j];
@@ -334,6 +334,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         if (kF32) clear = all_walls_miss32(R.P, R.ent32, R.off, R.ent_box, R.F, R.band, pos, disp, lo, hi, near_plane);
         else clear = all_walls_miss64(R.P, R.ent64, R.off, R.ent_box, pos, disp, lo, hi);
       }
+      const bool walls_clear = clear;
       if (clear && (sp.flags & SPF_CAN_VOLVOL) && !(sp.flags & SPF_TARGET_ONLY)) {
         // partners within the reaction radius of the segment: every one lies
         // in the cells of the segment's box widened by the radius
@@ -443,6 +444,7 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
         if (out.utime) out.utime[j] = utime;
       } else {
         hard = true;
+        partner_hard = walls_clear;
       }
     } else {
       store_pos(R, out.x, out.y, out.z, j, pos, false);
@@ -451,8 +453,13 @@ __global__ __launch_bounds__(256) void k_move_light(ReactParams R, Snapshot<PosT
       if (out.utime) out.utime[j] = utime;
     }
   }
-  const uint32_t shard = blockIdx.x % kQueueShards;
-  wave_append(hard, j, Q.slots + (size_t)shard * Q.shard_cap, Q.count + shard);
+  // movers queued for walls (and unimolecular events) and movers queued for
+  // partners go to separate halves of the shards, so that the full step's
+  // lanes mostly follow the same path
+  constexpr uint32_t kHalf = kQueueShards / 2;
+  const uint32_t s0 = blockIdx.x % kHalf, s1 = kHalf + blockIdx.x % kHalf;
+  wave_append(hard && !partner_hard, j, Q.slots + (size_t)s0 * Q.shard_cap, Q.count + s0);
+  wave_append(hard && partner_hard, j, Q.slots + (size_t)s1 * Q.shard_cap, Q.count + s1);
 }
 
 // A proposal whose mover and partner both carry its claim is accepted.
@@ -830,11 +837,11 @@ hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uin
   if (list == nullptr && snapshot_src) {
     // every snapshot slot: light pass, then the queued movers in full
     SlowQueue Q = queue;
-    Q.shard_cap = shard_capacity(max_movers);
+    Q.shard_cap = shard_capacity(max_movers, kQueueShards / 2);
     hipLaunchKernelGGL(k_fill_u32, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, 0u, kQueueShards);
     hipLaunchKernelGGL(k_move_light<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, M, count, species, barrier,
                        Q);
-    Q.nshards = std::min<uint32_t>(kQueueShards, blocks_for(max_movers).x);
+    Q.nshards = kQueueShards / 2 + std::min<uint32_t>(kQueueShards / 2, blocks_for(max_movers).x);
     hipLaunchKernelGGL(k_move_drain<PosT>, dim3(Q.nshards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
                        propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist,
                        hdr + kHdrUnimolDue);
