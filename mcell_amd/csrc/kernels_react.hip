@@ -1,4 +1,5 @@
-// HIP kernels of one diffuse-react iteration with volume-volume reactions.
+// HIP kernels of one diffuse-react iteration with reaction classes
+// (volume-volume, volume-surface and unimolecular).
 //
 // The reference moves molecules one after another, each seeing the others
 // where they are at that moment (diffuse_react_event.cpp:53-161).  Here all
@@ -6,19 +7,28 @@
 // iteration's start, and reactions are settled afterwards:
 //
 //   k_cell_keys / scan / k_scatter   counting sort of the live molecules by
-//                                    (replica, subpartition) into the snapshot
-//   k_move                           every mover's step (react_step.h); a mover
-//                                    whose reaction test succeeds stops with a
-//                                    proposal and claims itself and its partner
-//                                    (atomicMin of its id)
+//                                    bin = (replica, species, subpartition)
+//                                    into the snapshot
+//   k_move_light                     every mover: the displacement; a move that
+//                                    certainly meets no wall commits here, after
+//                                    up to two molecule collisions resolved in
+//                                    place; the others are queued (wall movers
+//                                    and partner movers in separate halves)
+//   k_move_drain, k_move             the full step of react_step.h for queued
+//                                    movers (and later for reruns and product
+//                                    movers); a mover whose reaction test
+//                                    succeeds stops with a proposal and claims
+//                                    itself and its partner (atomicMin of its id)
 //   k_accept                         a proposal is accepted when it holds both
 //                                    claims; its reactants are consumed
 //   k_requeue                        claims are released; movers whose proposal
 //                                    lost and that were not consumed move again
 //                                    (same stream, consumed partners skipped)
+//   k_unimol_fire                    due unimolecular events of unconsumed molecules
 //   products                         accepted reactions in (replica, mover id)
 //                                    order get product ids per replica; products
-//                                    move for the rest of the step as new movers
+//                                    move for the rest of the step as new movers;
+//                                    a surface product takes its receptor's tile
 //
 // The smallest mover id among open proposals always wins its claims, so every
 // round settles at least one proposal.  Molecules leave the snapshot order in
