@@ -133,6 +133,20 @@ __device__ __forceinline__ double next_toff(uint32_t iteration, double toff, dou
   return t - (double)(iteration + 1);
 }
 
+// The snapshot range of species t's molecules in the row of cells
+// (x0 .. x0 + nx, y, z) of replica rep (bins of one row are consecutive).
+__device__ __forceinline__ void row_range(const CellGrid& G, uint32_t rep, uint32_t t, int x0, int y, int z, int nx,
+                                          uint32_t& k0, uint32_t& k1) {
+  uint32_t c0;
+  if (!cell_of(G, x0, y, z, c0)) {
+    k0 = k1 = 0;
+    return;
+  }
+  const uint32_t b0 = bin_of(G, rep, t, c0);
+  k0 = G.start[b0];
+  k1 = G.start[b0 + (uint32_t)nx + 1];
+}
+
 // Diffusion-time offsets are zero for every molecule except products before
 // their next barrier; the offset array is only read and written for tags
 // that carry MF_TOFF.
@@ -382,21 +396,45 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
         int32_t crx[2];
         D3 ccp[2];
         bool cok[2];
-        // one species' partners in a row of cells along x are one range of the snapshot
+        // one species' partners in a row of cells along x are one range of the
+        // snapshot; the next row's range is loaded while the current row's
+        // partners are tested, and partner positions are loaded kGroup at a
+        // time, so that the dependent loads overlap (the candidates' order
+        // does not matter: at most two are kept and they are sorted below)
+        constexpr int kGroup = kF32 ? 4 : 2;
         const int nx = chi[0] - clo[0];
-        for (int z = clo[2]; z <= chi[2] && clear; ++z)
-          for (int y = clo[1]; y <= chi[1] && clear; ++y) {
-            uint32_t c0;
-            if (nx < 0 || !cell_of(R.G, clo[0], y, z, c0)) continue;
-            for (uint32_t t = 0; t < R.G.ns && clear; ++t) {
-              const int32_t rx = R.pair_rxn[my_sp * R.num_species + t];
-              if (rx < 0) continue;
-              const uint32_t b0 = bin_of(R.G, rep, t, c0);
-              const uint32_t k1 = R.G.start[b0 + (uint32_t)nx + 1];
-              for (uint32_t k = R.G.start[b0]; k < k1; ++k) {
-                const PosT px = S.x[k], py = S.y[k], pz = S.z[k];
-                if (box.outside(px, py, pz)) continue;
-                const double qx = (double)px, qy = (double)py, qz = (double)pz;
+        const bool rows = nx >= 0 && chi[1] >= clo[1] && chi[2] >= clo[2];
+        for (uint32_t t = 0; t < R.G.ns && rows && clear; ++t) {
+          const int32_t rx = R.pair_rxn[my_sp * R.num_species + t];
+          if (rx < 0) continue;
+          int y = clo[1], z = clo[2];
+          uint32_t n0, n1;
+          row_range(R.G, rep, t, clo[0], y, z, nx, n0, n1);
+          while (clear) {
+            const uint32_t k0 = n0, k1 = n1;
+            const bool last = y == chi[1] && z == chi[2];
+            if (!last) {
+              if (++y > chi[1]) {
+                y = clo[1];
+                ++z;
+              }
+              row_range(R.G, rep, t, clo[0], y, z, nx, n0, n1);
+            }
+            for (uint32_t kb = k0; kb < k1 && clear; kb += kGroup) {
+              PosT gx[kGroup], gy[kGroup], gz[kGroup];
+#pragma unroll
+              for (int u = 0; u < kGroup; ++u) {
+                const uint32_t k = kb + u < k1 ? kb + u : k1 - 1;
+                gx[u] = S.x[k];
+                gy[u] = S.y[k];
+                gz[u] = S.z[k];
+              }
+#pragma unroll
+              for (int u = 0; u < kGroup; ++u) {
+                const uint32_t k = kb + u;
+                if (k >= k1 || !clear) break;
+                if (box.outside(gx[u], gy[u], gz[u])) continue;
+                const double qx = (double)gx[u], qy = (double)gy[u], qz = (double)gz[u];
                 double tc;
                 D3 cp;
                 if (!collide_mol64(pos, disp, D3{qx, qy, qz}, sigma2, tc, cp)) continue;
@@ -416,7 +454,9 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
                 ++ncand;
               }
             }
+            if (last) break;
           }
+        }
         if (clear && ncand > 0) {
           // the reference's order: time, then the larger partner id first
           if (ncand == 2 && (ct[1] < ct[0] || (ct[1] == ct[0] && cid[1] > cid[0]))) {
