@@ -84,6 +84,22 @@ __device__ __forceinline__ bool closest_wall(const ReactParams& R, uint32_t s, u
   return closest_wall_in_subpart64(R.P, R.walls, R.off, R.idx, R.ent_box, s, last_hit, pos, disp, rng, hit, err);
 }
 
+// The snapshot range of species t's molecules in the k-th subpartition of the
+// set (empty outside the grid: no molecules there).
+__device__ __forceinline__ void set_range(const CellGrid& G, const SubpartSet& set, int k, uint32_t replica,
+                                          uint32_t t, uint32_t& j0, uint32_t& j1) {
+  const uint32_t s = set.at(k);
+  const int gx = (int)(s & 1023u), gy = (int)((s >> 10) & 1023u), gz = (int)(s >> 20);
+  uint32_t c;
+  if (!cell_of(G, gx, gy, gz, c)) {
+    j0 = j1 = 0;
+    return;
+  }
+  const uint32_t b = bin_of(G, replica, t, c);
+  j0 = G.start[b];
+  j1 = G.start[b + 1];
+}
+
 // The next molecule collision after key (t_last, id_last) in time order
 // (ties: larger partner id first, diffuse_react_event.cpp:354-358), strictly
 // before t_limit.  Scans the partner lists of the collected subpartitions.
@@ -103,37 +119,51 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
   const PartnerBox<PosT> box((pos.x < ex ? pos.x : ex) - R1, (pos.x < ex ? ex : pos.x) + R1,
                              (pos.y < ey ? pos.y : ey) - R1, (pos.y < ey ? ey : pos.y) + R1,
                              (pos.z < ez ? pos.z : ez) - R1, (pos.z < ez ? ez : pos.z) + R1);
-  for (int k = 0; k < set.n; ++k) {
-    const uint32_t s = set.at(k);
-    const int gx = (int)(s & 1023u), gy = (int)((s >> 10) & 1023u), gz = (int)(s >> 20);
-    uint32_t c;
-    if (!cell_of(R.G, gx, gy, gz, c)) continue;  // no molecules outside the grid
-    for (uint32_t t = 0; t < R.G.ns; ++t) {
+  // species by species; the next subpartition's range is loaded while the
+  // current one's partners are tested, and partner positions two at a time
+  // (the result does not depend on the order: the key is a total order)
+  constexpr int kGroup = 2;
+  for (uint32_t t = 0; t < R.G.ns && set.n > 0; ++t) {
     const int32_t rx = R.pair_rxn[my_species * R.num_species + t];
     if (rx < 0) continue;
-    const uint32_t b = bin_of(R.G, replica, t, c);
-    const uint32_t j1 = R.G.start[b + 1];
-    for (uint32_t j = R.G.start[b]; j < j1; ++j) {
-      const PosT px = S.x[j], py = S.y[j], pz = S.z[j];
-      if (box.outside(px, py, pz)) continue;
-      double t;
-      D3 cp;
-      if (!collide_mol64(pos, disp, D3{(double)px, (double)py, (double)pz}, sigma2, t, cp)) continue;
-      if (!(t < t_limit)) continue;
-      const uint32_t pid = S.id[j];
-      if (pid == my_id || S.consumed[j]) continue;
-      // strictly after the last processed key
-      if (!first && !(t > t_last || (t == t_last && pid < id_last))) continue;
-      ++candidates;
-      if (!found || t < t_best || (t == t_best && pid > id_best)) {
-        found = true;
-        t_best = t;
-        id_best = pid;
-        slot_best = j;
-        pos_best = cp;
-        rxn_best = rx;
+    uint32_t n0, n1;
+    set_range(R.G, set, 0, replica, t, n0, n1);
+    for (int k = 0; k < set.n; ++k) {
+      const uint32_t j0 = n0, j1 = n1;
+      if (k + 1 < set.n) set_range(R.G, set, k + 1, replica, t, n0, n1);
+      for (uint32_t jb = j0; jb < j1; jb += kGroup) {
+        PosT gx[kGroup], gy[kGroup], gz[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+          const uint32_t j = jb + u < j1 ? jb + u : j1 - 1;
+          gx[u] = S.x[j];
+          gy[u] = S.y[j];
+          gz[u] = S.z[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+          const uint32_t j = jb + u;
+          if (j >= j1) break;
+          if (box.outside(gx[u], gy[u], gz[u])) continue;
+          double tt;
+          D3 cp;
+          if (!collide_mol64(pos, disp, D3{(double)gx[u], (double)gy[u], (double)gz[u]}, sigma2, tt, cp)) continue;
+          if (!(tt < t_limit)) continue;
+          const uint32_t pid = S.id[j];
+          if (pid == my_id || S.consumed[j]) continue;
+          // strictly after the last processed key
+          if (!first && !(tt > t_last || (tt == t_last && pid < id_last))) continue;
+          ++candidates;
+          if (!found || tt < t_best || (tt == t_best && pid > id_best)) {
+            found = true;
+            t_best = tt;
+            id_best = pid;
+            slot_best = j;
+            pos_best = cp;
+            rxn_best = rx;
+          }
+        }
       }
-    }
     }
   }
   return found;
