@@ -352,8 +352,8 @@ __device__ __noinline__ double exact_disk_dev(const PartitionParams& PP, const W
   // the walls the reference keeps lie within R of loc on every axis (its
   // bounding-box test below), so the cube loc +- R bounds their entry boxes
   const uint32_t db = region_box(PP, csp, loc.x - R, loc.y - R, loc.z - R, loc.x + R, loc.y + R, loc.z + R);
-  for (uint32_t k = off[csp]; k < off[csp + 1]; ++k) {
-    if (!boxes_overlap(box[k], db)) continue;
+  const uint32_t e1 = off[csp + 1];
+  for (uint32_t k = next_overlap(box, off[csp], e1, db); k < e1; k = next_overlap(box, k + 1, e1, db)) {
     const WallRec64& w = walls[idx[k]];
     D3 wv[3] = {{w.p0x, w.p0y, w.p0z}, {w.p1x, w.p1y, w.p1z}, {w.p2x, w.p2y, w.p2z}};
     D3 n = {w.nx, w.ny, w.nz};

@@ -42,8 +42,8 @@ __device__ __forceinline__ void round_to_fp32_same_side(const PartitionParams& P
     const uint32_t pb = segment_box(P, s, px, py, pz, 0.0, 0.0, 0.0);
     for (int pass = 0; pass < 4; ++pass) {
       bool moved = false;
-      for (uint32_t k = off[s]; k < off[s + 1]; ++k) {
-        if (!boxes_overlap(box[k], pb)) continue;
+      const uint32_t e1 = off[s + 1];
+      for (uint32_t k = next_overlap(box, off[s], e1, pb); k < e1; k = next_overlap(box, k + 1, e1, pb)) {
         const WallRec64& w = walls[idx[k]];
         double sd = dot3(D3{w.nx, w.ny, w.nz}, D3{px, py, pz}) - w.d;
         double sf = dot3(D3{w.nx, w.ny, w.nz}, D3{(double)fx, (double)fy, (double)fz}) - w.d;

@@ -293,8 +293,7 @@ __device__ __forceinline__ bool closest_wall_in_subpart64(const PartitionParams&
     double closest = 1e20;  // TIME_FOREVER
     bool restarted = false;
     const uint32_t sb = segment_box(P, subpart, pos.x, pos.y, pos.z, move.x, move.y, move.z);
-    for (uint32_t k = k0; k < k1; ++k) {
-      if (!boxes_overlap(box[k], sb)) continue;
+    for (uint32_t k = next_overlap(box, k0, k1, sb); k < k1; k = next_overlap(box, k + 1, k1, sb)) {
       uint32_t wi = idx[k];
       if (wi == last_hit) continue;
       double t;

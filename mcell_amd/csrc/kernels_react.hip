@@ -479,12 +479,11 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
             const uint32_t csp = subpart_of(R.P, qx, qy, qz);
             const uint32_t db = region_box(R.P, csp, ccp[q].x - R1, ccp[q].y - R1, ccp[q].z - R1, ccp[q].x + R1,
                                            ccp[q].y + R1, ccp[q].z + R1);
-            for (uint32_t k = R.off[csp]; k < R.off[csp + 1]; ++k)
-              if (boxes_overlap(R.ent_box[k], db)) {
-                clear = false;
-                break;
-              }
-            if (!clear) break;
+            const uint32_t e1 = R.off[csp + 1];
+            if (next_overlap(R.ent_box, R.off[csp], e1, db) < e1) {
+              clear = false;
+              break;
+            }
             // test_bimolecular with factor 1: a reaction goes to the full step,
             // which re-draws the same numbers and proposes it
             if (test_bimolecular(R.rxns[rep * R.rxn_stride + crx[q]], r_rate_factor, rng) >= 0) clear = false;

@@ -32,6 +32,7 @@ def shim(tmp_path_factory):
     lib = ctypes.CDLL(str(out))
     for f in (lib.box_triangles, lib.box_segments, lib.box_overlap):
         f.restype = None
+    lib.overlap_walk.restype = ctypes.c_uint64
     return lib
 
 
@@ -50,6 +51,25 @@ def test_packed_overlap_equals_interval_test(shim):
     shim.box_overlap(ctypes.c_uint64(n), _p(a), _p(b), _p(out))
     expect = np.all((lo[0] <= hi[1]) & (lo[1] <= hi[0]), axis=1)
     assert np.array_equal(out.astype(bool), expect)
+
+
+def test_grouped_walk_visits_the_overlapping_entries_in_order(shim):
+    # next_overlap reads the boxes in groups; the walk it drives must visit
+    # exactly the entries a one-by-one loop keeps, in list order, for ranges
+    # of every length modulo the group size (including empty ones)
+    rng = np.random.default_rng(11)
+    for trial in range(400):
+        n = int(rng.integers(0, 40))
+        lo = rng.integers(0, 16, size=(n + 1, 3))
+        hi = np.minimum(lo + rng.integers(0, 6, size=(n + 1, 3)), 15)
+        boxes = np.array([sum((int(lo[i, a]) | (int(hi[i, a]) << 4)) << (8 * a) for a in range(3))
+                          for i in range(n + 1)], dtype=np.uint32)
+        sb, box = boxes[n], np.ascontiguousarray(boxes[:n]) if n else np.zeros(1, np.uint32)
+        k0 = int(rng.integers(0, n + 1))
+        keep = [k for k in range(k0, n) if np.all((lo[k] <= hi[n]) & (lo[n] <= hi[k]))]
+        out = np.zeros(max(n, 1), np.uint32)
+        cnt = shim.overlap_walk(_p(box), ctypes.c_uint32(k0), ctypes.c_uint32(n), ctypes.c_uint32(int(sb)), _p(out))
+        assert list(out[:cnt]) == keep, (trial, n, k0)
 
 
 def _segment_triangle_distance(p, m, t):
