@@ -116,7 +116,8 @@ __device__ __forceinline__ bool all_walls_miss64(const PartitionParams& P, const
         const uint32_t k0 = off[s], k1 = off[s + 1];
         if (k0 == k1) continue;
         const uint32_t sb = segment_box(P, s, pos.x, pos.y, pos.z, disp.x, disp.y, disp.z);
-        for (uint32_t k = next_overlap(box, k0, k1, sb); k < k1; k = next_overlap(box, k + 1, k1, sb))
+        for (uint32_t k = next_overlap<kLightOverlapGroup>(box, k0, k1, sb); k < k1;
+             k = next_overlap<kLightOverlapGroup>(box, k + 1, k1, sb))
           if (!wall_certain_miss64(ent[k], pos, disp)) return false;
       }
   return true;
@@ -137,7 +138,8 @@ __device__ __forceinline__ bool all_walls_miss32(const PartitionParams& P, const
         const uint32_t k0 = off[s], k1 = off[s + 1];
         if (k0 == k1) continue;
         const uint32_t sb = segment_box(P, s, pos.x, pos.y, pos.z, disp.x, disp.y, disp.z);
-        for (uint32_t k = next_overlap(box, k0, k1, sb); k < k1; k = next_overlap(box, k + 1, k1, sb)) {
+        for (uint32_t k = next_overlap<kLightOverlapGroup>(box, k0, k1, sb); k < k1;
+             k = next_overlap<kLightOverlapGroup>(box, k + 1, k1, sb)) {
           float de;
           if (!certain_miss32(ent[k], F, px, py, pz, mx, my, mz, &de)) return false;
           if (fabs_f(de) < band) near_plane = true;

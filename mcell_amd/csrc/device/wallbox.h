@@ -72,16 +72,21 @@ MCG_BOX_HD bool boxes_overlap(uint32_t a, uint32_t b) {
 }
 
 // The first entry k' >= k of the list range [k, k1) whose box overlaps sb,
-// or k1.  Boxes are read kOverlapGroup at a time, so that a long run of
-// skipped entries costs a few rounds of loads in flight together instead of
-// one load round per entry; the caller visits the entries in list order by
-// calling it again from k' + 1.
-constexpr uint32_t kOverlapGroup = 4;
+// or k1.  Boxes are read G at a time, so that a long run of skipped entries
+// costs a few rounds of loads in flight together instead of one load round
+// per entry; the caller visits the entries in list order by calling it again
+// from k' + 1.  The full steps, which meet lists of hundreds of entries next
+// to finely meshed surfaces, read kOverlapGroup at a time; the light passes,
+// whose lists are mostly a few entries long and whose instruction count
+// matters more, read kLightOverlapGroup.
+constexpr uint32_t kOverlapGroup = 8;
+constexpr uint32_t kLightOverlapGroup = 4;
+template <uint32_t G = kOverlapGroup>
 MCG_BOX_HD uint32_t next_overlap(const uint32_t* __restrict__ box, uint32_t k, uint32_t k1, uint32_t sb) {
-  for (; k < k1; k += kOverlapGroup) {
-    uint32_t bx[kOverlapGroup];
-    for (uint32_t u = 0; u < kOverlapGroup; ++u) bx[u] = box[k + u < k1 ? k + u : k1 - 1];
-    for (uint32_t u = 0; u < kOverlapGroup; ++u)
+  for (; k < k1; k += G) {
+    uint32_t bx[G];
+    for (uint32_t u = 0; u < G; ++u) bx[u] = box[k + u < k1 ? k + u : k1 - 1];
+    for (uint32_t u = 0; u < G; ++u)
       if (k + u < k1 && boxes_overlap(bx[u], sb)) return k + u;
   }
   return k1;

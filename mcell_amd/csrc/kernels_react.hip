@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
             const uint32_t db = region_box(R.P, csp, ccp[q].x - R1, ccp[q].y - R1, ccp[q].z - R1, ccp[q].x + R1,
                                            ccp[q].y + R1, ccp[q].z + R1);
             const uint32_t e1 = R.off[csp + 1];
-            if (next_overlap(R.ent_box, R.off[csp], e1, db) < e1) {
+            if (next_overlap<kLightOverlapGroup>(R.ent_box, R.off[csp], e1, db) < e1) {
               clear = false;
               break;
             }
