@@ -227,8 +227,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
-              file=sys.stderr)
+        print("bench.py: the engine drives exactly one GPU (no sharding across GPUs); --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
     from mcell_amd import workloads as W
     wl, cfg = build_workload(args)
