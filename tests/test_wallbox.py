@@ -32,7 +32,8 @@ def shim(tmp_path_factory):
     lib = ctypes.CDLL(str(out))
     for f in (lib.box_triangles, lib.box_segments, lib.box_overlap):
         f.restype = None
-    lib.overlap_walk.restype = ctypes.c_uint64
+    for f in (lib.overlap_walk, lib.overlap_walk_light):
+        f.restype = ctypes.c_uint64
     return lib
 
 
@@ -67,9 +68,10 @@ def test_grouped_walk_visits_the_overlapping_entries_in_order(shim):
         sb, box = boxes[n], np.ascontiguousarray(boxes[:n]) if n else np.zeros(1, np.uint32)
         k0 = int(rng.integers(0, n + 1))
         keep = [k for k in range(k0, n) if np.all((lo[k] <= hi[n]) & (lo[n] <= hi[k]))]
-        out = np.zeros(max(n, 1), np.uint32)
-        cnt = shim.overlap_walk(_p(box), ctypes.c_uint32(k0), ctypes.c_uint32(n), ctypes.c_uint32(int(sb)), _p(out))
-        assert list(out[:cnt]) == keep, (trial, n, k0)
+        for walk in (shim.overlap_walk, shim.overlap_walk_light):
+            out = np.zeros(max(n, 1), np.uint32)
+            cnt = walk(_p(box), ctypes.c_uint32(k0), ctypes.c_uint32(n), ctypes.c_uint32(int(sb)), _p(out))
+            assert list(out[:cnt]) == keep, (trial, n, k0, walk)
 
 
 def _segment_triangle_distance(p, m, t):

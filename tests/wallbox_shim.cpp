@@ -25,6 +25,15 @@ uint64_t overlap_walk(const uint32_t* box, uint32_t k0, uint32_t k1, uint32_t sb
   return n;
 }
 
+// The same walk with the light passes' group size.
+uint64_t overlap_walk_light(const uint32_t* box, uint32_t k0, uint32_t k1, uint32_t sb, uint32_t* out) {
+  uint64_t n = 0;
+  for (uint32_t k = next_overlap<kLightOverlapGroup>(box, k0, k1, sb); k < k1;
+       k = next_overlap<kLightOverlapGroup>(box, k + 1, k1, sb))
+    out[n++] = k;
+  return n;
+}
+
 // out[i] = region_box of the triangle tri[9i..9i+8] in subpartition s[i]
 void box_triangles(const double* pp, uint64_t n, const uint32_t* s, const double* tri, uint32_t* out) {
   const PartitionParams P = params(pp);
