@@ -63,21 +63,36 @@ __global__ __launch_bounds__(256) void k_cell_keys(RMolView<PosT> m, uint32_t n,
                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                                                    uint32_t* __restrict__ cell_count, uint32_t* __restrict__ err) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t tag = m.tag[i];
   uint32_t c = kInvalid;
-  if (!(tag_flags(tag) & MF_DEFUNCT)) {
-    int sx, sy, sz;
-    subpart_3d(P, D3{(double)m.x[i], (double)m.y[i], (double)m.z[i]}, sx, sy, sz);
-    if (!cell_of(G, sx, sy, sz, c)) {
-      atomicOr(err, ERR_GRID);
-      c = kInvalid;
-    } else {
-      c = bin_of(G, tag_replica(tag), tag_species(tag), c);
+  if (i < n) {
+    const uint32_t tag = m.tag[i];
+    if (!(tag_flags(tag) & MF_DEFUNCT)) {
+      int sx, sy, sz;
+      subpart_3d(P, D3{(double)m.x[i], (double)m.y[i], (double)m.z[i]}, sx, sy, sz);
+      if (!cell_of(G, sx, sy, sz, c)) {
+        atomicOr(err, ERR_GRID);
+        c = kInvalid;
+      } else {
+        c = bin_of(G, tag_replica(tag), tag_species(tag), c);
+      }
     }
+    key[i] = c;
   }
-  key[i] = c;
-  if (c != kInvalid) rank[i] = atomicAdd(cell_count + c, 1u);
+  // the molecules arrive mostly in bin order, so neighbouring lanes often
+  // share a bin: one atomic per run of equal bins in the wave, each lane's
+  // rank = the run's base + its place in the run
+  const int lane = __lane_id();
+  const uint32_t prev = __shfl_up(c, 1);
+  const bool head = lane == 0 || prev != c;
+  const uint64_t heads = __ballot(head);
+  const uint64_t upto = (2ull << lane) - 1;  // lanes 0..lane
+  const int my_head = 63 - __clzll((long long)(heads & upto));
+  const uint64_t after = heads & ~upto;
+  const int next_head = after ? __ffsll((long long)after) - 1 : 64;
+  uint32_t base = 0;
+  if (head && c != kInvalid) base = atomicAdd(cell_count + c, (uint32_t)(next_head - lane));
+  base = __shfl(base, my_head);
+  if (i < n && c != kInvalid) rank[i] = base + (uint32_t)(lane - my_head);
 }
 
 template <typename PosT>
