@@ -69,7 +69,7 @@ def build_workload(args):
     if args.workload == "free_diffusion_replicas":
         wl = W.free_diffusion_replicas(args.replicas, args.mols_per_replica, iterations=args.steps)
         cfg = {"workload": "free diffusion, %d replicas x %d A in a shared reflective 1 um cube (12 walls), "
-                           "0.1 um subpartitions, dt 1e-6 s" % (args.replicas, args.mols_per_replica),
+                           "dt 1e-6 s (subpartition edge: subpartition_um)" % (args.replicas, args.mols_per_replica),
                "replicas": args.replicas, "molecules": wl.num_molecules, "walls": int(len(wl.walls))}
     elif args.workload == "replicas_sphere":
         wl = W.replicas_sphere_in_cube(num_replicas=args.replicas, iterations=args.steps)
