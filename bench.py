@@ -218,7 +218,8 @@ def run_gpu(args, wl, precision):
     eng.close()
     steps_done = st.molecule_steps if reactive(wl) else wl.num_molecules * args.steps
     return {"secs": secs, "steps_done": steps_done, "value": steps_done / secs, "avg_ms": avg_ms,
-            "reactions": st.reactions, "rounds": st.wall_hits, "hard": st.slow_path_last}
+            "reactions": st.reactions, "rounds": st.settle_rounds, "hard": st.slow_path_last,
+            "reflections": st.wall_hits}
 
 
 def main():

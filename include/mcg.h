@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 3
+#define MCG_ABI_VERSION 4
 
 typedef struct mcg_ctx mcg_ctx;
 
@@ -39,8 +39,13 @@ enum {
   MCG_E_UNSUPPORTED = 5, /* feature not on this path */
   MCG_E_NOMEM = 6,       /* device allocation failed */
   MCG_E_LIMIT = 7,       /* a per-step safety limit was hit (reflections, re-draws, walk) */
-  MCG_E_STATE = 8        /* call out of order (e.g. step before geometry) */
+  MCG_E_STATE = 8        /* call out of order (e.g. step before geometry), or the context failed */
 };
+
+/* After an mcg_step that returned an error other than MCG_E_INVALID or
+ * MCG_E_STATE the molecule state is undefined (the reference terminates the
+ * process at this point); every later call on the context except
+ * mcg_destroy returns MCG_E_STATE. */
 
 enum {
   MCG_PREC_FP64 = 0,          /* plain FP64: positions, ray-triangle test, reflection */
@@ -64,10 +69,8 @@ typedef struct mcg_config {
   uint64_t seed;                 /* key of the per-molecule counter-based streams */
   uint64_t molecule_capacity;    /* volume molecules over all replicas (incl. products) */
   uint32_t flags;                /* MCG_FLAG_* */
-  uint32_t barrier_interval;     /* iterations between scheduler barriers (count events); 0 = every
-                                    iteration.  Bounds each molecule's time step as
-                                    Calendar::get_time_up_to_next_barrier does (scheduler.cpp:212-242),
-                                    at most 100 iterations ahead (diffuse_react_event.h:37). */
+  uint32_t start_iteration;      /* iteration of the first mcg_step (World's current iteration when
+                                    the event is created; 0 for a new simulation) */
 } mcg_config;
 
 #define MCG_FLAG_TRACE_HITS 1u    /* record the wall indices each molecule hits (parity tests) */
@@ -137,12 +140,15 @@ typedef struct mcg_rxn_event {
 /* Per-step counters mirroring SimulationStats (src4/simulation_stats.h). */
 typedef struct mcg_step_stats {
   uint64_t molecule_steps;       /* volume molecules diffused, summed over iterations */
-  uint64_t wall_hits;            /* reflections */
+  uint64_t wall_hits;            /* reflections off walls (SimulationStats::mol_wall_reflections) */
   uint64_t reactions;
   double kernel_ms;              /* device time of the diffuse kernels (HIP events) */
   double wall_ms;                /* host wall time of the call */
   uint32_t kernel_launches;
   uint32_t slow_path_last;       /* molecules that needed the full ray trace in the last iteration */
+  uint64_t transparent_crossings;/* walls crossed through a transparent surface class */
+  uint32_t settle_rounds;        /* claim-settling rounds of the reactive path, summed over iterations */
+  uint32_t reserved;
 } mcg_step_stats;
 
 const char* mcg_last_error(void);
@@ -252,13 +258,39 @@ int mcg_count_species(mcg_ctx* ctx, uint64_t* counts);
 
 /* Volume molecules of one replica (Partition::add_volume_molecule semantics:
  * each position must lie in the partition).  ids are the molecule ids that
- * key their random streams. */
+ * key their random streams; the replica's next product id becomes at least
+ * max(id) + 1 (partition.h:467-469).
+ *
+ * The time fields of Molecule (molecule.h:130-141) are optional (NULL gives
+ * the default): diffusion_times are absolute, in iterations, and lie in
+ * [clock, clock + 1) where clock is the iteration the next mcg_step starts
+ * at (default clock); unimol_times are absolute (TIME_INVALID = -256: no
+ * lifetime drawn yet or no unimolecular class; default -256); flags are
+ * MCG_MOLF_* (default MCG_MOLF_SCHEDULE_UNIMOL, a newly released molecule).
+ * Molecules given a diffusion time other than the clock make a context
+ * without reaction classes take the time-aware reactive pipeline. */
+#define MCG_MOLF_SCHEDULE_UNIMOL 1u  /* MOLECULE_FLAG_SCHEDULE_UNIMOL_RXN: lifetime not drawn yet */
 int mcg_add_volume_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids, const uint32_t* species,
-                             const double* positions, uint64_t count);
+                             const double* positions, const double* diffusion_times, const double* unimol_times,
+                             const uint32_t* flags, uint64_t count);
+
+/* Partition::next_molecule_id per replica (partition.h:420-470): the id the
+ * replica's next product gets.  Get fills num_replicas values; set raises
+ * each replica's counter to at least the given value (it never lowers it). */
+int mcg_get_next_molecule_ids(mcg_ctx* ctx, uint32_t* next_ids);
+int mcg_set_next_molecule_ids(mcg_ctx* ctx, const uint32_t* next_ids);
 
 /* Runs num_iterations diffuse-react iterations starting at first_iteration
- * (DiffuseReactEvent::step with event_time = iteration; barriers every
- * cfg.barrier_interval iterations).  stats may be NULL.
+ * (DiffuseReactEvent::step with event_time = iteration, diffuse_react_event.cpp:
+ * 53-64).  time_up_to_next_barrier is what Scheduler::handle_next_event hands
+ * the event through set_barrier_time_for_next_execution before the call's
+ * first iteration (scheduler.cpp:303-307): a whole number >= 1 of time units
+ * from first_iteration to the next barrier.  Iteration first_iteration + k
+ * clips every molecule's step at min(time_up_to_next_barrier - k, 100)
+ * (DIFFUSION_TIME_UPPER_LIMIT, diffuse_react_event.h:37), so a call must not
+ * run past its barrier: num_iterations <= time_up_to_next_barrier.
+ * first_iteration must be the engine's clock (cfg.start_iteration, then the
+ * iteration after the last call's).  stats may be NULL.
  *
  * Without reaction classes each molecule's step is the reference's step with
  * the molecule's own random stream.  With reaction classes, the molecules of
@@ -267,7 +299,8 @@ int mcg_add_volume_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids
  * proposal with the smaller mover id wins a contested molecule, and losers
  * repeat their step without the consumed molecules.  Products move for the
  * rest of the step afterwards, as the reference's new_diffuse_actions do. */
-int mcg_step(mcg_ctx* ctx, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats);
+int mcg_step(mcg_ctx* ctx, uint32_t first_iteration, uint32_t num_iterations, double time_up_to_next_barrier,
+             mcg_step_stats* stats);
 
 /* Waits for all queued work. */
 int mcg_sync(mcg_ctx* ctx);
@@ -277,9 +310,13 @@ int mcg_num_molecules(mcg_ctx* ctx, uint64_t* count);
 
 /* Copies the live volume molecules back (for Partition at a barrier).  Any
  * output pointer may be NULL.  Molecules come back in engine order (which
- * changes every iteration when reactions are on). */
+ * changes every iteration when reactions are on), with the time fields
+ * mcg_add_volume_molecules takes: an upload of what this returns into a new
+ * context (start_iteration = this clock, next molecule ids copied) continues
+ * the run bit for bit. */
 int mcg_get_volume_molecules(mcg_ctx* ctx, uint32_t* ids, uint32_t* species, uint32_t* replica, double* positions,
-                             uint32_t* subparts, uint64_t capacity, uint64_t* count);
+                             uint32_t* subparts, double* diffusion_times, double* unimol_times, uint32_t* flags,
+                             uint64_t capacity, uint64_t* count);
 
 /* Parity/inspection: wall constants (16 doubles per wall: normal, unit_u,
  * unit_v, distance_to_origin, uv_vert1_u, uv_vert2.u, uv_vert2.v, area, 0, 0)

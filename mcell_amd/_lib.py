@@ -36,7 +36,11 @@ EXPORTED = [
     "mcg_get_walls", "mcg_get_hit_trace", "mcg_last_kernel_time", "mcg_set_bimol_rxns", "mcg_set_rxn_log",
     "mcg_get_rxn_log", "mcg_count_species", "mcg_set_replica_params", "mcg_set_surface_classes",
     "mcg_set_unimol_rxns", "mcg_set_volsurf_rxns", "mcg_add_surface_molecules", "mcg_get_surface_molecules",
+    "mcg_get_next_molecule_ids", "mcg_set_next_molecule_ids",
 ]
+
+MCG_MOLF_SCHEDULE_UNIMOL = 1
+TIME_INVALID = -256.0
 
 
 class McgConfig(ctypes.Structure):
@@ -55,7 +59,7 @@ class McgConfig(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("molecule_capacity", ctypes.c_uint64),
         ("flags", ctypes.c_uint32),
-        ("barrier_interval", ctypes.c_uint32),
+        ("start_iteration", ctypes.c_uint32),
     ]
 
 
@@ -134,6 +138,9 @@ class McgStepStats(ctypes.Structure):
         ("wall_ms", ctypes.c_double),
         ("kernel_launches", ctypes.c_uint32),
         ("slow_path_last", ctypes.c_uint32),
+        ("transparent_crossings", ctypes.c_uint64),
+        ("settle_rounds", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -162,11 +169,14 @@ def load():
     lib.mcg_destroy.argtypes = [P]
     lib.mcg_set_geometry.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_uint32, P]
     lib.mcg_set_species.argtypes = [P, ctypes.POINTER(McgSpecies), ctypes.c_uint32]
-    lib.mcg_add_volume_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, ctypes.c_uint64]
-    lib.mcg_step.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(McgStepStats)]
+    lib.mcg_add_volume_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, ctypes.c_uint64]
+    lib.mcg_step.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(McgStepStats)]
     lib.mcg_sync.argtypes = [P]
     lib.mcg_num_molecules.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
-    lib.mcg_get_volume_molecules.argtypes = [P, P, P, P, P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    lib.mcg_get_volume_molecules.argtypes = [P, P, P, P, P, P, P, P, P, ctypes.c_uint64,
+                                             ctypes.POINTER(ctypes.c_uint64)]
+    lib.mcg_get_next_molecule_ids.argtypes = [P, P]
+    lib.mcg_set_next_molecule_ids.argtypes = [P, P]
     lib.mcg_get_walls.argtypes = [P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]
     lib.mcg_get_hit_trace.argtypes = [P, P, P, ctypes.c_uint64]
     lib.mcg_last_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32)]

@@ -29,18 +29,25 @@ struct Walls32View {
 // the box margin of the point).  The plane of a wall whose triangle is farther
 // away may be crossed: a later move from the rounded point meets that plane
 // within rounding distance of the point, outside the triangle, and misses.
-__device__ __forceinline__ void round_to_fp32_same_side(const PartitionParams& P, const WallRec64* __restrict__ walls,
+// Each pass steps the point one ulp per axis towards the FP64 side of every
+// plane it is on or across; near an acute edge one step can cross another
+// plane, so passes repeat until one moves nothing.  Returns false when
+// kRoundPasses passes did not settle (the caller reports ERR_ROUND32).
+constexpr int kRoundPasses = 8;
+__device__ __forceinline__ bool round_to_fp32_same_side(const PartitionParams& P, const WallRec64* __restrict__ walls,
                                                      const uint32_t* __restrict__ off,
                                                      const uint32_t* __restrict__ idx,
                                                      const uint32_t* __restrict__ box, double px, double py,
                                                      double pz, float* ox, float* oy, float* oz) {
   float fx = (float)px, fy = (float)py, fz = (float)pz;
+  bool settled = true;
   int sx, sy, sz;
   subpart_3d(P, D3{px, py, pz}, sx, sy, sz);
   if (!(sx < 0 || sy < 0 || sz < 0 || sx >= (int)P.n || sy >= (int)P.n || sz >= (int)P.n)) {
     uint32_t s = subpart_of(P, sx, sy, sz);
     const uint32_t pb = segment_box(P, s, px, py, pz, 0.0, 0.0, 0.0);
-    for (int pass = 0; pass < 4; ++pass) {
+    settled = false;
+    for (int pass = 0; pass < kRoundPasses && !settled; ++pass) {
       bool moved = false;
       const uint32_t e1 = off[s + 1];
       for (uint32_t k = next_overlap(box, off[s], e1, pb); k < e1; k = next_overlap(box, k + 1, e1, pb)) {
@@ -60,12 +67,13 @@ __device__ __forceinline__ void round_to_fp32_same_side(const PartitionParams& P
         else if (w.nz * dir < 0) fz = nextafterf(fz, -3.4e38f);
         moved = true;
       }
-      if (!moved) break;
+      settled = !moved;
     }
   }
   *ox = fx;
   *oy = fy;
   *oz = fz;
+  return settled;
 }
 
 }  // namespace mcg

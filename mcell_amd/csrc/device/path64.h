@@ -335,7 +335,8 @@ __device__ __forceinline__ bool diffuse_reflect64(const PartitionParams& P, cons
                                                   const uint32_t* __restrict__ idx,
                                                   const uint32_t* __restrict__ box, const SurfaceClasses& SC,
                                                   uint32_t species, D3& pos, D3 disp, PhiloxStream& rng,
-                                                  uint32_t& err, uint32_t* trace_walls, uint32_t& trace_n) {
+                                                  uint32_t& err, uint32_t* trace_walls, uint32_t& trace_n,
+                                                  uint32_t& n_reflect, uint32_t& n_cross) {
   uint32_t last_hit = kInvalid;
   int seg;
   for (seg = 0; seg < kMaxSegments; ++seg) {
@@ -391,11 +392,13 @@ __device__ __forceinline__ bool diffuse_reflect64(const PartitionParams& P, cons
     if (act == SC_TRANSPARENT) {
       // cross_transparent_wall (diffuse_react_event.cpp:3071-3094)
       disp = {disp.x * s, disp.y * s, disp.z * s};
+      ++n_cross;
       continue;
     }
     // reflect_from_wall (collision_utils.inl:1711-1747)
     double rf = -2.0 * dot3(disp, d3(w.nx, w.ny, w.nz));
     disp = {(disp.x + rf * w.nx) * s, (disp.y + rf * w.ny) * s, (disp.z + rf * w.nz) * s};
+    ++n_reflect;
   }
   if (seg == kMaxSegments) err |= ERR_SEGMENTS;
   if (!in_partition(P, pos)) err |= ERR_ESCAPED;

@@ -42,6 +42,7 @@ struct ReactParams {
   uint32_t total_tiles;
   const VolSurfRec* __restrict__ vs_rxns;
   const int32_t* __restrict__ pair_vs;   // [vol species * ns + surf species], -1 = none
+  unsigned long long* ctr;               // CTR_* statistics of the call
 };
 
 // xyz2grid_tile_index (grid_utils.inl:48-118) for a point on the wall.
@@ -68,6 +69,10 @@ __device__ __forceinline__ uint32_t xyz2grid(const WallRec64& w, const WallGrid&
   if ((uint32_t)idx >= g.num_tiles) idx = (int)g.num_tiles - 1;  // the reference asserts this cannot happen
   return (uint32_t)idx;
 }
+
+// Molecule collisions processed in one ray segment before the step reports
+// ERR_COLLISIONS (a runaway guard: a segment meets a few partners at most).
+constexpr int kMaxCollisions = 4096;
 
 struct Proposal {
   double time;    // absolute time of the collision
@@ -172,6 +177,7 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
 // Result of one mover's step.
 struct MoverEnd {
   D3 pos;          // end position; with a proposal, the start of the segment it was found in
+  uint32_t n_reflect, n_cross;  // walls reflected off and crossed (statistics)
   double t_steps;  // time the step covered (units of the time step), before reflections
   double utime;    // unimolecular reaction time after the step (kTimeInvalid: none)
   bool near_plane; // FP32 form: the end point lies near a wall plane
@@ -218,6 +224,8 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   out.absorbed = false;
   out.due = 0;
   out.t_steps = 0;
+  out.n_reflect = 0;
+  out.n_cross = 0;
   out.pos = pos;
   out.utime = start_unimol_time(R.unimol, spc, tag, utime, elapsed, rng);
   if (out.utime != kTimeInvalid && !(tag_flags(tag) & MF_SCHEDULE_UNIMOL) && out.utime <= elapsed) {
@@ -278,7 +286,11 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
       double t_last = 0;
       uint32_t id_last = 0;
       bool first = true;
-      for (int guard = 0; guard < 4096; ++guard) {
+      for (int guard = 0;; ++guard) {
+        if (guard == kMaxCollisions) {
+          err |= ERR_COLLISIONS;
+          break;
+        }
         double tc;
         uint32_t slot;
         D3 cp;
@@ -386,11 +398,13 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
       // cross_transparent_wall (diffuse_react_event.cpp:3071-3094)
       if (t_steps < 1e-12) t_steps = 1e-12;
       disp = {disp.x * s, disp.y * s, disp.z * s};
+      ++out.n_cross;
       continue;
     }
     // reflect_from_wall (collision_utils.inl:1711-1747)
     const double rf = -2.0 * dot3(disp, d3(w.nx, w.ny, w.nz));
     disp = {(disp.x + rf * w.nx) * s, (disp.y + rf * w.ny) * s, (disp.z + rf * w.nz) * s};
+    ++out.n_reflect;
   }
   if (seg == kMaxSegments) err |= ERR_SEGMENTS;
   if (!in_partition(R.P, pos)) err |= ERR_ESCAPED;

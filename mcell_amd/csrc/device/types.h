@@ -111,7 +111,18 @@ enum : uint32_t {
   ERR_GRID = 1u << 6,        // a reactive molecule outside the cell grid
   ERR_EXD = 1u << 7,         // exact_disk vertex pool exhausted
   ERR_QUEUE = 1u << 8,       // proposal / product capacity exceeded
+  ERR_ROUND32 = 1u << 9,     // FP32 form: a point could not be rounded onto its side of every nearby plane
+  ERR_COLLISIONS = 1u << 10, // more molecule collisions in one ray segment than the step's guard allows
 };
+
+// per-call statistics counters (SimulationStats::mol_wall_reflections and
+// the transparent crossings), one wave-summed atomic per wave
+enum : uint32_t { CTR_REFLECTIONS = 0, CTR_CROSSINGS = 1, CTR_WORDS = 2 };
+__device__ __forceinline__ void wave_count(unsigned long long* ctr, uint32_t v) {
+  if (ctr == nullptr) return;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (__lane_id() == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+}
 
 constexpr int kMaxSegments = 1024;
 constexpr int kMaxRedo = 4096;

@@ -1,6 +1,7 @@
 // The engine context and the C ABI declared in include/mcg.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -21,6 +22,11 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+int check_alive(const mcg_ctx* c) {
+  if (c && c->failed) return fail(MCG_E_STATE, "the context failed in an earlier step (" + c->fail_msg + ")");
+  return MCG_OK;
+}
+
 }  // namespace mcg
 
 mcg_ctx::~mcg_ctx() {
@@ -38,15 +44,15 @@ namespace {
 // every wall plane of its subpartition (a rounded point on a plane would let
 // the first move leave through the wall: collide_wall misses a move that
 // starts on the plane, collision_utils.inl:647-665).
-void round_same_side_host(const mcg_ctx* c, double px, double py, double pz, float& fx, float& fy, float& fz) {
+bool round_same_side_host(const mcg_ctx* c, double px, double py, double pz, float& fx, float& fy, float& fz) {
   const PartitionParams& P = c->P;
   fx = (float)px;
   fy = (float)py;
   fz = (float)pz;
   const int sx = (int)((px - P.ox) * P.sp_rcp), sy = (int)((py - P.oy) * P.sp_rcp), sz = (int)((pz - P.oz) * P.sp_rcp);
-  if (sx < 0 || sy < 0 || sz < 0 || sx >= (int)P.n || sy >= (int)P.n || sz >= (int)P.n) return;
+  if (sx < 0 || sy < 0 || sz < 0 || sx >= (int)P.n || sy >= (int)P.n || sz >= (int)P.n) return true;
   const uint32_t s = (uint32_t)sx + (uint32_t)sy * P.n + (uint32_t)sz * P.n2;
-  for (int pass = 0; pass < 4; ++pass) {
+  for (int pass = 0; pass < 8; ++pass) {
     bool moved = false;
     for (uint32_t k = c->off_host[s]; k < c->off_host[s + 1]; ++k) {
       const WallRec64& w = c->walls_host[c->idx_host[k]];
@@ -62,8 +68,9 @@ void round_same_side_host(const mcg_ctx* c, double px, double py, double pz, flo
       else if (w.nz * dir < 0) fz = std::nextafter(fz, -3.4e38f);
       moved = true;
     }
-    if (!moved) break;
+    if (!moved) return true;
   }
+  return false;
 }
 
 }  // namespace
@@ -85,6 +92,10 @@ int mcg_create(const mcg_config* cfg, mcg_ctx** out) {
   if (cfg->num_replicas == 0 || cfg->num_replicas > 65536) return fail(MCG_E_INVALID, "num_replicas out of range");
   if (cfg->molecule_capacity == 0 || cfg->molecule_capacity >= 0xFFFFFFFFull)
     return fail(MCG_E_INVALID, "molecule_capacity out of range");
+  // MCell4Converter's check (mcell4_converter.cpp:121-127): the expanded
+  // neighbour collection of a subpartition reaches one neighbour only
+  if (!(cfg->rxn_radius_3d * 1.41421356238 < cfg->subpart_edge_length / 2))
+    return fail(MCG_E_INVALID, "reaction radius multiplied by sqrt(2) must be less than half of the subpartition edge");
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
   if (ndev == 0) return fail(MCG_E_HIP, "no HIP device");
@@ -114,8 +125,9 @@ int mcg_create(const mcg_config* cfg, mcg_ctx** out) {
                             : (c->x.alloc(cap) || c->y.alloc(cap) || c->z.alloc(cap));
   if (pos_fail || c->id.alloc(cap) || c->tag.alloc(cap) || c->err.alloc(1) ||
       c->queue.alloc((size_t)kNumQueueShards * shard_capacity((uint32_t)cap, kNumQueueShards / 2)) ||
-      c->qcount.alloc(kNumQueueShards))
+      c->qcount.alloc(kNumQueueShards) || c->counters.alloc(CTR_WORDS))
     return cleanup_fail(fail(MCG_E_NOMEM, "molecule buffers"));
+  c->clock = cfg->start_iteration;
   double m = 0;
   for (double v : {P.ox, P.oy, P.oz, P.qx, P.qy, P.qz}) m = std::max(m, std::fabs(v));
   c->max_abs_coord = (float)m;
@@ -139,6 +151,7 @@ int mcg_destroy(mcg_ctx* ctx) {
 int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint32_t* walls, uint32_t nw,
                      const uint32_t* wall_class) {
   if (!c || (nv && !vertices) || (nw && !walls)) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   for (uint32_t i = 0; i < nw; ++i) {
     if (walls[3 * i] >= nv || walls[3 * i + 1] >= nv || walls[3 * i + 2] >= nv)
       return fail(MCG_E_INVALID, "wall vertex index out of range");
@@ -205,6 +218,7 @@ int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint
 
 int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
   if (!c || !sp || ns == 0 || ns > 256) return fail(MCG_E_INVALID, "bad species table");
+  MCG_ALIVE(c);
   std::vector<SpeciesRec> h(ns);
   for (uint32_t i = 0; i < ns; ++i) {
     if (!(sp[i].space_step >= 0)) return fail(MCG_E_INVALID, "negative space step");
@@ -223,29 +237,56 @@ int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
 }
 
 int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, const uint32_t* species,
-                             const double* pos, uint64_t count) {
+                             const double* pos, const double* dtimes, const double* utimes, const uint32_t* flags,
+                             uint64_t count) {
   if (!c || (count && (!ids || !species || !pos))) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if (replica >= c->cfg.num_replicas) return fail(MCG_E_INVALID, "replica out of range");
   if (c->n + count > c->cfg.molecule_capacity) return fail(MCG_E_CAPACITY, "molecule capacity exceeded");
   const PartitionParams& P = c->P;
-  std::vector<double> hx(count), hy(count), hz(count);
+  std::vector<double> hx(count), hy(count), hz(count), htoff(count, 0.0), hut(count, kTimeInvalid);
   std::vector<uint32_t> ht(count);
+  bool any_toff = false, any_time = false;
+  const double clock = (double)c->clock;
   for (uint64_t i = 0; i < count; ++i) {
     const double* p = pos + 3 * i;
     if (!(p[0] >= P.ox && p[1] >= P.oy && p[2] >= P.oz && p[0] < P.qx && p[1] < P.qy && p[2] < P.qz))
       return fail(MCG_E_INVALID, "molecule outside the partition (partition.h:559-563)");
     if (species[i] >= c->num_species) return fail(MCG_E_INVALID, "species id out of range");
     hx[i] = p[0]; hy[i] = p[1]; hz[i] = p[2];
-    ht[i] = make_tag(species[i], MF_SCHEDULE_UNIMOL, replica);
+    uint32_t f = MF_SCHEDULE_UNIMOL;
+    if (flags) {
+      if (flags[i] & ~MCG_MOLF_SCHEDULE_UNIMOL) return fail(MCG_E_INVALID, "unknown molecule flag");
+      f = (flags[i] & MCG_MOLF_SCHEDULE_UNIMOL) ? MF_SCHEDULE_UNIMOL : 0u;
+      any_time |= f == 0;
+    }
+    if (dtimes) {
+      // Molecule::diffusion_time as an offset past the clock (exact: both lie
+      // within one small range of whole iterations)
+      if (!(dtimes[i] >= clock)) return fail(MCG_E_INVALID, "diffusion time before the engine's clock");
+      htoff[i] = dtimes[i] - clock;
+      if (htoff[i] != 0.0) {
+        any_toff = true;
+        f |= MF_TOFF;
+      }
+    }
+    if (utimes) {
+      hut[i] = utimes[i];
+      any_time |= utimes[i] != kTimeInvalid;
+    }
+    ht[i] = make_tag(species[i], f, replica);
   }
+  // offsets and drawn lifetimes live in the reactive path's arrays
+  if (any_toff || any_time)
+    if (int st = ensure_react_buffers(c)) return st;
   if (c->rx.on && count) {
     // the device holds the id counters once steps have run
     if (!c->rx.next_id_dirty)
       HIP_TRY(hipMemcpy(c->rx.next_id_host.data(), c->rx.next_id.p, c->rx.next_id_host.size() * 4,
                         hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemsetAsync(c->rx.toff.p + c->n, 0, count * sizeof(double), c->stream));
-    std::vector<double> inv(count, kTimeInvalid);
-    HIP_TRY(hipMemcpyAsync(c->rx.utime.p + c->n, inv.data(), count * sizeof(double), hipMemcpyHostToDevice,
+    HIP_TRY(hipMemcpyAsync(c->rx.toff.p + c->n, htoff.data(), count * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->rx.utime.p + c->n, hut.data(), count * sizeof(double), hipMemcpyHostToDevice,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
@@ -266,7 +307,8 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
     if (!c->have_geometry) return fail(MCG_E_STATE, "the FP32 form needs the geometry before the molecules");
     fx.resize(count); fy.resize(count); fz.resize(count);
     for (uint64_t i = 0; i < count; ++i) {
-      round_same_side_host(c, hx[i], hy[i], hz[i], fx[i], fy[i], fz[i]);
+      if (!round_same_side_host(c, hx[i], hy[i], hz[i], fx[i], fy[i], fz[i]))
+        return fail(MCG_E_INVALID, "molecule cannot be rounded to FP32 on its side of the nearby wall planes");
       if (!(fx[i] < (float)P.qx && fy[i] < (float)P.qy && fz[i] < (float)P.qz))
         return fail(MCG_E_INVALID, "molecule rounds onto the partition boundary in FP32");
     }
@@ -290,11 +332,42 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
   return MCG_OK;
 }
 
-int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats) {
-  if (!c) return fail(MCG_E_INVALID, "null context");
-  if (!c->have_geometry || c->num_species == 0) return fail(MCG_E_STATE, "geometry and species must be set first");
-  c->stepped = true;
-  if (c->rx.on) return react_step(c, first_iteration, num_iterations, stats);
+int mcg_get_next_molecule_ids(mcg_ctx* c, uint32_t* next_ids) {
+  if (!c || !next_ids) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
+  ReactState& r = c->rx;
+  if (r.next_id_host.size() != c->cfg.num_replicas) r.next_id_host.assign(c->cfg.num_replicas, 0);
+  // the device holds the counters once reactive steps have run
+  if (r.on && !r.next_id_dirty) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(r.next_id_host.data(), r.next_id.p, r.next_id_host.size() * 4, hipMemcpyDeviceToHost));
+  }
+  std::memcpy(next_ids, r.next_id_host.data(), r.next_id_host.size() * 4);
+  return MCG_OK;
+}
+
+int mcg_set_next_molecule_ids(mcg_ctx* c, const uint32_t* next_ids) {
+  if (!c || !next_ids) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
+  std::vector<uint32_t> cur(c->cfg.num_replicas);
+  if (int st = mcg_get_next_molecule_ids(c, cur.data())) return st;
+  for (uint32_t k = 0; k < c->cfg.num_replicas; ++k) c->rx.next_id_host[k] = std::max(cur[k], next_ids[k]);
+  c->rx.next_id_dirty = true;
+  return MCG_OK;
+}
+
+namespace {
+
+// Marks the context failed after a step whose kernels ran into an error.
+int step_failed(mcg_ctx* c, int st) {
+  if (st != MCG_OK && st != MCG_E_INVALID && st != MCG_E_STATE) {
+    c->failed = true;
+    c->fail_msg = g_err;
+  }
+  return st;
+}
+
+int free_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats) {
   auto t0 = std::chrono::steady_clock::now();
   MolView64 m{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p};
   MolView32 mf{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p};
@@ -302,19 +375,25 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
   HitTrace tr{c->trace_wall.p, c->trace_count.p, (c->cfg.flags & MCG_FLAG_TRACE_HITS) ? 1u : 0u};
   uint32_t n = (uint32_t)c->n;
   SlowQueue Q{c->queue.p, c->qcount.p, 0};
+  HIP_TRY(launch_fill_u64(c->stream, c->counters.p, 0ull, CTR_WORDS));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  // every molecule has its diffusion time at the iteration start and the
+  // species time step 1, so the barrier (>= 1 ahead) never clips a step
   for (uint32_t k = 0; k < num_iterations; ++k) {
     if (c->fp32())
-      HIP_TRY(launch_diffuse_reflect32(c->stream, mf, n, c->species_table(), c->surface_classes(), c->P, W, c->walls32.p, c->max_abs_coord,
-                                       c->cfg.seed, first_iteration + k, Q, c->err.p, tr));
+      HIP_TRY(launch_diffuse_reflect32(c->stream, mf, n, c->species_table(), c->surface_classes(), c->P, W,
+                                       c->walls32.p, c->max_abs_coord, c->cfg.seed, first_iteration + k, Q, c->err.p,
+                                       tr, c->counters.p));
     else
-      HIP_TRY(launch_diffuse_reflect64(c->stream, m, n, c->species_table(), c->surface_classes(), c->P, W, c->cfg.seed, first_iteration + k,
-                                       Q, c->err.p, tr));
+      HIP_TRY(launch_diffuse_reflect64(c->stream, m, n, c->species_table(), c->surface_classes(), c->P, W,
+                                       c->cfg.seed, first_iteration + k, Q, c->err.p, tr, c->counters.p));
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   uint32_t err = 0, slow = 0;
+  unsigned long long ctr[CTR_WORDS] = {0, 0};
   std::vector<uint32_t> shard_counts(kNumQueueShards, 0);
   HIP_TRY(hipMemcpyAsync(&err, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(ctr, c->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, c->stream));
   if (num_iterations && n)
     HIP_TRY(hipMemcpyAsync(shard_counts.data(), c->qcount.p, kNumQueueShards * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -325,7 +404,8 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
   c->last_launches = num_iterations;
   if (stats) {
     stats->molecule_steps = (uint64_t)n * num_iterations;
-    stats->wall_hits = 0;
+    stats->wall_hits = ctr[CTR_REFLECTIONS];
+    stats->transparent_crossings = ctr[CTR_CROSSINGS];
     stats->reactions = 0;
     stats->kernel_ms = ms;
     stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -333,7 +413,7 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
     stats->slow_path_last = slow;
   }
   if (err) {
-    HIP_TRY(hipMemsetAsync(c->err.p, 0, 4, c->stream));
+    HIP_TRY(launch_fill_u32(c->stream, c->err.p, 0u, 1u));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (err & ERR_ESCAPED) return fail(MCG_E_ESCAPED, "a molecule escaped the partition");
     return fail(MCG_E_LIMIT, "per-step limit exceeded (error bits " + std::to_string(err) + ")");
@@ -341,8 +421,33 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_
   return MCG_OK;
 }
 
+}  // namespace
+
+int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, double time_up_to_next_barrier,
+             mcg_step_stats* stats) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
+  if (!c->have_geometry || c->num_species == 0) return fail(MCG_E_STATE, "geometry and species must be set first");
+  // DiffuseReactEvent::set_barrier_time_for_next_execution's contract
+  // (diffuse_react_event.h:142-150): a whole number > 0
+  if (!(time_up_to_next_barrier >= 1.0) || time_up_to_next_barrier != std::floor(time_up_to_next_barrier))
+    return fail(MCG_E_INVALID, "time_up_to_next_barrier must be a whole number >= 1");
+  if ((double)num_iterations > time_up_to_next_barrier)
+    return fail(MCG_E_INVALID, "a call must not run past its barrier (num_iterations > time_up_to_next_barrier)");
+  if (first_iteration != c->clock)
+    return fail(MCG_E_INVALID, "first_iteration " + std::to_string(first_iteration) + " is not the engine's clock " +
+                                   std::to_string(c->clock));
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  c->stepped = true;
+  int st = c->rx.on ? react_step(c, first_iteration, num_iterations, time_up_to_next_barrier, stats)
+                    : free_step(c, first_iteration, num_iterations, stats);
+  if (st == MCG_OK) c->clock = (uint64_t)first_iteration + num_iterations;
+  return step_failed(c, st);
+}
+
 int mcg_set_surface_classes(mcg_ctx* c, uint32_t num_classes, const uint8_t* action) {
   if (!c || (num_classes && !action)) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if (c->num_species == 0 || !c->have_geometry) return fail(MCG_E_STATE, "geometry and species must be set first");
   if (c->per_replica) return fail(MCG_E_STATE, "surface classes must be set before per-replica parameters");
   const uint32_t ns = c->num_species;
@@ -371,12 +476,14 @@ int mcg_set_surface_classes(mcg_ctx* c, uint32_t num_classes, const uint8_t* act
 
 int mcg_sync(mcg_ctx* c) {
   if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MCG_OK;
 }
 
 int mcg_num_molecules(mcg_ctx* c, uint64_t* count) {
   if (!c || !count) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if ((!c->rx.on && !c->sc_on) || c->num_species == 0) {
     *count = c->n;
     return MCG_OK;
@@ -391,10 +498,12 @@ int mcg_num_molecules(mcg_ctx* c, uint64_t* count) {
 }
 
 int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, double* pos,
-                             uint32_t* subparts, uint64_t capacity, uint64_t* count) {
+                             uint32_t* subparts, double* dtimes, double* utimes, uint32_t* flags, uint64_t capacity,
+                             uint64_t* count) {
   if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
   const uint64_t n = c->n;
-  std::vector<double> hx(n), hy(n), hz(n);
+  std::vector<double> hx(n), hy(n), hz(n), htoff, hut;
   std::vector<uint32_t> ht(n), hid(n);
   if (n && c->fp32()) {
     std::vector<float> fx(n), fy(n), fz(n);
@@ -404,6 +513,9 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (uint64_t i = 0; i < n; ++i) { hx[i] = fx[i]; hy[i] = fy[i]; hz[i] = fz[i]; }
   }
+  // unimolecular times are kept only while a unimolecular class exists
+  const bool have_ut = c->rx.on && std::any_of(c->rx.unimol.begin(), c->rx.unimol.end(),
+                                               [](const UnimolRec& u) { return u.num_pathways > 0; });
   if (n) {
     if (!c->fp32()) {
       HIP_TRY(hipMemcpyAsync(hx.data(), c->x.p, n * 8, hipMemcpyDeviceToHost, c->stream));
@@ -412,6 +524,14 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
     }
     HIP_TRY(hipMemcpyAsync(ht.data(), c->tag.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(hid.data(), c->id.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (dtimes && c->rx.on) {
+      htoff.resize(n);
+      HIP_TRY(hipMemcpyAsync(htoff.data(), c->rx.toff.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (utimes && have_ut) {
+      hut.resize(n);
+      HIP_TRY(hipMemcpyAsync(hut.data(), c->rx.utime.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
   uint64_t live = 0;
@@ -419,9 +539,11 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
   if (count) *count = live;
   if (capacity < live) return fail(MCG_E_CAPACITY, "output capacity too small");
   const PartitionParams& P = c->P;
+  const double clock = (double)c->clock;
   uint64_t k = 0;
   for (uint64_t i = 0; i < n; ++i) {
-    if (tag_flags(ht[i]) & MF_DEFUNCT) continue;
+    const uint32_t f = tag_flags(ht[i]);
+    if (f & MF_DEFUNCT) continue;
     if (ids) ids[k] = hid[i];
     if (species) species[k] = tag_species(ht[i]);
     if (replica) replica[k] = tag_replica(ht[i]);
@@ -431,6 +553,9 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
           iz = (int)((hz[i] - P.oz) * P.sp_rcp);
       subparts[k] = (uint32_t)ix + (uint32_t)iy * P.n + (uint32_t)iz * P.n2;
     }
+    if (dtimes) dtimes[k] = ((f & MF_TOFF) && !htoff.empty()) ? clock + htoff[i] : clock;
+    if (utimes) utimes[k] = hut.empty() ? kTimeInvalid : hut[i];
+    if (flags) flags[k] = (f & MF_SCHEDULE_UNIMOL) ? MCG_MOLF_SCHEDULE_UNIMOL : 0u;
     ++k;
   }
   return MCG_OK;
@@ -438,6 +563,7 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
 
 int mcg_get_walls(mcg_ctx* c, double* wc, uint32_t* offsets, uint32_t* indices, uint64_t* num_entries) {
   if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
   if (wc)
     for (size_t i = 0; i < c->walls_host.size(); ++i) {
       const WallRec64& w = c->walls_host[i];
@@ -453,6 +579,7 @@ int mcg_get_walls(mcg_ctx* c, double* wc, uint32_t* offsets, uint32_t* indices, 
 
 int mcg_get_hit_trace(mcg_ctx* c, uint32_t* walls, uint32_t* counts, uint64_t capacity) {
   if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
   if (!(c->cfg.flags & MCG_FLAG_TRACE_HITS)) return fail(MCG_E_STATE, "created without MCG_FLAG_TRACE_HITS");
   if (c->rx.on) return fail(MCG_E_UNSUPPORTED, "hit traces are recorded without reaction classes only");
   if (capacity < c->n) return fail(MCG_E_CAPACITY, "output capacity too small");
@@ -465,6 +592,7 @@ int mcg_get_hit_trace(mcg_ctx* c, uint32_t* walls, uint32_t* counts, uint64_t ca
 
 int mcg_last_kernel_time(mcg_ctx* c, double* avg_ms, uint32_t* launches) {
   if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
   if (avg_ms) *avg_ms = c->last_kernel_ms;
   if (launches) *launches = c->last_launches;
   return MCG_OK;

@@ -14,6 +14,13 @@ namespace mcg {
 // Records the error text for mcg_last_error and returns `code`.
 int fail(int code, const std::string& msg);
 
+// MCG_E_STATE when an earlier step left the context's state undefined.
+int check_alive(const mcg_ctx* c);
+#define MCG_ALIVE(c)                                   \
+  do {                                                 \
+    if (int st_ = ::mcg::check_alive(c)) return st_;   \
+  } while (0)
+
 #define HIP_TRY(expr)                                                                                  \
   do {                                                                                                 \
     hipError_t e_ = (expr);                                                                            \
@@ -150,13 +157,18 @@ struct mcg_ctx {
 
   mcg::ReactState rx;
   bool stepped = false;  // reaction classes are fixed once a step has run
+  uint64_t clock = 0;    // iteration the next mcg_step starts at (Molecule::diffusion_time base)
+  bool failed = false;   // a step failed after its kernels ran: the molecule state is undefined
+  std::string fail_msg;
+  mcg::DevBuf<unsigned long long> counters;  // CTR_* of the current mcg_step call
 
   ~mcg_ctx();
 };
 
 namespace mcg {
 // The reactive iteration loop (engine_react.cpp).
-int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats);
+int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, double time_up_to_next_barrier,
+               mcg_step_stats* stats);
 // Allocates the reactive path's buffers once (reaction classes of any kind).
 int ensure_react_buffers(mcg_ctx* c);
 // Live molecules per (replica, species) into counts (size num_replicas *

@@ -66,6 +66,7 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.R.F = make_filter_params(c->max_abs_coord);
   p.R.band = 2.0f * p.R.F.e_dd + 8.0f * p.R.F.eps * c->max_abs_coord;
   p.R.SC = c->surface_classes();
+  p.R.ctr = c->counters.p;
   p.species = c->species_table();
   // without unimolecular classes no molecule has a reaction time: the
   // arrays are neither copied nor read
@@ -142,8 +143,8 @@ int settle_rounds(mcg_ctx* c, ReactPipeline<PosT>& p, bool snapshot_src, double 
 }
 
 template <typename PosT>
-int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t num, mcg_step_stats* stats,
-                   double& move_ms) {
+int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t num, double time_up,
+                   mcg_step_stats* stats, double& move_ms) {
   ReactState& r = c->rx;
   uint32_t* h = r.hdr_host;
   const bool propose_only = (c->cfg.flags & MCG_FLAG_PROPOSE_ONLY) != 0;
@@ -152,15 +153,16 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
   uint32_t rounds = 0;
   r.log_count = 0;
   HIP_TRY(launch_fill_u32(c->stream, p.hdr, 0u, kHdrWords));
+  HIP_TRY(launch_fill_u64(c->stream, c->counters.p, 0ull, CTR_WORDS));
   // the kernels report errors into the header, which every iteration reads
   uint32_t* const err = p.hdr + kHdrErr;
   move_ms = 0;
   for (uint32_t k = 0; k < num; ++k) {
     const uint32_t it = first + k;
-    // Calendar::get_time_up_to_next_barrier (scheduler.cpp:212-242)
-    const uint64_t b = c->cfg.barrier_interval ? c->cfg.barrier_interval : 1;
-    const uint64_t next_barrier = ((uint64_t)it / b + 1) * b;
-    const double barrier = std::min((double)(next_barrier - it), 100.0);
+    // the scheduler's time up to the next barrier from this iteration,
+    // capped at DIFFUSION_TIME_UPPER_LIMIT (scheduler.cpp:212-242, 303-307;
+    // diffuse_react_event.h:37)
+    const double barrier = std::min(time_up - (double)k, 100.0);
     p.R.iteration = it;
     const uint32_t n_in = (uint32_t)c->n;
     HIP_TRY(p.sort(c->stream, n_in, r.num_cells, err));
@@ -210,8 +212,6 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->evm0, c->evm1));
     move_ms += ms;
-    c->n = h[kHdrNumOut];
-    steps += h[kHdrSnapshot];
     const uint32_t e = h[kHdrErr];
     if (e) {
       if (e & ERR_ESCAPED) return fail(MCG_E_ESCAPED, "a molecule escaped the partition");
@@ -219,12 +219,19 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
       if (e & ERR_QUEUE) return fail(MCG_E_CAPACITY, "molecule capacity exceeded by reaction products");
       return fail(MCG_E_LIMIT, "per-step limit exceeded (error bits " + std::to_string(e) + ")");
     }
+    // k_bump_ids keeps the used slots within the capacity
+    c->n = std::min<uint64_t>(h[kHdrNumOut], cap);
+    steps += h[kHdrSnapshot];
   }
   if (propose_only) r.log_count = h[kHdrLog];
   if (stats) {
+    unsigned long long ctr[CTR_WORDS] = {0, 0};
+    HIP_TRY(hipMemcpy(ctr, c->counters.p, sizeof(ctr), hipMemcpyDeviceToHost));
     stats->molecule_steps = steps;
     stats->reactions = reactions;
-    stats->wall_hits = rounds;  // settling rounds (claims + reruns) over the call
+    stats->wall_hits = ctr[CTR_REFLECTIONS];
+    stats->transparent_crossings = ctr[CTR_CROSSINGS];
+    stats->settle_rounds = rounds;
     // movers of the last iteration that took the full step
     std::vector<uint32_t> q(kNumQueueShards, 0);
     if (num && c->n)
@@ -272,7 +279,8 @@ int ensure_react_buffers(mcg_ctx* c) {
   return MCG_OK;
 }
 
-int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg_step_stats* stats) {
+int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, double time_up_to_next_barrier,
+               mcg_step_stats* stats) {
   if (c->cfg.flags & MCG_FLAG_TRACE_HITS) return fail(MCG_E_UNSUPPORTED, "hit traces need a context without reactions");
   ReactState& r = c->rx;
   auto t0 = std::chrono::steady_clock::now();
@@ -290,16 +298,15 @@ int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mc
     r.next_id_dirty = false;
   }
   if (int st = sync_tiles_to_device(c)) return st;
-  if (stats) std::memset(stats, 0, sizeof(*stats));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   double move_ms = 0;
   int st;
   if (c->fp32()) {
     auto p = make_pipeline<float>(c, c->xf.p, c->yf.p, c->zf.p, r.sxf.p, r.syf.p, r.szf.p);
-    st = run_iterations(c, p, first_iteration, num_iterations, stats, move_ms);
+    st = run_iterations(c, p, first_iteration, num_iterations, time_up_to_next_barrier, stats, move_ms);
   } else {
     auto p = make_pipeline<double>(c, c->x.p, c->y.p, c->z.p, r.sx.p, r.sy.p, r.sz.p);
-    st = run_iterations(c, p, first_iteration, num_iterations, stats, move_ms);
+    st = run_iterations(c, p, first_iteration, num_iterations, time_up_to_next_barrier, stats, move_ms);
   }
   if (st) return st;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -351,6 +358,7 @@ extern "C" {
 
 int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   if (c->stepped) return fail(MCG_E_STATE, "reaction classes must be set before the first step");
   if (!c->rx.rxns.empty()) return fail(MCG_E_STATE, "reaction classes are already set");
@@ -409,6 +417,7 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
 
 int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   if (c->stepped) return fail(MCG_E_STATE, "reaction classes must be set before the first step");
   if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
@@ -448,6 +457,7 @@ int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
 
 int mcg_set_replica_params(mcg_ctx* c, uint32_t replica, const mcg_species* species, const mcg_bimol_rxn* rxns) {
   if (!c || !species) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   if (replica >= c->cfg.num_replicas) return fail(MCG_E_INVALID, "replica out of range");
   const uint32_t ns = c->num_species, R = c->cfg.num_replicas;
@@ -503,6 +513,7 @@ int mcg_set_replica_params(mcg_ctx* c, uint32_t replica, const mcg_species* spec
 
 int mcg_set_rxn_log(mcg_ctx* c, uint64_t capacity) {
   if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
   if (capacity > 0xFFFFFFFFull) return fail(MCG_E_INVALID, "log capacity too large");
   if (c->rx.log.alloc(capacity)) return fail(MCG_E_NOMEM, "reaction log");
   c->rx.log_cap = capacity;
@@ -512,6 +523,7 @@ int mcg_set_rxn_log(mcg_ctx* c, uint64_t capacity) {
 
 int mcg_get_rxn_log(mcg_ctx* c, mcg_rxn_event* out, uint64_t capacity, uint64_t* count) {
   if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
   static_assert(sizeof(mcg_rxn_event) == sizeof(RxnEvent), "event layout");
   const uint64_t kept = std::min(c->rx.log_count, c->rx.log_cap);
   if (count) *count = c->rx.log_count;
@@ -524,6 +536,7 @@ int mcg_get_rxn_log(mcg_ctx* c, mcg_rxn_event* out, uint64_t capacity, uint64_t*
 
 int mcg_count_species(mcg_ctx* c, uint64_t* counts) {
   if (!c || !counts) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   return count_species(c, counts);
 }
@@ -590,6 +603,7 @@ extern "C" {
 
 int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   if (c->stepped) return fail(MCG_E_STATE, "reaction classes must be set before the first step");
   if (c->per_replica) return fail(MCG_E_STATE, "reaction classes must be set before per-replica parameters");
@@ -644,6 +658,7 @@ int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
 int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, const uint32_t* species,
                               const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, uint64_t count) {
   if (!c || (count && (!ids || !species || !walls || !tiles || !orients))) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
   if (!c->have_geometry) return fail(MCG_E_STATE, "surface molecules need the geometry first");
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   if (replica >= c->cfg.num_replicas) return fail(MCG_E_INVALID, "replica out of range");
@@ -699,6 +714,7 @@ int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids,
 int mcg_get_surface_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, uint32_t* walls,
                               uint32_t* tiles, int32_t* orients, uint64_t capacity, uint64_t* count) {
   if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
   ReactState& r = c->rx;
   if (int st = download_tiles(c)) return st;
   uint64_t k = 0;

@@ -50,13 +50,13 @@ uint32_t shard_capacity(uint32_t n, uint32_t shards = kNumQueueShards);
 hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, SpeciesTable species,
                                     const SurfaceClasses& SC,
                                     const PartitionParams& P, const WallView64& W, uint64_t seed, uint32_t iteration,
-                                    SlowQueue Q, uint32_t* err, const HitTrace& trace);
+                                    SlowQueue Q, uint32_t* err, const HitTrace& trace, unsigned long long* ctr);
 
 hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t n, SpeciesTable species,
                                     const SurfaceClasses& SC,
                                     const PartitionParams& P, const WallView64& W, const WallRec32* w32,
                                     float max_abs_coord, uint64_t seed, uint32_t iteration, SlowQueue Q,
-                                    uint32_t* err, const HitTrace& trace);
+                                    uint32_t* err, const HitTrace& trace, unsigned long long* ctr);
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 

@@ -73,7 +73,8 @@ __global__ __launch_bounds__(256) void k_light64(MolView64 m, uint32_t n, Specie
 
 __global__ __launch_bounds__(256) void k_hard64(MolView64 m, SpeciesTable species, SurfaceClasses SC,
                                                 PartitionParams P, WallView64 W, uint64_t seed, uint32_t iteration,
-                                                SlowQueue Q, uint32_t* __restrict__ err_out, HitTrace trace) {
+                                                SlowQueue Q, uint32_t* __restrict__ err_out, HitTrace trace,
+                                                unsigned long long* __restrict__ ctr) {
   __shared__ ZigguratTables T;
   __shared__ uint32_t pre[kQueueShards + 1];
   __shared__ uint32_t part[256];
@@ -81,7 +82,12 @@ __global__ __launch_bounds__(256) void k_hard64(MolView64 m, SpeciesTable specie
   const uint32_t total = queue_prefix(Q.count, Q.nshards, pre, part);
   if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_ziggurat(T);
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+  uint32_t n_reflect = 0, n_cross = 0;
+  // every thread of the block runs the same number of rounds (the counters
+  // are wave-summed after the loop)
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    if (q >= total) continue;
     const uint32_t i = queue_at(Q.slots, Q.shard_cap, pre, q);
     const uint32_t tag = m.tag[i];
     const SpeciesRec sp = species.of(tag);
@@ -91,7 +97,8 @@ __global__ __launch_bounds__(256) void k_hard64(MolView64 m, SpeciesTable specie
     const D3 disp = draw_displacement(rng, T, sp.space_step);
     uint32_t err = 0, tn = 0;
     uint32_t* tw = trace.enabled ? trace.wall + (size_t)i * kTraceHits : nullptr;
-    const bool alive = diffuse_reflect64(P, W.walls, W.off, W.idx, W.box, SC, tag_species(tag), pos, disp, rng, err, tw, tn);
+    const bool alive = diffuse_reflect64(P, W.walls, W.off, W.idx, W.box, SC, tag_species(tag), pos, disp, rng, err,
+                                         tw, tn, n_reflect, n_cross);
     m.x[i] = pos.x;
     m.y[i] = pos.y;
     m.z[i] = pos.z;
@@ -99,12 +106,15 @@ __global__ __launch_bounds__(256) void k_hard64(MolView64 m, SpeciesTable specie
     if (trace.enabled) trace.count[i] = tn;
     if (err) atomicOr(err_out, err);
   }
+  wave_count(ctr + CTR_REFLECTIONS, n_reflect);
+  wave_count(ctr + CTR_CROSSINGS, n_cross);
 }
 
 // ------------------------------------------------------------------- FP32 --
 __global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, SpeciesTable species,
                                                  PartitionParams P, Walls32View W, FilterParams F, float band,
-                                                 uint64_t seed, uint32_t iteration, SlowQueue Q, HitTrace trace) {
+                                                 uint64_t seed, uint32_t iteration, SlowQueue Q, HitTrace trace,
+                                                 uint32_t* __restrict__ err_out) {
   __shared__ ZigguratTables T;
   stage_ziggurat(T);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -126,7 +136,8 @@ __global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, Specie
         // full step's would
         float fx, fy, fz;
         if (near_plane) {
-          round_to_fp32_same_side(P, W.w64, W.off, W.idx, W.box, end.x, end.y, end.z, &fx, &fy, &fz);
+          if (!round_to_fp32_same_side(P, W.w64, W.off, W.idx, W.box, end.x, end.y, end.z, &fx, &fy, &fz))
+            atomicOr(err_out, ERR_ROUND32);
         } else {
           fx = (float)end.x;
           fy = (float)end.y;
@@ -148,7 +159,8 @@ __global__ __launch_bounds__(256) void k_light32(MolView32 m, uint32_t n, Specie
 __global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable species, SurfaceClasses SC,
                                                 PartitionParams P, Walls32View W,
                                                 uint64_t seed, uint32_t iteration, SlowQueue Q,
-                                                uint32_t* __restrict__ err_out, HitTrace trace) {
+                                                uint32_t* __restrict__ err_out, HitTrace trace,
+                                                unsigned long long* __restrict__ ctr) {
   __shared__ ZigguratTables T;
   __shared__ uint32_t pre[kQueueShards + 1];
   __shared__ uint32_t part[256];
@@ -156,7 +168,10 @@ __global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable specie
   const uint32_t total = queue_prefix(Q.count, Q.nshards, pre, part);
   if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_ziggurat(T);
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+  uint32_t n_reflect = 0, n_cross = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t q = base + threadIdx.x;
+    if (q >= total) continue;
     const uint32_t i = queue_at(Q.slots, Q.shard_cap, pre, q);
     const uint32_t tag = m.tag[i];
     const SpeciesRec sp = species.of(tag);
@@ -171,16 +186,19 @@ __global__ __launch_bounds__(256) void k_hard32(MolView32 m, SpeciesTable specie
     // it saved), then the end point rounded to FP32 on its side of nearby
     // wall planes
     const bool alive = diffuse_reflect64(P, W.w64, W.off, W.idx, W.box, SC, tag_species(tag), pos, disp, rng, err,
-                                         tw, tn);
+                                         tw, tn, n_reflect, n_cross);
     if (!alive) m.tag[i] = tag | (MF_DEFUNCT << 8);
     float fx, fy, fz;
-    round_to_fp32_same_side(P, W.w64, W.off, W.idx, W.box, pos.x, pos.y, pos.z, &fx, &fy, &fz);
+    if (!round_to_fp32_same_side(P, W.w64, W.off, W.idx, W.box, pos.x, pos.y, pos.z, &fx, &fy, &fz))
+      err |= ERR_ROUND32;
     m.x[i] = fx;
     m.y[i] = fy;
     m.z[i] = fz;
     if (trace.enabled) trace.count[i] = tn;
     if (err) atomicOr(err_out, err);
   }
+  wave_count(ctr + CTR_REFLECTIONS, n_reflect);
+  wave_count(ctr + CTR_CROSSINGS, n_cross);
 }
 
 // Copy kernel of known size (device-to-device bandwidth reference).
@@ -203,14 +221,15 @@ uint32_t shard_capacity(uint32_t n, uint32_t shards) {
 hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, SpeciesTable species,
                                     const SurfaceClasses& SC,
                                     const PartitionParams& P, const WallView64& W, uint64_t seed, uint32_t iteration,
-                                    SlowQueue Q, uint32_t* err, const HitTrace& trace) {
+                                    SlowQueue Q, uint32_t* err, const HitTrace& trace, unsigned long long* ctr) {
   if (n == 0) return hipSuccess;
   Q.shard_cap = shard_capacity(n);
   hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
   hipLaunchKernelGGL(k_light64, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W, seed, iteration, Q,
                      trace);
   Q.nshards = std::min<uint32_t>(kQueueShards, (n + 255) / 256);
-  hipLaunchKernelGGL(k_hard64, dim3(Q.nshards), dim3(256), 0, s, m, species, SC, P, W, seed, iteration, Q, err, trace);
+  hipLaunchKernelGGL(k_hard64, dim3(Q.nshards), dim3(256), 0, s, m, species, SC, P, W, seed, iteration, Q, err, trace,
+                     ctr);
   return hipGetLastError();
 }
 
@@ -218,7 +237,7 @@ hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t 
                                     const SurfaceClasses& SC,
                                     const PartitionParams& P, const WallView64& W, const WallRec32* w32,
                                     float max_abs_coord, uint64_t seed, uint32_t iteration, SlowQueue Q,
-                                    uint32_t* err, const HitTrace& trace) {
+                                    uint32_t* err, const HitTrace& trace, unsigned long long* ctr) {
   if (n == 0) return hipSuccess;
   FilterParams F = make_filter_params(max_abs_coord);
   float band = 2.0f * F.e_dd + 8.0f * F.eps * max_abs_coord;
@@ -227,10 +246,10 @@ hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t 
   Q.shard_cap = shard_capacity(n);
   hipLaunchKernelGGL(k_reset_counters, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, kQueueShards);
   hipLaunchKernelGGL(k_light32, dim3((n + 255) / 256), dim3(256), 0, s, m, n, species, P, W32, F, band, seed,
-                     iteration, Q, trace);
+                     iteration, Q, trace, err);
   Q.nshards = std::min<uint32_t>(kQueueShards, (n + 255) / 256);
   hipLaunchKernelGGL(k_hard32, dim3(Q.nshards), dim3(256), 0, s, m, species, SC, P, W32, seed, iteration, Q, err,
-                     trace);
+                     trace, ctr);
   return hipGetLastError();
 }
 

@@ -82,6 +82,7 @@ struct ReactPipeline {
 };
 
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n);
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v, uint32_t n);
 hipError_t launch_copy_u32(hipStream_t s, const uint32_t* src, uint32_t* dst);
 
 }  // namespace mcg

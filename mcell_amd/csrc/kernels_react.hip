@@ -56,6 +56,10 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 
 // ------------------------------------------------------------- cell sort --
 template <typename PosT>
@@ -116,18 +120,21 @@ __global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, c
 
 // ----------------------------------------------------------------- moves --
 // FP32 form: the end point in FP32, kept on the FP64 point's side of nearby
-// wall planes.
-__device__ __forceinline__ void store_pos(const ReactParams& R, double* x, double* y, double* z, uint32_t j, D3 p,
-                                          bool) {
+// wall planes.  Returns the error bit of a rounding that did not settle.
+__device__ __forceinline__ uint32_t store_pos(const ReactParams& R, double* x, double* y, double* z, uint32_t j, D3 p,
+                                              bool) {
   x[j] = p.x;
   y[j] = p.y;
   z[j] = p.z;
+  return 0;
 }
-__device__ __forceinline__ void store_pos(const ReactParams& R, float* x, float* y, float* z, uint32_t j, D3 p,
-                                          bool near_plane) {
+__device__ __forceinline__ uint32_t store_pos(const ReactParams& R, float* x, float* y, float* z, uint32_t j, D3 p,
+                                              bool near_plane) {
   float fx, fy, fz;
+  uint32_t e = 0;
   if (near_plane) {
-    round_to_fp32_same_side(R.P, R.walls, R.off, R.idx, R.ent_box, p.x, p.y, p.z, &fx, &fy, &fz);
+    if (!round_to_fp32_same_side(R.P, R.walls, R.off, R.idx, R.ent_box, p.x, p.y, p.z, &fx, &fy, &fz))
+      e = ERR_ROUND32;
   } else {
     fx = (float)p.x;
     fy = (float)p.y;
@@ -136,6 +143,7 @@ __device__ __forceinline__ void store_pos(const ReactParams& R, float* x, float*
   x[j] = fx;
   y[j] = fy;
   z[j] = fz;
+  return e;
 }
 
 // diffuse_single_molecule's time bookkeeping (diffuse_react_event.cpp:
@@ -146,6 +154,19 @@ __device__ __forceinline__ double next_toff(uint32_t iteration, double toff, dou
   const double r = rint(t);
   if (cmp_eq_d(t, r, 1e-6)) t = r;
   return t - (double)(iteration + 1);
+}
+
+// Partition::get_molecules_ready_for_diffusion (partition.h:232-246): the
+// molecule's diffusion time lies before the iteration's end,
+// cmp_lt(diffusion_time, iteration + 1, EPS).
+__device__ __forceinline__ bool ready_this_iteration(uint32_t iteration, double toff) {
+  const double t = (double)iteration + toff, e = (double)(iteration + 1);
+  return cmp_lt_d(t, e, 1e-12);
+}
+// The offset of an unmoved molecule's diffusion time past the next
+// iteration's start (exact: both are times of one iteration range).
+__device__ __forceinline__ double toff_past_iteration(uint32_t iteration, double toff) {
+  return ((double)iteration + toff) - (double)(iteration + 1);
 }
 
 // The snapshot range of species t's molecules in the row of cells
@@ -197,14 +218,17 @@ __device__ __forceinline__ void append_slot(bool want, uint32_t j, uint32_t* __r
 }
 
 // The full step of the mover in slot j (react_step.h): writes its end state,
-// or records its proposal and claims.  Returns whether it proposed.
+// or records its proposal and claims.  Returns whether it proposed.  The
+// reflections and crossings of a committed step are added to n_reflect /
+// n_cross; a proposal carries those before its reaction in its pad word
+// (counted when it is accepted: a losing mover repeats its step).
 template <typename PosT>
 __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<PosT>& S, const RMolView<PosT>& src,
                                          const RMolView<PosT>& out, uint32_t j, SpeciesTable species,
                                          const ZigguratTables& T, double barrier, uint32_t propose_only,
                                          Proposal* __restrict__ prop, uint32_t* __restrict__ owner,
                                          uint32_t* __restrict__ tile_owner, uint32_t* __restrict__ err_out,
-                                         bool& due, uint32_t* set_lds) {
+                                         bool& due, uint32_t* set_lds, uint32_t& n_reflect, uint32_t& n_cross) {
   const uint32_t tag = src.tag[j];
   const uint32_t idv = src.id[j];
   const double toff = load_toff(src.toff, tag, j);
@@ -219,6 +243,8 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   e.near_plane = false;
   e.absorbed = false;
   e.due = 0;
+  e.n_reflect = 0;
+  e.n_cross = 0;
   Proposal p;
   bool proposed = false;
   const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
@@ -242,7 +268,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   }
   if (proposed) {
     p.mover_slot = j;
-    p.pad = 0;
+    p.pad = (e.n_reflect > 0xFFFFu ? 0xFFFFu : e.n_reflect) | ((e.n_cross > 0xFFFFu ? 0xFFFFu : e.n_cross) << 16);
     prop[j] = p;
     if (!propose_only) {
       atomicMin(owner + j, idv);
@@ -251,12 +277,14 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
     }
   }
   if (!proposed || propose_only) {
-    store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
+    err |= store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
     out.id[j] = idv;
     uint32_t t2 = tag & ~(MF_SCHEDULE_UNIMOL << 8);
     t2 = store_toff(out.toff, j, sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff, t2);
     out.tag[j] = e.absorbed ? (t2 | (MF_DEFUNCT << 8)) : t2;
     if (out.utime) out.utime[j] = e.utime;
+    n_reflect += e.n_reflect;
+    n_cross += e.n_cross;
   }
   if (err) atomicOr(err_out, err);
   return proposed;
@@ -285,14 +313,16 @@ __global__ __launch_bounds__(256, 2) void k_move(ReactParams R, Snapshot<PosT> S
   const uint32_t n = *count;
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   bool proposed = false, due = false;
-  uint32_t j = 0;
+  uint32_t j = 0, n_reflect = 0, n_cross = 0;
   if (q < n) {
     j = list ? list[q] : q;
     proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner, err_out, due,
-                        sets + threadIdx.x);
+                        sets + threadIdx.x, n_reflect, n_cross);
   }
   append_slot(proposed, j, plist, pcount);
   append_slot(due, j, ulist, ucount);
+  wave_count(R.ctr + CTR_REFLECTIONS, n_reflect);
+  wave_count(R.ctr + CTR_CROSSINGS, n_cross);
 }
 
 // The full step for every queued mover (the shards read as one list,
@@ -313,6 +343,7 @@ __global__ __launch_bounds__(256, 2) void k_move_drain(ReactParams R, Snapshot<P
   const uint32_t total = queue_prefix(Q.count, Q.nshards, pre, part);
   if ((uint64_t)blockIdx.x * blockDim.x >= total) return;
   stage_zig(T);
+  uint32_t n_reflect = 0, n_cross = 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
     bool proposed = false, due = false;
@@ -320,11 +351,13 @@ __global__ __launch_bounds__(256, 2) void k_move_drain(ReactParams R, Snapshot<P
     if (q < total) {
       j = queue_at(Q.slots, Q.shard_cap, pre, q);
       proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner, err_out,
-                          due, sets + threadIdx.x);
+                          due, sets + threadIdx.x, n_reflect, n_cross);
     }
     append_slot(proposed, j, plist, pcount);
     append_slot(due, j, ulist, ucount);
   }
+  wave_count(R.ctr + CTR_REFLECTIONS, n_reflect);
+  wave_count(R.ctr + CTR_CROSSINGS, n_cross);
 }
 
 // The common step, register-light: movers whose straight move certainly
@@ -343,13 +376,14 @@ __global__ __launch_bounds__(256, 2) void k_move_drain(ReactParams R, Snapshot<P
 template <typename PosT>
 __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<PosT> S, RMolView<PosT> out,
                                                     const uint32_t* __restrict__ count, SpeciesTable species,
-                                                    double barrier, SlowQueue Q) {
+                                                    double barrier, SlowQueue Q, uint32_t* __restrict__ err_out) {
   constexpr bool kF32 = sizeof(PosT) == 4;
   __shared__ ZigguratTables T;
   stage_zig(T);
   const uint32_t n = *count;
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   bool hard = false, partner_hard = false;
+  uint32_t err = 0;
   if (j < n) {
     const uint32_t tag = S.tag[j];
     const uint32_t idv = S.id[j];
@@ -361,7 +395,14 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
     // lifetime is still to be drawn or ends within the step's time budget
     const bool unimol_now = R.unimol[tag_species(tag)].num_pathways > 0 &&
                             ((tag_flags(tag) & MF_SCHEDULE_UNIMOL) || utime < (double)R.iteration + barrier);
-    if (unimol_now) {
+    if (!ready_this_iteration(R.iteration, toff)) {
+      // scheduled for a later iteration (Partition::get_molecules_ready_for_
+      // diffusion, partition.h:232-246): stays where it is
+      store_pos(R, out.x, out.y, out.z, j, pos, false);
+      out.id[j] = idv;
+      out.tag[j] = store_toff(out.toff, j, toff_past_iteration(R.iteration, toff), tag);
+      if (out.utime) out.utime[j] = utime;
+    } else if (unimol_now) {
       hard = true;
     } else if (sp.space_step > 0) {
       PhiloxStream rng;
@@ -507,7 +548,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
       }
       if (clear) {
         // an end point next to a wall plane keeps its side in FP32
-        store_pos(R, out.x, out.y, out.z, j, end, near_plane);
+        err |= store_pos(R, out.x, out.y, out.z, j, end, near_plane);
         out.id[j] = idv;
         out.tag[j] = store_toff(out.toff, j, next_toff(R.iteration, toff, t_steps), tag & ~(MF_SCHEDULE_UNIMOL << 8));
         if (out.utime) out.utime[j] = utime;
@@ -525,6 +566,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
   // movers queued for walls (and unimolecular events) and movers queued for
   // partners go to separate halves of the shards, so that the full step's
   // lanes mostly follow the same path
+  if (err) atomicOr(err_out, err);
   constexpr uint32_t kHalf = kQueueShards / 2;
   const uint32_t s0 = blockIdx.x % kHalf, s1 = kHalf + blockIdx.x % kHalf;
   wave_append(hard && !partner_hard, j, Q.slots + (size_t)s0 * Q.shard_cap, Q.count + s0);
@@ -539,8 +581,9 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
                                                 uint32_t* __restrict__ consumed, uint32_t* __restrict__ tile_consumed,
                                                 uint32_t* __restrict__ out_tag,
                                                 const RxnRec* __restrict__ rxns, uint32_t* __restrict__ alist,
-                                                uint32_t* __restrict__ acount) {
+                                                uint32_t* __restrict__ acount, unsigned long long* __restrict__ ctr) {
   const uint32_t n = *pcount;
+  uint32_t n_reflect = 0, n_cross = 0;
   for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < n; q0 += gridDim.x * blockDim.x) {
   const uint32_t q = q0 + threadIdx.x;
   bool acc = false;
@@ -552,6 +595,8 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
     const uint32_t ps = p.partner_slot & ~kTileBit;
     if (owner[j] == p.mover_id && (tile ? tile_owner[ps] : owner[ps]) == p.mover_id) {
       acc = true;
+      n_reflect += p.pad & 0xFFFFu;  // the walls the mover met before the reaction
+      n_cross += p.pad >> 16;
       // outcome_bimolecular (diffuse_react_event.cpp:1833-1895): both
       // reactants are consumed (reactions keeping a reactant are refused at
       // set-up); a surface reactant's tile goes to its product
@@ -575,6 +620,8 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
     if (acc) alist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
   }
   }
+  wave_count(ctr + CTR_REFLECTIONS, n_reflect);
+  wave_count(ctr + CTR_CROSSINGS, n_cross);
 }
 
 // Unimolecular events of the phase (react_unimol_single_molecule +
@@ -751,7 +798,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
       atomicOr(err, ERR_QUEUE);
       break;
     }
-    store_pos(R, out.x, out.y, out.z, slot, D3{p.x, p.y, p.z}, true);
+    if (store_pos(R, out.x, out.y, out.z, slot, D3{p.x, p.y, p.z}, true)) atomicOr(err, ERR_ROUND32);
     out.id[slot] = first_id + i;
     out.tag[slot] = store_toff(out.toff, slot, p.time - (double)(R.iteration + (moves_now ? 0u : 1u)),
                                make_tag(prods[i], MF_SCHEDULE_UNIMOL, rep));
@@ -777,14 +824,19 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
   }
 }
 
+// The used output slots never exceed the capacity: k_products wrote no
+// product past it (and reported ERR_QUEUE).
 __global__ void k_bump_ids(const uint64_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ pre,
                            const uint32_t* __restrict__ head, uint32_t* __restrict__ next_id,
-                           uint32_t* __restrict__ hdr) {
+                           uint32_t* __restrict__ hdr, uint32_t capacity) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const uint32_t rep = (uint32_t)(keys[k] >> 32);
   if (k == n - 1 || (uint32_t)(keys[k + 1] >> 32) != rep) next_id[rep] += pre[k + 1] - pre[head[rep]];
-  if (k == n - 1) hdr[kHdrNumOut] += pre[n];
+  if (k == n - 1) {
+    const uint64_t used = (uint64_t)hdr[kHdrNumOut] + pre[n];
+    hdr[kHdrNumOut] = used < capacity ? (uint32_t)used : capacity;
+  }
 }
 
 // Proposals logged without being carried out (the snapshot parity mode).
@@ -909,7 +961,7 @@ hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uin
     Q.shard_cap = shard_capacity(max_movers, kQueueShards / 2);
     hipLaunchKernelGGL(k_fill_u32, dim3(kQueueShards / 256), dim3(256), 0, s, Q.count, 0u, kQueueShards);
     hipLaunchKernelGGL(k_move_light<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, M, count, species, barrier,
-                       Q);
+                       Q, err);
     Q.nshards = kQueueShards / 2 + std::min<uint32_t>(kQueueShards / 2, blocks_for(max_movers).x);
     hipLaunchKernelGGL(k_move_drain<PosT>, dim3(Q.nshards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
                        propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist,
@@ -927,7 +979,7 @@ hipError_t ReactPipeline<PosT>::settle(hipStream_t s, uint32_t max_props, uint32
   // grid-stride over the device's proposal count; max_props bounds it
   const dim3 grid = dim3(std::min<uint32_t>(blocks_for(max_props).x, 1024u));
   hipLaunchKernelGGL(k_accept<PosT>, grid, dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
-                     tile_owner, consumed, tile_consumed, M.tag, R.rxns, alist, hdr + kHdrAccepted);
+                     tile_owner, consumed, tile_consumed, M.tag, R.rxns, alist, hdr + kHdrAccepted, R.ctr);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, rcount, 0u, 1u);
   hipLaunchKernelGGL(k_requeue, grid, dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
                      tile_owner, consumed, rlist, rcount);
@@ -960,7 +1012,8 @@ hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t
   hipLaunchKernelGGL(k_products<PosT>, blocks_for(n_acc), dim3(256), 0, s, R, alist_sorted, keys_b, n_acc, prop, pre,
                      head, next_id, hdr, capacity, M, consumed, owner, mlist, hdr + kHdrProductMovers, log, log_base,
                      log_cap, err, tiles, tile_id, tile_consumed);
-  hipLaunchKernelGGL(k_bump_ids, blocks_for(n_acc), dim3(256), 0, s, keys_b, n_acc, pre, head, next_id, hdr);
+  hipLaunchKernelGGL(k_bump_ids, blocks_for(n_acc), dim3(256), 0, s, keys_b, n_acc, pre, head, next_id, hdr,
+                     capacity);
   return hipGetLastError();
 }
 
@@ -992,6 +1045,12 @@ template struct ReactPipeline<float>;
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, blocks_for(n), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, blocks_for(n), dim3(256), 0, s, p, v, n);
   return hipGetLastError();
 }
 

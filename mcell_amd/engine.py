@@ -24,7 +24,11 @@ def _ptr(a: Optional[np.ndarray]):
 class DiffuseReactEngine:
     def __init__(self, part: PartitionConfig, precision: str = "fp64", seed: int = 1, num_replicas: int = 1,
                  capacity: int = 1 << 20, trace: bool = False, barrier_interval: int = 1,
-                 propose_only: bool = False):
+                 propose_only: bool = False, start_iteration: int = 0, barriers: Optional[Sequence[int]] = None):
+        """barrier_interval / barriers: the scheduler's barriers (count
+        events): every barrier_interval iterations, or at the iterations
+        listed in barriers.  step() hands the engine the time up to the next
+        one, as Scheduler::handle_next_event does (scheduler.cpp:303-307)."""
         self.lib = _lib.load()
         self.part = part
         cfg = _lib.McgConfig()
@@ -43,21 +47,25 @@ class DiffuseReactEngine:
         cfg.seed = seed
         cfg.molecule_capacity = capacity
         cfg.flags = (_lib.MCG_FLAG_TRACE_HITS if trace else 0) | (_lib.MCG_FLAG_PROPOSE_ONLY if propose_only else 0)
-        cfg.barrier_interval = barrier_interval
+        cfg.start_iteration = start_iteration
+        self.barrier_interval = barrier_interval
+        self.barriers = None if barriers is None else sorted(int(b) for b in barriers)
+        self.num_replicas = num_replicas
         self.ctx = ctypes.c_void_p()
         _lib.check(self.lib.mcg_create(ctypes.byref(cfg), ctypes.byref(self.ctx)))
         self.trace = trace
 
     @classmethod
     def from_workload(cls, wl, precision: str = "fp64", seed: Optional[int] = None, capacity: Optional[int] = None,
-                      propose_only: bool = False, log_capacity: int = 0, trace: bool = False):
+                      propose_only: bool = False, log_capacity: int = 0, trace: bool = False,
+                      add_molecules: bool = True, start_iteration: int = 0, barriers=None):
         """An engine holding the workload's geometry, species, reaction
-        classes and molecules (every replica)."""
+        classes and (with add_molecules) molecules of every replica."""
         from .workloads import rxn_table, species_steps, unimol_table, volsurf_table
         cap = capacity if capacity is not None else max(2 * wl.num_molecules, 1024)
         eng = cls(wl.part, precision=precision, seed=wl.config.seed if seed is None else seed,
                   num_replicas=wl.num_replicas, capacity=cap, trace=trace, barrier_interval=wl.barrier_interval,
-                  propose_only=propose_only)
+                  propose_only=propose_only, start_iteration=start_iteration, barriers=barriers)
         eng.set_geometry(wl.vertices, wl.walls, wl.wall_class)
         eng.set_species(wl.species_steps)
         if wl.surface_classes is not None:
@@ -73,6 +81,8 @@ class DiffuseReactEngine:
                 eng.set_replica_params(r, species_steps(wl, r), rxn_table(wl, r) if wl.rxns else None)
         if log_capacity:
             eng.set_rxn_log(log_capacity)
+        if not add_molecules:
+            return eng
         for r in range(wl.num_replicas):
             sel = wl.replica == r
             if sel.any():
@@ -211,17 +221,66 @@ class DiffuseReactEngine:
         _lib.check(self.lib.mcg_count_species(self.ctx, _ptr(out)))
         return out
 
-    def add_volume_molecules(self, replica: int, ids: np.ndarray, species: np.ndarray, positions: np.ndarray):
+    def add_volume_molecules(self, replica: int, ids: np.ndarray, species: np.ndarray, positions: np.ndarray,
+                             diffusion_times=None, unimol_times=None, flags=None):
         ids = np.ascontiguousarray(ids, dtype=np.uint32)
         sp = np.ascontiguousarray(species, dtype=np.uint32)
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
         assert len(ids) == len(sp) == len(pos)
-        _lib.check(self.lib.mcg_add_volume_molecules(self.ctx, replica, _ptr(ids), _ptr(sp), _ptr(pos), len(ids)))
+        dt = None if diffusion_times is None else np.ascontiguousarray(diffusion_times, dtype=np.float64)
+        ut = None if unimol_times is None else np.ascontiguousarray(unimol_times, dtype=np.float64)
+        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint32)
+        for a in (dt, ut, fl):
+            assert a is None or len(a) == len(ids)
+        _lib.check(self.lib.mcg_add_volume_molecules(self.ctx, replica, _ptr(ids), _ptr(sp), _ptr(pos), _ptr(dt),
+                                                     _ptr(ut), _ptr(fl), len(ids)))
+
+    def next_molecule_ids(self) -> np.ndarray:
+        out = np.zeros(self.num_replicas, np.uint32)
+        _lib.check(self.lib.mcg_get_next_molecule_ids(self.ctx, _ptr(out)))
+        return out
+
+    def set_next_molecule_ids(self, next_ids):
+        a = np.ascontiguousarray(next_ids, dtype=np.uint32)
+        assert len(a) == self.num_replicas
+        _lib.check(self.lib.mcg_set_next_molecule_ids(self.ctx, _ptr(a)))
 
     # --------------------------------------------------------------- run --
+    def next_barrier(self, it: int) -> Optional[int]:
+        """The first barrier strictly after iteration it (Calendar::
+        get_time_up_to_next_barrier, scheduler.cpp:212-242), or None."""
+        if self.barriers is not None:
+            for b in self.barriers:
+                if b > it:
+                    return b
+            return None
+        b = max(int(self.barrier_interval), 1)
+        return (it // b + 1) * b
+
     def step(self, first_iteration: int, num_iterations: int = 1) -> _lib.McgStepStats:
+        """Runs iterations first_iteration .. first_iteration + num_iterations
+        - 1, one mcg_step call per stretch between barriers, each handed the
+        time up to its barrier.  Returns the summed statistics."""
+        total = _lib.McgStepStats()
+        it, end = first_iteration, first_iteration + num_iterations
+        while it < end:
+            nb = self.next_barrier(it)
+            time_up = (nb - it) if nb is not None else (end - it) + 100
+            chunk = min(end - it, time_up)
+            st = _lib.McgStepStats()
+            _lib.check(self.lib.mcg_step(self.ctx, it, chunk, float(time_up), ctypes.byref(st)))
+            for f in ("molecule_steps", "wall_hits", "reactions", "kernel_ms", "wall_ms", "kernel_launches",
+                      "transparent_crossings", "settle_rounds"):
+                setattr(total, f, getattr(total, f) + getattr(st, f))
+            total.slow_path_last = st.slow_path_last
+            it += chunk
+        return total
+
+    def step_raw(self, first_iteration: int, num_iterations: int, time_up_to_next_barrier: float):
+        """One mcg_step call as the C ABI takes it."""
         st = _lib.McgStepStats()
-        _lib.check(self.lib.mcg_step(self.ctx, first_iteration, num_iterations, ctypes.byref(st)))
+        _lib.check(self.lib.mcg_step(self.ctx, first_iteration, num_iterations, float(time_up_to_next_barrier),
+                                     ctypes.byref(st)))
         return st
 
     def sync(self):
@@ -246,10 +305,14 @@ class DiffuseReactEngine:
         rep = np.zeros(n, np.uint32)
         pos = np.zeros((n, 3), np.float64)
         sub = np.zeros(n, np.uint32)
+        dt = np.zeros(n, np.float64)
+        ut = np.zeros(n, np.float64)
+        fl = np.zeros(n, np.uint32)
         cnt = ctypes.c_uint64()
         _lib.check(self.lib.mcg_get_volume_molecules(self.ctx, _ptr(ids), _ptr(sp), _ptr(rep), _ptr(pos), _ptr(sub),
-                                                     n, ctypes.byref(cnt)))
-        return {"id": ids, "species": sp, "replica": rep, "pos": pos, "subpart": sub}
+                                                     _ptr(dt), _ptr(ut), _ptr(fl), n, ctypes.byref(cnt)))
+        return {"id": ids, "species": sp, "replica": rep, "pos": pos, "subpart": sub, "diffusion_time": dt,
+                "unimol_time": ut, "flags": fl}
 
     def walls(self):
         cnt = ctypes.c_uint64()

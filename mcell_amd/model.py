@@ -29,6 +29,7 @@ N_AV = 6.0221417930e23
 EPS = 1e-12
 PARTITION_EDGE_EXTRA_MARGIN_UM = 0.01
 MAX_SUBPARTS_PER_PARTITION = 300
+POS_SQRT2 = 1.41421356238  # src4/defines.h:148
 
 
 # --------------------------------------------------------------- geometry ---
@@ -213,6 +214,10 @@ def derive_partition(cfg: Config, bbox_um, has_bimol_vol_rxns: bool) -> Partitio
     sp_edge = pel / float(n)
     rcp = 1.0 / sp_edge
     pel = sp_edge * n
+    # MCell4Converter's check (mcell4_converter.cpp:121-127)
+    if not rxn_radius_3d * POS_SQRT2 < sp_edge / 2:
+        raise ValueError("Reaction radius multiplied by sqrt(2) %g must be less than half of subpartition edge "
+                         "length %g" % (rxn_radius_3d * length_unit * POS_SQRT2, sp_edge * length_unit / 2))
     return PartitionConfig(time_unit, length_unit, rcp_length_unit, grid_density, rxn_radius_3d, pel,
                            tuple(origin), sp_edge, rcp, n, 1 if has_bimol_vol_rxns else 0)
 

@@ -103,6 +103,33 @@ def free_diffusion_box(scale: float = 1.0, iterations: int = 1000, seed: int = 1
                     config=cfg, objects=[box])
 
 
+def box_at_partition_faces(n: int = 20000, iterations: int = 50, seed: int = 7) -> Workload:
+    """Free diffusion in the reflective 1 um cube with the partition sized to
+    the cube (partition_dimension below the cube's extent: the converter
+    takes the bounding box plus its 0.01 um margin, mcell4_converter.cpp:
+    280-310) and 0.02 um subpartitions, so that every face of the cube lies
+    2 internal units inside a partition face.  The molecules start within
+    1.5 units of the walls: a move towards a wall often ends outside the
+    partition, and ray_trace_vol clips it at the partition face before the
+    wall search (get_displacement_up_to_partition_boundary,
+    collision_utils.inl:48-114)."""
+    cfg = M.Config(time_step=1e-6, subpartition_dimension=0.02, partition_dimension=0.5, seed=seed)
+    box = M.create_box("box", edge_dimension=1.0)
+    part = M.derive_partition(cfg, M.bounding_box([box]), has_bimol_vol_rxns=False)
+    verts, walls, classes = M.vertices_internal([box], part)
+    species = [M.Species("A", 1e-6)]
+    rng = np.random.default_rng(seed)
+    half = 0.5 / part.length_unit
+    pos = rng.uniform(-half, half, size=(n, 3))
+    # push one coordinate of every molecule to within 1.5 units of a face
+    axis = rng.integers(0, 3, n)
+    side = np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    pos[np.arange(n), axis] = side * (half - rng.uniform(0.01, 1.5, n))
+    return Workload("box_at_partition_faces", part, verts, walls, classes, species,
+                    _species_table(species, part, []), np.arange(n, dtype=np.uint32), np.zeros(n, np.uint32), pos,
+                    np.zeros(n, np.uint32), iterations, config=cfg, objects=[box])
+
+
 def free_diffusion_replicas(num_replicas: int, mols_per_replica: int, iterations: int = 100, seed: int = 4,
                             subpartition_dimension: float = 0.1, box_um: float = 1.0) -> Workload:
     """Many independent free-diffusion instances in one shared reflective cube

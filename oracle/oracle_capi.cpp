@@ -230,6 +230,9 @@ void orc_set_rng(void* wp, int mode, uint64_t seed, uint32_t replica) {
 void orc_set_snapshot(void* wp, int on) { static_cast<World*>(wp)->snapshot = on != 0; }
 void orc_set_trace(void* wp, int on) { static_cast<World*>(wp)->trace = on != 0; }
 void orc_set_barrier_interval(void* wp, uint32_t b) { static_cast<World*>(wp)->barrier_interval = b; }
+void orc_set_barriers(void* wp, const uint64_t* b, uint32_t n) {
+  static_cast<World*>(wp)->barrier_list.assign(b, b + n);
+}
 
 // reaction records: u32 x4 (iteration, mover, partner, pathway) + rxn index,
 // f64 x4 (time, pos xyz)
@@ -301,12 +304,12 @@ uint64_t orc_get_hits(void* wp, uint32_t* u, double* t) {
 void orc_clear_hits(void* wp) { static_cast<World*>(wp)->hits.clear(); }
 
 // ray_voxel_tests, ray_polygon_tests, ray_polygon_colls, mol_wall_reflections,
-// vol_mol_vol_mol_collisions, diffuse_3d_calls, redo_count, rng draws
+// vol_mol_vol_mol_collisions, diffuse_3d_calls, redo_count, rng draws, partition clips
 void orc_get_stats(void* wp, uint64_t* out) {
   World* w = static_cast<World*>(wp);
   out[0] = w->ray_voxel_tests; out[1] = w->ray_polygon_tests; out[2] = w->ray_polygon_colls;
   out[3] = w->mol_wall_reflections; out[4] = w->vol_mol_vol_mol_collisions; out[5] = w->diffuse_3d_calls;
-  out[6] = w->redo_count; out[7] = w->isaac.draws;
+  out[6] = w->redo_count; out[7] = w->isaac.draws; out[8] = w->partition_clips;
 }
 
 // ------------------------------------------------------------ unit functions

@@ -341,6 +341,7 @@ class World {
   uint64_t ray_voxel_tests = 0, ray_polygon_tests = 0, ray_polygon_colls = 0;
   uint64_t mol_wall_reflections = 0, vol_mol_vol_mol_collisions = 0, diffuse_3d_calls = 0;
   uint64_t redo_count = 0;
+  uint64_t partition_clips = 0;  // ray segments clipped at the partition face
 
   // snapshot mode: each mover sees the start-of-iteration positions and no
   // molecule is consumed while the iteration runs; the first reaction a mover
@@ -348,8 +349,10 @@ class World {
   bool snapshot = false;
 
   // iterations between scheduler barriers (count events); the time a product
-  // may diffuse past its creation is clipped at the next barrier
+  // may diffuse past its creation is clipped at the next barrier.  A non-empty
+  // barrier list (ascending iterations) replaces the periodic rule.
   uint32_t barrier_interval = 1;
+  std::vector<uint64_t> barrier_list;
 
   explicit World(const Config& c);
   void set_rng_isaac(uint32_t seed) { mode = DRAW_ISAAC; isaac.s.init(seed); }

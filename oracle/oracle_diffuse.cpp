@@ -121,9 +121,19 @@ void World::run_iteration() {
   event_time = (double)iteration;
   // Calendar::get_time_up_to_next_barrier (scheduler.cpp:212-242): the next
   // count event strictly after this iteration, at most 100 iterations away
-  uint64_t b = barrier_interval ? barrier_interval : 1;
-  uint64_t next_barrier = (iteration / b + 1) * b;
-  time_up_to_next_barrier = std::min((double)(next_barrier - iteration), 100.0);
+  if (!barrier_list.empty()) {
+    // the first listed barrier after this iteration; none: TIME_FOREVER
+    time_up_to_next_barrier = 100.0;
+    for (uint64_t b : barrier_list)
+      if (b > iteration) {
+        time_up_to_next_barrier = std::min((double)(b - iteration), 100.0);
+        break;
+      }
+  } else {
+    uint64_t b = barrier_interval ? barrier_interval : 1;
+    uint64_t next_barrier = (iteration / b + 1) * b;
+    time_up_to_next_barrier = std::min((double)(next_barrier - iteration), 100.0);
+  }
   calls_this_iteration.assign(mols.size(), 0);
   if (snapshot) {
     snapshot_pos.resize(mols.size());
@@ -340,7 +350,10 @@ int World::ray_trace_vol(uint32_t id, bool can_vol_react, uint32_t last_hit, V3&
   collisions.clear();
   Mol vm = mols[id];  // copy: the position is committed below
   V3 part_disp = remaining;
-  if (!in_partition(vm.pos + remaining)) part_disp = displacement_up_to_partition_boundary(vm.pos, remaining);
+  if (!in_partition(vm.pos + remaining)) {
+    part_disp = displacement_up_to_partition_boundary(vm.pos, remaining);
+    ++partition_clips;
+  }
 
   std::vector<uint32_t> walls_sp, mols_sp;
   uint32_t last_subpart = collect_crossed_subparts(vm, part_disp, can_vol_react, true, walls_sp, mols_sp);

@@ -69,6 +69,7 @@ def load():
         "orc_set_snapshot": (None, [P, ctypes.c_int]),
         "orc_set_trace": (None, [P, ctypes.c_int]),
         "orc_set_barrier_interval": (None, [P, u32]),
+        "orc_set_barriers": (None, [P, P, u32]),
         "orc_get_rxn_log": (u64, [P, P, P, P]),
         "orc_clear_rxn_log": (None, [P]),
         "orc_exact_disk": (d, [P, P, P, u32]),
@@ -256,6 +257,12 @@ class OracleWorld:
     def set_barrier_interval(self, b):
         self.lib.orc_set_barrier_interval(self.w, b)
 
+    def set_barriers(self, barriers):
+        """Scheduler barriers at the listed iterations (ascending) instead of
+        every barrier_interval iterations."""
+        a = np.ascontiguousarray(sorted(barriers), dtype=np.uint64)
+        self.lib.orc_set_barriers(self.w, a.ctypes.data_as(ctypes.c_void_p), len(a))
+
     def run(self, n):
         secs = ctypes.c_double()
         if self.lib.orc_run(self.w, n, ctypes.byref(secs)) != 0:
@@ -291,10 +298,10 @@ class OracleWorld:
         return u[:n], rx[:n], d[:n]
 
     def stats(self):
-        out = np.zeros(8, np.uint64)
+        out = np.zeros(9, np.uint64)
         self.lib.orc_get_stats(self.w, _p(out))
         keys = ["ray_voxel_tests", "ray_polygon_tests", "ray_polygon_colls", "mol_wall_reflections",
-                "vol_mol_vol_mol_collisions", "diffuse_3d_calls", "redo_count", "isaac_draws"]
+                "vol_mol_vol_mol_collisions", "diffuse_3d_calls", "redo_count", "isaac_draws", "partition_clips"]
         return dict(zip(keys, out.tolist()))
 
     def collide_wall(self, pos, wall, move):

@@ -31,14 +31,14 @@ def test_abi_version_and_struct_sizes(tmp_path):
     import shutil
     import subprocess
     lib = _lib.load()
-    assert lib.mcg_abi_version() == 3
+    assert lib.mcg_abi_version() == 4
     sizes = (ctypes.sizeof(_lib.McgConfig), ctypes.sizeof(_lib.McgSpecies), ctypes.sizeof(_lib.McgStepStats),
              ctypes.sizeof(_lib.McgBimolRxn), ctypes.sizeof(_lib.McgRxnEvent), ctypes.sizeof(_lib.McgUnimolRxn),
              ctypes.sizeof(_lib.McgVolSurfRxn))
     import numpy as np
     assert np.dtype(_lib.RXN_EVENT_DTYPE).itemsize == ctypes.sizeof(_lib.McgRxnEvent)
     if shutil.which("gcc") is None:
-        assert sizes == (112, 24, 48, 112, 64, 104, 88)
+        assert sizes == (112, 24, 64, 112, 64, 104, 88)
         return
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "mcg.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu", '
@@ -58,3 +58,31 @@ def test_create_rejects_bad_config_without_touching_a_device():
     assert lib.mcg_create(ctypes.byref(cfg), ctypes.byref(ctx)) == 1
     assert b"num_subparts_per_edge" in lib.mcg_last_error()
     assert not ctx.value
+
+
+def test_create_enforces_the_radius_rule_without_touching_a_device():
+    """MCell4Converter's check (mcell4_converter.cpp:121-127): the reaction
+    radius times sqrt(2) must be below half the subpartition edge."""
+    lib = _lib.load()
+    cfg = _lib.McgConfig()
+    cfg.num_subparts_per_edge = 10
+    cfg.subpart_edge_length = 1.5
+    cfg.subpart_edge_length_rcp = 1 / 1.5
+    cfg.partition_edge_length = 15.0
+    cfg.rxn_radius_3d = 0.56419
+    cfg.precision = 0
+    cfg.num_replicas = 1
+    cfg.molecule_capacity = 10
+    ctx = ctypes.c_void_p()
+    assert lib.mcg_create(ctypes.byref(cfg), ctypes.byref(ctx)) == 1
+    assert b"sqrt(2)" in lib.mcg_last_error()
+    assert not ctx.value
+
+
+def test_model_layer_enforces_the_radius_rule():
+    import pytest
+    from mcell_amd import model as M
+    box = M.create_box("box", edge_dimension=1.0)
+    with pytest.raises(ValueError):
+        M.derive_partition(M.Config(partition_dimension=1.2, subpartition_dimension=0.015), M.bounding_box([box]), True)
+    M.derive_partition(M.Config(subpartition_dimension=0.02), M.bounding_box([box]), True)
