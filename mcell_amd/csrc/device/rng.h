@@ -75,6 +75,81 @@ struct PhiloxStream {
   __device__ __forceinline__ double dbl() { return 2.3283064365386962890625e-10 * (double)u32(); }
 };
 
+// log1p(x) for x in (-1, 0] with the host C library's rounding.  The
+// Ziggurat tail (src/rng.c:205) feeds log1p straight into the Gaussian, and
+// the device library's log1p differs from the host's in the last bit for
+// about one input in a thousand, so the tail is evaluated here in IEEE basic
+// operations only: the fdlibm method (1 + x = 2^k (1 + f) with
+// sqrt(2)/2 <= 1 + f < sqrt(2), s = f / (2 + f),
+// log(1 + f) = f - (f^2/2 - s (f^2/2 + R(s^2))) with R the degree-7 minimax
+// polynomial of fdlibm's Lp1..Lp7, k ln 2 in a hi + lo split, and the
+// rounding error of 1 + x carried as c / u), with the polynomial summed in
+// the split order of glibc's dbl-64 log1p (R1 + z^2 R2 + z^4 R3 + z^6 R4).
+// Equal to glibc's log1p(-u) for every u = k * 2^-32 (all 2^32 k checked).
+__host__ __device__ __forceinline__ double log1p_neg(double x) {
+  constexpr double ln2_hi = 6.93147180369123816490e-01;  // 0x3FE62E42FEE00000
+  constexpr double ln2_lo = 1.90821492927058770002e-10;  // 0x3DEA39EF35793C76
+  constexpr double Lp1 = 6.666666666666735130e-01, Lp2 = 3.999999999940941908e-01,
+                   Lp3 = 2.857142874366239149e-01, Lp4 = 2.222219843214978396e-01,
+                   Lp5 = 1.818357216161805012e-01, Lp6 = 1.531383769920937332e-01,
+                   Lp7 = 1.479819860511658591e-01;
+  const uint64_t xb = __builtin_bit_cast(uint64_t, x);
+  const int32_t hx = (int32_t)(uint32_t)(xb >> 32);
+  const int32_t ax = hx & 0x7fffffff;
+  int k = 1;
+  double f = 0.0, c = 0.0;
+  uint32_t hu = 0;
+  if (ax >= 0x3ff00000) return -__builtin_inf();  // x = -1
+  if (ax < 0x3e200000) {                          // |x| < 2^-29
+    if (ax < 0x3c900000) return x;
+    return x - x * x * 0.5;
+  }
+  // (the sign bit makes hx negative: hx <= 0xbfd2bec3 as a signed word is
+  // -0.2929 <= x < 0)
+  if (hx > (int32_t)0xbfd2bec3) {  // x < -0.2929: reduce
+    const double u0 = 1.0 + x;
+    uint64_t ub = __builtin_bit_cast(uint64_t, u0);
+    hu = (uint32_t)(ub >> 32);
+    k = (int)(hu >> 20) - 1023;
+    c = (k > 0) ? 1.0 - (u0 - x) : x - (u0 - 1.0);
+    c /= u0;
+    hu &= 0x000fffffu;
+    double u;
+    if (hu < 0x6a09eu) {
+      u = __builtin_bit_cast(double, ((uint64_t)(hu | 0x3ff00000u) << 32) | (ub & 0xffffffffull));
+    } else {
+      k += 1;
+      u = __builtin_bit_cast(double, ((uint64_t)(hu | 0x3fe00000u) << 32) | (ub & 0xffffffffull));
+      hu = (0x00100000u - hu) >> 2;
+    }
+    f = u - 1.0;
+  } else {  // -0.2929 <= x < 0: no reduction
+    k = 0;
+    f = x;
+    hu = 1;
+  }
+  const double hfsq = 0.5 * f * f;
+  if (hu == 0) {  // |f| < 2^-20
+    if (f == 0.0) {
+      if (k == 0) return 0.0;
+      c += k * ln2_lo;
+      return k * ln2_hi + c;
+    }
+    const double R = hfsq * (1.0 - 0.66666666666666666 * f);
+    if (k == 0) return f - R;
+    return k * ln2_hi - ((R - (k * ln2_lo + c)) - f);
+  }
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  const double R1 = z * Lp1, z2 = z * z;
+  const double R2 = Lp2 + z * Lp3, z4 = z2 * z2;
+  const double R3 = Lp4 + z * Lp5, z6 = z4 * z2;
+  const double R4 = Lp6 + z * Lp7;
+  const double R = R1 + z2 * R2 + z4 * R3 + z6 * R4;
+  if (k == 0) return f - (hfsq - s * (hfsq + R));
+  return k * ln2_hi - ((hfsq - (s * (hfsq + R) + (k * ln2_lo + c))) - f);
+}
+
 // src/rng.c:173-218.  The table reads go through the read-only constant
 // tables; the fast path (about 98.8% of draws) is one uint32, one multiply and
 // one compare.
@@ -95,7 +170,7 @@ __device__ __forceinline__ double gauss(PhiloxStream& r, const ZigguratTables& T
       double yR = T.y[region - 1] - yB;
       y = yB + yR * r.dbl();
     } else {
-      x = SCALE_FACTOR - log1p(-r.dbl()) * RECIP_SCALE_FACTOR;
+      x = SCALE_FACTOR - log1p_neg(-r.dbl()) * RECIP_SCALE_FACTOR;
       y = exp(-SCALE_FACTOR * (x - 0.5 * SCALE_FACTOR)) * r.dbl();
     }
     if (!(y >= exp(-0.5 * x * x))) break;
