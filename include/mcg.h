@@ -256,6 +256,19 @@ int mcg_get_rxn_log(mcg_ctx* ctx, mcg_rxn_event* events, uint64_t capacity, uint
  * whole partition). */
 int mcg_count_species(mcg_ctx* ctx, uint64_t* counts);
 
+/* mcg_count_species without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The counters live in a device array of the library, at most
+ * 65,536 of them (num_replicas * num_species; larger contexts get
+ * MCG_E_LIMIT), 512 KiB that each GPU the library runs on holds until the
+ * process ends.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (from several host threads or contexts) race.  Surface molecules
+ * added since the last step are uploaded by this call only when their tile
+ * buffers already exist (MCG_E_STATE otherwise: call mcg_step or
+ * mcg_count_species once first). */
+int mcg_count_species_noalloc(mcg_ctx* ctx, uint64_t* counts);
+
 /* Volume molecules of one replica (Partition::add_volume_molecule semantics:
  * each position must lie in the partition).  ids are the molecule ids that
  * key their random streams; the replica's next product id becomes at least

@@ -36,7 +36,7 @@ EXPORTED = [
     "mcg_get_walls", "mcg_get_hit_trace", "mcg_last_kernel_time", "mcg_set_bimol_rxns", "mcg_set_rxn_log",
     "mcg_get_rxn_log", "mcg_count_species", "mcg_set_replica_params", "mcg_set_surface_classes",
     "mcg_set_unimol_rxns", "mcg_set_volsurf_rxns", "mcg_add_surface_molecules", "mcg_get_surface_molecules",
-    "mcg_get_next_molecule_ids", "mcg_set_next_molecule_ids",
+    "mcg_get_next_molecule_ids", "mcg_set_next_molecule_ids", "mcg_count_species_noalloc",
 ]
 
 MCG_MOLF_SCHEDULE_UNIMOL = 1
@@ -184,6 +184,7 @@ def load():
     lib.mcg_set_rxn_log.argtypes = [P, ctypes.c_uint64]
     lib.mcg_get_rxn_log.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
     lib.mcg_count_species.argtypes = [P, P]
+    lib.mcg_count_species_noalloc.argtypes = [P, P]
     lib.mcg_set_surface_classes.argtypes = [P, ctypes.c_uint32, P]
     lib.mcg_set_unimol_rxns.argtypes = [P, ctypes.POINTER(McgUnimolRxn), ctypes.c_uint32]
     lib.mcg_set_replica_params.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(McgSpecies), P]

@@ -127,6 +127,8 @@ int mcg_create(const mcg_config* cfg, mcg_ctx** out) {
       c->queue.alloc((size_t)kNumQueueShards * shard_capacity((uint32_t)cap, kNumQueueShards / 2)) ||
       c->qcount.alloc(kNumQueueShards) || c->counters.alloc(CTR_WORDS))
     return cleanup_fail(fail(MCG_E_NOMEM, "molecule buffers"));
+  if ((e = count_scratch_address(&c->count_scratch)) != hipSuccess)
+    return cleanup_fail(fail(MCG_E_HIP, std::string("count array address: ") + hipGetErrorString(e)));
   c->clock = cfg->start_iteration;
   double m = 0;
   for (double v : {P.ox, P.oy, P.oz, P.qx, P.qy, P.qz}) m = std::max(m, std::fabs(v));

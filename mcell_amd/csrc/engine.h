@@ -161,6 +161,7 @@ struct mcg_ctx {
   bool failed = false;   // a step failed after its kernels ran: the molecule state is undefined
   std::string fail_msg;
   mcg::DevBuf<unsigned long long> counters;  // CTR_* of the current mcg_step call
+  uint64_t* count_scratch = nullptr;  // the library's count array on this context's device
 
   ~mcg_ctx();
 };
@@ -174,6 +175,10 @@ int ensure_react_buffers(mcg_ctx* c);
 // Live molecules per (replica, species) into counts (size num_replicas *
 // num_species); surface molecules too when with_surface.
 int count_species(mcg_ctx* c, uint64_t* counts, bool with_surface = true);
-// Uploads the host copy of the tiles when it is newer (before steps and counts).
-int sync_tiles_to_device(mcg_ctx* c);
+// The same into the device buffer dcounts (num_replicas * num_species
+// entries, cleared here); allocates nothing when allow_alloc is false.
+int count_species_into(mcg_ctx* c, uint64_t* counts, uint64_t* dcounts, bool with_surface, bool allow_alloc);
+// Uploads the host copy of the tiles when it is newer (before steps and
+// counts); with allow_alloc false, MCG_E_STATE when that needs new buffers.
+int sync_tiles_to_device(mcg_ctx* c, bool allow_alloc = true);
 }  // namespace mcg

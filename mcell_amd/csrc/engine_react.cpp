@@ -325,29 +325,35 @@ int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, do
 
 int count_species(mcg_ctx* c, uint64_t* counts, bool with_surface) {
   const size_t nc = (size_t)c->cfg.num_replicas * c->num_species;
-  ReactState& r = c->rx;
-  if (with_surface)
-    if (int st = sync_tiles_to_device(c)) return st;
-  const uint32_t tiles = with_surface ? r.total_tiles : 0u;
   DevBuf<uint64_t> d;
   if (d.alloc(std::max<size_t>(nc, 1))) return fail(MCG_E_NOMEM, "count buffer");
-  HIP_TRY(hipMemsetAsync(d.p, 0, nc * 8, c->stream));
+  return count_species_into(c, counts, d.p, with_surface, true);
+}
+
+int count_species_into(mcg_ctx* c, uint64_t* counts, uint64_t* dcounts, bool with_surface, bool allow_alloc) {
+  const size_t nc = (size_t)c->cfg.num_replicas * c->num_species;
+  ReactState& r = c->rx;
+  if (with_surface)
+    if (int st = sync_tiles_to_device(c, allow_alloc)) return st;
+  const uint32_t tiles = with_surface ? r.total_tiles : 0u;
+  // cleared by a kernel (no memset node if a caller captures the stream)
+  HIP_TRY(launch_fill_u64(c->stream, reinterpret_cast<unsigned long long*>(dcounts), 0ull, (uint32_t)nc));
   if (c->n || tiles) {
     if (c->fp32()) {
       ReactPipeline<float> p{};
       p.M = RMolView<float>{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p, nullptr, nullptr};
       p.R.total_tiles = tiles;
       p.tiles = r.tiles.p;
-      HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, d.p));
+      HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, dcounts));
     } else {
       ReactPipeline<double> p{};
       p.M = RMolView<double>{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p, nullptr, nullptr};
       p.R.total_tiles = tiles;
       p.tiles = r.tiles.p;
-      HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, d.p));
+      HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, dcounts));
     }
   }
-  HIP_TRY(hipMemcpyAsync(counts, d.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(counts, dcounts, nc * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MCG_OK;
 }
@@ -541,6 +547,18 @@ int mcg_count_species(mcg_ctx* c, uint64_t* counts) {
   return count_species(c, counts);
 }
 
+int mcg_count_species_noalloc(mcg_ctx* c, uint64_t* counts) {
+  if (!c || !counts) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
+  if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
+  const uint64_t nc = (uint64_t)c->cfg.num_replicas * c->num_species;
+  if (nc > kCountScratchMax)
+    return fail(MCG_E_LIMIT, "replicas x species = " + std::to_string(nc) + " exceeds the " +
+                                 std::to_string(kCountScratchMax) +
+                                 " counters of the allocation-free count (use mcg_count_species)");
+  return count_species_into(c, counts, c->count_scratch, true, false);
+}
+
 }  // extern "C"
 
 namespace mcg {
@@ -576,11 +594,14 @@ int download_tiles(mcg_ctx* c) {
 
 }  // namespace
 
-int sync_tiles_to_device(mcg_ctx* c) {
+int sync_tiles_to_device(mcg_ctx* c, bool allow_alloc) {
   ReactState& r = c->rx;
   if (!r.tiles_dirty) return MCG_OK;
   const size_t nt = (size_t)r.total_tiles * c->cfg.num_replicas;
   if (nt > 0xFFFFFFFFull) return fail(MCG_E_LIMIT, "too many tiles (tiles x replicas >= 2^32)");
+  if (!allow_alloc && (r.tiles.n != nt || r.grids_d.n != r.grids_host.size()))
+    return fail(MCG_E_STATE, "surface molecules on new walls need tile buffers: run mcg_step or "
+                             "mcg_count_species once before the allocation-free count");
   if (r.tiles.n != nt) {
     if (r.tiles.alloc(nt) || r.tile_id.alloc(nt) || r.tile_owner.alloc(nt) || r.tile_consumed.alloc(nt))
       return fail(MCG_E_NOMEM, "tile buffers");

@@ -81,6 +81,13 @@ struct ReactPipeline {
   hipError_t count_species(hipStream_t s, uint32_t n, uint32_t num_species, uint32_t num_replicas, uint64_t* counts);
 };
 
+// Allocation-free species counts: at most kCountScratchMax (replica,
+// species) counters in a device array of the library; its address on the
+// current device (looked up once per device).
+constexpr uint32_t kCountScratchMax = 65536;  // 512 KiB per device
+constexpr int kMaxDevices = 64;
+hipError_t count_scratch_address(uint64_t** out);
+
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v, uint32_t n);
 hipError_t launch_copy_u32(hipStream_t s, const uint32_t* src, uint32_t* dst);

@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include <hipcub/hipcub.hpp>
 
@@ -1041,6 +1042,32 @@ hipError_t ReactPipeline<PosT>::count_species(hipStream_t s, uint32_t n, uint32_
 
 template struct ReactPipeline<double>;
 template struct ReactPipeline<float>;
+
+// ------------------------------------------------- allocation-free counts --
+// The counters of mcg_count_species_noalloc live in this file-scope device
+// array (each GPU the library runs on holds its own copy until the process
+// ends).  Host code never passes the array's name to a kernel: it is a
+// host-side stand-in, not a device address.  count_scratch_address looks the
+// address up once per device (with that device current) and keeps it.
+static __device__ __align__(256) uint64_t scratch[kCountScratchMax];
+
+hipError_t count_scratch_address(uint64_t** out) {
+  static std::mutex mu;
+  static uint64_t* per_device[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (per_device[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    per_device[dev] = static_cast<uint64_t*>(p);
+  }
+  *out = per_device[dev];
+  return hipSuccess;
+}
 
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n) {
   if (n == 0) return hipSuccess;
