@@ -389,6 +389,29 @@ uint32_t orc_crossed_subparts(void* wp, const double* pos, const double* disp, u
   return (uint32_t)ws.size();
 }
 
+// collide_wall as the ray trace calls it (update_move: a REDO case returns
+// 3 and draws its jitter sign from the world's stream) for n rays against
+// one wall; t and hit are written for FRONT (1) and BACK (2).
+void orc_collide_wall_rays(void* wp, uint32_t wall, uint64_t n, const double* pos, const double* move,
+                           int32_t* types, double* t, double* hit) {
+  World* w = static_cast<World*>(wp);
+  for (uint64_t i = 0; i < n; ++i) {
+    V3 mv{move[3 * i], move[3 * i + 1], move[3 * i + 2]};
+    V3 h{0, 0, 0};
+    double tt = 0;
+    types[i] = (int32_t)w->collide_wall({pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, wall, true, mv, tt, h);
+    t[i] = tt;
+    hit[3 * i] = h.x; hit[3 * i + 1] = h.y; hit[3 * i + 2] = h.z;
+  }
+}
+
+// test_bimolecular for each of n given draws u[i]; out[i] = pathway or -1
+void orc_test_bimolecular_u(double max_fixed_p, const double* cum_probs, uint32_t num_pathways, double scaling,
+                            uint64_t n, const double* u, int32_t* out) {
+  std::vector<double> cum(cum_probs, cum_probs + num_pathways);
+  for (uint64_t i = 0; i < n; ++i) out[i] = test_bimolecular_u(max_fixed_p, cum, scaling, u[i]);
+}
+
 uint32_t orc_subpart_index(void* wp, const double* pos) {
   return static_cast<World*>(wp)->subpart_index({pos[0], pos[1], pos[2]});
 }

@@ -450,4 +450,7 @@ class World {
   double exact_disk(V3 loc, V3 mv, double R, const Mol& moving, const Mol& target) const;
 };
 
+// test_bimolecular (rxn_utils.inl:336-414) with its draw u = rng_dbl() given.
+int test_bimolecular_u(double max_fixed_p, const std::vector<double>& cum_probs, double scaling, double u);
+
 }  // namespace orc

@@ -525,17 +525,21 @@ int World::test_bimolecular(const BimolRxn& rx, double scaling) {
 }
 
 int World::test_bimolecular_p(double max_fixed_p, const std::vector<double>& cum_probs, double scaling) {
+  return test_bimolecular_u(max_fixed_p, cum_probs, scaling, rng().dbl());
+}
+
+// The same with its one rng_dbl draw u given (every branch draws exactly once).
+int test_bimolecular_u(double max_fixed_p, const std::vector<double>& cum_probs, double scaling, double u) {
   double prob;
-  DrawSource& r = rng();
   if (max_fixed_p < scaling) {
-    prob = r.dbl() * scaling;
+    prob = u * scaling;
     if (prob >= max_fixed_p) return -1;
   } else {
     float max_p = (float)max_fixed_p;  // rxn_utils.inl:367 keeps max_p as float
     if (max_p >= scaling) {
-      prob = r.dbl() * max_p;
+      prob = u * max_p;
     } else {
-      prob = r.dbl() * scaling;
+      prob = u * scaling;
       if (prob >= max_p) return -1;
     }
   }

@@ -47,7 +47,6 @@ def load():
         "orc_add_walls": (None, [P, P, u32, u32, u32, u32]),
         "orc_finalize_walls": (None, [P]),
         "orc_num_walls": (u32, [P]),
-        "orc_get_wall_vertices": (None, [P, P]),
         "orc_get_walls": (None, [P, P]),
         "orc_get_vertices": (None, [P, P]),
         "orc_get_subpart_walls": (u64, [P, P, P]),
@@ -91,6 +90,9 @@ def load():
         "orc_collide_wall": (ctypes.c_int, [P, P, u32, P, P, P]),
         "orc_crossed_subparts": (u32, [P, P, P, P, u32]),
         "orc_subpart_index": (u32, [P, P]),
+        "orc_collide_wall_rays": (None, [P, u32, u64, P, P, P, P, P]),
+        "orc_test_bimolecular_u": (None, [d, P, u32, d, u64, P, P]),
+        "orc_get_wall_vertices": (None, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -319,8 +321,35 @@ class OracleWorld:
         n = self.lib.orc_crossed_subparts(self.w, _p(pos), _p(disp), _p(out), cap)
         return out[:min(n, cap)]
 
+    def collide_wall_rays(self, wall, pos, move):
+        """collide_wall with the ray trace's REDO handling: types (0 MISS,
+        1 FRONT, 2 BACK, 3 REDO), t and hit point per ray."""
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        move = np.ascontiguousarray(move, np.float64).reshape(-1, 3)
+        n = len(pos)
+        types = np.zeros(n, np.int32)
+        t = np.zeros(n)
+        hit = np.zeros((n, 3))
+        self.lib.orc_collide_wall_rays(self.w, wall, n, _p(pos), _p(move), _p(types), _p(t), _p(hit))
+        return types, t, hit
+
+    def wall_vertex_indices(self):
+        out = np.zeros((self.lib.orc_num_walls(self.w), 3), np.uint32)
+        self.lib.orc_get_wall_vertices(self.w, _p(out))
+        return out
+
     def subpart_index(self, pos):
         return self.lib.orc_subpart_index(self.w, _p(np.asarray(pos, np.float64)))
+
+
+def test_bimolecular_u(max_fixed_p, cum_probs, scaling, u):
+    """The oracle's test_bimolecular for given draws u: pathway or -1."""
+    lib = load()
+    cum = np.ascontiguousarray(cum_probs, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.zeros(len(u), np.int32)
+    lib.orc_test_bimolecular_u(max_fixed_p, _p(cum), len(cum), scaling, len(u), _p(u), _p(out))
+    return out
 
 
 def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):

@@ -1,5 +1,8 @@
 """Writes tests/golden/bimol_ensemble.npz: the CPU oracle's 32-seed ensemble
-of A + B -> C in the 1 um reflective cube.
+of A + B -> C in the 1 um reflective cube.  With --full it writes
+tests/golden/bimol_ensemble_full.npz instead: the same at the full size of
+the bimolecular benchmark case (50,000 A + 50,000 B, 2,000 iterations;
+counts [32, 201, 3]).
 
 The oracle follows the reference's sequential diffuse-react loop with the
 reference's ISAAC64 + Ziggurat streams (one stream per run, seeded 1..32).
@@ -28,6 +31,8 @@ ITERATIONS = 1000
 SAMPLE = 10
 RUNS = 32
 PLACEMENT_BASE = 1000
+# the full-size ensemble (--full)
+FULL = dict(N_A=50000, N_B=50000, ITERATIONS=2000, OUT="bimol_ensemble_full.npz")
 
 
 def workload(num_replicas=1, base=PLACEMENT_BASE):
@@ -36,12 +41,14 @@ def workload(num_replicas=1, base=PLACEMENT_BASE):
                        barrier_interval=SAMPLE)
 
 
-def run_one(r):
+def run_one(r, iterations=None):
+    """Run r's counts; iterations < ITERATIONS gives the run's first samples."""
     from oracle import pyoracle as po
     wl = workload(base=PLACEMENT_BASE + r)
     w = po.world_from_workload(wl, draw_mode=po.DRAW_ISAAC, seed=r + 1)
     w.set_barrier_interval(SAMPLE)
-    out = np.zeros((ITERATIONS // SAMPLE + 1, 3), np.uint32)
+    its = ITERATIONS if iterations is None else iterations
+    out = np.zeros((its // SAMPLE + 1, 3), np.uint32)
 
     def count():
         m = w.molecules()
@@ -49,20 +56,35 @@ def run_one(r):
         return np.bincount(m["species"][live], minlength=3)[:3]
 
     out[0] = count()
-    for k in range(ITERATIONS // SAMPLE):
+    for k in range(its // SAMPLE):
         w.run(SAMPLE)
         out[k + 1] = count()
     w.close()
     return out
 
 
+def use_full():
+    """Switches the module to the full-size ensemble (before workload/run_one)."""
+    global N_A, N_B, ITERATIONS
+    N_A, N_B, ITERATIONS = FULL["N_A"], FULL["N_B"], FULL["ITERATIONS"]
+
+
+def _init(full):
+    if full:
+        use_full()
+
+
 def main():
     from oracle import pyoracle
+    full = "--full" in sys.argv
+    if full:
+        use_full()
     pyoracle.build()
-    with Pool(min(8, os.cpu_count() or 1)) as p:
+    with Pool(min(8, os.cpu_count() or 1), initializer=_init, initargs=(full,)) as p:
         counts = np.stack(p.map(run_one, range(RUNS)))
     its = np.arange(0, ITERATIONS + 1, SAMPLE, dtype=np.uint32)
-    np.savez_compressed(os.path.join(HERE, "bimol_ensemble.npz"), counts=counts, iterations=its)
+    out = FULL["OUT"] if full else "bimol_ensemble.npz"
+    np.savez_compressed(os.path.join(HERE, out), counts=counts, iterations=its)
     print("final A mean %.1f sd %.1f" % (counts[:, -1, 0].mean(), counts[:, -1, 0].std()))
 
 

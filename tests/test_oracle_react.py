@@ -152,6 +152,32 @@ def test_ensemble_fixture_is_reproduced_by_the_oracle(oracle_lib):
     assert np.array_equal(run0, ref["counts"][0])
 
 
+def test_full_size_ensemble_kinetics_and_reproduction(oracle_lib):
+    """bimol_ensemble_full.npz (50,000 + 50,000, 2,000 iterations): A + C
+    and B + C conserved in every run, the mean within 3 % of the mean-field
+    law A(t) = A0 / (1 + k A0 t) (the finite-radius collision rate runs a
+    little above mean field), and run 0's first 200 iterations re-run by the
+    oracle give the fixture's first 21 samples exactly."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(HERE, "golden", "make_bimol_ensemble.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    mk.use_full()
+    ref = np.load(os.path.join(HERE, "golden", "bimol_ensemble_full.npz"))
+    counts = ref["counts"].astype(np.float64)
+    assert counts.shape == (32, 201, 3)
+    A0 = counts[0, 0, 0]
+    assert np.all(counts[:, :, 0] + counts[:, :, 2] == A0)
+    assert np.all(counts[:, :, 1] + counts[:, :, 2] == counts[:, 0, 1:2])
+    kA0 = 1e8 * A0 / (6.0221417930e23 * 1e-15)
+    mf = A0 / (1 + kA0 * ref["iterations"].astype(np.float64) * 1e-6)
+    mean = counts[:, :, 0].mean(axis=0)
+    print("full ensemble: final A %.1f, mean field %.1f" % (mean[-1], mf[-1]))
+    assert np.all(np.abs(mean - mf) / mf < 0.03), (np.abs(mean - mf) / mf).max()
+    run0 = mk.run_one(0, iterations=200)
+    assert np.array_equal(run0, ref["counts"][0, :21])
+
+
 def test_unimolecular_decay_is_exponential(oracle_lib):
     """E -> A + B at 1e3 /s: E(t) ~ Binomial(E0, exp(-k t)) with the
     lifetimes -ln(u)/k_tot of rxn_utils.inl:720-735; every decay leaves one A
