@@ -64,6 +64,48 @@ def test_crowded_snapshot_proposals_match_oracle(gpu_available, oracle_lib):
     eng.close()
 
 
+def test_full_crowded_mesh_snapshot_proposals_match_oracle(gpu_available, oracle_lib):
+    """The crowded case's full mesh: 64 perturbed icospheres of 20,480
+    triangles (1,310,732 walls with the cube), with 300,000 volume and
+    50,000 surface molecules.  Every proposal and position of 4 snapshot
+    iterations equals the oracle's."""
+    from oracle import pyoracle as po
+    seed = 17
+    wl = W.crowded_spheres(n_molecules=300000, n_surface=50000, iterations=4)
+    assert len(wl.walls) == 1310732
+    wl = dataclasses.replace(wl, unimol_rxns=[], rxns=[M.BimolReaction(0, 1, 1e9, [2])],
+                             volsurf_rxns=[M.VolSurfReaction(0, 3, 1e9, 4, product_orient=1, geom_vol=0, geom_surf=1)])
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed, propose_only=True, log_capacity=1 << 16)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
+    w.set_snapshot(True)
+    w.set_barrier_interval(wl.barrier_interval)
+    total = {"bimol": 0, "volsurf": 0}
+    for it in range(wl.iterations):
+        eng.step(it, 1)
+        w.run(1)
+        g = eng.rxn_log()
+        u, rx, d = w.rxn_log()
+        w.lib.orc_clear_rxn_log(w.w)
+        g = g[np.argsort(g["mover_id"], kind="stable")]
+        o = np.argsort(u[:, 1], kind="stable")
+        assert np.array_equal(g["mover_id"], u[o, 1]), "iteration %d: movers" % it
+        assert np.array_equal(g["partner_id"], u[o, 2]), "iteration %d: partners" % it
+        assert np.array_equal(g["rxn"], rx[o].astype(np.uint32)), "iteration %d: classes" % it
+        vs = (g["rxn"] & 0x40000000) != 0
+        total["volsurf"] += int(vs.sum())
+        total["bimol"] += int((~vs).sum())
+        m = eng.molecules()
+        om = w.molecules()
+        live = np.nonzero(((om["flags"] & 1) == 0) & (om["species"] < 3))[0]
+        oo = np.argsort(m["id"], kind="stable")
+        assert np.array_equal(m["id"][oo], live)
+        assert np.abs(m["pos"][oo] - om["pos"][live]).max() <= 1e-12 * 1000, "iteration %d: positions" % it
+        print("iteration %d: %s" % (it, total), flush=True)
+    assert total["bimol"] > 20 and total["volsurf"] > 20
+    w.close()
+    eng.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_crowded_invariants(gpu_available, precision):
     wl = _small(n_molecules=400000, iterations=30)

@@ -95,6 +95,53 @@ def test_volsurf_full_run_matches_oracle(gpu_available, oracle_lib):
     eng.close()
 
 
+def test_volume_surface_case_matches_oracle(gpu_available, oracle_lib):
+    """The volume-surface benchmark case itself (100,000 L outside the
+    5,120-triangle sphere, 20,000 R' on it, 10,000 T crossing it through a
+    transparent surface class, L' + R' -> LR' at 1e8 /M/s): 20 iterations
+    of the whole run equal the oracle's, ligand for ligand and tile for
+    tile, and so do the crossings of T."""
+    from oracle import pyoracle as po
+    seed = 37
+    wl = W.volume_surface(iterations=20)
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
+    w.set_barrier_interval(wl.barrier_interval)
+    crossings = 0
+    for it in range(wl.iterations):
+        st = eng.step(it, 1)
+        crossings += st.transparent_crossings
+        w.run(1)
+        g = eng.molecules()
+        o = np.argsort(g["id"], kind="stable")
+        om = w.molecules()
+        live = np.nonzero(((om["flags"] & 1) == 0) & ((om["species"] == 0) | (om["species"] == 3)))[0]
+        assert np.array_equal(g["id"][o], live), "iteration %d: the same volume molecules" % it
+        assert np.array_equal(g["species"][o], om["species"][live])
+        d = np.abs(g["pos"][o] - om["pos"][live]).max(initial=0)
+        assert d <= 1e-12 * 100, "iteration %d: max |dpos| %g" % (it, d)
+        assert np.array_equal(_engine_surface(eng), _oracle_surface(w, wl)), "iteration %d: surface" % it
+    n_rx = w.volsurf_reactions()
+    print("%d reactions, %d transparent crossings (oracle %d)" % (n_rx, crossings,
+                                                                  w.wall_stats()["transparent_crossings"]), flush=True)
+    assert n_rx > 20 and crossings == w.wall_stats()["transparent_crossings"] > 100
+    w.close()
+    eng.close()
+
+
+def test_volume_surface_case_fp32_takes_the_fp64_wall_hits(gpu_available):
+    """The volume-surface case's molecules and geometry (transparent sphere
+    for T) without its reaction classes: over 20 iterations the FP32+FP64-
+    fallback form hits exactly the walls the FP64 form hits, molecule-step by
+    molecule-step (the hit-index disagreement rate the benchmark reports)."""
+    import bench
+    r = bench.fp32_hit_agreement(W.volume_surface(iterations=20))
+    print(r, flush=True)
+    assert r["steps_with_hits"] > 10000
+    assert r["hit_index_disagreements"] == 0
+    assert r["position_rms_lu"] < 1e-3
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_volsurf_conservation_and_orientation(gpu_available, precision):
     wl = W.receptor_sphere(n_l=200000, n_r=40000, iterations=40, rate=4e8)
