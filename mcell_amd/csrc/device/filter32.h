@@ -35,13 +35,15 @@ struct FilterParams {
   float eps;     // 2^-23
   float S;       // max |coordinate| in the partition
   float e_dd;    // error bound of n.p - d
+  float mrel;    // relative error of the move components against the FP64 move (0: the FP64 move rounded)
 };
 
-MCG_HD FilterParams make_filter_params(float S) {
+MCG_HD FilterParams make_filter_params(float S, float move_ulps = 0.0f) {
   FilterParams f;
   f.eps = 1.1920928955078125e-07f;  // 2^-23
   f.S = S;
   f.e_dd = 64.0f * f.eps * S;
+  f.mrel = move_ulps * f.eps;
   return f;
 }
 
@@ -56,7 +58,9 @@ MCG_HD bool certain_miss32(const FilterWall& w, const FilterParams& F, float px,
   float dd = ((w.nx * px + w.ny * py) + w.nz * pz) - w.d;
   float m1 = (fabs_f(mx) + fabs_f(my)) + fabs_f(mz);
   float dv = (w.nx * mx + w.ny * my) + w.nz * mz;
-  float e_dv = 16.0f * F.eps * m1;
+  // (a move computed in FP32 is within mrel of the FP64 move per component:
+  // dv moves by at most mrel * m1, the hit point by t * mrel * max|m|)
+  float e_dv = 16.0f * F.eps * m1 + F.mrel * m1;
   float M = 2.0f * (F.e_dd + e_dv);
   float de = dd + dv;
   *end_dist = de;
@@ -68,7 +72,7 @@ MCG_HD bool certain_miss32(const FilterWall& w, const FilterParams& F, float px,
   // bound of the error of t, then of the hit point (max norm)
   float et = 2.0f * (F.e_dd + (at + 1.0f) * e_dv) / adv + 8.0f * F.eps * (at + 1.0f);
   float mxa = fmax_f(fmax_f(fabs_f(mx), fabs_f(my)), fabs_f(mz));
-  float eh = et * mxa + 8.0f * F.eps * (F.S + mxa);
+  float eh = et * mxa + 8.0f * F.eps * (F.S + mxa) + at * F.mrel * mxa;
   float hx = px + t * mx, hy = py + t * my, hz = pz + t * mz;
   float lx = hx - w.p0x, ly = hy - w.p0y, lz = hz - w.p0z;
   float L = (fabs_f(lx) + fabs_f(ly)) + fabs_f(lz);

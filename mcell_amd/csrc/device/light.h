@@ -148,4 +148,74 @@ __device__ __forceinline__ bool all_walls_miss32(const PartitionParams& P, const
   return true;
 }
 
+// The partition's geometry in FP32 for the light passes' conservative
+// index and entry boxes.
+struct PartitionParams32 {
+  float ox, oy, oz;
+  float sp_len, sp_rcp;
+  float box_scale;  // kSubCells / subpartition edge
+  float nf;         // subpartitions per edge
+  uint32_t n, n2;
+};
+
+__host__ __device__ inline PartitionParams32 make_partition32(const PartitionParams& P) {
+  PartitionParams32 Q;
+  Q.ox = (float)P.ox;
+  Q.oy = (float)P.oy;
+  Q.oz = (float)P.oz;
+  Q.sp_len = (float)P.sp_len;
+  Q.sp_rcp = (float)P.sp_rcp;
+  Q.box_scale = (float)((double)kSubCells * P.sp_rcp);
+  Q.nf = (float)P.n;
+  Q.n = P.n;
+  Q.n2 = P.n2;
+  return Q;
+}
+
+// Subpartition index box of the segment p -> e, computed in FP32 and widened
+// by kIndexMargin subpartition widths on every side: every subpartition the
+// segment touches lies inside it (the FP32 rounding of the cell coordinates
+// is below 1e-4 widths for partitions of up to ~10^3 subpartitions per
+// edge).  False when the widened box reaches a face of the partition, so a
+// true result also means the end point lies inside the partition: the move
+// takes the full step otherwise (clip at the partition face, escape checks).
+constexpr float kIndexMargin = 1e-3f;
+__device__ __forceinline__ bool index_box32(const PartitionParams32& Q, float px, float py, float pz, float ex,
+                                            float ey, float ez, int lo[3], int hi[3]) {
+  const float a[3] = {px, py, pz}, b[3] = {ex, ey, ez}, o[3] = {Q.ox, Q.oy, Q.oz};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float t0 = (a[k] - o[k]) * Q.sp_rcp, t1 = (b[k] - o[k]) * Q.sp_rcp;
+    const float mn = (t0 < t1 ? t0 : t1) - kIndexMargin, mx = (t0 < t1 ? t1 : t0) + kIndexMargin;
+    if (!(mn >= 0.0f && mx < Q.nf)) return false;  // (a NaN fails too)
+    lo[k] = (int)mn;
+    hi[k] = (int)mx;
+  }
+  return true;
+}
+
+// Walks the wall lists of the index box: for every entry whose box overlaps
+// the segment's (FP32 segment boxes), test(k) must return true (the wall
+// certainly cannot be hit); false as soon as one does not.
+template <typename Test>
+__device__ __forceinline__ bool walls_clear(const PartitionParams32& Q, const uint32_t* __restrict__ off,
+                                            const uint32_t* __restrict__ box, float px, float py, float pz,
+                                            float ex, float ey, float ez, const int lo[3], const int hi[3],
+                                            Test test) {
+  for (int z = lo[2]; z <= hi[2]; ++z)
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int x = lo[0]; x <= hi[0]; ++x) {
+        const uint32_t s = (uint32_t)x + (uint32_t)y * Q.n + (uint32_t)z * Q.n2;
+        const uint32_t k0 = off[s], k1 = off[s + 1];
+        if (k0 == k1) continue;
+        const uint32_t sb =
+            segment_box32(Q.ox + (float)x * Q.sp_len, Q.oy + (float)y * Q.sp_len, Q.oz + (float)z * Q.sp_len,
+                          Q.box_scale, px, py, pz, ex, ey, ez);
+        for (uint32_t k = next_overlap<kLightOverlapGroup>(box, k0, k1, sb); k < k1;
+             k = next_overlap<kLightOverlapGroup>(box, k + 1, k1, sb))
+          if (!test(k)) return false;
+      }
+  return true;
+}
+
 }  // namespace mcg

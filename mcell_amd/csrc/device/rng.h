@@ -45,10 +45,14 @@ struct PhiloxStream {
         ka += 0x9E3779B9u;
         kb += 0xBB67AE85u;
       }
-      uint32_t lo0 = 0xD2511F53u * x0;
-      uint32_t hi0 = __umulhi(0xD2511F53u, x0);
-      uint32_t lo1 = 0xCD9E8D57u * x2;
-      uint32_t hi1 = __umulhi(0xCD9E8D57u, x2);
+      // one 32 x 32 -> 64 multiply per product (v_mad_u64_u32) instead of
+      // separate low and high halves
+      const uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+      const uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+      uint32_t lo0 = (uint32_t)p0;
+      uint32_t hi0 = (uint32_t)(p0 >> 32);
+      uint32_t lo1 = (uint32_t)p1;
+      uint32_t hi1 = (uint32_t)(p1 >> 32);
       uint32_t n0 = hi1 ^ x1 ^ ka;
       uint32_t n2 = hi0 ^ x3 ^ kb;
       x0 = n0;

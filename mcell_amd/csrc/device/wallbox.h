@@ -61,6 +61,28 @@ MCG_BOX_HD uint32_t segment_box(const PartitionParams& P, uint32_t s, double px,
                     py < ey ? ey : py, pz < ez ? ez : pz);
 }
 
+// The same boxes computed in FP32 (the light passes of the FP32 form).  The
+// coordinates' FP32 rounding (below 1e-4 internal units for partitions up to
+// a few thousand units) is far inside kBoxMargin, so a segment box computed
+// here still overlaps the entry box of every wall the segment comes within
+// ~1e-3 of.
+MCG_BOX_HD uint32_t cell_range32(float a, float b, float origin, float scale) {
+  float ta = (a - (float)kBoxMargin - origin) * scale;
+  float tb = (b + (float)kBoxMargin - origin) * scale;
+  ta = ta < 0.0f ? 0.0f : (ta > kSubCells - 1 ? (float)(kSubCells - 1) : ta);
+  tb = tb < 0.0f ? 0.0f : (tb > kSubCells - 1 ? (float)(kSubCells - 1) : tb);
+  return (uint32_t)(int)ta | ((uint32_t)(int)tb << 4);
+}
+
+// Box of the segment p -> e (end point given) within the subpartition whose
+// low corner is (cx, cy, cz); scale = kSubCells / subpartition edge.
+MCG_BOX_HD uint32_t segment_box32(float cx, float cy, float cz, float scale, float px, float py, float pz, float ex,
+                                  float ey, float ez) {
+  return cell_range32(px < ex ? px : ex, px < ex ? ex : px, cx, scale) |
+         (cell_range32(py < ey ? py : ey, py < ey ? ey : py, cy, scale) << 8) |
+         (cell_range32(pz < ez ? pz : ez, pz < ez ? ez : pz, cz, scale) << 16);
+}
+
 // True when the two boxes share a cell on every axis.
 MCG_BOX_HD bool boxes_overlap(uint32_t a, uint32_t b) {
   // per 4-bit field: a_lo <= b_hi and b_lo <= a_hi

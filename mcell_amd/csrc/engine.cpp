@@ -125,7 +125,8 @@ int mcg_create(const mcg_config* cfg, mcg_ctx** out) {
                             : (c->x.alloc(cap) || c->y.alloc(cap) || c->z.alloc(cap));
   if (pos_fail || c->id.alloc(cap) || c->tag.alloc(cap) || c->err.alloc(1) ||
       c->queue.alloc((size_t)kNumQueueShards * shard_capacity((uint32_t)cap, kNumQueueShards / 2)) ||
-      c->qcount.alloc(kNumQueueShards) || c->counters.alloc(CTR_WORDS))
+      c->qcount.alloc(kNumQueueShards) || c->counters.alloc(CTR_WORDS) ||
+      c->mqueue.alloc((size_t)kNumQueueShards * shard_capacity((uint32_t)cap)) || c->mqcount.alloc(kNumQueueShards))
     return cleanup_fail(fail(MCG_E_NOMEM, "molecule buffers"));
   if ((e = count_scratch_address(&c->count_scratch)) != hipSuccess)
     return cleanup_fail(fail(MCG_E_HIP, std::string("count array address: ") + hipGetErrorString(e)));
@@ -377,6 +378,7 @@ int free_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg
   HitTrace tr{c->trace_wall.p, c->trace_count.p, (c->cfg.flags & MCG_FLAG_TRACE_HITS) ? 1u : 0u};
   uint32_t n = (uint32_t)c->n;
   SlowQueue Q{c->queue.p, c->qcount.p, 0};
+  SlowQueue MQ{c->mqueue.p, c->mqcount.p, 0};
   HIP_TRY(launch_fill_u64(c->stream, c->counters.p, 0ull, CTR_WORDS));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   // every molecule has its diffusion time at the iteration start and the
@@ -384,11 +386,11 @@ int free_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, mcg
   for (uint32_t k = 0; k < num_iterations; ++k) {
     if (c->fp32())
       HIP_TRY(launch_diffuse_reflect32(c->stream, mf, n, c->species_table(), c->surface_classes(), c->P, W,
-                                       c->walls32.p, c->max_abs_coord, c->cfg.seed, first_iteration + k, Q, c->err.p,
-                                       tr, c->counters.p));
+                                       c->walls32.p, c->max_abs_coord, c->cfg.seed, first_iteration + k, MQ, Q,
+                                       c->err.p, tr, c->counters.p));
     else
       HIP_TRY(launch_diffuse_reflect64(c->stream, m, n, c->species_table(), c->surface_classes(), c->P, W,
-                                       c->cfg.seed, first_iteration + k, Q, c->err.p, tr, c->counters.p));
+                                       c->cfg.seed, first_iteration + k, MQ, Q, c->err.p, tr, c->counters.p));
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   uint32_t err = 0, slow = 0;

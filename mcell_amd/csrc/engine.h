@@ -151,6 +151,7 @@ struct mcg_ctx {
   // diagnostics
   mcg::DevBuf<uint32_t> err;
   mcg::DevBuf<uint32_t> queue, qcount;  // hard-path queue of molecule indices
+  mcg::DevBuf<uint32_t> mqueue, mqcount;  // middle-pass queue (free diffusion)
   mcg::DevBuf<uint32_t> trace_wall, trace_count;
   double last_kernel_ms = 0;
   uint32_t last_launches = 0;

@@ -36,6 +36,9 @@ struct WallView64 {
 };
 
 // Slow-path queue: kQueueShards counters and shards of shard_cap slots.
+// Free diffusion runs three passes: light, middle (displacements off the
+// Ziggurat's fast path) and full step, with one queue in front of each of
+// the last two.
 struct SlowQueue {
   uint32_t* slots;
   uint32_t* count;
@@ -50,13 +53,14 @@ uint32_t shard_capacity(uint32_t n, uint32_t shards = kNumQueueShards);
 hipError_t launch_diffuse_reflect64(hipStream_t s, const MolView64& m, uint32_t n, SpeciesTable species,
                                     const SurfaceClasses& SC,
                                     const PartitionParams& P, const WallView64& W, uint64_t seed, uint32_t iteration,
-                                    SlowQueue Q, uint32_t* err, const HitTrace& trace, unsigned long long* ctr);
+                                    SlowQueue MQ, SlowQueue Q, uint32_t* err, const HitTrace& trace,
+                                    unsigned long long* ctr);
 
 hipError_t launch_diffuse_reflect32(hipStream_t s, const MolView32& m, uint32_t n, SpeciesTable species,
                                     const SurfaceClasses& SC,
                                     const PartitionParams& P, const WallView64& W, const WallRec32* w32,
-                                    float max_abs_coord, uint64_t seed, uint32_t iteration, SlowQueue Q,
-                                    uint32_t* err, const HitTrace& trace, unsigned long long* ctr);
+                                    float max_abs_coord, uint64_t seed, uint32_t iteration, SlowQueue MQ,
+                                    SlowQueue Q, uint32_t* err, const HitTrace& trace, unsigned long long* ctr);
 
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 

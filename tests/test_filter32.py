@@ -24,6 +24,9 @@ def shim(tmp_path_factory):
     lib.filter_batch.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_uint64, ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.filter_batch.restype = None
+    lib.filter_batch_perturbed.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.filter_batch_perturbed.restype = None
     return lib
 
 
@@ -37,7 +40,7 @@ def _world_with_triangles(tris_um):
     return w
 
 
-def _check(shim, w, wall, S, pos, move):
+def _check(shim, w, wall, S, pos, move, move_ulps=0.0):
     n = len(pos)
     pos = np.ascontiguousarray(pos, np.float64)
     move = np.ascontiguousarray(move, np.float64)
@@ -52,8 +55,8 @@ def _check(shim, w, wall, S, pos, move):
                  np.float32)
     miss = np.zeros(n, np.int32)
     de = np.zeros(n, np.float32)
-    shim.filter_batch(f.ctypes.data, ctypes.c_float(S), n, pos.ctypes.data, move.ctypes.data, miss.ctypes.data,
-                      de.ctypes.data)
+    shim.filter_batch_perturbed(f.ctypes.data, ctypes.c_float(S), ctypes.c_float(move_ulps), n, pos.ctypes.data,
+                                move.ctypes.data, miss.ctypes.data, de.ctypes.data)
     exact_nonmiss = types != 0
     bad = exact_nonmiss & (miss == 1)
     return types, miss, bad
@@ -75,8 +78,11 @@ def _rays_at_points(rng, targets, normal, n_per, step):
     return start, L[:, None] * d
 
 
-@pytest.mark.parametrize("seed", [0, 1, 2])
-def test_filter_is_conservative_near_edges_and_vertices(shim, seed):
+@pytest.mark.parametrize("seed,move_ulps", [(0, 0.0), (1, 0.0), (2, 0.0), (3, 4.0), (4, 4.0)])
+def test_filter_is_conservative_near_edges_and_vertices(shim, seed, move_ulps):
+    """move_ulps 4: the filter gets the move moved by up to 4 FP32 ulps per
+    component (the light pass's FP32 displacement) and must still keep every
+    FP64 hit and re-draw of the unmoved FP64 move."""
     rng = np.random.default_rng(seed)
     # triangles in um, well inside a 10 um partition (internal units x100)
     tris = []
@@ -101,7 +107,7 @@ def test_filter_is_conservative_near_edges_and_vertices(shim, seed):
                                   V[0] + rng.dirichlet([1, 1, 1], 300) @ (V - V[0]) * 1.0])
         step = float(np.linalg.norm(V[1] - V[0]) + 2.0)
         pos, move = _rays_at_points(rng, targets, normal, 8, step)
-        types, miss, bad = _check(shim, w, wi, S, pos, move)
+        types, miss, bad = _check(shim, w, wi, S, pos, move, move_ulps)
         assert not bad.any(), "filter rejected %d FP64 hits/redos on wall %d" % (bad.sum(), wi)
         total_nonmiss += int((types != 0).sum())
         # the filter must also be useful: most far misses are rejected
