@@ -44,24 +44,60 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (known HBM bytes) to scale rocprof counters")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) that fill roofline.traffic")
+    ap.add_argument("--pmc-inner", action="store_true",
+                    help="(run by the traffic passes under rocprofv3) a few iterations, then the 1 GiB copy")
+    ap.add_argument("--cpu-processes", type=int, default=None,
+                    help="concurrent single-threaded oracle processes for the replica batch's CPU baseline "
+                         "(default: min(usable cores, 16))")
     return ap.parse_args()
 
 
+COPY_BYTES = 1 << 30
+
+
+def _hip():
+    import ctypes
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            return ctypes.CDLL(name)
+        except OSError:
+            continue
+    raise OSError("libamdhip64.so not found")
+
+
+def copy_peak(reps=10, quiet=False):
+    """A plain device copy of known size (1 GiB read + 1 GiB written per
+    launch, hipMemcpy device to device): the measured HBM copy rate, and the
+    reference launch for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = _hip()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(COPY_BYTES)
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        raise RuntimeError("hipMalloc of the copy buffers failed")
+    try:
+        hip.hipMemset(a, 1, n)
+        hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hip.hipMemcpy(b, a, n, 3)
+        hip.hipDeviceSynchronize()
+        secs = (time.perf_counter() - t0) / reps
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    gbps = 2 * COPY_BYTES / secs / 1e9
+    if not quiet:
+        print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "launches": reps,
+                          "GBps": gbps}), flush=True)
+    return gbps
+
+
 def copy_calibration():
-    """A plain copy of known size: 1 GiB read + 1 GiB written per launch."""
-    import torch
-    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    x.fill_(1.0)
-    torch.cuda.synchronize()
-    reps = 10
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    secs = (time.perf_counter() - t0) / reps
-    print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "launches": reps,
-                      "GBps": 2 * (1 << 30) / secs / 1e9}), flush=True)
+    copy_peak()
 
 
 def build_workload(args):
@@ -75,8 +111,8 @@ def build_workload(args):
         wl = W.replicas_sphere_in_cube(num_replicas=args.replicas, iterations=args.steps)
         cfg = {"workload": "%d replicas of A + B -> C (1e8 /M/s) over one mesh: reflective 2 um cube + icosphere "
                            "r 0.5 um (5,120 triangles); per replica N_A ~ U[1e3, 2e5], N_B = N_A/2, "
-                           "D ~ U[1e-7, 5e-6] cm^2/s (seed 4 + i), C with D 5e-7, counts every 10 iterations, "
-                           "dt 1e-6 s" % args.replicas,
+                           "D ~ U[1e-7, 5e-6] cm^2/s (seed 4 + i), C with D 5e-7, species counts of every "
+                           "replica every 10 iterations, dt 1e-6 s" % args.replicas,
                "replicas": args.replicas, "molecules": wl.num_molecules, "walls": int(len(wl.walls))}
     elif args.workload == "volume_surface":
         wl = W.volume_surface(iterations=args.steps)
@@ -105,6 +141,70 @@ def build_workload(args):
     else:
         raise SystemExit("unknown workload %s" % args.workload)
     return wl, cfg
+
+
+def host_cpu():
+    """nproc, the cores this process may use, and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "usable_cores": usable, "model": model}
+
+
+def _oracle_instance(job):
+    """One single-threaded oracle process on one instance of the replica
+    batch for about `seconds`: (molecule-steps, seconds)."""
+    r, seconds, subpart = job
+    sys.path.insert(0, ROOT)
+    from mcell_amd import workloads as W
+    from oracle import pyoracle as po
+    wl = W.replicas_sphere_in_cube(num_replicas=r + 1)
+    if subpart is not None:
+        wl = W.with_subpartition(wl, subpart)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_ISAAC, replica=r)
+    w.set_barrier_interval(wl.barrier_interval)
+    secs = 0.0
+    done = 0
+    while secs < seconds and done < 20000:
+        secs += w.run(10)
+        done += 10
+    steps = w.stats()["diffuse_3d_calls"]
+    w.close()
+    return steps, secs
+
+
+def cpu_baseline_replicas(seconds, processes, subpart):
+    """The replica batch on the CPU as MCell4 runs a sweep: one
+    single-threaded process per instance, `processes` at a time
+    (mcell4_runner.py -j): instances 0 .. processes-1, each for about
+    `seconds`; the aggregate molecule-timesteps/s of the wave."""
+    from multiprocessing import get_context
+    jobs = [(r, seconds, subpart) for r in range(processes)]
+    t0 = time.perf_counter()
+    with get_context("spawn").Pool(processes) as pool:
+        res = pool.map(_oracle_instance, jobs)
+    wall = time.perf_counter() - t0
+    steps = sum(r[0] for r in res)
+    loop_secs = max(r[1] for r in res)
+    one = res[0][0] / res[0][1]
+    host = host_cpu()
+    return {"value": steps / loop_secs, "unit": "molecule-timesteps/s", "cores": processes, "kind": "port",
+            "processes": processes, "single_process_value": one, "host": host,
+            "sample": "oracle (FP64 restatement of the reference CPU path, ISAAC64 stream), instances 0-%d of the "
+                      "replica batch (A + B -> C, sphere-in-cube mesh, 5,132 walls), %d concurrent single-threaded "
+                      "processes (one per instance, as mcell4_runner -j runs a sweep) on %d of %d cores (%s), about "
+                      "%.0f s of iterations each (%.1f s wall for the wave)"
+                      % (processes - 1, processes, processes, host["nproc"], host["model"], seconds, wall)}
 
 
 def cpu_baseline(seconds, workload):
@@ -139,9 +239,11 @@ def cpu_baseline(seconds, workload):
         done += chunk
     steps = w.stats()["diffuse_3d_calls"]
     rate = steps / secs
-    return {"value": rate, "unit": "molecule-timesteps/s", "cores": 1, "kind": "port",
+    host = host_cpu()
+    return {"value": rate, "unit": "molecule-timesteps/s", "cores": 1, "kind": "port", "host": host,
             "sample": "oracle (FP64 restatement of the reference CPU path, ISAAC64 stream), %s, %d iterations, "
-                      "%.1f s" % (what, done, secs)}
+                      "%.1f s, one single-threaded process on a %s (%d cores)"
+                      % (what, done, secs, host["model"], host["nproc"])}
 
 
 def fp32_hit_agreement(wl, iterations=20, max_molecules=500000):
@@ -198,28 +300,115 @@ def kernel_name(precision, wl):
     return "k_light64 + k_hard64" if precision == "fp64" else "k_light32 + k_hard32"
 
 
+def counts_every(wl):
+    """Iterations between species counts (the count barriers of the
+    reaction workloads), or None."""
+    return wl.barrier_interval if reactive(wl) and wl.barrier_interval > 1 else None
+
+
 def run_gpu(args, wl, precision):
-    """Warm-up, then K timed iterations of one precision form; returns the
+    """Warm-up, then K timed iterations of one precision form, with the
+    species counts of the reaction workloads taken every count barrier
+    (on the device, allocation-free) inside the timed region; returns the
     step statistics and timings."""
+    import numpy as np
     from mcell_amd.engine import DiffuseReactEngine
     eng = DiffuseReactEngine.from_workload(wl, precision=precision,
                                            capacity=wl.num_molecules * (2 if reactive(wl) else 1))
     print("bench: %s, %d molecules, precision %s, warmup %d, steps %d" % (args.workload, wl.num_molecules, precision,
                                                                           args.warmup, args.steps),
           file=sys.stderr, flush=True)
+    every = counts_every(wl)
+    counts = np.zeros((wl.num_replicas, len(wl.species_steps)), np.uint64)
     if args.warmup:
         eng.step(0, args.warmup)
     eng.sync()
+    it, end = args.warmup, args.warmup + args.steps
+    kernel_ms = 0.0
+    n_counts = 0
+    st_sum = {"molecule_steps": 0, "reactions": 0, "settle_rounds": 0, "wall_hits": 0}
+    hard = 0
     t0 = time.perf_counter()
-    st = eng.step(args.warmup, args.steps)
+    b = max(int(wl.barrier_interval), 1)
+    while it < end:
+        chunk = min(end - it, b - it % b)  # one mcg_step call: up to the next barrier
+        st = eng.step(it, chunk)
+        avg_ms, _ = eng.last_kernel_time()  # per iteration of that call
+        kernel_ms += avg_ms * chunk
+        for k in st_sum:
+            st_sum[k] += getattr(st, k)
+        hard = st.slow_path_last
+        it += chunk
+        if every and it % every == 0:
+            eng.count_species_noalloc(counts)
+            n_counts += 1
     eng.sync()
     secs = time.perf_counter() - t0
-    avg_ms, launches = eng.last_kernel_time()
     eng.close()
-    steps_done = st.molecule_steps if reactive(wl) else wl.num_molecules * args.steps
-    return {"secs": secs, "steps_done": steps_done, "value": steps_done / secs, "avg_ms": avg_ms,
-            "reactions": st.reactions, "rounds": st.settle_rounds, "hard": st.slow_path_last,
-            "reflections": st.wall_hits}
+    steps_done = st_sum["molecule_steps"] if reactive(wl) else wl.num_molecules * args.steps
+    return {"secs": secs, "steps_done": steps_done, "value": steps_done / secs, "avg_ms": kernel_ms / args.steps,
+            "reactions": st_sum["reactions"], "rounds": st_sum["settle_rounds"], "hard": hard,
+            "reflections": st_sum["wall_hits"], "counts": n_counts}
+
+
+def traffic_passes(args, wl_subpart, main_prec, iterations):
+    """HBM bytes per iteration of the dominant kernels from two rocprofv3
+    --pmc passes over a short run of the same workload (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by what the counter reads for
+    the 1 GiB copy in the same process.  None (with the reason) when the
+    profiler is not available or fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    names = ("k_move_light<float>", "k_move_drain<float>") if main_prec == "fp32" else \
+        ("k_move_light<double>", "k_move_drain<double>")
+    wl_name = args.workload
+    if wl_name not in ("replicas_sphere", "bimol_box", "crowded_spheres", "volume_surface"):
+        names = ("k_light32", "k_mid32", "k_hard32") if main_prec == "fp32" else ("k_light64", "k_mid64", "k_hard64")
+    out = {}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+                   sys.executable, os.path.join(ROOT, "bench.py"), "--pmc-inner", "--workload", wl_name,
+                   "--steps", str(iterations), "--warmup", "1", "--precision", main_prec,
+                   "--replicas", str(args.replicas), "--mols-per-replica", str(args.mols_per_replica)]
+            if wl_subpart is not None:
+                cmd += ["--subpart", str(wl_subpart)]
+            print("bench: traffic pass %s" % counter, file=sys.stderr, flush=True)
+            try:
+                r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, timeout=300)
+            except subprocess.TimeoutExpired:
+                return None, "rocprofv3 %s pass timed out" % counter
+            files = []
+            for root, _, fs in os.walk(d):
+                files += [os.path.join(root, f) for f in fs if f.endswith("counter_collection.csv")]
+            if r.returncode != 0 or not files:
+                return None, "rocprofv3 %s pass failed (rc %d)" % (counter, r.returncode)
+            kern, copy = 0.0, []
+            for row in csv.DictReader(open(files[0])):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row["Kernel_Name"]
+                v = float(row["Counter_Value"]) * 1024.0  # KiB per dispatch
+                if any(k in name for k in names):
+                    kern += v
+                elif "copy" in name.lower() or "elementwise" in name.lower():
+                    copy.append(v)
+            copy = [c for c in copy if c > 0.25 * COPY_BYTES]
+            if not copy:
+                return None, "no copy dispatch in the %s pass" % counter
+            factor = COPY_BYTES / (sum(copy) / len(copy))
+            out[counter] = (kern / (iterations + 1), factor)  # the warm-up iteration is profiled too
+    fetch = out["FETCH_SIZE"][0] * out["FETCH_SIZE"][1]
+    write = out["WRITE_SIZE"][0] * out["WRITE_SIZE"][1]
+    return {"bytes_per_iteration": fetch + write, "fetch": fetch, "write": write,
+            "fetch_factor": out["FETCH_SIZE"][1], "write_factor": out["WRITE_SIZE"][1], "kernels": list(names),
+            "iterations_profiled": iterations + 1}, None
 
 
 def main():
@@ -234,6 +423,13 @@ def main():
     from mcell_amd import workloads as W
     wl, cfg = build_workload(args)
     main_prec = "fp32" if args.precision in ("fp32", "both") else "fp64"
+    if args.pmc_inner:
+        if args.subpart is not None:
+            wl = W.with_subpartition(wl, args.subpart)
+        args.no_autotune = True
+        run_gpu(args, wl, main_prec)
+        copy_peak(reps=2, quiet=True)
+        return
     if args.subpart is not None:
         wl = W.with_subpartition(wl, args.subpart)
     elif wl.num_replicas > 1 and not args.no_autotune:
@@ -268,6 +464,7 @@ def main():
     if reactive(wl):
         out["reactions"] = r["reactions"]
         out["settle_rounds"] = r["rounds"]
+        out["species_counts_taken"] = r["counts"]
         out["roofline"]["full_step_fraction"] = r["hard"] / max(per_step, 1)
     else:
         out["roofline"]["slow_path_fraction"] = r["hard"] / max(wl.num_molecules, 1)
@@ -276,13 +473,36 @@ def main():
         out["fp64"] = {"value": r64["value"], "ms_per_step": 1e3 * r64["secs"] / args.steps,
                        "kernel_avg_ms": r64["avg_ms"], "reactions": r64["reactions"]}
         out["fp32_over_fp64"] = r["value"] / r64["value"]
+    # the measured copy rate beside the published peak
+    try:
+        out["roofline"]["copy_peak_GBps"] = copy_peak(quiet=True)
+    except Exception as e:  # a side measurement
+        out["roofline"]["copy_peak_GBps"] = None
+        out["roofline"]["copy_peak_error"] = str(e)
+    # wall records and partner positions are read through the caches, not
+    # staged in LDS (LDS holds the Ziggurat tables and the full step's
+    # subpartition sets): the LDS share of wall/candidate reads is zero
+    out["roofline"]["lds_hit_rate"] = 0.0
+    if not args.no_traffic:
+        t, why = traffic_passes(args, cfg["subpartition_um"], main_prec, 3)
+        if t is not None:
+            out["roofline"]["traffic"] = t["bytes_per_iteration"] / 1e9
+            out["roofline"]["traffic_unit"] = "GB per iteration of the dominant kernels"
+            out["roofline"]["traffic_detail"] = t
+            out["roofline"]["traffic_over_algorithmic"] = t["bytes_per_iteration"] / (B_ALG[main_prec] * per_step)
+        else:
+            out["roofline"]["traffic_error"] = why
     if not args.no_agreement:
         try:
             out["fp32_vs_fp64"] = fp32_hit_agreement(wl)
         except Exception as e:  # a side measurement; the benchmark line stands without it
             out["fp32_vs_fp64"] = {"error": str(e)}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, args.workload)
+        if args.workload == "replicas_sphere":
+            procs = args.cpu_processes or max(1, min(host_cpu()["usable_cores"], 16, args.replicas))
+            out["cpu_baseline"] = cpu_baseline_replicas(args.cpu_seconds, procs, cfg["subpartition_um"])
+        else:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, args.workload)
     print(json.dumps(out), flush=True)
 
 

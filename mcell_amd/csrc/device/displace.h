@@ -19,8 +19,8 @@ __device__ __forceinline__ D3 draw_displacement(PhiloxStream& rng, const Ziggura
 // A step clipped to max_time (units of the time step): steps and t_steps as
 // compute_vol_displacement sets them; scale = sqrt(steps) * space_step when
 // steps != 1.  Returns r_rate_factor.
-__device__ __forceinline__ double clipped_displacement(PhiloxStream& rng, const ZigguratTables& T, double space_step,
-                                                       double time_step, double max_time, double& t_steps, D3& d) {
+__device__ __forceinline__ double step_scale(double space_step, double time_step, double max_time, double& t_steps,
+                                             double& scale) {
   const double EPS = 1e-12;
   double steps = 1.0;
   t_steps = steps * time_step;
@@ -32,13 +32,21 @@ __device__ __forceinline__ double clipped_displacement(PhiloxStream& rng, const 
     steps = EPS;
     t_steps = EPS * time_step;
   }
-  double rate_factor = 1.0, r_rate_factor = 1.0;
-  double scale = space_step;
+  double r_rate_factor = 1.0;
+  scale = space_step;
   if (!(steps == 1.0)) {
-    rate_factor = sqrt(steps);
+    const double rate_factor = sqrt(steps);
     r_rate_factor = 1.0 / rate_factor;
     scale = rate_factor * space_step;
   }
+  return r_rate_factor;
+}
+
+// The displacement of such a step drawn through rng_gauss.
+__device__ __forceinline__ double clipped_displacement(PhiloxStream& rng, const ZigguratTables& T, double space_step,
+                                                       double time_step, double max_time, double& t_steps, D3& d) {
+  double scale;
+  const double r_rate_factor = step_scale(space_step, time_step, max_time, t_steps, scale);
   d = draw_displacement(rng, T, scale);
   return r_rate_factor;
 }

@@ -221,6 +221,13 @@ class DiffuseReactEngine:
         _lib.check(self.lib.mcg_count_species(self.ctx, _ptr(out)))
         return out
 
+    def count_species_noalloc(self, out: np.ndarray) -> np.ndarray:
+        """The same counts into out (uint64 [replicas, species]) without any
+        device allocation (mcg_count_species_noalloc)."""
+        assert out.dtype == np.uint64 and out.flags.c_contiguous
+        _lib.check(self.lib.mcg_count_species_noalloc(self.ctx, _ptr(out)))
+        return out
+
     def add_volume_molecules(self, replica: int, ids: np.ndarray, species: np.ndarray, positions: np.ndarray,
                              diffusion_times=None, unimol_times=None, flags=None):
         ids = np.ascontiguousarray(ids, dtype=np.uint32)
