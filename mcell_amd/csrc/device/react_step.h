@@ -182,8 +182,20 @@ struct MoverEnd {
   double utime;    // unimolecular reaction time after the step (kTimeInvalid: none)
   bool near_plane; // FP32 form: the end point lies near a wall plane
   bool absorbed;   // destroyed by an absorptive surface class
+  bool exd;        // deferred: a collision needs exact_disk (the caller repeats the step with it)
   uint32_t due;    // 0, or 1 + the stream pass that draws the unimolecular pathway
 };
+
+// exact_disk is exactly 1 unless an entry of the collision point's
+// subpartition has a box overlapping the point's +-R cube (the walls it
+// would process; exact_disk_dev returns 1 when it processes none).
+__device__ __forceinline__ bool disk_meets_walls(const ReactParams& R, D3 cp) {
+  const double r = R.P.rxn_radius;
+  const uint32_t csp = subpart_index(R.P, cp);
+  const uint32_t db = region_box(R.P, csp, cp.x - r, cp.y - r, cp.z - r, cp.x + r, cp.y + r, cp.z + r);
+  const uint32_t e1 = R.off[csp + 1];
+  return next_overlap(R.ent_box, R.off[csp], e1, db) < e1;
+}
 
 // The molecule's unimolecular time at the start of its step
 // (diffuse_single_molecule, diffuse_react_event.cpp:213-241): a newly created
@@ -206,7 +218,11 @@ __device__ __forceinline__ double start_unimol_time(const UnimolRec* __restrict_
 // reactions the molecule collisions of each segment in time order, each
 // reaction test drawing from the mover's stream.  The first successful test
 // ends the step with a proposal (return true).
-template <typename PosT>
+// kExd false (the drain's form): a collision whose disk meets walls does not
+// call exact_disk -- the step stops with out.exd set and writes nothing, and
+// a second pass repeats it with kExd true.  Keeping exact_disk's vertex pool
+// and its call out of the drain keeps the drain's registers off the stack.
+template <typename PosT, bool kExd>
 __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot<PosT>& S, const SpeciesRec& sp,
                                             const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
                                             double utime, double barrier, Proposal& prop, MoverEnd& out,
@@ -222,6 +238,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   double elapsed = (double)R.iteration + toff;
   out.near_plane = kF32;
   out.absorbed = false;
+  out.exd = false;
   out.due = 0;
   out.t_steps = 0;
   out.n_reflect = 0;
@@ -306,10 +323,16 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
           if (candidates == 1) break;
           continue;
         }
-        bool full = false;
-        const D3 target = {(double)S.x[slot], (double)S.y[slot], (double)S.z[slot]};
-        const double factor = exact_disk_dev(R.P, R.walls, R.off, R.idx, R.ent_box, cp, disp, R.P.rxn_radius, target, &full);
-        if (full) err |= ERR_EXD;
+        double factor = 1.0;
+        if constexpr (kExd) {
+          bool full = false;
+          const D3 target = {(double)S.x[slot], (double)S.y[slot], (double)S.z[slot]};
+          factor = exact_disk_dev(R.P, R.walls, R.off, R.idx, R.ent_box, cp, disp, R.P.rxn_radius, target, &full);
+          if (full) err |= ERR_EXD;
+        } else if (disk_meets_walls(R, cp)) {
+          out.exd = true;
+          return false;
+        }
         if (factor < 0) {  // the disk is occluded
           if (candidates == 1) break;
           continue;

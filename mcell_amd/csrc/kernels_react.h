@@ -47,6 +47,7 @@ enum : uint32_t {
   kHdrLog = 7,            // events logged (snapshot mode)
   kHdrUnimolDue = 8,      // molecules whose unimolecular reaction falls in this phase
   kHdrErr = 9,            // error bits (ERR_*) of the call's kernels
+  kHdrExd = 10,           // movers the full step deferred to its exact_disk pass
   kHdrWords = 12,
 };
 

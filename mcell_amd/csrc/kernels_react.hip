@@ -223,13 +223,16 @@ __device__ __forceinline__ void append_slot(bool want, uint32_t j, uint32_t* __r
 // reflections and crossings of a committed step are added to n_reflect /
 // n_cross; a proposal carries those before its reaction in its pad word
 // (counted when it is accepted: a losing mover repeats its step).
-template <typename PosT>
+// With kExd false a step that needs exact_disk writes nothing and sets
+// `deferred` (the caller queues the mover for the exact_disk pass).
+template <typename PosT, bool kExd>
 __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<PosT>& S, const RMolView<PosT>& src,
                                          const RMolView<PosT>& out, uint32_t j, SpeciesTable species,
                                          const ZigguratTables& T, double barrier, uint32_t propose_only,
                                          Proposal* __restrict__ prop, uint32_t* __restrict__ owner,
                                          uint32_t* __restrict__ tile_owner, uint32_t* __restrict__ err_out,
-                                         bool& due, uint32_t* set_lds, uint32_t& n_reflect, uint32_t& n_cross) {
+                                         bool& due, bool& deferred, uint32_t* set_lds, uint32_t& n_reflect,
+                                         uint32_t& n_cross) {
   const uint32_t tag = src.tag[j];
   const uint32_t idv = src.id[j];
   const double toff = load_toff(src.toff, tag, j);
@@ -243,13 +246,20 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   e.utime = utime;
   e.near_plane = false;
   e.absorbed = false;
+  e.exd = false;
   e.due = 0;
   e.n_reflect = 0;
   e.n_cross = 0;
   Proposal p;
   bool proposed = false;
   const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
-  if (moves) proposed = react_mover(R, S, sp, T, pos, idv, tag, toff, utime, barrier, p, e, err, set_lds);
+  if (moves) proposed = react_mover<PosT, kExd>(R, S, sp, T, pos, idv, tag, toff, utime, barrier, p, e, err, set_lds);
+  deferred = e.exd;
+  if (deferred) {
+    due = false;
+    if (err) atomicOr(err_out, err);
+    return false;
+  }
   due = e.due != 0;
   if (due) {
     // the unimolecular event, settled after the phase's bimolecular claims
@@ -310,18 +320,24 @@ __global__ __launch_bounds__(256, 2) void k_move(ReactParams R, Snapshot<PosT> S
                                               uint32_t* __restrict__ ucount) {
   __shared__ ZigguratTables T;
   __shared__ uint32_t sets[SubpartSet::kCap * 256];
-  stage_zig(T);
   const uint32_t n = *count;
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  bool proposed = false, due = false;
-  uint32_t j = 0, n_reflect = 0, n_cross = 0;
-  if (q < n) {
-    j = list ? list[q] : q;
-    proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner, err_out, due,
-                        sets + threadIdx.x, n_reflect, n_cross);
+  if ((uint64_t)blockIdx.x * blockDim.x >= n) return;  // (uniform per block)
+  stage_zig(T);
+  uint32_t n_reflect = 0, n_cross = 0;
+  // grid-stride: the launch is sized by a host bound on the count, so a
+  // small grid walks a list whose actual length is known only on the device
+  for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
+    const uint32_t q = b + threadIdx.x;
+    bool proposed = false, due = false, deferred = false;
+    uint32_t j = 0;
+    if (q < n) {
+      j = list ? list[q] : q;
+      proposed = move_one<PosT, true>(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner,
+                                      err_out, due, deferred, sets + threadIdx.x, n_reflect, n_cross);
+    }
+    append_slot(proposed, j, plist, pcount);
+    append_slot(due, j, ulist, ucount);
   }
-  append_slot(proposed, j, plist, pcount);
-  append_slot(due, j, ulist, ucount);
   wave_count(R.ctr + CTR_REFLECTIONS, n_reflect);
   wave_count(R.ctr + CTR_CROSSINGS, n_cross);
 }
@@ -336,7 +352,8 @@ __global__ __launch_bounds__(256, 2) void k_move_drain(ReactParams R, Snapshot<P
                                                     uint32_t* __restrict__ plist, uint32_t* __restrict__ pcount,
                                                     uint32_t* __restrict__ owner, uint32_t* __restrict__ tile_owner,
                                                     uint32_t* __restrict__ err_out, uint32_t* __restrict__ ulist,
-                                                    uint32_t* __restrict__ ucount) {
+                                                    uint32_t* __restrict__ ucount, uint32_t* __restrict__ xlist,
+                                                    uint32_t* __restrict__ xcount) {
   __shared__ ZigguratTables T;
   __shared__ uint32_t sets[SubpartSet::kCap * 256];
   __shared__ uint32_t pre[kQueueShards + 1];
@@ -347,15 +364,16 @@ __global__ __launch_bounds__(256, 2) void k_move_drain(ReactParams R, Snapshot<P
   uint32_t n_reflect = 0, n_cross = 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
     const uint32_t q = base + threadIdx.x;
-    bool proposed = false, due = false;
+    bool proposed = false, due = false, deferred = false;
     uint32_t j = 0;
     if (q < total) {
       j = queue_at(Q.slots, Q.shard_cap, pre, q);
-      proposed = move_one(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner, err_out,
-                          due, sets + threadIdx.x, n_reflect, n_cross);
+      proposed = move_one<PosT, false>(R, S, src, out, j, species, T, barrier, propose_only, prop, owner,
+                                       tile_owner, err_out, due, deferred, sets + threadIdx.x, n_reflect, n_cross);
     }
     append_slot(proposed, j, plist, pcount);
     append_slot(due, j, ulist, ucount);
+    append_slot(deferred, j, xlist, xcount);
   }
   wave_count(R.ctr + CTR_REFLECTIONS, n_reflect);
   wave_count(R.ctr + CTR_CROSSINGS, n_cross);
@@ -921,6 +939,11 @@ __global__ __launch_bounds__(256) void k_count_tiles(const uint32_t* __restrict_
 
 // --------------------------------------------------------------- launchers --
 static inline dim3 blocks_for(uint32_t n) { return dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+// The full-step list kernel walks its list grid-stride: at most two
+// workgroups per CU fit (its LDS), and every workgroup launched beyond the
+// list's actual length (a device count under a host bound) still costs a
+// workgroup launch.
+static inline dim3 move_grid(uint32_t max_movers) { return dim3(std::min<uint32_t>(blocks_for(max_movers).x, 512u)); }
 
 template <typename PosT>
 hipError_t ReactPipeline<PosT>::sort(hipStream_t s, uint32_t n_in, uint32_t num_cells, uint32_t* err) {
@@ -964,12 +987,18 @@ hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uin
     hipLaunchKernelGGL(k_move_light<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, M, count, species, barrier,
                        Q, err);
     Q.nshards = kQueueShards / 2 + std::min<uint32_t>(kQueueShards / 2, blocks_for(max_movers).x);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrExd, 0u, 1u);
     hipLaunchKernelGGL(k_move_drain<PosT>, dim3(Q.nshards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
                        propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist,
-                       hdr + kHdrUnimolDue);
+                       hdr + kHdrUnimolDue, mlist, hdr + kHdrExd);
+    // the steps the drain deferred, with exact_disk (mlist is free until the
+    // products are listed)
+    hipLaunchKernelGGL(k_move<PosT>, move_grid(max_movers), dim3(256), 0, s, R, S, Sm, M, mlist, hdr + kHdrExd,
+                       species, barrier, propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, tile_owner,
+                       err, ulist, hdr + kHdrUnimolDue);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_move<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, snapshot_src ? Sm : M, M, list,
+  hipLaunchKernelGGL(k_move<PosT>, move_grid(max_movers), dim3(256), 0, s, R, S, snapshot_src ? Sm : M, M, list,
                      count, species, barrier, propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner,
                      tile_owner, err, ulist, hdr + kHdrUnimolDue);
   return hipGetLastError();
