@@ -328,6 +328,7 @@ def run_gpu(args, wl, precision):
     n_counts = 0
     st_sum = {"molecule_steps": 0, "reactions": 0, "settle_rounds": 0, "wall_hits": 0}
     hard = 0
+    form = 0
     t0 = time.perf_counter()
     b = max(int(wl.barrier_interval), 1)
     while it < end:
@@ -338,6 +339,7 @@ def run_gpu(args, wl, precision):
         for k in st_sum:
             st_sum[k] += getattr(st, k)
         hard = st.slow_path_last
+        form = st.drain_form
         it += chunk
         if every and it % every == 0:
             eng.count_species_noalloc(counts)
@@ -348,7 +350,8 @@ def run_gpu(args, wl, precision):
     steps_done = st_sum["molecule_steps"] if reactive(wl) else wl.num_molecules * args.steps
     return {"secs": secs, "steps_done": steps_done, "value": steps_done / secs, "avg_ms": kernel_ms / args.steps,
             "reactions": st_sum["reactions"], "rounds": st_sum["settle_rounds"], "hard": hard,
-            "reflections": st_sum["wall_hits"], "counts": n_counts}
+            "reflections": st_sum["wall_hits"], "counts": n_counts,
+            "drain_form": {0: None, 1: "exact_disk in place", 2: "exact_disk deferred"}.get(form)}
 
 
 def traffic_passes(args, wl_subpart, main_prec, iterations):
@@ -464,6 +467,7 @@ def main():
     if reactive(wl):
         out["reactions"] = r["reactions"]
         out["settle_rounds"] = r["rounds"]
+        out["drain_form"] = r["drain_form"]
         out["species_counts_taken"] = r["counts"]
         out["roofline"]["full_step_fraction"] = r["hard"] / max(per_step, 1)
     else:

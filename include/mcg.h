@@ -77,6 +77,14 @@ typedef struct mcg_config {
 #define MCG_FLAG_PROPOSE_ONLY 2u  /* parity mode: every molecule moves against the iteration-start
                                      snapshot and successful reaction tests are logged, not carried
                                      out (a reacting molecule stops where its last ray segment began) */
+/* The full step's drain (the movers the light pass queues) runs exact_disk
+ * either in place, with a small vertex pool, or defers every step with a
+ * disk that meets walls to a pass of its own: the results are the same, the
+ * speed depends on how often disks meet walls.  By default the engine times
+ * both forms on the first iterations after the geometry or the molecules are
+ * set and keeps the faster; these flags fix one. */
+#define MCG_FLAG_DRAIN_INLINE 4u
+#define MCG_FLAG_DRAIN_DEFER 8u
 
 /* BNG::Species fields on this path (space_step/time_step from
  * Species::update_space_and_time_step, MCell3 src/mcell_species.c:237-300). */
@@ -148,7 +156,7 @@ typedef struct mcg_step_stats {
   uint32_t slow_path_last;       /* molecules that needed the full ray trace in the last iteration */
   uint64_t transparent_crossings;/* walls crossed through a transparent surface class */
   uint32_t settle_rounds;        /* claim-settling rounds of the reactive path, summed over iterations */
-  uint32_t reserved;
+  uint32_t drain_form;           /* drain form of the last iteration: 1 in place, 2 deferred (MCG_FLAG_DRAIN_*) */
 } mcg_step_stats;
 
 const char* mcg_last_error(void);

@@ -17,6 +17,8 @@ MCG_PREC_FP64 = 0
 MCG_PREC_FP32_FALLBACK = 1
 MCG_FLAG_TRACE_HITS = 1
 MCG_FLAG_PROPOSE_ONLY = 2
+MCG_FLAG_DRAIN_INLINE = 4
+MCG_FLAG_DRAIN_DEFER = 8
 
 MCG_SPF_CAN_VOLVOL = 1
 MCG_SPF_CAN_VOLWALL = 2
@@ -140,7 +142,7 @@ class McgStepStats(ctypes.Structure):
         ("slow_path_last", ctypes.c_uint32),
         ("transparent_crossings", ctypes.c_uint64),
         ("settle_rounds", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("drain_form", ctypes.c_uint32),
     ]
 
 

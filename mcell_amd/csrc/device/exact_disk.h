@@ -3,7 +3,8 @@
 // cut off by walls, or -1 when a wall lies between the mover and its target.
 // Runs only for collisions with a wall of the collision subpartition within
 // the reaction radius; the reference's linked lists become a fixed pool of
-// vertex records addressed by index.
+// vertex records addressed by index.  Host-callable too (the pool-bound
+// test runs it on the CPU).
 #pragma once
 #include "path64.h"
 
@@ -26,18 +27,24 @@ static_assert(sizeof(V) == 40, "vertex record");
 
 // 2 vertices per wall edge plus crossings and spans.  A disk near a finely
 // meshed surface (triangle edges below the reaction radius) is cut by tens
-// of walls; the pool lives in private memory, so its size costs nothing until
-// used.  Running out is reported as an error.
+// of walls.  The pool lives in private memory, sized per kernel: the passes
+// of the full step run with small pools and hand a disk that overflows one
+// to a pass with the large pool (kPool), where running out is reported as an
+// error.  A pool that runs out stops exact_disk with *full set.
 constexpr int kPool = 256;
 
-struct Pool {
-  V p[kPool];
+template <int N>
+struct PoolT {
+  static constexpr int kCap = N;
+  V p[N];
   int n;
   bool full;
-  __device__ __forceinline__ int make() {
-    if (n >= kPool) {
+  // On overflow `full` is set and the last slot handed out again; every
+  // caller checks `full` right after and stops before linking it anywhere.
+  __host__ __device__ __forceinline__ int make() {
+    if (n >= N) {
       full = true;
-      return kPool - 1;  // reuse the last slot; the caller reports the failure
+      return N - 1;
     }
     V& x = p[n];
     x.next = -1; x.u = 0; x.v = 0; x.r2 = 0; x.zeta = 0; x.e = -1; x.span = -1; x.role = UNDEFINED;
@@ -46,7 +53,7 @@ struct Pool {
 };
 
 // exact_disk_utils.inl:54-80
-__device__ __forceinline__ double zetize(double y, double x) {
+__host__ __device__ __forceinline__ double zetize(double y, double x) {
   if (y >= 0) {
     if (x >= 0) {
       if (x < y) return 1 - 0.5 * x / y;
@@ -64,7 +71,7 @@ __device__ __forceinline__ double zetize(double y, double x) {
 }
 
 // exact_disk_utils.inl:94-145
-__device__ __forceinline__ void coordize(D3 mv, D3& m, D3& u, D3& v) {
+__host__ __device__ __forceinline__ void coordize(D3 mv, D3& m, D3& u, D3& v) {
   double a = 1 / sqrt(dot3(mv, mv));
   m = {a * mv.x, a * mv.y, a * mv.z};
   double mx2 = m.x * m.x, my2 = m.y * m.y, mz2 = m.z * m.z;
@@ -85,7 +92,7 @@ __device__ __forceinline__ void coordize(D3 mv, D3& m, D3& u, D3& v) {
 }
 
 // src4/defines.h:766-807
-__device__ __forceinline__ bool distinguishable_vec3(D3 a, D3 b, double eps) {
+__host__ __device__ __forceinline__ bool distinguishable_vec3(D3 a, D3 b, double eps) {
   double c = fabs(a.x), d;
   d = fabs(a.y); if (d > c) c = d;
   d = fabs(a.z); if (d > c) c = d;
@@ -99,16 +106,19 @@ __device__ __forceinline__ bool distinguishable_vec3(D3 a, D3 b, double eps) {
   return c * eps < cc;
 }
 
-__device__ __forceinline__ double span(const Pool& P, int v1, int v2, int p) {
+template <class Pool>
+__host__ __device__ __forceinline__ double span(const Pool& P, int v1, int v2, int p) {
   return (P.p[v1].u - P.p[p].u) * (P.p[v2].v - P.p[p].v) - (P.p[v2].u - P.p[p].u) * (P.p[v1].v - P.p[p].v);
 }
-__device__ __forceinline__ double time_span(const Pool& P, int v1, int v2, int p) {
+template <class Pool>
+__host__ __device__ __forceinline__ double time_span(const Pool& P, int v1, int v2, int p) {
   return (P.p[p].u * P.p[v1].v - P.p[p].v * P.p[v1].u) /
          (P.p[p].v * (P.p[v2].u - P.p[v1].u) - P.p[p].u * (P.p[v2].v - P.p[v1].v));
 }
 
 // exact_disk_utils.inl:517-817
-__device__ double area_multiple(Pool& P, int head, double R2) {
+template <class Pool>
+__host__ __device__ double area_multiple(Pool& P, int head, double R2) {
   const double PI = 3.14159265358979323846;
   int vp = P.p[head].next;
   int ppa = head, ppb = head;
@@ -134,15 +144,16 @@ __device__ double area_multiple(Pool& P, int head, double R2) {
   P.p[ppb].next = ppa;
 
   int pa = P.make(), pb = P.make();
+  if (P.full) return 0;  // (a reused slot would link the lists into cycles)
   ppb = -1;
   int guard = 0;
   for (ppa = head; ppa != head || ppb == -1; ppa = P.p[ppa].next) {
-    if (++guard > 4 * kPool) { P.full = true; return 0; }
+    if (++guard > 4 * Pool::kCap) { P.full = true; return 0; }
     if (P.p[ppa].role != HEAD) continue;
     ppb = P.p[ppa].e;
     int pqa;
     for (pqa = P.p[ppa].next; pqa != ppb; pqa = P.p[pqa].next) {
-      if (++guard > 16 * kPool) { P.full = true; return 0; }
+      if (++guard > 16 * Pool::kCap) { P.full = true; return 0; }
       if (P.p[pqa].role != HEAD) continue;
       int pqb = P.p[pqa].e;
       P.p[pa].u = P.p[ppb].u - P.p[ppa].u;
@@ -178,6 +189,7 @@ __device__ double area_multiple(Pool& P, int head, double R2) {
         if (s <= POS_EPS * R2 || s >= P.p[pa].v * P.p[pa].v * (1 - POS_EPS)) continue;
       }
       int vq = P.make();
+      if (P.full) return 0;
       P.p[vq].u = P.p[pqa].u + t * P.p[pb].u;
       P.p[vq].v = P.p[pqa].v + t * P.p[pb].v;
       P.p[vq].r2 = P.p[vq].u * P.p[vq].u + P.p[vq].v * P.p[vq].v;
@@ -218,6 +230,7 @@ __device__ double area_multiple(Pool& P, int head, double R2) {
       if (!distinguishable(P.p[vq].zeta, P.p[P.p[vp].e].zeta, POS_EPS)) break;
       if (P.p[vq].role == OTHER) continue;
       int vr = P.make();
+      if (P.full) return 0;
       P.p[vr].next = P.p[vq].span;
       P.p[vq].span = vr;
       P.p[vr].e = vp;
@@ -230,7 +243,7 @@ __device__ double area_multiple(Pool& P, int head, double R2) {
   int vs = -1;
   guard = 0;
   for (vp = head; zeta < 4 - POS_EPS; vp = P.p[vp].next) {
-    if (++guard > 8 * kPool) { P.full = true; return 0; }
+    if (++guard > 8 * Pool::kCap) { P.full = true; return 0; }
     if (P.p[vp].role == OTHER) continue;
     if (!distinguishable(P.p[vp].zeta, last_zeta, POS_EPS)) continue;
     last_zeta = P.p[vp].zeta;
@@ -321,13 +334,14 @@ __device__ double area_multiple(Pool& P, int head, double R2) {
 }  // namespace exd
 
 // exact_disk_utils.inl:840-1145; `full` set when the pool ran out.
-__device__ __noinline__ double exact_disk_dev(const PartitionParams& PP, const WallRec64* __restrict__ walls,
+template <int N = exd::kPool>
+__host__ __device__ __noinline__ double exact_disk_dev(const PartitionParams& PP, const WallRec64* __restrict__ walls,
                                               const uint32_t* __restrict__ off, const uint32_t* __restrict__ idx,
                                               const uint32_t* __restrict__ box, D3 loc, D3 mv, double R, D3 target,
                                               bool* full) {
   using namespace exd;
   const double PI = 3.14159265358979323846;
-  Pool P;
+  PoolT<N> P;
   P.n = 0;
   P.full = false;
   *full = false;
@@ -429,6 +443,7 @@ __device__ __noinline__ double exact_disk_dev(const PartitionParams& PP, const W
     }
     // construct_final_endpoints (exact_disk_utils.inl:275-307)
     int ia = P.make(), ib = P.make();
+    if (P.full) break;  // reported below; the lists are not walked
     if (ti > 0) {
       P.p[ia].u = pa_u + ti * (pb_u - pa_u);
       P.p[ia].v = pa_v + ti * (pb_v - pa_v);

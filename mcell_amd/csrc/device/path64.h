@@ -13,19 +13,19 @@ namespace mcg {
 struct D3 {
   double x, y, z;
 };
-__device__ __forceinline__ D3 d3(double x, double y, double z) { return {x, y, z}; }
-__device__ __forceinline__ double dot3(D3 a, D3 b) {
+__host__ __device__ __forceinline__ D3 d3(double x, double y, double z) { return {x, y, z}; }
+__host__ __device__ __forceinline__ double dot3(D3 a, D3 b) {
   double tx = a.x * b.x, ty = a.y * b.y, tz = a.z * b.z;
   return (tx + ty) + tz;
 }
-__device__ __forceinline__ double maxd(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ double maxd(double a, double b) { return (a < b) ? b : a; }
 
 constexpr double POS_EPS = 1e-12;
 constexpr double STIME_EPS = 1e-12;
 constexpr double FLT_MIN_D = 1.17549435082228750797e-38;
 
 // src/util.c:449-462 (MCell3 distinguishable, == libbng distinguishable_p)
-__device__ __forceinline__ bool distinguishable(double a, double b, double eps) {
+__host__ __device__ __forceinline__ bool distinguishable(double a, double b, double eps) {
   double c = fabs(a - b);
   a = fabs(a);
   if (a < 1) a = 1;
@@ -35,20 +35,20 @@ __device__ __forceinline__ bool distinguishable(double a, double b, double eps) 
   return c > eps;
 }
 
-__device__ __forceinline__ bool in_partition(const PartitionParams& P, D3 p) {
+__host__ __device__ __forceinline__ bool in_partition(const PartitionParams& P, D3 p) {
   return p.x >= P.ox && p.y >= P.oy && p.z >= P.oz && p.x < P.qx && p.y < P.qy && p.z < P.qz;
 }
 
 // Partition::get_subpart_3d_indices: IVec3((pos - origin) * rcp), truncating
-__device__ __forceinline__ void subpart_3d(const PartitionParams& P, D3 p, int& ix, int& iy, int& iz) {
+__host__ __device__ __forceinline__ void subpart_3d(const PartitionParams& P, D3 p, int& ix, int& iy, int& iz) {
   ix = (int)((p.x - P.ox) * P.sp_rcp);
   iy = (int)((p.y - P.oy) * P.sp_rcp);
   iz = (int)((p.z - P.oz) * P.sp_rcp);
 }
-__device__ __forceinline__ uint32_t subpart_of(const PartitionParams& P, int ix, int iy, int iz) {
+__host__ __device__ __forceinline__ uint32_t subpart_of(const PartitionParams& P, int ix, int iy, int iz) {
   return (uint32_t)ix + (uint32_t)iy * P.n + (uint32_t)iz * P.n2;
 }
-__device__ __forceinline__ uint32_t subpart_index(const PartitionParams& P, D3 p) {
+__host__ __device__ __forceinline__ uint32_t subpart_index(const PartitionParams& P, D3 p) {
   int ix, iy, iz;
   subpart_3d(P, p, ix, iy, iz);
   return subpart_of(P, ix, iy, iz);

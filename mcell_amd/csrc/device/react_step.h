@@ -218,11 +218,14 @@ __device__ __forceinline__ double start_unimol_time(const UnimolRec* __restrict_
 // reactions the molecule collisions of each segment in time order, each
 // reaction test drawing from the mover's stream.  The first successful test
 // ends the step with a proposal (return true).
-// kExd false (the drain's form): a collision whose disk meets walls does not
-// call exact_disk -- the step stops with out.exd set and writes nothing, and
-// a second pass repeats it with kExd true.  Keeping exact_disk's vertex pool
-// and its call out of the drain keeps the drain's registers off the stack.
-template <typename PosT, bool kExd>
+// kPoolN is the vertex pool of exact_disk in this form.  0: a collision
+// whose disk meets walls stops the step with out.exd set (nothing written)
+// and a later pass repeats the step with a pool; 0 < kPoolN < exd::kPool:
+// exact_disk with kPoolN records, and a disk that overflows them stops the
+// step the same way; exd::kPool: the last pass, where running out is an
+// error.  The pool is private memory per lane, so the small forms keep the
+// scratch of the kernels that run them small.
+template <typename PosT, int kPoolN>
 __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot<PosT>& S, const SpeciesRec& sp,
                                             const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
                                             double utime, double barrier, Proposal& prop, MoverEnd& out,
@@ -324,14 +327,24 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
           continue;
         }
         double factor = 1.0;
-        if constexpr (kExd) {
-          bool full = false;
-          const D3 target = {(double)S.x[slot], (double)S.y[slot], (double)S.z[slot]};
-          factor = exact_disk_dev(R.P, R.walls, R.off, R.idx, R.ent_box, cp, disp, R.P.rxn_radius, target, &full);
-          if (full) err |= ERR_EXD;
-        } else if (disk_meets_walls(R, cp)) {
-          out.exd = true;
-          return false;
+        if (disk_meets_walls(R, cp)) {
+          if constexpr (kPoolN == 0) {
+            out.exd = true;
+            return false;
+          } else {
+            bool full = false;
+            const D3 target = {(double)S.x[slot], (double)S.y[slot], (double)S.z[slot]};
+            factor = exact_disk_dev<kPoolN>(R.P, R.walls, R.off, R.idx, R.ent_box, cp, disp, R.P.rxn_radius, target,
+                                            &full);
+            if (full) {
+              if constexpr (kPoolN >= exd::kPool) {
+                err |= ERR_EXD;
+              } else {  // the pool ran out: a pass with a larger one repeats the step
+                out.exd = true;
+                return false;
+              }
+            }
+          }
         }
         if (factor < 0) {  // the disk is occluded
           if (candidates == 1) break;

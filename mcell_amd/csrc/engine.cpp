@@ -174,6 +174,7 @@ int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint
   if (nw)
     HIP_TRY(hipMemcpy(c->wall_class_d.p, c->wall_class_host.data(), nw * 4, hipMemcpyHostToDevice));
   c->rx.grid_dirty = true;
+  c->rx.reset_drain_form();
   c->walls_host.resize(nw);
   for (uint32_t i = 0; i < nw; ++i)
     c->walls_host[i] = wall_constants(vertices + 3 * walls[3 * i], vertices + 3 * walls[3 * i + 1],
@@ -201,15 +202,8 @@ int mcg_set_geometry(mcg_ctx* c, const double* vertices, uint32_t nv, const uint
     e32[k] = w32[c->idx_host[k]];
   }
   // sub-cell box of every entry: the wall's bounding box within the entry's subpartition
-  std::vector<uint32_t> eb(std::max<size_t>(ne, 1), kBoxAll);
-  const size_t n3 = c->off_host.size() - 1;
-  for (size_t s = 0; s < n3; ++s)
-    for (uint32_t k = c->off_host[s]; k < c->off_host[s + 1]; ++k) {
-      const WallRec64& w = c->walls_host[c->idx_host[k]];
-      eb[k] = region_box(P, (uint32_t)s, std::min({w.p0x, w.p1x, w.p2x}), std::min({w.p0y, w.p1y, w.p2y}),
-                         std::min({w.p0z, w.p1z, w.p2z}), std::max({w.p0x, w.p1x, w.p2x}),
-                         std::max({w.p0y, w.p1y, w.p2y}), std::max({w.p0z, w.p1z, w.p2z}));
-    }
+  std::vector<uint32_t> eb;
+  entry_boxes(P, c->walls_host, c->off_host, c->idx_host, eb);
   if (c->ent64.alloc(e64.size()) || c->ent32.alloc(e32.size()) || c->ent_box.alloc(eb.size()))
     return fail(MCG_E_NOMEM, "wall entry buffers");
   HIP_TRY(hipMemcpy(c->ent64.p, e64.data(), e64.size() * sizeof(WallRec64), hipMemcpyHostToDevice));
@@ -304,6 +298,7 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
   if (count) {
     c->rx.next_id_dirty = true;
     c->rx.grid_dirty = true;
+    c->rx.reset_drain_form();
   }
   std::vector<float> fx, fy, fz;
   if (c->fp32()) {

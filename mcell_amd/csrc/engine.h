@@ -80,6 +80,20 @@ struct ReactState {
   std::vector<uint32_t> next_id_host;  // per replica: 1 + largest id seen
   bool next_id_dirty = true;
   uint32_t* hdr_host = nullptr;        // pinned mirror of hdr
+  // the drain's form (1 exact_disk in place, 2 deferred): fixed by a flag,
+  // else timed on the first iterations after the geometry or the molecules
+  // are set (0 while undecided; the trials alternate the forms)
+  uint32_t drain_form = 0;
+  uint32_t form_trials = 0;
+  double form_ms[2] = {0, 0};
+  uint32_t form_n[2] = {0, 0};
+  uint32_t last_form = 0;
+  void reset_drain_form() {
+    drain_form = 0;
+    form_trials = 0;
+    form_ms[0] = form_ms[1] = 0;
+    form_n[0] = form_n[1] = 0;
+  }
   // volume-surface reaction classes
   std::vector<VolSurfRec> vs;
   std::vector<int32_t> pair_vs;  // vol species x surf species -> class or -1

@@ -118,6 +118,28 @@ int read_hdr(mcg_ctx* c) {
   return MCG_OK;
 }
 
+// The drain's form for the next iteration: the flag's, the decided one, or
+// (while undecided) the trials' alternation.
+uint32_t drain_form_for(const mcg_ctx* c) {
+  if (c->cfg.flags & MCG_FLAG_DRAIN_INLINE) return 1;
+  if (c->cfg.flags & MCG_FLAG_DRAIN_DEFER) return 2;
+  return c->rx.drain_form ? c->rx.drain_form : 1 + (c->rx.form_trials & 1);
+}
+
+// Books the move phase's time of an undecided form's trial.  The first trial
+// of each form is not counted (its kernels' first launch loads their code);
+// after three counted trials of each the faster form is kept.
+void drain_form_timed(ReactState& r, uint32_t form, float ms) {
+  constexpr uint32_t kTrials = 8;
+  if (r.drain_form) return;
+  if (r.form_trials >= 2) {
+    r.form_ms[form - 1] += ms;
+    ++r.form_n[form - 1];
+  }
+  if (++r.form_trials == kTrials)
+    r.drain_form = r.form_ms[0] * r.form_n[1] <= r.form_ms[1] * r.form_n[0] ? 1u : 2u;
+}
+
 // Claims, acceptance and repeated moves until every proposal of the phase
 // is settled.  max_props bounds the phase's proposals (its movers); the
 // settling kernels read the actual count on the device, so the header is
@@ -169,6 +191,8 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
     HIP_TRY(launch_copy_u32(c->stream, p.cell_start + r.num_cells, p.hdr + kHdrNumOut));
     HIP_TRY(launch_copy_u32(c->stream, p.cell_start + r.num_cells, p.hdr + kHdrSnapshot));
     HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
+    const uint32_t form = drain_form_for(c);
+    p.drain_inline = form == 1;
     HIP_TRY(hipEventRecord(c->evm0, c->stream));
     HIP_TRY(p.move(c->stream, true, nullptr, p.cell_start + r.num_cells, n_in, barrier, propose_only, err));
     HIP_TRY(hipEventRecord(c->evm1, c->stream));
@@ -212,6 +236,8 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->evm0, c->evm1));
     move_ms += ms;
+    if (n_in) drain_form_timed(r, form, ms);
+    r.last_form = form;
     const uint32_t e = h[kHdrErr];
     if (e) {
       if (e & ERR_ESCAPED) return fail(MCG_E_ESCAPED, "a molecule escaped the partition");
@@ -232,6 +258,7 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
     stats->wall_hits = ctr[CTR_REFLECTIONS];
     stats->transparent_crossings = ctr[CTR_CROSSINGS];
     stats->settle_rounds = rounds;
+    stats->drain_form = r.last_form;
     // movers of the last iteration that took the full step
     std::vector<uint32_t> q(kNumQueueShards, 0);
     if (num && c->n)

@@ -2,6 +2,8 @@
 // per-subpartition wall lists, computed once when the geometry is handed over.
 #include "geometry.h"
 
+#include "device/wallbox.h"
+
 #include <algorithm>
 #include <cmath>
 
@@ -186,6 +188,19 @@ void bin_walls(const PartitionParams& P, const std::vector<WallRec64>& walls, st
   for (auto& p : pairs) off[p.first + 1]++;
   for (size_t i = 0; i < n3; ++i) off[i + 1] += off[i];
   for (size_t i = 0; i < pairs.size(); ++i) idx[i] = pairs[i].second;
+}
+
+void entry_boxes(const PartitionParams& P, const std::vector<WallRec64>& walls, const std::vector<uint32_t>& off,
+                 const std::vector<uint32_t>& idx, std::vector<uint32_t>& box) {
+  box.assign(std::max<size_t>(idx.size(), 1), kBoxAll);
+  const size_t n3 = off.size() - 1;
+  for (size_t s = 0; s < n3; ++s)
+    for (uint32_t k = off[s]; k < off[s + 1]; ++k) {
+      const WallRec64& w = walls[idx[k]];
+      box[k] = region_box(P, (uint32_t)s, std::min({w.p0x, w.p1x, w.p2x}), std::min({w.p0y, w.p1y, w.p2y}),
+                          std::min({w.p0z, w.p1z, w.p2z}), std::max({w.p0x, w.p1x, w.p2x}),
+                          std::max({w.p0y, w.p1y, w.p2y}), std::max({w.p0z, w.p1z, w.p2z}));
+    }
 }
 
 }  // namespace mcg

@@ -19,4 +19,9 @@ WallRec32 wall_to_fp32(const WallRec64& w);
 void bin_walls(const PartitionParams& P, const std::vector<WallRec64>& walls, std::vector<uint32_t>& off,
                std::vector<uint32_t>& idx);
 
+// The sub-cell box (device/wallbox.h) of every list entry: the wall's
+// bounding box within the entry's subpartition.
+void entry_boxes(const PartitionParams& P, const std::vector<WallRec64>& walls, const std::vector<uint32_t>& off,
+                 const std::vector<uint32_t>& idx, std::vector<uint32_t>& box);
+
 }  // namespace mcg

@@ -66,6 +66,7 @@ struct ReactPipeline {
   uint32_t *cnt, *pre, *head, *next_id;
   uint32_t* hdr;
   SlowQueue queue;  // light-pass queue of movers that need the full step
+  bool drain_inline;  // the drain runs exact_disk in place (else it defers those steps)
   void* scan_tmp;
   size_t scan_tmp_bytes;
 

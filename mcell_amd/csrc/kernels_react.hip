@@ -223,9 +223,10 @@ __device__ __forceinline__ void append_slot(bool want, uint32_t j, uint32_t* __r
 // reflections and crossings of a committed step are added to n_reflect /
 // n_cross; a proposal carries those before its reaction in its pad word
 // (counted when it is accepted: a losing mover repeats its step).
-// With kExd false a step that needs exact_disk writes nothing and sets
-// `deferred` (the caller queues the mover for the exact_disk pass).
-template <typename PosT, bool kExd>
+// A step that needs a larger exact_disk pool than kPoolN (react_mover)
+// writes nothing and sets `deferred` (the caller queues the mover for a pass
+// with a larger pool).
+template <typename PosT, int kPoolN>
 __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<PosT>& S, const RMolView<PosT>& src,
                                          const RMolView<PosT>& out, uint32_t j, SpeciesTable species,
                                          const ZigguratTables& T, double barrier, uint32_t propose_only,
@@ -253,7 +254,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   Proposal p;
   bool proposed = false;
   const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
-  if (moves) proposed = react_mover<PosT, kExd>(R, S, sp, T, pos, idv, tag, toff, utime, barrier, p, e, err, set_lds);
+  if (moves) proposed = react_mover<PosT, kPoolN>(R, S, sp, T, pos, idv, tag, toff, utime, barrier, p, e, err, set_lds);
   deferred = e.exd;
   if (deferred) {
     due = false;
@@ -304,11 +305,13 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
 // One mover per thread.  Movers are snapshot slots [0, *count) when list is
 // null, else slots list[0, *count).  src holds the movers' start state (the
 // snapshot, or the output arrays for products); out receives end states.
+// Movers whose exact_disk needs more than kPoolN vertex records are appended
+// to xlist (kPoolN < exd::kPool).
 // The full-step kernels hold ~60 KB of LDS per workgroup, so at most two
 // workgroups share a CU (2 waves per SIMD); their registers are capped at
 // what two waves per SIMD allow (256), the rest spilling to scratch, which
 // measured ~10 % faster than the uncapped one-wave form.
-template <typename PosT>
+template <typename PosT, int kPoolN>
 __global__ __launch_bounds__(256, 2) void k_move(ReactParams R, Snapshot<PosT> S, RMolView<PosT> src,
                                               RMolView<PosT> out, const uint32_t* __restrict__ list,
                                               const uint32_t* __restrict__ count,
@@ -317,7 +320,8 @@ __global__ __launch_bounds__(256, 2) void k_move(ReactParams R, Snapshot<PosT> S
                                               uint32_t* __restrict__ plist, uint32_t* __restrict__ pcount,
                                               uint32_t* __restrict__ owner, uint32_t* __restrict__ tile_owner,
                                               uint32_t* __restrict__ err_out, uint32_t* __restrict__ ulist,
-                                              uint32_t* __restrict__ ucount) {
+                                              uint32_t* __restrict__ ucount, uint32_t* __restrict__ xlist,
+                                              uint32_t* __restrict__ xcount) {
   __shared__ ZigguratTables T;
   __shared__ uint32_t sets[SubpartSet::kCap * 256];
   const uint32_t n = *count;
@@ -332,11 +336,12 @@ __global__ __launch_bounds__(256, 2) void k_move(ReactParams R, Snapshot<PosT> S
     uint32_t j = 0;
     if (q < n) {
       j = list ? list[q] : q;
-      proposed = move_one<PosT, true>(R, S, src, out, j, species, T, barrier, propose_only, prop, owner, tile_owner,
-                                      err_out, due, deferred, sets + threadIdx.x, n_reflect, n_cross);
+      proposed = move_one<PosT, kPoolN>(R, S, src, out, j, species, T, barrier, propose_only, prop, owner,
+                                        tile_owner, err_out, due, deferred, sets + threadIdx.x, n_reflect, n_cross);
     }
     append_slot(proposed, j, plist, pcount);
     append_slot(due, j, ulist, ucount);
+    if constexpr (kPoolN < exd::kPool) append_slot(deferred, j, xlist, xcount);
   }
   wave_count(R.ctr + CTR_REFLECTIONS, n_reflect);
   wave_count(R.ctr + CTR_CROSSINGS, n_cross);
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(256, 2) void k_move(ReactParams R, Snapshot<PosT> S
 // The full step for every queued mover (the shards read as one list,
 // grid-stride; consecutive threads take consecutive slots of a shard, which
 // came from neighbouring cells).
-template <typename PosT>
+template <typename PosT, int kPoolN>
 __global__ __launch_bounds__(256, 2) void k_move_drain(ReactParams R, Snapshot<PosT> S, RMolView<PosT> src,
                                                     RMolView<PosT> out, SlowQueue Q, SpeciesTable species, double barrier,
                                                     uint32_t propose_only, Proposal* __restrict__ prop,
@@ -368,8 +373,8 @@ __global__ __launch_bounds__(256, 2) void k_move_drain(ReactParams R, Snapshot<P
     uint32_t j = 0;
     if (q < total) {
       j = queue_at(Q.slots, Q.shard_cap, pre, q);
-      proposed = move_one<PosT, false>(R, S, src, out, j, species, T, barrier, propose_only, prop, owner,
-                                       tile_owner, err_out, due, deferred, sets + threadIdx.x, n_reflect, n_cross);
+      proposed = move_one<PosT, kPoolN>(R, S, src, out, j, species, T, barrier, propose_only, prop, owner,
+                                        tile_owner, err_out, due, deferred, sets + threadIdx.x, n_reflect, n_cross);
     }
     append_slot(proposed, j, plist, pcount);
     append_slot(due, j, ulist, ucount);
@@ -973,12 +978,29 @@ size_t ReactPipeline<PosT>::scan_bytes(uint32_t num_cells, uint32_t capacity) {
   return std::max(a, std::max(b, c));
 }
 
+// Vertex pools of exact_disk (40 bytes a record, private memory per lane)
+// in the full-step passes: the drain runs with kDrainPool records in place
+// (drain_inline) or defers every step with a disk that meets walls, the list
+// moves (reruns, products) start with kListPool, and the steps they defer
+// run again with exd::kPool.  The pool in place costs the drain about a
+// quarter of its time even where disks rarely meet walls; deferring costs a
+// second, sparse pass over the deferred steps.
+constexpr int kDrainPool = 48;
+constexpr int kListPool = 48;
+
 template <typename PosT>
 hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uint32_t* list, const uint32_t* count,
                                      uint32_t max_movers, double barrier, bool propose_only, uint32_t* err) {
   if (max_movers == 0) return hipSuccess;
   Snapshot<PosT> S{Sm.x, Sm.y, Sm.z, Sm.id, Sm.tag, consumed, Sm.toff, Sm.utime};
+  const uint32_t po = propose_only ? 1u : 0u;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrProposals, 0u, 1u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrExd, 0u, 1u);
+  // the deferred steps go to a list that is free during this move: mlist
+  // until the products are listed, else the rerun list settle() does not
+  // read next
+  uint32_t* const xlist = (list == nullptr) ? mlist : (list == rlist0 ? rlist1 : rlist0);
+  const RMolView<PosT> src = snapshot_src ? Sm : M;
   if (list == nullptr && snapshot_src) {
     // every snapshot slot: light pass, then the queued movers in full
     SlowQueue Q = queue;
@@ -987,20 +1009,23 @@ hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uin
     hipLaunchKernelGGL(k_move_light<PosT>, blocks_for(max_movers), dim3(256), 0, s, R, S, M, count, species, barrier,
                        Q, err);
     Q.nshards = kQueueShards / 2 + std::min<uint32_t>(kQueueShards / 2, blocks_for(max_movers).x);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrExd, 0u, 1u);
-    hipLaunchKernelGGL(k_move_drain<PosT>, dim3(Q.nshards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
-                       propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist,
-                       hdr + kHdrUnimolDue, mlist, hdr + kHdrExd);
-    // the steps the drain deferred, with exact_disk (mlist is free until the
-    // products are listed)
-    hipLaunchKernelGGL(k_move<PosT>, move_grid(max_movers), dim3(256), 0, s, R, S, Sm, M, mlist, hdr + kHdrExd,
-                       species, barrier, propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner, tile_owner,
-                       err, ulist, hdr + kHdrUnimolDue);
-    return hipGetLastError();
+    if (drain_inline)
+      hipLaunchKernelGGL((k_move_drain<PosT, kDrainPool>), dim3(Q.nshards), dim3(256), 0, s, R, S, Sm, M, Q, species,
+                         barrier, po, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist,
+                         hdr + kHdrUnimolDue, xlist, hdr + kHdrExd);
+    else
+      hipLaunchKernelGGL((k_move_drain<PosT, 0>), dim3(Q.nshards), dim3(256), 0, s, R, S, Sm, M, Q, species, barrier,
+                         po, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist, hdr + kHdrUnimolDue,
+                         xlist, hdr + kHdrExd);
+  } else {
+    hipLaunchKernelGGL((k_move<PosT, kListPool>), move_grid(max_movers), dim3(256), 0, s, R, S, src, M, list, count,
+                       species, barrier, po, prop, plist, hdr + kHdrProposals, owner, tile_owner, err, ulist,
+                       hdr + kHdrUnimolDue, xlist, hdr + kHdrExd);
   }
-  hipLaunchKernelGGL(k_move<PosT>, move_grid(max_movers), dim3(256), 0, s, R, S, snapshot_src ? Sm : M, M, list,
-                     count, species, barrier, propose_only ? 1u : 0u, prop, plist, hdr + kHdrProposals, owner,
-                     tile_owner, err, ulist, hdr + kHdrUnimolDue);
+  // the deferred steps, with the large pool
+  hipLaunchKernelGGL((k_move<PosT, exd::kPool>), move_grid(max_movers), dim3(256), 0, s, R, S, src, M, xlist,
+                     hdr + kHdrExd, species, barrier, po, prop, plist, hdr + kHdrProposals, owner, tile_owner, err,
+                     ulist, hdr + kHdrUnimolDue, nullptr, nullptr);
   return hipGetLastError();
 }
 

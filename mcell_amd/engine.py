@@ -24,7 +24,8 @@ def _ptr(a: Optional[np.ndarray]):
 class DiffuseReactEngine:
     def __init__(self, part: PartitionConfig, precision: str = "fp64", seed: int = 1, num_replicas: int = 1,
                  capacity: int = 1 << 20, trace: bool = False, barrier_interval: int = 1,
-                 propose_only: bool = False, start_iteration: int = 0, barriers: Optional[Sequence[int]] = None):
+                 propose_only: bool = False, start_iteration: int = 0, barriers: Optional[Sequence[int]] = None,
+                 drain: str = "auto"):
         """barrier_interval / barriers: the scheduler's barriers (count
         events): every barrier_interval iterations, or at the iterations
         listed in barriers.  step() hands the engine the time up to the next
@@ -46,7 +47,8 @@ class DiffuseReactEngine:
         cfg.num_replicas = num_replicas
         cfg.seed = seed
         cfg.molecule_capacity = capacity
-        cfg.flags = (_lib.MCG_FLAG_TRACE_HITS if trace else 0) | (_lib.MCG_FLAG_PROPOSE_ONLY if propose_only else 0)
+        cfg.flags = ((_lib.MCG_FLAG_TRACE_HITS if trace else 0) | (_lib.MCG_FLAG_PROPOSE_ONLY if propose_only else 0) |
+                     {"auto": 0, "inline": _lib.MCG_FLAG_DRAIN_INLINE, "defer": _lib.MCG_FLAG_DRAIN_DEFER}[drain])
         cfg.start_iteration = start_iteration
         self.barrier_interval = barrier_interval
         self.barriers = None if barriers is None else sorted(int(b) for b in barriers)
@@ -58,14 +60,14 @@ class DiffuseReactEngine:
     @classmethod
     def from_workload(cls, wl, precision: str = "fp64", seed: Optional[int] = None, capacity: Optional[int] = None,
                       propose_only: bool = False, log_capacity: int = 0, trace: bool = False,
-                      add_molecules: bool = True, start_iteration: int = 0, barriers=None):
+                      add_molecules: bool = True, start_iteration: int = 0, barriers=None, drain: str = "auto"):
         """An engine holding the workload's geometry, species, reaction
         classes and (with add_molecules) molecules of every replica."""
         from .workloads import rxn_table, species_steps, unimol_table, volsurf_table
         cap = capacity if capacity is not None else max(2 * wl.num_molecules, 1024)
         eng = cls(wl.part, precision=precision, seed=wl.config.seed if seed is None else seed,
                   num_replicas=wl.num_replicas, capacity=cap, trace=trace, barrier_interval=wl.barrier_interval,
-                  propose_only=propose_only, start_iteration=start_iteration, barriers=barriers)
+                  propose_only=propose_only, start_iteration=start_iteration, barriers=barriers, drain=drain)
         eng.set_geometry(wl.vertices, wl.walls, wl.wall_class)
         eng.set_species(wl.species_steps)
         if wl.surface_classes is not None:
@@ -280,6 +282,7 @@ class DiffuseReactEngine:
                       "transparent_crossings", "settle_rounds"):
                 setattr(total, f, getattr(total, f) + getattr(st, f))
             total.slow_path_last = st.slow_path_last
+            total.drain_form = st.drain_form
             it += chunk
         return total
 

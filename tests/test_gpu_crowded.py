@@ -26,13 +26,14 @@ def _small(**kw):
     return W.crowded_spheres(**args)
 
 
-def test_crowded_snapshot_proposals_match_oracle(gpu_available, oracle_lib):
+@pytest.mark.parametrize("drain", ["inline", "defer"])
+def test_crowded_snapshot_proposals_match_oracle(gpu_available, oracle_lib, drain):
     from oracle import pyoracle as po
     seed = 13
     wl = _small(iterations=5)
     wl = dataclasses.replace(wl, unimol_rxns=[], rxns=[M.BimolReaction(0, 1, 1e9, [2])],
                              volsurf_rxns=[M.VolSurfReaction(0, 3, 1e9, 4, product_orient=1, geom_vol=0, geom_surf=1)])
-    eng = DiffuseReactEngine.from_workload(wl, seed=seed, propose_only=True, log_capacity=1 << 16)
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed, propose_only=True, log_capacity=1 << 16, drain=drain)
     w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
     w.set_snapshot(True)
     w.set_barrier_interval(wl.barrier_interval)
@@ -121,3 +122,36 @@ def test_crowded_invariants(gpu_available, precision):
     assert c[4] > 0 and st.reactions > 100
     assert eng.num_molecules() == c[0] + c[1] + c[2]
     eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_drain_forms_give_the_same_run(gpu_available, precision):
+    """The drain's two forms (exact_disk in place with a small vertex pool, or
+    every step whose disk meets walls deferred to a pass of its own) and the
+    engine's timed choice between them give the same run, bit for bit:
+    molecules, surface tiles and counts after 12 full iterations."""
+    wl = _small(n_molecules=300000, iterations=12)
+    runs = {}
+    for drain in ("inline", "defer", "auto"):
+        eng = DiffuseReactEngine.from_workload(wl, precision=precision, seed=6, drain=drain)
+        st = eng.step(0, wl.iterations)
+        if drain != "auto":
+            assert st.drain_form == (1 if drain == "inline" else 2)
+        m = eng.molecules()
+        o = np.argsort(m["id"], kind="stable")
+        sm = eng.surface_molecules()
+        runs[drain] = (m["id"][o], m["species"][o], m["pos"][o], {k: np.asarray(v) for k, v in sm.items()},
+                       eng.count_species(1, 5), st.reactions, st.wall_hits)
+        print("%s %s: %d reactions, %d reflections, form %d" % (precision, drain, st.reactions, st.wall_hits,
+                                                                st.drain_form), flush=True)
+        eng.close()
+    a = runs["inline"]
+    assert a[5] > 100
+    for other in ("defer", "auto"):
+        b = runs[other]
+        for k in (0, 1, 2, 4):
+            assert np.array_equal(a[k], b[k]), (other, k)
+        assert a[3].keys() == b[3].keys()
+        for k in a[3]:
+            assert np.array_equal(a[3][k], b[3][k]), (other, k)
+        assert a[5:] == b[5:]
