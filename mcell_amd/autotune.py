@@ -21,8 +21,10 @@ from .engine import DiffuseReactEngine
 DEFAULT_EDGES_UM = (0.05, 0.1, 0.2)
 
 
-def measure(wl: W.Workload, precision: str, warmup: int = 3, steps: int = 5) -> float:
-    """Molecule-timesteps/s of `steps` iterations after `warmup` ones."""
+def measure(wl: W.Workload, precision: str, warmup: int = 9, steps: int = 5) -> float:
+    """Molecule-timesteps/s of `steps` iterations after `warmup` ones (the
+    warm-up covers the engine's timed choice of its drain form, 8
+    iterations)."""
     eng = DiffuseReactEngine.from_workload(wl, precision=precision,
                                            capacity=wl.num_molecules * (2 if wl.rxns else 1))
     try:
@@ -39,7 +41,7 @@ def measure(wl: W.Workload, precision: str, warmup: int = 3, steps: int = 5) -> 
 
 
 def choose_subpartition(wl: W.Workload, precision: str = "fp64", edges_um: Sequence[float] = DEFAULT_EDGES_UM,
-                        warmup: int = 3, steps: int = 5, log=None) -> Tuple[float, Dict[float, float]]:
+                        warmup: int = 9, steps: int = 5, log=None) -> Tuple[float, Dict[float, float]]:
     """Returns (best edge in um, {edge: molecule-timesteps/s})."""
     rates = {}
     for e in edges_um:

@@ -19,17 +19,25 @@ struct RxnRec {
   float max_p_f;  // rxn_utils.inl:367 holds max_p in a float
   uint32_t num_pathways;
   uint32_t reactant_a, reactant_b;
-  uint32_t num_products[kMaxPathways];
+  uint32_t num_products[kMaxPathways];   // new molecules (kept reactants excluded)
   uint32_t products[kMaxPathways][kMaxProducts];
+  uint32_t keep[kMaxPathways];           // kKeepA | kKeepB: reactants that stay
 };
+// A reactant is kept when its species is among the pathway's products
+// (outcome_products_random, diffuse_react_event.cpp:2551-2565, 2686-2727):
+// the first product of that species is the reactant itself, not a new
+// molecule.  Reactant a takes the first match; for a == b, b is kept only
+// when the species appears twice.
+constexpr uint32_t kKeepA = 1u, kKeepB = 2u;
 
 // The unimolecular class of one species: cumulative pathway rates per
 // internal time unit (k_tot = cum[num_pathways - 1]) and products.
 struct UnimolRec {
   double cum[kMaxPathways];
   uint32_t num_pathways;  // 0: the species has no unimolecular reaction
-  uint32_t num_products[kMaxPathways];
+  uint32_t num_products[kMaxPathways];  // new molecules (a kept reactant excluded)
   uint32_t products[kMaxPathways][kMaxProducts];
+  uint32_t keep[kMaxPathways];          // 1: the reactant stays and diffuses on
 };
 constexpr uint32_t kUnimolBit = 0x80000000u;  // Proposal::rxn of a unimolecular event: bit | species
 
@@ -100,7 +108,7 @@ struct Snapshot {
   const PosT* __restrict__ z;
   const uint32_t* __restrict__ id;
   const uint32_t* __restrict__ tag;
-  const uint32_t* __restrict__ consumed;  // nonzero: taken by an accepted reaction this iteration
+  const uint32_t* __restrict__ consumed;  // nonzero: taken by an accepted reaction this iteration (its mover's id + 1)
   const double* __restrict__ toff;        // diffusion-time offsets of the snapshot's molecules
   const double* __restrict__ utime;       // unimolecular reaction times (absolute) or kTimeInvalid
 };

@@ -40,6 +40,10 @@ struct ReactParams {
   const uint32_t* __restrict__ tile_id;
   const uint32_t* __restrict__ tile_consumed;
   uint32_t total_tiles;
+  // per output slot: reactions already carried out in this step by a mover
+  // that stayed (its class keeps it); the mover's repeated step passes
+  // through that many successful tests
+  const uint32_t* __restrict__ kept;
   const VolSurfRec* __restrict__ vs_rxns;
   const int32_t* __restrict__ pair_vs;   // [vol species * ns + surf species], -1 = none
   unsigned long long* ctr;               // CTR_* statistics of the call
@@ -80,8 +84,10 @@ struct Proposal {
   uint32_t mover_slot, partner_slot;
   uint32_t mover_id, partner_id;
   uint32_t rxn, pathway;
-  uint32_t replica, pad;
+  uint32_t replica;
+  uint32_t pad;   // walls reflected off (16 bits) | crossed (15 bits) << 16 | kPadMoverIsB
 };
+constexpr uint32_t kPadMoverIsB = 1u << 31;  // the mover is the class's reactant b
 
 // Closest wall in one subpartition (FP64; the entry-box pre-test first).
 __device__ __forceinline__ bool closest_wall(const ReactParams& R, uint32_t s, uint32_t last_hit, D3 pos, D3& disp,
@@ -155,7 +161,11 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
           if (!collide_mol64(pos, disp, D3{(double)gx[u], (double)gy[u], (double)gz[u]}, sigma2, tt, cp)) continue;
           if (!(tt < t_limit)) continue;
           const uint32_t pid = S.id[j];
-          if (pid == my_id || S.consumed[j]) continue;
+          // consumed partners are gone, except those consumed by this
+          // mover's own earlier reactions of the step (it stayed and
+          // repeats the step up to there: consumed holds consumer id + 1)
+          const uint32_t cj = S.consumed[j];
+          if (pid == my_id || (cj != 0 && cj != my_id + 1)) continue;
           // strictly after the last processed key
           if (!first && !(tt > t_last || (tt == t_last && pid < id_last))) continue;
           ++candidates;
@@ -225,19 +235,30 @@ __device__ __forceinline__ double start_unimol_time(const UnimolRec* __restrict_
 // step the same way; exd::kPool: the last pass, where running out is an
 // error.  The pool is private memory per lane, so the small forms keep the
 // scratch of the kernels that run them small.
+//
+// The molecule's stream is pass n of its n-th step in the iteration (the
+// tag's pass bits; MF_SKIP_DRAW: the pathway draw of a unimolecular
+// reaction that kept it was the pass's first draw).  pass_through: the
+// step's successful bimolecular tests already carried out (the mover
+// stayed); they are passed as the reference's ray trace goes on after them
+// (collide_and_react_with_vol_mol returns false for a kept mover,
+// diffuse_react_event.cpp:463-470, 823-827).
 template <typename PosT, int kPoolN>
 __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot<PosT>& S, const SpeciesRec& sp,
                                             const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
-                                            double utime, double barrier, Proposal& prop, MoverEnd& out,
-                                            uint32_t& err, uint32_t* set_lds) {
+                                            double utime, double barrier, uint32_t pass_through, Proposal& prop,
+                                            MoverEnd& out, uint32_t& err, uint32_t* set_lds) {
   // The full step tests its candidate walls in FP64 in both forms: after the
   // entry-box pre-test a mover here meets few walls, and the FP32 filter in
   // front of them cost more than it saved.  In the FP32 form the end point
   // is then always rounded to FP32 on its FP64 side of nearby wall planes.
   constexpr bool kF32 = sizeof(PosT) == 4;
   const uint32_t rep = tag_replica(tag), spc = tag_species(tag);
+  const uint32_t pass = (tag_flags(tag) & MF_PASS_MASK) >> MF_PASS_SHIFT;
   PhiloxStream rng;
-  rng.init(R.seed, idv, R.iteration, rep, 0);
+  rng.init(R.seed, idv, R.iteration, rep, pass);
+  if (tag_flags(tag) & MF_SKIP_DRAW) (void)rng.u32();
+  uint32_t succ = 0;
   double elapsed = (double)R.iteration + toff;
   out.near_plane = kF32;
   out.absorbed = false;
@@ -249,7 +270,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   out.pos = pos;
   out.utime = start_unimol_time(R.unimol, spc, tag, utime, elapsed, rng);
   if (out.utime != kTimeInvalid && !(tag_flags(tag) & MF_SCHEDULE_UNIMOL) && out.utime <= elapsed) {
-    out.due = 1;  // due at the start: react_unimol_single_molecule before any move
+    out.due = pass + 1;  // due at the start: react_unimol_single_molecule before any move
     return false;
   }
   // get_max_time (diffuse_react_event.cpp:164-198)
@@ -350,8 +371,11 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
           if (candidates == 1) break;
           continue;
         }
-        const int pw = test_bimolecular(R.rxns[rep * R.rxn_stride + rx], factor * r_rate_factor, rng);
-        if (pw >= 0) {
+        const RxnRec& rr = R.rxns[rep * R.rxn_stride + rx];
+        const int pw = test_bimolecular(rr, factor * r_rate_factor, rng);
+        if (pw >= 0 && succ < pass_through) {
+          ++succ;  // carried out already: the mover stayed, its step goes on
+        } else if (pw >= 0 && S.consumed[slot] == 0) {  // (else: consumed by this mover's earlier reaction)
           prop.time = elapsed + t_steps * tc;
           prop.x = cp.x;
           prop.y = cp.y;
@@ -362,6 +386,9 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
           prop.rxn = (uint32_t)rx;
           prop.pathway = (uint32_t)pw;
           prop.replica = rep;
+          // outcome_products_random's reactant order (diffuse_react_event.cpp:
+          // 2528-2547): the mover is reactant b unless both match in order
+          prop.pad = (spc == rr.reactant_a && tag_species(S.tag[slot]) == rr.reactant_b) ? 0u : kPadMoverIsB;
           // ray_trace_vol commits the move before the collisions are
           // processed when the segment hit no wall (diffuse_react_event.cpp:773-777)
           out.pos = found ? pos : D3{pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
@@ -407,6 +434,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
               prop.rxn = kVolSurfBit | (uint32_t)ri;
               prop.pathway = (uint32_t)pw;
               prop.replica = rep;
+              prop.pad = 0;
               out.pos = pos;
               return true;
             }
@@ -447,7 +475,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   out.pos = pos;
   // the reaction is due within this iteration: the reference diffuses the
   // molecule again right away and reacts (diffuse_react_event.cpp:298-306)
-  if (out.utime != kTimeInvalid && out.utime < (double)R.iteration + 1.0) out.due = 2;
+  if (out.utime != kTimeInvalid && out.utime < (double)R.iteration + 1.0) out.due = pass + 2;
   return false;
 }
 

@@ -78,6 +78,14 @@ enum : uint32_t {
   MF_DEFUNCT = 1u << 0,
   MF_SCHEDULE_UNIMOL = 1u << 1,  // lifetime not drawn yet (MOLECULE_FLAG_SCHEDULE_UNIMOL_RXN)
   MF_TOFF = 1u << 2,             // reactive path: the diffusion-time offset array holds a non-zero value
+  // reactive path, within one iteration only: the stream pass of the
+  // molecule's next step (its n-th diffusion in the iteration reads pass n)
+  // and whether that pass's first draw is already spent (the unimolecular
+  // pathway draw of a reactant that stays)
+  MF_PASS_SHIFT = 3,
+  MF_PASS_MASK = 7u << 3,
+  MF_SKIP_DRAW = 1u << 6,
+  MF_STEP_FLAGS = MF_SCHEDULE_UNIMOL | MF_PASS_MASK | MF_SKIP_DRAW,  // cleared when a step ends
 };
 constexpr double kTimeInvalid = -256.0;   // TIME_INVALID (defines.h:144)
 constexpr double kTimeForever = 1e20;     // TIME_FOREVER (defines.h:145)
@@ -113,6 +121,7 @@ enum : uint32_t {
   ERR_QUEUE = 1u << 8,       // proposal / product capacity exceeded
   ERR_ROUND32 = 1u << 9,     // FP32 form: a point could not be rounded onto its side of every nearby plane
   ERR_COLLISIONS = 1u << 10, // more molecule collisions in one ray segment than the step's guard allows
+  ERR_PASSES = 1u << 11,     // a molecule that stays after its reactions diffused more often in one iteration than its tag counts
 };
 
 // per-call statistics counters (SimulationStats::mol_wall_reflections and

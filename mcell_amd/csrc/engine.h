@@ -68,6 +68,11 @@ struct ReactState {
   DevBuf<double> stoff;
   DevBuf<uint32_t> key, rank, consumed, owner;
   DevBuf<Proposal> prop;
+  // classes that keep a reactant: per-slot counts of the reactions a mover
+  // stayed through, and copies of the accepted proposals (allocated with
+  // the first such class)
+  DevBuf<uint32_t> kept;
+  DevBuf<Proposal> aprop;
   DevBuf<uint32_t> plist, alist, alist_sorted, rlist0, rlist1, mlist, ulist;
   DevBuf<uint64_t> keys_a, keys_b;
   DevBuf<uint32_t> cnt, pre, head, next_id;

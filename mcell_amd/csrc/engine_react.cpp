@@ -93,6 +93,9 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.consumed = r.consumed.p;
   p.owner = r.owner.p;
   p.prop = r.prop.p;
+  p.kept = r.kept.p;
+  p.aprop = r.aprop.p;
+  p.R.kept = r.kept.p;
   p.plist = r.plist.p;
   p.alist = r.alist.p;
   p.alist_sorted = r.alist_sorted.p;
@@ -215,7 +218,7 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
         reactions += na;
         const uint64_t log_base = std::min<uint64_t>(r.log_count, 0xFFFFFFFFull);
         HIP_TRY(p.products(c->stream, na, (uint32_t)cap, r.log.p, (uint32_t)log_base, (uint32_t)r.log_cap,
-                           err));
+                           snapshot_src, err));
         r.log_count += na;
         HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
         st = read_hdr(c);
@@ -389,6 +392,52 @@ int count_species_into(mcg_ctx* c, uint64_t* counts, uint64_t* dcounts, bool wit
 
 extern "C" {
 
+namespace {
+
+// Splits a pathway's products into the reactants it keeps and the new
+// molecules (outcome_products_random, diffuse_react_event.cpp:2551-2565,
+// 2686-2727): reactant a is kept when its species is a product and takes
+// the first such product; reactant b likewise takes the next product of its
+// species (for a == b, b stays only when the species appears twice).
+// Returns kKeepA | kKeepB; b = kInvalid for a unimolecular class.
+uint32_t split_products(uint32_t a, uint32_t b, const uint32_t* prods, uint32_t n, uint32_t* out, uint32_t& n_out) {
+  bool keep_a = false, keep_b = false;
+  uint32_t n_a = 0;
+  for (uint32_t q = 0; q < n; ++q) {
+    keep_a |= prods[q] == a;
+    keep_b |= prods[q] == b;
+    n_a += prods[q] == a;
+  }
+  if (a == b) keep_b = n_a > 1;
+  bool used_a = false, used_b = false;
+  n_out = 0;
+  for (uint32_t q = 0; q < n; ++q) {
+    const uint32_t ps = prods[q];
+    if (keep_a && !used_a && ps == a) {
+      used_a = true;
+      continue;
+    }
+    if (keep_b && !used_b && ps == b) {
+      used_b = true;
+      continue;
+    }
+    out[n_out++] = ps;
+  }
+  return (keep_a ? kKeepA : 0u) | (keep_b ? kKeepB : 0u);
+}
+
+// The per-slot buffers of classes that keep a reactant, allocated once.
+int ensure_keep_buffers(mcg_ctx* c) {
+  ReactState& r = c->rx;
+  if (r.kept.p) return MCG_OK;
+  const size_t cap = c->cfg.molecule_capacity;
+  if (r.kept.alloc(cap) || r.aprop.alloc(cap)) return fail(MCG_E_NOMEM, "buffers of classes that keep a reactant");
+  HIP_TRY(hipMemset(r.kept.p, 0, cap * sizeof(uint32_t)));
+  return MCG_OK;
+}
+
+}  // namespace
+
 int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   if (!c || (n && !rxns)) return fail(MCG_E_INVALID, "null argument");
   MCG_ALIVE(c);
@@ -401,6 +450,7 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   std::vector<RxnRec> h(n);
   std::vector<int32_t> pair((size_t)ns * ns, -1);
   std::vector<SpeciesRec> sp = c->species_host;
+  bool any_keep = false;
   for (uint32_t i = 0; i < n; ++i) {
     const mcg_bimol_rxn& x = rxns[i];
     if (x.reactant_a >= ns || x.reactant_b >= ns) return fail(MCG_E_INVALID, "reactant species out of range");
@@ -414,14 +464,11 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
       last = x.cum_probs[k];
       r.cum_probs[k] = x.cum_probs[k];
       if (x.num_products[k] > (uint32_t)kMaxProducts) return fail(MCG_E_INVALID, "at most 3 products per pathway");
-      r.num_products[k] = x.num_products[k];
-      for (uint32_t q = 0; q < x.num_products[k]; ++q) {
-        const uint32_t ps = x.products[k][q];
-        if (ps >= ns) return fail(MCG_E_INVALID, "product species out of range");
-        if (ps == x.reactant_a || ps == x.reactant_b)
-          return fail(MCG_E_UNSUPPORTED, "reaction classes that keep a reactant are not on this path yet");
-        r.products[k][q] = ps;
-      }
+      for (uint32_t q = 0; q < x.num_products[k]; ++q)
+        if (x.products[k][q] >= ns) return fail(MCG_E_INVALID, "product species out of range");
+      r.keep[k] = split_products(x.reactant_a, x.reactant_b, x.products[k], x.num_products[k], r.products[k],
+                                 r.num_products[k]);
+      any_keep |= r.keep[k] != 0;
     }
     r.num_pathways = x.num_pathways;
     r.max_fixed_p = r.cum_probs[x.num_pathways - 1];
@@ -438,6 +485,7 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
   ReactState& r = c->rx;
   int st = ensure_react_buffers(c);
   if (st) return st;
+  if (any_keep && (st = ensure_keep_buffers(c))) return st;
   if (r.rxns_d.alloc(n)) return fail(MCG_E_NOMEM, "reaction buffers");
   HIP_TRY(hipMemcpy(r.rxns_d.p, h.data(), n * sizeof(RxnRec), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(r.pair_d.p, pair.data(), pair.size() * 4, hipMemcpyHostToDevice));
@@ -457,6 +505,7 @@ int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
   const uint32_t ns = c->num_species;
   std::vector<UnimolRec> u(ns);
   std::memset(u.data(), 0, ns * sizeof(UnimolRec));
+  bool any_keep = false;
   for (uint32_t i = 0; i < n; ++i) {
     const mcg_unimol_rxn& x = rxns[i];
     if (x.reactant >= ns) return fail(MCG_E_INVALID, "reactant species out of range");
@@ -467,14 +516,12 @@ int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
       if (!(x.cum_rates[k] >= last)) return fail(MCG_E_INVALID, "cumulative rates must not decrease");
       last = x.cum_rates[k];
       if (x.num_products[k] > (uint32_t)kMaxProducts) return fail(MCG_E_INVALID, "at most 3 products per pathway");
-      for (uint32_t q = 0; q < x.num_products[k]; ++q) {
+      for (uint32_t q = 0; q < x.num_products[k]; ++q)
         if (x.products[k][q] >= ns) return fail(MCG_E_INVALID, "product species out of range");
-        if (x.products[k][q] == x.reactant)
-          return fail(MCG_E_UNSUPPORTED, "unimolecular classes that keep the reactant are not on this path yet");
-        u[x.reactant].products[k][q] = x.products[k][q];
-      }
+      u[x.reactant].keep[k] = split_products(x.reactant, kInvalid, x.products[k], x.num_products[k],
+                                             u[x.reactant].products[k], u[x.reactant].num_products[k]) & kKeepA;
+      any_keep |= u[x.reactant].keep[k] != 0;
       u[x.reactant].cum[k] = x.cum_rates[k];
-      u[x.reactant].num_products[k] = x.num_products[k];
     }
     u[x.reactant].num_pathways = x.num_pathways;
   }
@@ -483,6 +530,7 @@ int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
       return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
   int st = ensure_react_buffers(c);
   if (st) return st;
+  if (any_keep && (st = ensure_keep_buffers(c))) return st;
   c->rx.unimol = u;
   HIP_TRY(hipMemcpy(c->rx.unimol_d.p, u.data(), ns * sizeof(UnimolRec), hipMemcpyHostToDevice));
   return MCG_OK;
@@ -509,9 +557,12 @@ int mcg_set_replica_params(mcg_ctx* c, uint32_t replica, const mcg_species* spec
       return fail(MCG_E_INVALID, "per-replica classes must keep the base table's reactants and pathways");
     double last = 0;
     for (uint32_t k = 0; k < q.num_pathways; ++k) {
-      if (x.num_products[k] != q.num_products[k]) return fail(MCG_E_INVALID, "per-replica products differ");
-      for (uint32_t m = 0; m < q.num_products[k]; ++m)
-        if (x.products[k][m] != q.products[k][m]) return fail(MCG_E_INVALID, "per-replica products differ");
+      if (x.num_products[k] > (uint32_t)kMaxProducts) return fail(MCG_E_INVALID, "per-replica products differ");
+      uint32_t xp[kMaxProducts], nxp = 0;
+      const uint32_t xkeep = split_products(x.reactant_a, x.reactant_b, x.products[k], x.num_products[k], xp, nxp);
+      if (nxp != q.num_products[k] || xkeep != q.keep[k]) return fail(MCG_E_INVALID, "per-replica products differ");
+      for (uint32_t m = 0; m < nxp; ++m)
+        if (xp[m] != q.products[k][m]) return fail(MCG_E_INVALID, "per-replica products differ");
       if (!(x.cum_probs[k] >= last)) return fail(MCG_E_INVALID, "cumulative probabilities must not decrease");
       last = x.cum_probs[k];
       q.cum_probs[k] = x.cum_probs[k];

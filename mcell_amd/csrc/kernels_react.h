@@ -59,6 +59,8 @@ struct ReactPipeline {
   RMolView<PosT> Sm;  // cell-sorted snapshot
   uint32_t *key, *rank, *cell_count, *cell_start;
   uint32_t *consumed, *owner;
+  uint32_t* kept;   // per slot: reactions its mover stayed through this iteration (null: no class keeps a reactant)
+  Proposal* aprop;  // accepted proposals (null: read in place from prop)
   uint32_t *tiles, *tile_id, *tile_owner, *tile_consumed;  // surface molecules per (replica, tile)
   Proposal* prop;
   uint32_t *plist, *alist, *alist_sorted, *rlist0, *rlist1, *mlist, *ulist;
@@ -77,7 +79,7 @@ struct ReactPipeline {
   hipError_t settle(hipStream_t s, uint32_t max_props, uint32_t* rlist, uint32_t* rcount);
   hipError_t fire_unimol(hipStream_t s, uint32_t max_due);
   hipError_t products(hipStream_t s, uint32_t n_acc, uint32_t capacity, RxnEvent* log, uint32_t log_base,
-                      uint32_t log_cap, uint32_t* err);
+                      uint32_t log_cap, bool first_phase, uint32_t* err);
   hipError_t log_proposals(hipStream_t s, uint32_t max_props, uint32_t iteration, RxnEvent* log, uint32_t* log_count,
                            uint32_t log_cap);
   hipError_t count_species(hipStream_t s, uint32_t n, uint32_t num_species, uint32_t num_replicas, uint64_t* counts);

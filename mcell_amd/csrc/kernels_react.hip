@@ -254,7 +254,10 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   Proposal p;
   bool proposed = false;
   const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
-  if (moves) proposed = react_mover<PosT, kPoolN>(R, S, sp, T, pos, idv, tag, toff, utime, barrier, p, e, err, set_lds);
+  const uint32_t pass_through = R.kept ? R.kept[j] : 0u;
+  if (moves)
+    proposed = react_mover<PosT, kPoolN>(R, S, sp, T, pos, idv, tag, toff, utime, barrier, pass_through, p, e, err,
+                                         set_lds);
   deferred = e.exd;
   if (deferred) {
     due = false;
@@ -280,7 +283,8 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   }
   if (proposed) {
     p.mover_slot = j;
-    p.pad = (e.n_reflect > 0xFFFFu ? 0xFFFFu : e.n_reflect) | ((e.n_cross > 0xFFFFu ? 0xFFFFu : e.n_cross) << 16);
+    p.pad = (p.pad & kPadMoverIsB) | (e.n_reflect > 0xFFFFu ? 0xFFFFu : e.n_reflect) |
+            ((e.n_cross > 0x7FFFu ? 0x7FFFu : e.n_cross) << 16);
     prop[j] = p;
     if (!propose_only) {
       atomicMin(owner + j, idv);
@@ -291,7 +295,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   if (!proposed || propose_only) {
     err |= store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
     out.id[j] = idv;
-    uint32_t t2 = tag & ~(MF_SCHEDULE_UNIMOL << 8);
+    uint32_t t2 = tag & ~(MF_STEP_FLAGS << 8);
     t2 = store_toff(out.toff, j, sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff, t2);
     out.tag[j] = e.absorbed ? (t2 | (MF_DEFUNCT << 8)) : t2;
     if (out.utime) out.utime[j] = e.utime;
@@ -574,7 +578,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
         // an end point next to a wall plane keeps its side in FP32
         err |= store_pos(R, out.x, out.y, out.z, j, end, near_plane);
         out.id[j] = idv;
-        out.tag[j] = store_toff(out.toff, j, next_toff(R.iteration, toff, t_steps), tag & ~(MF_SCHEDULE_UNIMOL << 8));
+        out.tag[j] = store_toff(out.toff, j, next_toff(R.iteration, toff, t_steps), tag & ~(MF_STEP_FLAGS << 8));
         if (out.utime) out.utime[j] = utime;
       } else {
         hard = true;
@@ -583,7 +587,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
     } else {
       store_pos(R, out.x, out.y, out.z, j, pos, false);
       out.id[j] = idv;
-      out.tag[j] = store_toff(out.toff, j, toff, tag & ~(MF_SCHEDULE_UNIMOL << 8));
+      out.tag[j] = store_toff(out.toff, j, toff, tag & ~(MF_STEP_FLAGS << 8));
       if (out.utime) out.utime[j] = utime;
     }
   }
@@ -597,6 +601,30 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
   wave_append(hard && partner_hard, j, Q.slots + (size_t)s1 * Q.shard_cap, Q.count + s1);
 }
 
+// Appends slot j (or, with aprop, a copy of prop[j]) to the accepted list.
+// Movers that stay after a reaction propose again in the same phase, so
+// with such classes the accepted proposals are copied out of prop.
+__device__ __forceinline__ void append_accepted(bool acc, uint32_t j, const Proposal* __restrict__ prop,
+                                                Proposal* __restrict__ aprop, uint32_t* __restrict__ alist,
+                                                uint32_t* __restrict__ acount) {
+  const uint64_t mask = __ballot(acc);
+  if (!mask) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(acount, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  if (acc) {
+    const uint32_t a = base + __popcll(mask & ((1ull << lane) - 1));
+    if (aprop) {
+      aprop[a] = prop[j];
+      alist[a] = a;
+    } else {
+      alist[a] = j;
+    }
+  }
+}
+
 // A proposal whose mover and partner both carry its claim is accepted.
 template <typename PosT>
 __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ plist, const uint32_t* __restrict__ pcount,
@@ -604,7 +632,9 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
                                                 const uint32_t* __restrict__ tile_owner,
                                                 uint32_t* __restrict__ consumed, uint32_t* __restrict__ tile_consumed,
                                                 uint32_t* __restrict__ out_tag,
-                                                const RxnRec* __restrict__ rxns, uint32_t* __restrict__ alist,
+                                                const RxnRec* __restrict__ rxns, uint32_t rxn_stride,
+                                                uint32_t* __restrict__ kept, Proposal* __restrict__ aprop,
+                                                uint32_t* __restrict__ alist,
                                                 uint32_t* __restrict__ acount, unsigned long long* __restrict__ ctr) {
   const uint32_t n = *pcount;
   uint32_t n_reflect = 0, n_cross = 0;
@@ -619,30 +649,35 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
     const uint32_t ps = p.partner_slot & ~kTileBit;
     if (owner[j] == p.mover_id && (tile ? tile_owner[ps] : owner[ps]) == p.mover_id) {
       acc = true;
-      n_reflect += p.pad & 0xFFFFu;  // the walls the mover met before the reaction
-      n_cross += p.pad >> 16;
-      // outcome_bimolecular (diffuse_react_event.cpp:1833-1895): both
-      // reactants are consumed (reactions keeping a reactant are refused at
-      // set-up); a surface reactant's tile goes to its product
-      consumed[j] = 1;
-      out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
+      // outcome_bimolecular (diffuse_react_event.cpp:1833-1895): the
+      // reactants are consumed unless the pathway keeps them (bit 0: the
+      // mover, bit 1: the partner, from the class's a / b bits); a surface
+      // reactant's tile goes to its product.  consumed holds the consuming
+      // mover's id + 1.
+      uint32_t keep = 0;
+      if (!tile) {
+        const uint32_t k = rxns[p.replica * rxn_stride + p.rxn].keep[p.pathway];
+        keep = (p.pad & kPadMoverIsB) ? ((k >> 1) & 1u) | ((k & 1u) << 1) : k;
+      }
+      if (keep & 1u) {
+        // the mover stays: its step is repeated through this reaction and
+        // goes on (the repeat counts its walls)
+        ++kept[j];
+      } else {
+        n_reflect += p.pad & 0xFFFFu;  // the walls the mover met before the reaction
+        n_cross += (p.pad >> 16) & 0x7FFFu;
+        consumed[j] = p.mover_id + 1;
+        out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
+      }
       if (tile) {
         tile_consumed[ps] = 1;
-      } else {
-        consumed[ps] = 1;
+      } else if (!(keep & 2u)) {
+        consumed[ps] = p.mover_id + 1;
         out_tag[ps] = out_tag[ps] | (MF_DEFUNCT << 8);
       }
     }
   }
-  const uint64_t mask = __ballot(acc);
-  if (mask) {
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(acount, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (acc) alist[base + __popcll(mask & ((1ull << lane) - 1))] = j;
-  }
+  append_accepted(acc, j, prop, aprop, alist, acount);
   }
   wave_count(ctr + CTR_REFLECTIONS, n_reflect);
   wave_count(ctr + CTR_CROSSINGS, n_cross);
@@ -653,11 +688,16 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
 // due molecule that no accepted bimolecular reaction consumed reacts at its
 // scheduled time; the pathway is drawn from the molecule's stream when the
 // class has several (which_unimolecular, rxn_utils.inl:772-783).
+// A pathway that keeps the reactant leaves it alive: the products phase
+// lets it diffuse on from its reaction time in the stream pass of the
+// pathway draw (outcome_unimolecular, diffuse_react_event.cpp:2990-3003, and
+// the rest of diffuse_single_molecule); pad records that pass (bits 0-7)
+// and whether the draw was made (bit 8).
 __global__ __launch_bounds__(256) void k_unimol_fire(ReactParams R, const uint32_t* __restrict__ ulist,
                                                      const uint32_t* __restrict__ ucount,
                                                      Proposal* __restrict__ prop, uint32_t* __restrict__ consumed,
-                                                     uint32_t* __restrict__ out_tag, uint32_t* __restrict__ alist,
-                                                     uint32_t* __restrict__ acount) {
+                                                     uint32_t* __restrict__ out_tag, Proposal* __restrict__ aprop,
+                                                     uint32_t* __restrict__ alist, uint32_t* __restrict__ acount) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   bool fire = false;
   uint32_t j = 0;
@@ -673,13 +713,16 @@ __global__ __launch_bounds__(256) void k_unimol_fire(ReactParams R, const uint32
         const double match = rng.dbl() * u.cum[u.num_pathways - 1];
         pw = (uint32_t)binary_search_double(u.cum, match, (int)u.num_pathways - 1, 1);
       }
+      p.pad = p.pathway | (u.num_pathways > 1 ? 0x100u : 0u);
       p.pathway = pw;
-      consumed[j] = 1;
-      out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
+      if (!u.keep[pw]) {
+        consumed[j] = p.mover_id + 1;
+        out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
+      }
       fire = true;
     }
   }
-  append_slot(fire, j, alist, acount);
+  append_accepted(fire, j, prop, aprop, alist, acount);
 }
 
 // Releases the claims of every proposal of the round and queues the movers
@@ -716,12 +759,31 @@ __global__ __launch_bounds__(256) void k_requeue(const uint32_t* __restrict__ pl
 }
 
 // ---------------------------------------------------------------- products --
+// The order in which the reference would carry the phase's reactions out,
+// which numbers their products: per replica, by the diffusion call they
+// happen in (pass: a unimolecular event due after the molecule's first move
+// happens in its second call, which the reference queues behind every
+// first call), then the movers in the order diffuse_molecules takes them
+// (diffuse_react_event.cpp:68-124) -- in the iteration's first phase those
+// starting at the iteration's start, then those starting later (late = 1),
+// each by id; in later phases new products before reactants that stayed --
+// and a mover's reactions in the order they were accepted (the sort is
+// stable).  key = replica << 36 | pass << 33 | late << 32 | mover id.
+__device__ __forceinline__ uint32_t key_replica(uint64_t key) { return (uint32_t)(key >> 36); }
+
 __global__ __launch_bounds__(256) void k_acc_keys(const uint32_t* __restrict__ alist, uint32_t n,
-                                                  const Proposal* __restrict__ prop, uint64_t* __restrict__ keys) {
+                                                  const Proposal* __restrict__ prop, const uint32_t* __restrict__ stag,
+                                                  const double* __restrict__ stoff, const uint32_t* __restrict__ hdr,
+                                                  uint32_t first_phase, uint64_t* __restrict__ keys) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const Proposal& p = prop[alist[k]];
-  keys[k] = ((uint64_t)p.replica << 32) | p.mover_id;
+  const uint32_t j = p.mover_slot;
+  const bool snap = j < hdr[kHdrSnapshot];
+  const bool late = first_phase ? (snap && load_toff(stoff, stag[j], j) != 0.0) : snap;
+  // a fired unimolecular event keeps its draw's pass in pad (k_unimol_fire)
+  const uint32_t pass = (first_phase && (p.rxn & kUnimolBit)) ? min(p.pad & 0xFFu, 7u) : 0u;
+  keys[k] = ((uint64_t)p.replica << 36) | ((uint64_t)pass << 33) | ((uint64_t)late << 32) | p.mover_id;
 }
 
 __device__ __forceinline__ uint32_t new_products(const RxnRec& r, uint32_t pathway) {
@@ -749,8 +811,8 @@ __global__ __launch_bounds__(256) void k_seg_heads(const uint64_t* __restrict__ 
                                                    uint32_t* __restrict__ head) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const uint32_t rep = (uint32_t)(keys[k] >> 32);
-  if (k == 0 || (uint32_t)(keys[k - 1] >> 32) != rep) head[rep] = k;
+  const uint32_t rep = key_replica(keys[k]);
+  if (k == 0 || key_replica(keys[k - 1]) != rep) head[rep] = k;
 }
 
 // Products of accepted reaction k (in (replica, mover id) order) get ids
@@ -768,7 +830,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
                                                   uint32_t* __restrict__ mcount, RxnEvent* __restrict__ log,
                                                   uint32_t log_base, uint32_t log_cap, uint32_t* __restrict__ err,
                                                   uint32_t* __restrict__ tiles, uint32_t* __restrict__ tile_id,
-                                                  uint32_t* __restrict__ tile_consumed) {
+                                                  uint32_t* __restrict__ tile_consumed, uint32_t* __restrict__ kept) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const Proposal& p = prop[sorted_slots[k]];
@@ -776,7 +838,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
     // the surface product takes the reactant's tile (outcome_products_random
     // recycles it, diffuse_react_event.cpp:2145-2155); its output slot stays a hole
     const VolSurfRec& vs = R.vs_rxns[p.rxn & ~kVolSurfBit];
-    const uint32_t rep = (uint32_t)(keys[k] >> 32);
+    const uint32_t rep = key_replica(keys[k]);
     const uint32_t id = next_id[rep] + pre[k] - pre[head[rep]];
     const uint32_t ts = p.partner_slot & ~kTileBit;
     tiles[ts] = make_tile(vs.product[p.pathway], vs.product_orient[p.pathway]);
@@ -812,7 +874,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
                               : R.rxns[p.replica * R.rxn_stride + p.rxn].products[p.pathway];
   const uint32_t np = uni ? R.unimol[p.rxn & ~kUnimolBit].num_products[p.pathway]
                           : R.rxns[p.replica * R.rxn_stride + p.rxn].num_products[p.pathway];
-  const uint32_t rep = (uint32_t)(keys[k] >> 32);
+  const uint32_t rep = key_replica(keys[k]);
   const uint32_t base = hdr[kHdrNumOut];
   const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
   const bool moves_now = cmp_lt_d(p.time, (double)R.iteration + 1.0, 1e-12);
@@ -829,7 +891,23 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
     if (out.utime) out.utime[slot] = kTimeInvalid;
     consumed[slot] = 0;
     owner[slot] = kInvalid;
+    if (kept) kept[slot] = 0;
     if (moves_now) mlist[atomicAdd(mcount, 1u)] = slot;
+  }
+  if (uni && R.unimol[p.rxn & ~kUnimolBit].keep[p.pathway] && moves_now) {
+    // the reactant stays and diffuses on from its reaction time, drawing a
+    // new lifetime first, in the stream pass of the pathway draw after it
+    const uint32_t j = p.mover_slot;
+    const uint32_t pass = p.pad & 0xFFu;
+    if (pass > (MF_PASS_MASK >> MF_PASS_SHIFT)) {
+      atomicOr(err, ERR_PASSES);
+    } else {
+      const uint32_t f = MF_SCHEDULE_UNIMOL | (pass << MF_PASS_SHIFT) | ((p.pad & 0x100u) ? MF_SKIP_DRAW : 0u);
+      const uint32_t t = (out.tag[j] & ~(MF_STEP_FLAGS << 8)) | (f << 8);
+      out.tag[j] = store_toff(out.toff, j, p.time - (double)R.iteration, t);
+      if (out.utime) out.utime[j] = kTimeInvalid;
+      mlist[atomicAdd(mcount, 1u)] = j;
+    }
   }
   if (log && log_base + k < log_cap) {
     RxnEvent& e = log[log_base + k];
@@ -855,8 +933,8 @@ __global__ void k_bump_ids(const uint64_t* __restrict__ keys, uint32_t n, const 
                            uint32_t* __restrict__ hdr, uint32_t capacity) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const uint32_t rep = (uint32_t)(keys[k] >> 32);
-  if (k == n - 1 || (uint32_t)(keys[k + 1] >> 32) != rep) next_id[rep] += pre[k + 1] - pre[head[rep]];
+  const uint32_t rep = key_replica(keys[k]);
+  if (k == n - 1 || key_replica(keys[k + 1]) != rep) next_id[rep] += pre[k + 1] - pre[head[rep]];
   if (k == n - 1) {
     const uint64_t used = (uint64_t)hdr[kHdrNumOut] + pre[n];
     hdr[kHdrNumOut] = used < capacity ? (uint32_t)used : capacity;
@@ -963,6 +1041,7 @@ hipError_t ReactPipeline<PosT>::sort(hipStream_t s, uint32_t n_in, uint32_t num_
     hipLaunchKernelGGL(k_scatter<PosT>, blocks_for(n_in), dim3(256), 0, s, M, n_in, key, rank, cell_start, Sm);
     // claim words of the snapshot's slots (n_in bounds it), written in order
     hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, consumed, 0u, n_in);
+    if (kept) hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, kept, 0u, n_in);
     hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, owner, kInvalid, n_in);
   }
   return hipGetLastError();
@@ -1034,7 +1113,8 @@ hipError_t ReactPipeline<PosT>::settle(hipStream_t s, uint32_t max_props, uint32
   // grid-stride over the device's proposal count; max_props bounds it
   const dim3 grid = dim3(std::min<uint32_t>(blocks_for(max_props).x, 1024u));
   hipLaunchKernelGGL(k_accept<PosT>, grid, dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
-                     tile_owner, consumed, tile_consumed, M.tag, R.rxns, alist, hdr + kHdrAccepted, R.ctr);
+                     tile_owner, consumed, tile_consumed, M.tag, R.rxns, R.rxn_stride, kept, aprop, alist,
+                     hdr + kHdrAccepted, R.ctr);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, rcount, 0u, 1u);
   hipLaunchKernelGGL(k_requeue, grid, dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
                      tile_owner, consumed, rlist, rcount);
@@ -1045,28 +1125,32 @@ template <typename PosT>
 hipError_t ReactPipeline<PosT>::fire_unimol(hipStream_t s, uint32_t max_due) {
   if (max_due == 0) return hipSuccess;
   hipLaunchKernelGGL(k_unimol_fire, blocks_for(max_due), dim3(256), 0, s, R, ulist, hdr + kHdrUnimolDue, prop,
-                     consumed, M.tag, alist, hdr + kHdrAccepted);
+                     consumed, M.tag, aprop, alist, hdr + kHdrAccepted);
   return hipGetLastError();
 }
 
 template <typename PosT>
 hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t capacity, RxnEvent* log,
-                                         uint32_t log_base, uint32_t log_cap, uint32_t* err) {
+                                         uint32_t log_base, uint32_t log_cap, bool first_phase, uint32_t* err) {
   if (n_acc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_acc_keys, blocks_for(n_acc), dim3(256), 0, s, alist, n_acc, prop, keys_a);
+  // the accepted proposals: copies when movers can stay, else in place
+  const Proposal* const P = aprop ? aprop : prop;
+  hipLaunchKernelGGL(k_acc_keys, blocks_for(n_acc), dim3(256), 0, s, alist, n_acc, P, Sm.tag, Sm.toff, hdr,
+                     first_phase ? 1u : 0u, keys_a);
   size_t tmp = scan_tmp_bytes;
   hipError_t e =
       hipcub::DeviceRadixSort::SortPairs(scan_tmp, tmp, keys_a, keys_b, alist, alist_sorted, n_acc, 0, 64, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_prod_counts, blocks_for(n_acc + 1), dim3(256), 0, s, alist_sorted, n_acc, prop, R.rxns, R.rxn_stride, R.unimol, cnt);
+  hipLaunchKernelGGL(k_prod_counts, blocks_for(n_acc + 1), dim3(256), 0, s, alist_sorted, n_acc, P, R.rxns,
+                     R.rxn_stride, R.unimol, cnt);
   tmp = scan_tmp_bytes;
   e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt, pre, n_acc + 1, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_seg_heads, blocks_for(n_acc), dim3(256), 0, s, keys_b, n_acc, head);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrProductMovers, 0u, 1u);
-  hipLaunchKernelGGL(k_products<PosT>, blocks_for(n_acc), dim3(256), 0, s, R, alist_sorted, keys_b, n_acc, prop, pre,
+  hipLaunchKernelGGL(k_products<PosT>, blocks_for(n_acc), dim3(256), 0, s, R, alist_sorted, keys_b, n_acc, P, pre,
                      head, next_id, hdr, capacity, M, consumed, owner, mlist, hdr + kHdrProductMovers, log, log_base,
-                     log_cap, err, tiles, tile_id, tile_consumed);
+                     log_cap, err, tiles, tile_id, tile_consumed, kept);
   hipLaunchKernelGGL(k_bump_ids, blocks_for(n_acc), dim3(256), 0, s, keys_b, n_acc, pre, head, next_id, hdr,
                      capacity);
   return hipGetLastError();

@@ -150,17 +150,19 @@ def free_diffusion_replicas(num_replicas: int, mols_per_replica: int, iterations
 
 def bimol_box(n_a: int = 50000, n_b: int = 50000, iterations: int = 2000, seed: int = 2,
               num_replicas: int = 1, rate: float = 1e8, subpartition_dimension: float = 0.05,
-              barrier_interval: int = 10) -> Workload:
-    """A + B -> C in the 1 um reflective cube: A and B with D = 1e-6 cm^2/s,
-    C with D = 5e-7, uniform placement, counts every ``barrier_interval``
-    iterations.  Replica r is placed with seed ``seed + r`` (replica 0 is the
+              barrier_interval: int = 10, products=(2,), d_a: float = 1e-6, d_b: float = 1e-6) -> Workload:
+    """A + B -> C in the 1 um reflective cube: A and B with D = 1e-6 cm^2/s
+    (d_a, d_b), C with D = 5e-7, uniform placement, counts every
+    ``barrier_interval`` iterations.  ``products`` replaces C (a product of a
+    reactant's species keeps that reactant, e.g. (0, 2): A + B -> A + C).
+    Replica r is placed with seed ``seed + r`` (replica 0 is the
     single-instance workload); ids are 0..n_a-1 for A, then B."""
     cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
     box = M.create_box("box", edge_dimension=1.0)
     part = M.derive_partition(cfg, M.bounding_box([box]), has_bimol_vol_rxns=True)
     verts, walls, classes = M.vertices_internal([box], part)
-    species = [M.Species("A", 1e-6), M.Species("B", 1e-6), M.Species("C", 5e-7)]
-    rxns = [M.BimolReaction(0, 1, rate, [2])]
+    species = [M.Species("A", d_a), M.Species("B", d_b), M.Species("C", 5e-7)]
+    rxns = [M.BimolReaction(0, 1, rate, list(products))]
     n = n_a + n_b
     pos = np.concatenate([M.place_uniform_in_box(n, 0.5, seed + r, part) for r in range(num_replicas)])
     sp1 = np.concatenate([np.zeros(n_a, np.uint32), np.ones(n_b, np.uint32)])
@@ -437,10 +439,12 @@ def crowded_spheres(n_molecules: int = 10_000_000, n_surface: int = 200_000, ite
 
 
 def decay_box(n_e: int = 20000, iterations: int = 2000, seed: int = 6, rate: float = 1e3,
-              d_e: float = 2e-7, back_rate: float = 0.0) -> Workload:
+              d_e: float = 2e-7, back_rate: float = 0.0, products=(0, 1), more_pathways=()) -> Workload:
     """E -> A + B at `rate` /s in the reflective 1 um cube (E with D = d_e
     cm^2/s, A and B with 1e-6), optionally A + B -> E at `back_rate` /M/s;
-    E uniform (seed), counts every 10 iterations."""
+    E uniform (seed), counts every 10 iterations.  ``products`` replaces
+    A + B (E among them keeps E: E -> E + A); ``more_pathways`` adds
+    (rate, products) pathways of E."""
     cfg = M.Config(time_step=1e-6, subpartition_dimension=0.05, seed=seed)
     box = M.create_box("box", edge_dimension=1.0)
     rxns = [M.BimolReaction(0, 1, back_rate, [2])] if back_rate > 0 else []
@@ -451,4 +455,5 @@ def decay_box(n_e: int = 20000, iterations: int = 2000, seed: int = 6, rate: flo
     return Workload("decay_box", part, verts, walls, classes, species, _species_table(species, part, rxns),
                     np.arange(n_e, dtype=np.uint32), np.full(n_e, 2, np.uint32), pos, np.zeros(n_e, np.uint32),
                     iterations, rxns=rxns, config=cfg, objects=[box], barrier_interval=10,
-                    unimol_rxns=[M.UnimolReaction(2, rate, [0, 1])])
+                    unimol_rxns=[M.UnimolReaction(2, rate, list(products))] +
+                    [M.UnimolReaction(2, r, list(pr)) for r, pr in more_pathways])

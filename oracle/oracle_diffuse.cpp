@@ -88,7 +88,8 @@ void World::pick_unimol_time(Mol& m, double current_time) {
 // react_unimol_single_molecule + outcome_unimolecular
 // (diffuse_react_event.cpp:1760-1826, 2939-3003): the pathway (a draw only with
 // several), products at the molecule's position, created at the scheduled
-// time, then the molecule is destroyed.  Returns whether it survived.
+// time, then the molecule is destroyed unless the pathway keeps it.  Returns
+// whether it survived.
 bool World::react_unimol(uint32_t id) {
   Mol& m = mols[id];
   double t = m.unimol_rxn_time;
@@ -102,13 +103,26 @@ bool World::react_unimol(uint32_t id) {
   V3 pos = m.pos;
   const uint32_t sp = m.species;  // `m` dangles once products are added
   std::vector<uint32_t> prods = u.products[pw];
+  // outcome_unimolecular + outcome_products_random (diffuse_react_event.cpp:
+  // 2686-2727, 2939-3003): a product of the reactant's species is the
+  // reactant itself, which stays and diffuses on with a new lifetime
+  const bool keep = std::find(prods.begin(), prods.end(), sp) != prods.end();
+  bool used = false;
   for (uint32_t ps : prods) {
+    if (keep && !used && ps == sp) {
+      used = true;
+      continue;
+    }
     uint32_t nid = add_volume_molecule(ps, pos, t);
     if (cmp_lt(t, event_time + 1.0, EPS)) new_diffuse_actions.push_back({nid});
   }
   ++unimol_reactions;
   rxn_log.push_back({(uint32_t)iteration, id, INVALID_U32, (int32_t)(rxns.size() + sp), pw, t, pos});
   Mol& mm = mols[id];
+  if (keep) {
+    mm.flags |= MF_SCHEDULE_UNIMOL;
+    return true;
+  }
   mm.flags |= MF_DEFUNCT;
   register_reactant(mm, false);
   return false;
@@ -203,7 +217,8 @@ void World::diffuse_single_molecule(uint32_t id) {
     pick_unimol_time(mols[id], diffusion_start_time);
   }
   double max_time = get_max_time(mols[id]);
-  if (species[m0.species].is_vol) diffuse_vol_molecule(id, max_time, diffusion_start_time);
+  // (m0 dangles once a reaction that kept the molecule added products)
+  if (species[mols[id].species].is_vol) diffuse_vol_molecule(id, max_time, diffusion_start_time);
   Mol& m = mols[id];
   if (m.flags & MF_DEFUNCT) return;
   const Species& sp = species[m.species];

@@ -197,3 +197,67 @@ def test_unimolecular_decay_is_exponential(oracle_lib):
         assert abs(c[2] - n0 * p) < 4 * sd, (k, c, n0 * p)
         assert c[0] == c[1] == n0 - c[2] == w.unimol_reactions()
     w.close()
+
+
+def test_unimolecular_class_that_keeps_its_reactant(oracle_lib):
+    """E -> E + A at k1 = 1e3 /s beside E -> B at k2 = 500 /s (the reactant
+    stays in the first pathway: outcome_unimolecular,
+    diffuse_react_event.cpp:2990-3003).  Each E is a Poisson source of A
+    while it lives and dies at rate k2, so E(t) ~ Binomial(E0, exp(-k2 t)),
+    B = E0 - E, and the expected A count is E0 k1 (1 - exp(-k2 t)) / k2 with
+    variance E0 k1 / k2^2 (...) bounded by its Poisson-mixture form."""
+    from oracle import pyoracle as po
+    n0, k1, k2 = 4000, 1e3, 500.0
+    wl = W.decay_box(n_e=n0, iterations=400, products=(2, 0), rate=k1, more_pathways=[(k2, (1,))])
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_ISAAC, seed=5)
+    w.set_barrier_interval(wl.barrier_interval)
+    for k in range(1, 3):
+        w.run(200)
+        t = 1e-6 * 200 * k
+        m = w.molecules()
+        live = (m["flags"] & 1) == 0
+        c = np.bincount(m["species"][live], minlength=3)
+        p = math.exp(-k2 * t)
+        sd_e = math.sqrt(n0 * p * (1 - p))
+        assert abs(c[2] - n0 * p) < 4 * sd_e, (k, c)
+        assert c[1] == n0 - c[2]
+        # A per E: Poisson(k1 * lifetime within t), lifetime ~ min(Exp(k2), t)
+        mean_life = (1 - p) / k2
+        e_life2 = 2 * (1 - p - k2 * t * p) / k2 ** 2 + t * t * p  # E[min(L, t)^2]
+        mean_a = n0 * k1 * mean_life
+        var_a = n0 * (k1 * mean_life + k1 * k1 * (e_life2 - mean_life ** 2))
+        print("t=%g: E %d (%.0f), A %d (%.0f +- %.0f), B %d" % (t, c[2], n0 * p, c[0], mean_a, math.sqrt(var_a),
+                                                                 c[1]))
+        assert abs(c[0] - mean_a) < 4 * math.sqrt(var_a)
+        assert w.unimol_reactions() == c[0] + c[1]
+    w.close()
+
+
+@pytest.mark.parametrize("keep", ["a", "b"])
+def test_catalysis_keeps_the_catalyst_and_is_pseudo_first_order(oracle_lib, keep):
+    """A + B -> A + C (keep a) or A + B -> B + C (keep b) in the 1 um cube:
+    the kept species' count never changes and the other decays as
+    exp(-k [kept] t), [kept] = N / (N_A V) (k = 1e8 /M/s, the mean-field law
+    the unkept A + B -> C ensemble follows)."""
+    from oracle import pyoracle as po
+    n_cat, n_sub = 2000, 2000
+    products = (0, 2) if keep == "a" else (1, 2)
+    wl = W.bimol_box(n_a=n_cat if keep == "a" else n_sub, n_b=n_sub if keep == "a" else n_cat, iterations=500,
+                     products=products, barrier_interval=100)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_ISAAC, seed=9)
+    w.set_barrier_interval(wl.barrier_interval)
+    conc = n_cat / (6.02214076e23 * 1e-15)  # molar, 1 um^3 = 1e-15 l
+    cat, sub = (0, 1) if keep == "a" else (1, 0)
+    for k in range(1, 3):
+        w.run(250)
+        m = w.molecules()
+        live = (m["flags"] & 1) == 0
+        c = np.bincount(m["species"][live], minlength=3)
+        p = math.exp(-1e8 * conc * 1e-6 * 250 * k)
+        sd = math.sqrt(n_sub * p * (1 - p))
+        print("keep %s t=%d: catalyst %d, substrate %d (%.0f +- %.0f), C %d"
+              % (keep, 250 * k, c[cat], c[sub], n_sub * p, sd, c[2]))
+        assert c[cat] == n_cat
+        assert c[sub] + c[2] == n_sub
+        assert abs(c[sub] - n_sub * p) < 4 * sd + 0.01 * n_sub
+    w.close()
