@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 4
+#define MCG_ABI_VERSION 5
 
 typedef struct mcg_ctx mcg_ctx;
 
@@ -215,23 +215,37 @@ int mcg_set_replica_params(mcg_ctx* ctx, uint32_t replica, const mcg_species* sp
  * of every kind are set before the first mcg_step (MCG_E_STATE after it). */
 int mcg_set_unimol_rxns(mcg_ctx* ctx, const mcg_unimol_rxn* rxns, uint32_t num_rxns);
 
-/* A volume + surface reaction class: the volume reactant hits a wall tile
- * holding the surface reactant (collide_and_react_with_surf_mol,
- * diffuse_react_event.cpp:845-975).  Pathway k replaces the surface reactant
- * on its tile by product[k] with orientation product_orient[k] and destroys
- * the volume reactant.  cum_probs are cumulative reaction probabilities per
- * hit (rate x pb_factor, MCell3 src/react_util.c:132-157) before the tile
- * binding factor; geom_vol and geom_surf are the orientations in the rule
- * (0 none, +1, -1), matched as trigger_bimolecular does (rxn_utils.inl:58-84).
- * Volume reactants get MCG_SPF_CAN_VOLSURF. */
+/* A volume + surface reaction class, V + S -> products in that reactant
+ * order: the volume reactant hits a wall tile holding the surface reactant
+ * (collide_and_react_with_surf_mol, diffuse_react_event.cpp:845-975).
+ * cum_probs are cumulative reaction probabilities per hit (rate x pb_factor,
+ * MCell3 src/react_util.c:132-157) before the tile binding factor; geom_vol
+ * and geom_surf are the orientations in the rule (0 none, +1, -1), matched
+ * as trigger_bimolecular does (rxn_utils.inl:58-84).  Volume reactants get
+ * MCG_SPF_CAN_VOLSURF.
+ *
+ * Pathway k has num_products[k] (0..3) products in the rule's order:
+ * species, orientation (+1, -1, or 0: none, drawn per reaction, rng_uint & 1)
+ * and bit q of surf_products[k] set for a surface species
+ * (outcome_products_random, diffuse_react_event.cpp:2446-2933).  A given
+ * orientation is flipped when the surface reactant's orientation differs
+ * from geom_surf (:2636-2658).  The volume reactant is consumed (classes
+ * that keep it are refused).  A surface product of S's species keeps S (its
+ * orientation becomes the product's); at most one other surface product,
+ * which takes S's tile (no neighbour-tile search on this path).  A volume
+ * product starts at the hit point moved off the wall to the side its
+ * orientation names (update_vol_mol_after_rxn_with_surf_mol, :2291-2320,
+ * tiny_diffuse_3D) and does not react with that tile on its first hit. */
 typedef struct mcg_volsurf_rxn {
   uint32_t vol_species, surf_species;
   int32_t geom_vol, geom_surf;
   uint32_t num_pathways;           /* 1..4 */
   uint32_t reserved;
   double cum_probs[4];
-  uint32_t product[4];             /* surface product species per pathway */
-  int32_t product_orient[4];       /* +1 or -1 */
+  uint32_t num_products[4];
+  uint32_t products[4][3];
+  int32_t product_orient[4][3];
+  uint32_t surf_products[4];
 } mcg_volsurf_rxn;
 
 int mcg_set_volsurf_rxns(mcg_ctx* ctx, const mcg_volsurf_rxn* rxns, uint32_t num_rxns);

@@ -105,8 +105,10 @@ class McgVolSurfRxn(ctypes.Structure):
         ("num_pathways", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
         ("cum_probs", ctypes.c_double * 4),
-        ("product", ctypes.c_uint32 * 4),
-        ("product_orient", ctypes.c_int32 * 4),
+        ("num_products", ctypes.c_uint32 * 4),
+        ("products", (ctypes.c_uint32 * 3) * 4),
+        ("product_orient", (ctypes.c_int32 * 3) * 4),
+        ("surf_products", ctypes.c_uint32 * 4),
     ]
 
 

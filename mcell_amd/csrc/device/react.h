@@ -51,17 +51,27 @@ struct WallGrid {
 static_assert(sizeof(WallGrid) == 64, "grid record");
 
 // A volume + surface reaction class: the volume reactant hitting the tile of
-// the surface reactant; the surface product takes the tile, the volume
-// reactant is destroyed.
+// the surface reactant.  The volume reactant is consumed; the surface
+// reactant stays (keep_surf: its product index + 1) or its tile goes to the
+// one new surface product; volume products start next to the wall.
+// Products are in the rule's order; orientation 0 is drawn per reaction.
 struct VolSurfRec {
   double max_fixed_p;
   double cum_probs[kMaxPathways];
   float max_p_f;
   uint32_t num_pathways;
   int32_t geom_vol, geom_surf;  // orientations in the rule (0 none, +1, -1)
-  uint32_t product[kMaxPathways];
-  int32_t product_orient[kMaxPathways];
+  uint32_t surf_species;
+  uint32_t num_products[kMaxPathways];
+  uint32_t products[kMaxPathways][kMaxProducts];
+  int32_t product_orient[kMaxPathways][kMaxProducts];
+  uint32_t surf_mask[kMaxPathways];  // bit q: product q is a surface species
+  uint32_t keep_surf[kMaxPathways];  // 0, or 1 + the product index that is the kept surface reactant
+  uint32_t num_new[kMaxPathways];    // new molecules (the kept reactant excluded)
 };
+// Proposal::pathway of a volume-surface reaction: pathway | drawn
+// orientation bits << 8 (bit q: product q drawn up)
+constexpr uint32_t kPathwayMask = 0xFFu;
 constexpr uint32_t kVolSurfBit = 0x40000000u;  // Proposal::rxn of a volume-surface reaction: bit | class
 constexpr uint32_t kTileBit = 0x80000000u;     // Proposal::partner_slot naming a tile: bit | tile slot
 constexpr uint32_t kTileEmpty = 0xFFFFFFFFu;

@@ -44,6 +44,10 @@ struct ReactParams {
   // that stayed (its class keeps it); the mover's repeated step passes
   // through that many successful tests
   const uint32_t* __restrict__ kept;
+  const uint32_t* __restrict__ tile_wall;  // per tile (of one replica): its wall
+  // per output slot: 1 + the tile slot a volume product was created on (it
+  // does not react with that tile on its first hit there), or 0
+  uint32_t* __restrict__ born;
   const VolSurfRec* __restrict__ vs_rxns;
   const int32_t* __restrict__ pair_vs;   // [vol species * ns + surf species], -1 = none
   unsigned long long* ctr;               // CTR_* statistics of the call
@@ -77,6 +81,59 @@ __device__ __forceinline__ uint32_t xyz2grid(const WallRec64& w, const WallGrid&
 // Molecule collisions processed in one ray segment before the step reports
 // ERR_COLLISIONS (a runaway guard: a segment meets a few partners at most).
 constexpr int kMaxCollisions = 4096;
+
+// The first wall the segment pos + disp meets (ray_trace_vol's wall search,
+// diffuse_react_event.cpp:713-734): the start subpartition's walls, then
+// those of the subpartitions the segment crosses.  `skip` is the wall the
+// segment starts on.
+__device__ __forceinline__ bool segment_first_wall(const ReactParams& R, D3 pos, D3& disp, uint32_t skip,
+                                                   PhiloxStream& rng, WallHit& hit, uint32_t& err) {
+  const D3 end = {pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
+  const D3 part_disp = in_partition(R.P, end) ? disp : displacement_up_to_partition_boundary(R.P, pos, disp);
+  int sx, sy, sz;
+  subpart_3d(R.P, pos, sx, sy, sz);
+  if (sx < 0 || sy < 0 || sz < 0 || sx >= (int)R.P.n || sy >= (int)R.P.n || sz >= (int)R.P.n) {
+    err |= ERR_ESCAPED;
+    return false;
+  }
+  const uint32_t start = subpart_of(R.P, sx, sy, sz);
+  if (closest_wall_in_subpart64(R.P, R.walls, R.off, R.idx, R.ent_box, start, skip, pos, disp, rng, hit, err))
+    return true;
+  const D3 dst = {pos.x + part_disp.x, pos.y + part_disp.y, pos.z + part_disp.z};
+  int dx, dy, dz;
+  subpart_3d(R.P, dst, dx, dy, dz);
+  if (subpart_of(R.P, dx, dy, dz) == start) return false;
+  SubpartWalk64 walk;
+  walk.init(R.P, pos, start, part_disp);
+  while (walk.next(R.P, err))
+    if (closest_wall_in_subpart64(R.P, R.walls, R.off, R.idx, R.ent_box, walk.cur, skip, pos, disp, rng, hit, err))
+      return true;
+  return false;
+}
+
+// The stream pass of the wall search that places a volume product (its
+// edge re-draws): Philox(seed, product id, iteration, replica, kAuxPass).
+constexpr uint32_t kAuxPass = 0xFFu;
+
+// Where a volume product of a volume-surface reaction starts
+// (update_vol_mol_after_rxn_with_surf_mol, diffuse_react_event.cpp:2291-
+// 2320, with tiny_diffuse_3D, diffusion_utils.inl:551-595): the hit point p
+// on wall wi moved by 2 * 16 POS_EPS along the wall normal to the side
+// `orient` names, or half the way to the first other wall on that move.
+// rng only serves the wall search's edge re-draws.
+__device__ __forceinline__ D3 volume_product_position(const ReactParams& R, D3 p, uint32_t wi, int orient,
+                                                      PhiloxStream& rng, uint32_t& err) {
+  const WallRec64& w = R.walls[wi];
+  const double bump = orient > 0 ? 16.0 * POS_EPS : -16.0 * POS_EPS;
+  const D3 d0 = {2.0 * bump * w.nx, 2.0 * bump * w.ny, 2.0 * bump * w.nz};
+  D3 disp = d0;
+  WallHit hit;
+  if (segment_first_wall(R, p, disp, wi, rng, hit, err) && (hit.t <= 1.0 || fabs(hit.t - 1.0) < 1e-12)) {
+    const double h = hit.t * 0.5;
+    return {p.x + d0.x * h, p.y + d0.y * h, p.z + d0.z * h};
+  }
+  return {p.x + d0.x, p.y + d0.y, p.z + d0.z};
+}
 
 struct Proposal {
   double time;    // absolute time of the collision
@@ -246,8 +303,8 @@ __device__ __forceinline__ double start_unimol_time(const UnimolRec* __restrict_
 template <typename PosT, int kPoolN>
 __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot<PosT>& S, const SpeciesRec& sp,
                                             const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
-                                            double utime, double barrier, uint32_t pass_through, Proposal& prop,
-                                            MoverEnd& out, uint32_t& err, uint32_t* set_lds) {
+                                            double utime, double barrier, uint32_t pass_through, uint32_t born_tile,
+                                            Proposal& prop, MoverEnd& out, uint32_t& err, uint32_t* set_lds) {
   // The full step tests its candidate walls in FP64 in both forms: after the
   // entry-box pre-test a mover here meets few walls, and the FP32 filter in
   // front of them cost more than it saved.  In the FP32 form the end point
@@ -409,7 +466,9 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
       const WallGrid& g = R.grids[hit.wall];
       const uint32_t slot = rep * R.total_tiles + g.tile_base + xyz2grid(R.walls[hit.wall], g, hit.pos);
       const uint32_t tw = R.tiles[slot];
-      if (tw != kTileEmpty && !R.tile_consumed[slot]) {
+      if (tw != kTileEmpty && !R.tile_consumed[slot] && born_tile == slot + 1) {
+        born_tile = 0;  // created on this tile: no rebinding on the first hit (:878-885)
+      } else if (tw != kTileEmpty && !R.tile_consumed[slot]) {
         const int32_t ri = R.pair_vs[spc * R.num_species + tile_species(tw)];
         if (ri >= 0) {
           const VolSurfRec& vs = R.vs_rxns[ri];
@@ -432,7 +491,12 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
               prop.mover_id = idv;
               prop.partner_id = R.tile_id[slot];
               prop.rxn = kVolSurfBit | (uint32_t)ri;
-              prop.pathway = (uint32_t)pw;
+              // products without an orientation in the rule draw one each,
+              // in the rule's order (outcome_products_random, :2620-2625)
+              uint32_t drawn = 0;
+              for (uint32_t q = 0; q < vs.num_products[pw]; ++q)
+                if (vs.product_orient[pw][q] == 0 && (rng.u32() & 1u)) drawn |= 1u << q;
+              prop.pathway = (uint32_t)pw | (drawn << 8);
               prop.replica = rep;
               prop.pad = 0;
               out.pos = pos;

@@ -73,6 +73,8 @@ struct ReactState {
   // the first such class)
   DevBuf<uint32_t> kept;
   DevBuf<Proposal> aprop;
+  DevBuf<uint32_t> born;       // volume products of volume-surface classes: per slot, 1 + creation tile
+  DevBuf<uint32_t> tile_wall;  // per tile of one replica: its wall
   DevBuf<uint32_t> plist, alist, alist_sorted, rlist0, rlist1, mlist, ulist;
   DevBuf<uint64_t> keys_a, keys_b;
   DevBuf<uint32_t> cnt, pre, head, next_id;

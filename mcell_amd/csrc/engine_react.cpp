@@ -96,6 +96,8 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.kept = r.kept.p;
   p.aprop = r.aprop.p;
   p.R.kept = r.kept.p;
+  p.R.born = r.born.p;
+  p.R.tile_wall = r.tile_wall.p;
   p.plist = r.plist.p;
   p.alist = r.alist.p;
   p.alist_sorted = r.alist_sorted.p;
@@ -687,6 +689,14 @@ int sync_tiles_to_device(mcg_ctx* c, bool allow_alloc) {
   if (r.grids_d.n != r.grids_host.size() && r.grids_d.alloc(r.grids_host.size()))
     return fail(MCG_E_NOMEM, "grid buffer");
   HIP_TRY(hipMemcpy(r.grids_d.p, r.grids_host.data(), r.grids_host.size() * sizeof(WallGrid), hipMemcpyHostToDevice));
+  if (r.tile_wall.n != r.total_tiles) {
+    if (!allow_alloc) return fail(MCG_E_STATE, "surface molecules on new walls need tile buffers");
+    std::vector<uint32_t> tw(r.total_tiles, kInvalid);
+    for (size_t w = 0; w < r.grids_host.size(); ++w)
+      for (uint32_t t = 0; t < r.grids_host[w].num_tiles; ++t) tw[r.grids_host[w].tile_base + t] = (uint32_t)w;
+    if (r.tile_wall.alloc(r.total_tiles)) return fail(MCG_E_NOMEM, "tile buffers");
+    if (r.total_tiles) HIP_TRY(hipMemcpy(r.tile_wall.p, tw.data(), tw.size() * 4, hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipMemcpy(r.tiles.p, r.tiles_host.data(), nt * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(r.tile_id.p, r.tile_id_host.data(), nt * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(r.tile_owner.p, 0xFF, nt * 4));
@@ -727,24 +737,53 @@ int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
     for (uint32_t k = 0; k < x.num_pathways; ++k) {
       if (!(x.cum_probs[k] >= last)) return fail(MCG_E_INVALID, "cumulative probabilities must not decrease");
       last = x.cum_probs[k];
-      if (x.product[k] >= ns || x.product[k] > 0xFFu) return fail(MCG_E_INVALID, "product species out of range");
-      if (x.product_orient[k] != 1 && x.product_orient[k] != -1)
-        return fail(MCG_E_INVALID, "surface products are oriented +1 or -1");
       v.cum_probs[k] = x.cum_probs[k];
-      v.product[k] = x.product[k];
-      v.product_orient[k] = x.product_orient[k];
+      const uint32_t np = x.num_products[k];
+      if (np > (uint32_t)kMaxProducts) return fail(MCG_E_INVALID, "at most 3 products per pathway");
+      uint32_t new_surf = 0;
+      for (uint32_t q = 0; q < np; ++q) {
+        const uint32_t ps = x.products[k][q];
+        const bool surf = (x.surf_products[k] >> q) & 1u;
+        if (ps >= ns || ps > 0xFFu) return fail(MCG_E_INVALID, "product species out of range");
+        if (!orient_ok(x.product_orient[k][q])) return fail(MCG_E_INVALID, "orientations are 0, +1 or -1");
+        v.products[k][q] = ps;
+        v.product_orient[k][q] = x.product_orient[k][q];
+        if (surf) v.surf_mask[k] |= 1u << q;
+        // the first surface product of the surface reactant's species is that reactant
+        if (surf && ps == x.surf_species && !v.keep_surf[k]) {
+          v.keep_surf[k] = q + 1;
+          continue;
+        }
+        if (!surf && ps == x.vol_species)
+          return fail(MCG_E_UNSUPPORTED, "volume-surface classes that keep the volume reactant are not on this path");
+        if (surf) ++new_surf;
+      }
+      if (new_surf > (v.keep_surf[k] ? 0u : 1u))
+        return fail(MCG_E_UNSUPPORTED, "surface products beyond the reactant's tile need a neighbour-tile search");
+      v.num_products[k] = np;
+      v.num_new[k] = np - (v.keep_surf[k] ? 1u : 0u);
     }
     v.num_pathways = x.num_pathways;
     v.max_fixed_p = v.cum_probs[x.num_pathways - 1];
     v.max_p_f = (float)v.max_fixed_p;
     v.geom_vol = x.geom_vol;
     v.geom_surf = x.geom_surf;
+    v.surf_species = x.surf_species;
     pair[x.vol_species * ns + x.surf_species] = (int32_t)i;
     sp[x.vol_species].flags |= SPF_CAN_VOLSURF;
   }
   int st = ensure_react_buffers(c);
   if (st) return st;
   if (r.vs_d.alloc(n) || r.pair_vs_d.alloc(pair.size())) return fail(MCG_E_NOMEM, "reaction buffers");
+  bool vol_products = false;
+  for (const VolSurfRec& v : h)
+    for (uint32_t k = 0; k < v.num_pathways; ++k)
+      for (uint32_t q = 0; q < v.num_products[k]; ++q) vol_products |= !((v.surf_mask[k] >> q) & 1u);
+  if (vol_products && !r.born.p) {
+    const size_t cap = c->cfg.molecule_capacity;
+    if (r.born.alloc(cap)) return fail(MCG_E_NOMEM, "reaction buffers");
+    HIP_TRY(hipMemset(r.born.p, 0, cap * 4));
+  }
   HIP_TRY(hipMemcpy(r.vs_d.p, h.data(), n * sizeof(VolSurfRec), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(r.pair_vs_d.p, pair.data(), pair.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->species.p, sp.data(), ns * sizeof(SpeciesRec), hipMemcpyHostToDevice));

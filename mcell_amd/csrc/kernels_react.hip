@@ -255,9 +255,10 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   bool proposed = false;
   const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
   const uint32_t pass_through = R.kept ? R.kept[j] : 0u;
+  const uint32_t born = R.born ? R.born[j] : 0u;
   if (moves)
-    proposed = react_mover<PosT, kPoolN>(R, S, sp, T, pos, idv, tag, toff, utime, barrier, pass_through, p, e, err,
-                                         set_lds);
+    proposed = react_mover<PosT, kPoolN>(R, S, sp, T, pos, idv, tag, toff, utime, barrier, pass_through, born, p, e,
+                                         err, set_lds);
   deferred = e.exd;
   if (deferred) {
     due = false;
@@ -299,6 +300,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
     t2 = store_toff(out.toff, j, sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff, t2);
     out.tag[j] = e.absorbed ? (t2 | (MF_DEFUNCT << 8)) : t2;
     if (out.utime) out.utime[j] = e.utime;
+    if (born) R.born[j] = 0;  // the step that could use it is over
     n_reflect += e.n_reflect;
     n_cross += e.n_cross;
   }
@@ -633,6 +635,7 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
                                                 uint32_t* __restrict__ consumed, uint32_t* __restrict__ tile_consumed,
                                                 uint32_t* __restrict__ out_tag,
                                                 const RxnRec* __restrict__ rxns, uint32_t rxn_stride,
+                                                const VolSurfRec* __restrict__ vs_rxns,
                                                 uint32_t* __restrict__ kept, Proposal* __restrict__ aprop,
                                                 uint32_t* __restrict__ alist,
                                                 uint32_t* __restrict__ acount, unsigned long long* __restrict__ ctr) {
@@ -670,7 +673,8 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
         out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
       }
       if (tile) {
-        tile_consumed[ps] = 1;
+        // the surface reactant stays when a product of its species keeps it
+        if (!vs_rxns[p.rxn & ~kVolSurfBit].keep_surf[p.pathway & kPathwayMask]) tile_consumed[ps] = 1;
       } else if (!(keep & 2u)) {
         consumed[ps] = p.mover_id + 1;
         out_tag[ps] = out_tag[ps] | (MF_DEFUNCT << 8);
@@ -794,6 +798,7 @@ __global__ __launch_bounds__(256) void k_prod_counts(const uint32_t* __restrict_
                                                      const Proposal* __restrict__ prop,
                                                      const RxnRec* __restrict__ rxns, uint32_t rxn_stride,
                                                      const UnimolRec* __restrict__ unimol,
+                                                     const VolSurfRec* __restrict__ vs_rxns,
                                                      uint32_t* __restrict__ cnt) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > n) return;
@@ -803,7 +808,7 @@ __global__ __launch_bounds__(256) void k_prod_counts(const uint32_t* __restrict_
   }
   const Proposal& p = prop[sorted_slots[k]];
   cnt[k] = (p.rxn & kUnimolBit)    ? unimol[p.rxn & ~kUnimolBit].num_products[p.pathway]
-           : (p.rxn & kVolSurfBit) ? 1u
+           : (p.rxn & kVolSurfBit) ? vs_rxns[p.rxn & ~kVolSurfBit].num_new[p.pathway & kPathwayMask]
                                    : new_products(rxns[p.replica * rxn_stride + p.rxn], p.pathway);
 }
 
@@ -835,22 +840,59 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
   if (k >= n) return;
   const Proposal& p = prop[sorted_slots[k]];
   if (p.rxn & kVolSurfBit) {
-    // the surface product takes the reactant's tile (outcome_products_random
-    // recycles it, diffuse_react_event.cpp:2145-2155); its output slot stays a hole
+    // outcome_products_random (diffuse_react_event.cpp:2446-2933) for V + S:
+    // new products in the rule's order get ids and output slots; a new
+    // surface product takes S's tile (its output slot stays a hole), a kept
+    // S takes its product's orientation, a volume product starts next to
+    // the wall and moves for the rest of the step
     const VolSurfRec& vs = R.vs_rxns[p.rxn & ~kVolSurfBit];
+    const uint32_t pw = p.pathway & kPathwayMask, drawn = p.pathway >> 8;
     const uint32_t rep = key_replica(keys[k]);
-    const uint32_t id = next_id[rep] + pre[k] - pre[head[rep]];
+    const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
     const uint32_t ts = p.partner_slot & ~kTileBit;
-    tiles[ts] = make_tile(vs.product[p.pathway], vs.product_orient[p.pathway]);
-    tile_id[ts] = id;
-    tile_consumed[ts] = 0;  // the product reacts with later movers
-    const uint32_t slot = hdr[kHdrNumOut] + pre[k];
-    if (slot < capacity) {
-      out.tag[slot] = make_tag(0, MF_DEFUNCT, rep);
-      consumed[slot] = 1;
+    const uint32_t wall = R.tile_wall[ts - rep * R.total_tiles];
+    // a given orientation flips when S is not oriented as in the rule (:2636-2658)
+    const int flip = (vs.geom_surf != 0 && tile_orient(tiles[ts]) != vs.geom_surf) ? -1 : 1;
+    const bool moves_now = cmp_lt_d(p.time, (double)R.iteration + 1.0, 1e-12);
+    uint32_t i = 0;
+    for (uint32_t q = 0; q < vs.num_products[pw]; ++q) {
+      const int o = vs.product_orient[pw][q] != 0 ? vs.product_orient[pw][q] * flip : (((drawn >> q) & 1u) ? 1 : -1);
+      const uint32_t ps = vs.products[pw][q];
+      if (vs.keep_surf[pw] == q + 1) {
+        tiles[ts] = make_tile(ps, o);
+        continue;
+      }
+      const uint32_t id = first_id + i;
+      const uint32_t slot = hdr[kHdrNumOut] + pre[k] + i;
+      ++i;
+      if (slot >= capacity) {
+        atomicOr(err, ERR_QUEUE);
+        break;
+      }
       owner[slot] = kInvalid;
-    } else {
-      atomicOr(err, ERR_QUEUE);
+      if (kept) kept[slot] = 0;
+      if ((vs.surf_mask[pw] >> q) & 1u) {
+        tiles[ts] = make_tile(ps, o);
+        tile_id[ts] = id;
+        tile_consumed[ts] = 0;  // the product reacts with later movers
+        out.tag[slot] = make_tag(0, MF_DEFUNCT, rep);
+        consumed[slot] = kInvalid;
+        if (R.born) R.born[slot] = 0;
+        continue;
+      }
+      PhiloxStream aux;  // the wall search's edge re-draws only
+      aux.init(R.seed, id, R.iteration, rep, kAuxPass);
+      uint32_t e2 = 0;
+      const D3 vp = volume_product_position(R, D3{p.x, p.y, p.z}, wall, o, aux, e2);
+      if (store_pos(R, out.x, out.y, out.z, slot, vp, true)) e2 |= ERR_ROUND32;
+      if (e2) atomicOr(err, e2);
+      out.id[slot] = id;
+      out.tag[slot] = store_toff(out.toff, slot, p.time - (double)(R.iteration + (moves_now ? 0u : 1u)),
+                                 make_tag(ps, MF_SCHEDULE_UNIMOL, rep));
+      if (out.utime) out.utime[slot] = kTimeInvalid;
+      consumed[slot] = 0;
+      if (R.born) R.born[slot] = moves_now ? ts + 1 : 0u;
+      if (moves_now) mlist[atomicAdd(mcount, 1u)] = slot;
     }
     if (log && log_base + k < log_cap) {
       RxnEvent& e = log[log_base + k];
@@ -863,9 +905,9 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
       e.mover_id = p.mover_id;
       e.partner_id = p.partner_id;
       e.rxn = p.rxn;
-      e.pathway = p.pathway;
-      e.first_product_id = id;
-      e.num_products = 1;
+      e.pathway = pw;
+      e.first_product_id = first_id;
+      e.num_products = vs.num_new[pw];
     }
     return;
   }
@@ -962,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_log_proposals(const uint32_t* __restric
   e.mover_id = p.mover_id;
   e.partner_id = p.partner_id;
   e.rxn = p.rxn;
-  e.pathway = p.pathway;
+  e.pathway = p.pathway & kPathwayMask;
   e.first_product_id = kInvalid;
   e.num_products = 0;
 }
@@ -1042,6 +1084,7 @@ hipError_t ReactPipeline<PosT>::sort(hipStream_t s, uint32_t n_in, uint32_t num_
     // claim words of the snapshot's slots (n_in bounds it), written in order
     hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, consumed, 0u, n_in);
     if (kept) hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, kept, 0u, n_in);
+    if (R.born) hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, R.born, 0u, n_in);
     hipLaunchKernelGGL(k_fill_u32, blocks_for(n_in), dim3(256), 0, s, owner, kInvalid, n_in);
   }
   return hipGetLastError();
@@ -1113,8 +1156,8 @@ hipError_t ReactPipeline<PosT>::settle(hipStream_t s, uint32_t max_props, uint32
   // grid-stride over the device's proposal count; max_props bounds it
   const dim3 grid = dim3(std::min<uint32_t>(blocks_for(max_props).x, 1024u));
   hipLaunchKernelGGL(k_accept<PosT>, grid, dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
-                     tile_owner, consumed, tile_consumed, M.tag, R.rxns, R.rxn_stride, kept, aprop, alist,
-                     hdr + kHdrAccepted, R.ctr);
+                     tile_owner, consumed, tile_consumed, M.tag, R.rxns, R.rxn_stride, R.vs_rxns, kept, aprop,
+                     alist, hdr + kHdrAccepted, R.ctr);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, rcount, 0u, 1u);
   hipLaunchKernelGGL(k_requeue, grid, dim3(256), 0, s, plist, hdr + kHdrProposals, prop, owner,
                      tile_owner, consumed, rlist, rcount);
@@ -1142,7 +1185,7 @@ hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t
       hipcub::DeviceRadixSort::SortPairs(scan_tmp, tmp, keys_a, keys_b, alist, alist_sorted, n_acc, 0, 64, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_prod_counts, blocks_for(n_acc + 1), dim3(256), 0, s, alist_sorted, n_acc, P, R.rxns,
-                     R.rxn_stride, R.unimol, cnt);
+                     R.rxn_stride, R.unimol, R.vs_rxns, cnt);
   tmp = scan_tmp_bytes;
   e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, tmp, cnt, pre, n_acc + 1, s);
   if (e != hipSuccess) return e;

@@ -171,10 +171,10 @@ class DiffuseReactEngine:
         _lib.check(self.lib.mcg_set_unimol_rxns(self.ctx, arr, len(rxns)))
 
     def set_volsurf_rxns(self, rxns):
-        """rxns: (vol, surf, geom_vol, geom_surf, cum_probs, products,
-        product orientations) per class."""
+        """rxns: (vol, surf, geom_vol, geom_surf, cum_probs, products per
+        pathway as [(species, orientation, is_surface), ...]) per class."""
         arr = (_lib.McgVolSurfRxn * max(len(rxns), 1))()
-        for i, (v, sf, gv, gs, cum, prods, orients) in enumerate(rxns):
+        for i, (v, sf, gv, gs, cum, prods) in enumerate(rxns):
             arr[i].vol_species = v
             arr[i].surf_species = sf
             arr[i].geom_vol = gv
@@ -182,8 +182,12 @@ class DiffuseReactEngine:
             arr[i].num_pathways = len(cum)
             for k, p in enumerate(cum):
                 arr[i].cum_probs[k] = p
-                arr[i].product[k] = prods[k]
-                arr[i].product_orient[k] = orients[k]
+                arr[i].num_products[k] = len(prods[k])
+                for q, (ps, o, surf) in enumerate(prods[k]):
+                    arr[i].products[k][q] = ps
+                    arr[i].product_orient[k][q] = o
+                    if surf:
+                        arr[i].surf_products[k] |= 1 << q
         _lib.check(self.lib.mcg_set_volsurf_rxns(self.ctx, arr, len(rxns)))
 
     def add_surface_molecules(self, replica: int, ids, species, walls, tiles, orients):

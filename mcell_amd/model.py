@@ -284,19 +284,28 @@ class UnimolReaction:
 
 @dataclass
 class VolSurfReaction:
-    """vol + surf -> surface product (rate in M^-1 s^-1): the volume reactant
-    hitting the tile of the surface reactant; the product takes the tile with
-    orientation product_orient and the volume reactant is destroyed.
-    geom_vol / geom_surf are the orientations written in the rule (0 none,
-    +1 up ('), -1 down (,))."""
+    """vol + surf -> products (rate in M^-1 s^-1): the volume reactant
+    hitting the tile of the surface reactant, which is consumed.  By default
+    one surface product that takes the tile with orientation product_orient;
+    ``products`` instead lists (species, orientation (+1, -1, 0 none), is
+    surface) in the rule's order -- a surface product of the surface
+    reactant's species keeps that reactant, volume products start next to
+    the wall.  geom_vol / geom_surf are the orientations written in the rule
+    (0 none, +1 up ('), -1 down (,))."""
 
     vol: int
     surf: int
     rate: float
-    product: int
+    product: int = -1
     product_orient: int = 1
     geom_vol: int = 0
     geom_surf: int = 0
+    products: Optional[List[tuple]] = None
+
+    def product_list(self) -> List[tuple]:
+        if self.products is not None:
+            return [tuple(p) for p in self.products]
+        return [(self.product, self.product_orient, True)]
 
     def pb_factor(self, part: PartitionConfig, sp_vol: Species) -> float:
         """Vol-surf pb_factor (MCell3 src/react_util.c:132-157): the volume

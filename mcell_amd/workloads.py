@@ -279,11 +279,12 @@ def unimol_table(wl: Workload):
 
 def volsurf_table(wl: Workload):
     """Volume-surface classes as the engine takes them: (vol, surf, geom_vol,
-    geom_surf, cum_probs, products, product orientations), P = rate * pb_factor."""
+    geom_surf, cum_probs, products per pathway as (species, orientation, is
+    surface) lists), P = rate * pb_factor."""
     out = []
     for v in wl.volsurf_rxns:
         pb = v.pb_factor(wl.part, wl.species[v.vol])
-        out.append((v.vol, v.surf, v.geom_vol, v.geom_surf, [v.rate * pb], [v.product], [v.product_orient]))
+        out.append((v.vol, v.surf, v.geom_vol, v.geom_surf, [v.rate * pb], [v.product_list()]))
     return out
 
 
@@ -307,7 +308,8 @@ def place_on_walls(n: int, wall_ids: np.ndarray, vertices: np.ndarray, walls: np
 
 def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, seed: int = 3,
                     subpartition_dimension: float = 0.1, sphere_subdivisions: int = 5, rate: float = 1e8,
-                    down_fraction: float = 0.5, n_t: int = 0, t_action: int = 1) -> Workload:
+                    down_fraction: float = 0.5, n_t: int = 0, t_action: int = 1, products=None,
+                    more_species=(), more_volsurf=()) -> Workload:
     """Ligand-receptor binding on a sphere: the reflective 2 um cube holding
     the icosphere of radius 0.5 um; L (D = 1e-6 cm^2/s) starts outside the
     sphere and is reflected by it, the surface receptors R sit on distinct
@@ -319,7 +321,10 @@ def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, s
     acts on it with `t_action` from both sides (1 = transparent, 2 =
     absorptive).  Species L, R, LR, T; placement seeds: seed (L), seed + 1
     (T), seed + 2 (receptor tiles), seed + 3 (orientations); ids 0..n_l-1 (L),
-    then T, then the receptors."""
+    then T, then the receptors.  ``products`` replaces LR' by a product list
+    of the class ((species, orientation, is surface), model.VolSurfReaction);
+    ``more_species`` ((name, D), ...) follow the others and ``more_volsurf``
+    adds classes."""
     cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
     box = M.create_box("box", edge_dimension=2.0)
     ball = M.create_icosphere("sphere", 0.5, sphere_subdivisions)
@@ -331,7 +336,9 @@ def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, s
     species = [M.Species("L", 1e-6), M.Species("R", 0.0), M.Species("LR", 0.0)]
     if n_t:
         species.append(M.Species("T", 1e-6))
-    rx = [M.VolSurfReaction(0, 1, rate, 2, product_orient=1, geom_vol=1, geom_surf=1)]
+    species += [M.Species(name, d) for name, d in more_species]
+    rx = [M.VolSurfReaction(0, 1, rate, 2, product_orient=1, geom_vol=1, geom_surf=1, products=products)]
+    rx += list(more_volsurf)
     sphere_walls = np.arange(len(box.wall_list), len(walls), dtype=np.uint32)
     sw, st = place_on_walls(n_r, sphere_walls, verts, walls, seed + 2)
     rng = np.random.Generator(np.random.PCG64(seed + 3))
@@ -347,7 +354,8 @@ def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, s
     surf = {"ids": np.arange(n_vol, n_vol + n_r, dtype=np.uint32), "species": np.ones(n_r, np.uint32),
             "walls": sw, "tiles": st, "orients": orients, "replica": np.zeros(n_r, np.uint32)}
     steps = _species_table(species, part, [])
-    steps[0] = (steps[0][0], steps[0][1], steps[0][2] | 4)
+    for v in rx:
+        steps[v.vol] = (steps[v.vol][0], steps[v.vol][1], steps[v.vol][2] | 4)
     if n_t:
         steps[3] = (steps[3][0], steps[3][1], steps[3][2] | 2)
     return Workload("receptor_sphere", part, verts, walls, classes, species, steps,

@@ -148,9 +148,12 @@ void orc_add_unimol_rxn(void* wp, uint32_t species, double rate, const uint32_t*
 
 uint64_t orc_unimol_reactions(void* wp) { return static_cast<World*>(wp)->unimol_reactions; }
 
-int orc_add_volsurf_rxn(void* wp, uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate,
-                        uint32_t product, int product_orient) {
-  return static_cast<World*>(wp)->add_volsurf_rxn(vol, surf, geom_vol, geom_surf, rate, product, product_orient);
+// products: n entries of species, orientation (+1, -1, 0) and surface flag
+int orc_add_volsurf_rxn(void* wp, uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate, uint32_t n,
+                        const uint32_t* species, const int32_t* orients, const uint32_t* is_surf) {
+  std::vector<VolSurfRxn::Product> prods;
+  for (uint32_t q = 0; q < n; ++q) prods.push_back({species[q], orients[q], is_surf[q] != 0});
+  return static_cast<World*>(wp)->add_volsurf_rxn(vol, surf, geom_vol, geom_surf, rate, prods);
 }
 
 // out: pb_factor, max_fixed_p

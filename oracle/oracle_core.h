@@ -226,8 +226,14 @@ struct VolSurfRxn {
   double rate, pb_factor;
   double max_fixed_p;
   std::vector<double> cum_probs;
-  std::vector<uint32_t> product;       // surface product species per pathway
-  std::vector<int> product_orient;     // its orientation per pathway
+  // the products in the rule's order: species, orientation (+1, -1, 0 =
+  // none: drawn per reaction), surface species or not
+  struct Product {
+    uint32_t species;
+    int orient;
+    bool surf;
+  };
+  std::vector<Product> products;
 };
 
 // A unimolecular reaction class of one species: cumulative pathway rates
@@ -363,8 +369,8 @@ class World {
   void finalize_walls();
   uint32_t add_species(double D_cm2s, bool is_vol, uint32_t extra_flags);
   int add_bimol_rxn(uint32_t a, uint32_t b, double rate, const std::vector<uint32_t>& products);
-  int add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate, uint32_t product,
-                      int product_orient);
+  int add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate,
+                      const std::vector<VolSurfRxn::Product>& products);
   uint32_t add_surface_molecule(uint32_t species, uint32_t wall, uint32_t tile, int orient);
   void init_grid(uint32_t wall);
   uint32_t xyz2grid(V3 v, uint32_t wall) const;
@@ -416,7 +422,13 @@ class World {
   // the mover species can react with (Partition::get_volume_molecule_reactants)
   std::vector<std::vector<std::vector<uint32_t>>> reactants;  // [species][subpart] (lazily sized)
   std::vector<uint32_t> schedulable;
-  struct DiffuseAction { uint32_t id; };
+  // where_created_this_iteration: a volume product of a volume-surface
+  // reaction does not react with its creation tile on its first hit there
+  struct DiffuseAction {
+    uint32_t id;
+    uint32_t born_wall = INVALID_U32, born_tile = INVALID_U32;
+  };
+  uint32_t born_wall = INVALID_U32, born_tile = INVALID_U32;  // of the molecule diffusing now
   std::vector<DiffuseAction> new_diffuse_actions;
   std::vector<V3> snapshot_pos;
   double event_time = 0;
@@ -435,6 +447,7 @@ class World {
   void diffuse_vol_molecule(uint32_t id, double& max_time, double diffusion_start_time);
   int ray_trace_vol(uint32_t id, bool can_vol_react, uint32_t last_hit, V3& remaining_disp,
                     std::vector<Collision>& collisions);
+  V3 volume_product_position(uint32_t id, uint32_t wall, int orient);
   void jump_away_line(V3 p, double k, V3 A, V3 B, V3 n, V3& v);
   bool collide_mol(const Mol& vm, V3 displacement, uint32_t partner, double& t, V3& pos) const;
   void reflect_from_wall(const Collision& c, Mol& vm, V3& disp, double& t_steps, uint32_t& last_hit);

@@ -167,7 +167,12 @@ void World::run_iteration() {
     else delayed.push_back(id);
   }
   for (uint32_t id : delayed) diffuse_single_molecule(id);
-  for (size_t i = 0; i < new_diffuse_actions.size(); ++i) diffuse_single_molecule(new_diffuse_actions[i].id);
+  for (size_t i = 0; i < new_diffuse_actions.size(); ++i) {
+    born_wall = new_diffuse_actions[i].born_wall;
+    born_tile = new_diffuse_actions[i].born_tile;
+    diffuse_single_molecule(new_diffuse_actions[i].id);
+    born_wall = born_tile = INVALID_U32;
+  }
   new_diffuse_actions.clear();
   if (snapshot) {
     // reactant sets follow the final positions once the iteration is over
@@ -419,6 +424,29 @@ int World::ray_trace_vol(uint32_t id, bool can_vol_react, uint32_t last_hit, V3&
     m.subpart = subpart_index(m.pos);
   }
   return res;
+}
+
+// update_vol_mol_after_rxn_with_surf_mol + tiny_diffuse_3D
+// (diffuse_react_event.cpp:2291-2320, diffusion_utils.inl:551-595): the
+// product at its creation point moves by 2 * 16 POS_EPS along the wall's
+// normal to the side `orient` names, or half the way to the first other
+// wall on that move.  In the per-molecule draw mode the wall search's edge
+// re-draws read the product's pass 255 (the reference uses its auxiliary
+// generator); the mover's own stream is left where it was.
+V3 World::volume_product_position(uint32_t id, uint32_t wall, int orient) {
+  const double bump = orient > 0 ? 16.0 * POS_EPS : -16.0 * POS_EPS;
+  const V3 disp = v3(2.0 * bump) * walls[wall].normal;
+  const V3 start = mols[id].pos;
+  V3 temp = disp;
+  std::vector<Collision> collisions;
+  const PhiloxSource saved = philox;
+  if (mode == DRAW_PHILOX) philox.reset(philox_seed, id, (uint32_t)iteration, replica, 0xFFu);
+  ray_trace_vol(id, false, wall, temp, collisions);
+  if (mode == DRAW_PHILOX) philox = saved;
+  V3 nd = disp;
+  for (const Collision& c : collisions)
+    if (c.type != CollType::VOLVOL && (c.time <= 1.0 || cmp_eq(c.time, 1.0, EPS))) nd = disp * v3(c.time * 0.5);
+  return start + nd;
 }
 
 // collision_utils.inl:48-114

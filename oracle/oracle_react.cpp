@@ -637,6 +637,12 @@ int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, d
   uint32_t j = xyz2grid(c.pos, c.wall);
   uint32_t occ = g.tile_mol[j];
   if (occ == INVALID_U32) return -1;
+  // a volume product created on this tile does not rebind on its first hit
+  // (diffuse_react_event.cpp:876-885)
+  if (born_wall == c.wall && born_tile == j) {
+    born_wall = born_tile = INVALID_U32;
+    return -1;
+  }
   const Mol& sm = mols[occ];
   const Mol& vm = mols[c.mover];
   size_t n = species.size();
@@ -658,13 +664,50 @@ int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, d
     rxn_log.push_back({(uint32_t)iteration, c.mover, occ, (int32_t)code, pw, time, c.pos});
     return 1;
   }
-  // outcome: the surface product takes the reactant's tile (outcome_products_random
-  // recycles it, diffuse_react_event.cpp:2145-2155); both reactants are destroyed
-  uint32_t wall = sm.wall, tile = sm.tile;
-  mols[occ].flags |= MF_DEFUNCT;
-  grids[wall].tile_mol[tile] = INVALID_U32;
-  add_surface_molecule(rx.product[pw], wall, tile, rx.product_orient[pw]);
-  mols.back().birthday = time;
+  // outcome_products_random (diffuse_react_event.cpp:2446-2933): products
+  // without an orientation draw one each in the rule's order (:2620-2625); a
+  // given orientation flips when the surface reactant is not oriented as in
+  // the rule (:2636-2658).  The first surface product of the reactant's
+  // species is the reactant itself (it takes the orientation); another
+  // surface product takes the reactant's tile; volume products start next
+  // to the wall (update_vol_mol_after_rxn_with_surf_mol, :2291-2320).
+  const std::vector<VolSurfRxn::Product>& prods = rx.products;
+  std::vector<int> orient(prods.size());
+  for (size_t q = 0; q < prods.size(); ++q)
+    if (prods[q].orient == 0) orient[q] = (rng().u32() & 1u) ? 1 : -1;
+  const int flip = (rx.geom_surf != 0 && sm.orient != rx.geom_surf) ? -1 : 1;
+  for (size_t q = 0; q < prods.size(); ++q)
+    if (prods[q].orient != 0) orient[q] = prods[q].orient * flip;
+  size_t keep = prods.size();
+  for (size_t q = 0; q < prods.size(); ++q)
+    if (prods[q].surf && prods[q].species == rx.surf) {
+      keep = q;
+      break;
+    }
+  const uint32_t wall = sm.wall, tile = sm.tile;
+  if (keep == prods.size()) {
+    mols[occ].flags |= MF_DEFUNCT;
+    grids[wall].tile_mol[tile] = INVALID_U32;
+  }
+  for (size_t q = 0; q < prods.size(); ++q) {
+    if (q == keep) {
+      mols[occ].orient = orient[q];
+      continue;
+    }
+    if (prods[q].surf) {
+      add_surface_molecule(prods[q].species, wall, tile, orient[q]);
+      mols.back().birthday = time;
+      continue;
+    }
+    const uint32_t nid = add_volume_molecule(prods[q].species, c.pos, time);
+    Mol& pm = mols[nid];
+    pm.pos = volume_product_position(nid, wall, orient[q]);
+    register_reactant(pm, false);
+    pm.subpart = subpart_index(pm.pos);
+    pm.reactant_subpart = pm.subpart;
+    register_reactant(pm, true);
+    if (cmp_lt(time, event_time + 1.0, EPS)) new_diffuse_actions.push_back({nid, wall, tile});
+  }
   Mol& mv = mols[c.mover];
   mv.flags |= MF_DEFUNCT;
   register_reactant(mv, false);

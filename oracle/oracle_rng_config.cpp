@@ -464,8 +464,8 @@ void World::init_grid(uint32_t wi) {
 
 // vol-surf pb_factor (MCell3 src/react_util.c:132-157): the volume reactant's
 // D and time step; doubled when both reactants are oriented the same way.
-int World::add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate, uint32_t product,
-                           int product_orient) {
+int World::add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate,
+                           const std::vector<VolSurfRxn::Product>& products) {
   VolSurfRxn r;
   r.vol = vol;
   r.surf = surf;
@@ -479,8 +479,7 @@ int World::add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_s
   r.pb_factor = pb;
   r.cum_probs.push_back(rate * pb);
   r.max_fixed_p = r.cum_probs.back();
-  r.product.push_back(product);
-  r.product_orient.push_back(product_orient);
+  r.products = products;
   vs_rxns.push_back(r);
   size_t n = species.size();
   if (pair_vs.size() != n * n) pair_vs.assign(n * n, -1);

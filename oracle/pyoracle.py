@@ -56,7 +56,7 @@ def load():
         "orc_get_rxn": (None, [P, ctypes.c_int, P]),
         "orc_add_unimol_rxn": (None, [P, u32, d, P, u32]),
         "orc_unimol_reactions": (u64, [P]),
-        "orc_add_volsurf_rxn": (ctypes.c_int, [P, u32, u32, ctypes.c_int, ctypes.c_int, d, u32, ctypes.c_int]),
+        "orc_add_volsurf_rxn": (ctypes.c_int, [P, u32, u32, ctypes.c_int, ctypes.c_int, d, u32, P, P, P]),
         "orc_get_volsurf_rxn": (None, [P, ctypes.c_int, P]),
         "orc_add_surface_molecules": (ctypes.c_int, [P, u32, u32, P, P, P]),
         "orc_grid_info": (None, [P, u32, P]),
@@ -202,8 +202,13 @@ class OracleWorld:
         p = np.ascontiguousarray(products, np.uint32)
         self.lib.orc_add_unimol_rxn(self.w, species, rate, _p(p) if len(p) else None, len(p))
 
-    def add_volsurf_rxn(self, vol, surf, geom_vol, geom_surf, rate, product, product_orient):
-        return self.lib.orc_add_volsurf_rxn(self.w, vol, surf, geom_vol, geom_surf, rate, product, product_orient)
+    def add_volsurf_rxn(self, vol, surf, geom_vol, geom_surf, rate, products):
+        """products: [(species, orientation, is_surface), ...] in the rule's order."""
+        sp = np.ascontiguousarray([p[0] for p in products] or [0], np.uint32)
+        orient = np.ascontiguousarray([p[1] for p in products] or [0], np.int32)
+        surf_ = np.ascontiguousarray([1 if p[2] else 0 for p in products] or [0], np.uint32)
+        return self.lib.orc_add_volsurf_rxn(self.w, vol, surf, geom_vol, geom_surf, rate, len(products), _p(sp),
+                                            _p(orient), _p(surf_))
 
     def volsurf_rxn(self, r):
         out = np.zeros(2)
@@ -370,7 +375,7 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
     for u in getattr(wl, "unimol_rxns", []) or []:
         w.add_unimol_rxn(u.reactant, u.rate, u.products)
     for v in getattr(wl, "volsurf_rxns", []) or []:
-        w.add_volsurf_rxn(v.vol, v.surf, v.geom_vol, v.geom_surf, v.rate, v.product, v.product_orient)
+        w.add_volsurf_rxn(v.vol, v.surf, v.geom_vol, v.geom_surf, v.rate, v.product_list())
     if getattr(wl, "surface_classes", None) is not None:
         w.set_surface_classes(wl.surface_classes)
     sel = wl.replica == replica

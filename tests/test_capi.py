@@ -31,14 +31,14 @@ def test_abi_version_and_struct_sizes(tmp_path):
     import shutil
     import subprocess
     lib = _lib.load()
-    assert lib.mcg_abi_version() == 4
+    assert lib.mcg_abi_version() == 5
     sizes = (ctypes.sizeof(_lib.McgConfig), ctypes.sizeof(_lib.McgSpecies), ctypes.sizeof(_lib.McgStepStats),
              ctypes.sizeof(_lib.McgBimolRxn), ctypes.sizeof(_lib.McgRxnEvent), ctypes.sizeof(_lib.McgUnimolRxn),
              ctypes.sizeof(_lib.McgVolSurfRxn))
     import numpy as np
     assert np.dtype(_lib.RXN_EVENT_DTYPE).itemsize == ctypes.sizeof(_lib.McgRxnEvent)
     if shutil.which("gcc") is None:
-        assert sizes == (112, 24, 64, 112, 64, 104, 88)
+        assert sizes == (112, 24, 64, 112, 64, 104, 184)
         return
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "mcg.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu", '

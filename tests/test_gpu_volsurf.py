@@ -163,3 +163,45 @@ def test_volsurf_conservation_and_orientation(gpu_available, precision):
             assert orient == start[(int(wall), int(tile))]
     assert len(s["id"]) == len(sm["ids"]) and len(np.unique(s["id"])) == len(s["id"])
     eng.close()
+
+
+@pytest.mark.parametrize("case", ["enzyme", "release"])
+def test_volume_products_match_oracle(gpu_available, oracle_lib, case):
+    """Volume products of volume-surface reactions, whole runs against the
+    oracle: 'enzyme' L' + R' -> R' + P (the receptor stays, P's orientation
+    drawn per reaction) with P + R' -> RP' (a new P does not rebind to its
+    creation tile on its first hit there); 'release' L' + R' -> LR' + P, (P
+    to the back side, LR' takes the tile).  Every volume molecule (L and P)
+    and every tile equal the oracle's each iteration."""
+    from oracle import pyoracle as po
+    from mcell_amd import model as M
+    seed = 41
+    if case == "enzyme":
+        wl = W.receptor_sphere(n_l=60000, n_r=30000, iterations=12, rate=4e8,
+                               products=[(1, 1, True), (3, 0, False)], more_species=[("P", 1e-6), ("RP", 0.0)],
+                               more_volsurf=[M.VolSurfReaction(3, 1, 4e8, 4, product_orient=1, geom_vol=0,
+                                                               geom_surf=1)])
+    else:
+        wl = W.receptor_sphere(n_l=60000, n_r=30000, iterations=12, rate=4e8,
+                               products=[(2, 1, True), (3, -1, False)], more_species=[("P", 1e-6)])
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
+    w.set_barrier_interval(wl.barrier_interval)
+    for it in range(wl.iterations):
+        eng.step(it, 1)
+        w.run(1)
+        g = eng.molecules()
+        o = np.argsort(g["id"], kind="stable")
+        om = w.molecules()
+        live = np.nonzero(((om["flags"] & 1) == 0) & ((om["species"] == 0) | (om["species"] == 3)))[0]
+        assert np.array_equal(g["id"][o], live), "iteration %d: the same volume molecules" % it
+        assert np.array_equal(g["species"][o], om["species"][live])
+        d = np.abs(g["pos"][o] - om["pos"][live]).max(initial=0)
+        assert d <= 1e-12 * 100, "iteration %d: max |dpos| %g" % (it, d)
+        assert np.array_equal(_engine_surface(eng), _oracle_surface(w, wl)), "iteration %d: surface" % it
+    n_rx = w.volsurf_reactions()
+    n_p = int(np.sum(om["species"][live] == 3))
+    print("%s: %d reactions, %d P alive" % (case, n_rx, n_p), flush=True)
+    assert n_rx > 100 and n_p > 50
+    w.close()
+    eng.close()

@@ -151,3 +151,39 @@ def test_first_iteration_follows_mean_field(oracle_lib):
     print("first-iteration reactions: %.1f per run, mean field %.1f" % (mean, expected))
     # Poisson noise plus 10% for the sphere's curvature and its 5120 facets
     assert abs(mean - expected) < 4 * sd + 0.1 * expected
+
+
+@pytest.mark.parametrize("orient", [1, -1, 0])
+def test_volume_product_starts_next_to_the_wall(oracle_lib, orient):
+    """L' + R' -> R' + P (the receptor stays, P a volume product that does
+    not move): P starts 2 * 16 POS_EPS from the hit point along the wall
+    normal, on the side its orientation names (outside the sphere for +1,
+    inside for -1, each side about half the time when drawn), R is never
+    consumed and every reaction leaves exactly one P
+    (update_vol_mol_after_rxn_with_surf_mol, diffuse_react_event.cpp:2291-
+    2320; outcome_products_random, :2620-2625)."""
+    from oracle import pyoracle as po
+    wl = W.receptor_sphere(n_l=60000, n_r=20000, iterations=6, rate=4e8, down_fraction=0.0,
+                           products=[(1, 1, True), (3, orient, False)], more_species=[("P", 0.0)])
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=7)
+    w.set_barrier_interval(wl.barrier_interval)
+    w.run(wl.iterations)
+    u, rx, d = w.rxn_log()
+    om = w.molecules()
+    p_ids = np.nonzero(om["species"] == 3)[0]
+    assert len(p_ids) == len(u) > 100
+    assert np.sum(om["species"] == 1) == 20000
+    # products are numbered in reaction order; P i belongs to reaction i
+    hit = d[:, 1:4]
+    off = om["pos"][p_ids] - hit
+    dist = np.linalg.norm(off, axis=1)
+    assert np.abs(dist - 32e-12).max() < 1e-13
+    centre = (np.asarray(wl.objects[1].vertices()).mean(axis=0)) / wl.part.length_unit
+    radial = np.einsum("ij,ij->i", off, hit - centre)
+    side = np.sign(radial)
+    if orient:
+        assert np.all(side == orient)
+    else:
+        n_out = int(np.sum(side > 0))
+        assert abs(n_out - len(side) / 2) < 4 * math.sqrt(len(side) / 4), (n_out, len(side))
+    w.close()
