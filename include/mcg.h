@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 5
+#define MCG_ABI_VERSION 6
 
 typedef struct mcg_ctx mcg_ctx;
 
@@ -303,11 +303,32 @@ int mcg_count_species_noalloc(mcg_ctx* ctx, uint64_t* counts);
  * lifetime drawn yet or no unimolecular class; default -256); flags are
  * MCG_MOLF_* (default MCG_MOLF_SCHEDULE_UNIMOL, a newly released molecule).
  * Molecules given a diffusion time other than the clock make a context
- * without reaction classes take the time-aware reactive pipeline. */
+ * without reaction classes take the time-aware reactive pipeline.
+ * counted_volumes (Molecule::v.counted_volume_index, molecule.h:76) may be
+ * NULL (0, outside every counted object); with mcg_set_counted_volumes they
+ * follow the molecules through transparent walls and into their products. */
 #define MCG_MOLF_SCHEDULE_UNIMOL 1u  /* MOLECULE_FLAG_SCHEDULE_UNIMOL_RXN: lifetime not drawn yet */
 int mcg_add_volume_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids, const uint32_t* species,
                              const double* positions, const double* diffusion_times, const double* unimol_times,
-                             const uint32_t* flags, uint64_t count);
+                             const uint32_t* flags, const uint32_t* counted_volumes, uint64_t count);
+
+/* Counted volumes (GeometryObject::counted_volume_index_inside / _outside,
+ * geometry.h:102-107): for each wall (mcg_set_geometry order) the index of
+ * the counted volume inside and outside its object, or 0xFFFFFFFF for both
+ * when the object is not a counted volume; indices < num_volumes.  A
+ * molecule crossing a transparent wall of a counted object takes the index
+ * on the other side (update_counted_volume_id_when_crossing_wall,
+ * collision_utils.inl:1637-1694; objects whose volumes intersect are not on
+ * this path); products take the volume the reaction happened in, a volume
+ * product of a volume-surface reaction on a counted wall the side its
+ * orientation names.  Set after the geometry and before the first step; a
+ * context with counted volumes runs the reactive pipeline. */
+int mcg_set_counted_volumes(mcg_ctx* ctx, uint32_t num_volumes, const uint32_t* wall_inside,
+                            const uint32_t* wall_outside);
+
+/* Live volume molecules per (replica, counted volume, species):
+ * counts[(replica * num_volumes + volume) * num_species + species]. */
+int mcg_count_species_by_volume(mcg_ctx* ctx, uint64_t* counts);
 
 /* Partition::next_molecule_id per replica (partition.h:420-470): the id the
  * replica's next product gets.  Get fills num_replicas values; set raises
@@ -351,7 +372,7 @@ int mcg_num_molecules(mcg_ctx* ctx, uint64_t* count);
  * the run bit for bit. */
 int mcg_get_volume_molecules(mcg_ctx* ctx, uint32_t* ids, uint32_t* species, uint32_t* replica, double* positions,
                              uint32_t* subparts, double* diffusion_times, double* unimol_times, uint32_t* flags,
-                             uint64_t capacity, uint64_t* count);
+                             uint32_t* counted_volumes, uint64_t capacity, uint64_t* count);
 
 /* Parity/inspection: wall constants (16 doubles per wall: normal, unit_u,
  * unit_v, distance_to_origin, uv_vert1_u, uv_vert2.u, uv_vert2.v, area, 0, 0)

@@ -39,6 +39,7 @@ EXPORTED = [
     "mcg_get_rxn_log", "mcg_count_species", "mcg_set_replica_params", "mcg_set_surface_classes",
     "mcg_set_unimol_rxns", "mcg_set_volsurf_rxns", "mcg_add_surface_molecules", "mcg_get_surface_molecules",
     "mcg_get_next_molecule_ids", "mcg_set_next_molecule_ids", "mcg_count_species_noalloc",
+    "mcg_set_counted_volumes", "mcg_count_species_by_volume",
 ]
 
 MCG_MOLF_SCHEDULE_UNIMOL = 1
@@ -173,12 +174,14 @@ def load():
     lib.mcg_destroy.argtypes = [P]
     lib.mcg_set_geometry.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_uint32, P]
     lib.mcg_set_species.argtypes = [P, ctypes.POINTER(McgSpecies), ctypes.c_uint32]
-    lib.mcg_add_volume_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, ctypes.c_uint64]
+    lib.mcg_add_volume_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, P, ctypes.c_uint64]
     lib.mcg_step.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(McgStepStats)]
     lib.mcg_sync.argtypes = [P]
     lib.mcg_num_molecules.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
-    lib.mcg_get_volume_molecules.argtypes = [P, P, P, P, P, P, P, P, P, ctypes.c_uint64,
+    lib.mcg_get_volume_molecules.argtypes = [P, P, P, P, P, P, P, P, P, P, ctypes.c_uint64,
                                              ctypes.POINTER(ctypes.c_uint64)]
+    lib.mcg_set_counted_volumes.argtypes = [P, ctypes.c_uint32, P, P]
+    lib.mcg_count_species_by_volume.argtypes = [P, P]
     lib.mcg_get_next_molecule_ids.argtypes = [P, P]
     lib.mcg_set_next_molecule_ids.argtypes = [P, P]
     lib.mcg_get_walls.argtypes = [P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]

@@ -121,6 +121,7 @@ struct Snapshot {
   const uint32_t* __restrict__ consumed;  // nonzero: taken by an accepted reaction this iteration (its mover's id + 1)
   const double* __restrict__ toff;        // diffusion-time offsets of the snapshot's molecules
   const double* __restrict__ utime;       // unimolecular reaction times (absolute) or kTimeInvalid
+  const uint32_t* __restrict__ cvol;      // counted volume indices, or null
 };
 
 // A small set of subpartitions (the reference's SubpartIndicesSet), stored

@@ -45,6 +45,10 @@ struct ReactParams {
   // through that many successful tests
   const uint32_t* __restrict__ kept;
   const uint32_t* __restrict__ tile_wall;  // per tile (of one replica): its wall
+  // counted volumes: per wall the index inside / outside its object
+  // (kInvalid: not a counted object), or null
+  const uint32_t* __restrict__ cv_in;
+  const uint32_t* __restrict__ cv_out;
   // per output slot: 1 + the tile slot a volume product was created on (it
   // does not react with that tile on its first hit there), or 0
   uint32_t* __restrict__ born;
@@ -143,6 +147,8 @@ struct Proposal {
   uint32_t rxn, pathway;
   uint32_t replica;
   uint32_t pad;   // walls reflected off (16 bits) | crossed (15 bits) << 16 | kPadMoverIsB
+  uint32_t cvol;  // the mover's counted volume at the reaction
+  uint32_t pad2;
 };
 constexpr uint32_t kPadMoverIsB = 1u << 31;  // the mover is the class's reactant b
 
@@ -251,6 +257,7 @@ struct MoverEnd {
   bool absorbed;   // destroyed by an absorptive surface class
   bool exd;        // deferred: a collision needs exact_disk (the caller repeats the step with it)
   uint32_t due;    // 0, or 1 + the stream pass that draws the unimolecular pathway
+  uint32_t cvol;   // counted volume (in: at the start; out: at the end or the reaction)
 };
 
 // exact_disk is exactly 1 unless an entry of the collision point's
@@ -446,6 +453,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
           // outcome_products_random's reactant order (diffuse_react_event.cpp:
           // 2528-2547): the mover is reactant b unless both match in order
           prop.pad = (spc == rr.reactant_a && tag_species(S.tag[slot]) == rr.reactant_b) ? 0u : kPadMoverIsB;
+          prop.cvol = out.cvol;
           // ray_trace_vol commits the move before the collisions are
           // processed when the segment hit no wall (diffuse_react_event.cpp:773-777)
           out.pos = found ? pos : D3{pos.x + disp.x, pos.y + disp.y, pos.z + disp.z};
@@ -499,6 +507,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
               prop.pathway = (uint32_t)pw | (drawn << 8);
               prop.replica = rep;
               prop.pad = 0;
+              prop.cvol = out.cvol;
               out.pos = pos;
               return true;
             }
@@ -523,7 +532,11 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
     last_hit = hit.wall;
     const double s = 1.0 - t_hit;
     if (act == SC_TRANSPARENT) {
-      // cross_transparent_wall (diffuse_react_event.cpp:3071-3094)
+      // cross_transparent_wall (diffuse_react_event.cpp:3071-3094) with
+      // update_counted_volume_id_when_crossing_wall (collision_utils.inl:
+      // 1637-1694): a front hit enters the wall's object, a back hit leaves it
+      if (R.cv_in && R.cv_in[hit.wall] != kInvalid)
+        out.cvol = hit.type == WALL_FRONT ? R.cv_in[hit.wall] : R.cv_out[hit.wall];
       if (t_steps < 1e-12) t_steps = 1e-12;
       disp = {disp.x * s, disp.y * s, disp.z * s};
       ++out.n_cross;

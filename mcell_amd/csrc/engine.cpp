@@ -235,7 +235,7 @@ int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
 
 int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, const uint32_t* species,
                              const double* pos, const double* dtimes, const double* utimes, const uint32_t* flags,
-                             uint64_t count) {
+                             const uint32_t* cvols, uint64_t count) {
   if (!c || (count && (!ids || !species || !pos))) return fail(MCG_E_INVALID, "null argument");
   MCG_ALIVE(c);
   if (replica >= c->cfg.num_replicas) return fail(MCG_E_INVALID, "replica out of range");
@@ -271,6 +271,8 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
       hut[i] = utimes[i];
       any_time |= utimes[i] != kTimeInvalid;
     }
+    if (cvols && c->rx.num_volumes && cvols[i] >= c->rx.num_volumes)
+      return fail(MCG_E_INVALID, "counted volume index out of range");
     ht[i] = make_tag(species[i], f, replica);
   }
   // offsets and drawn lifetimes live in the reactive path's arrays
@@ -285,6 +287,11 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
                            c->stream));
     HIP_TRY(hipMemcpyAsync(c->rx.utime.p + c->n, hut.data(), count * sizeof(double), hipMemcpyHostToDevice,
                            c->stream));
+    if (c->rx.num_volumes) {
+      std::vector<uint32_t> hcv(count, 0u);
+      if (cvols) std::copy(cvols, cvols + count, hcv.begin());
+      HIP_TRY(hipMemcpyAsync(c->rx.cvol.p + c->n, hcv.data(), count * 4, hipMemcpyHostToDevice, c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
   if (c->rx.next_id_host.size() != c->cfg.num_replicas) c->rx.next_id_host.assign(c->cfg.num_replicas, 0);
@@ -497,13 +504,13 @@ int mcg_num_molecules(mcg_ctx* c, uint64_t* count) {
 }
 
 int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, double* pos,
-                             uint32_t* subparts, double* dtimes, double* utimes, uint32_t* flags, uint64_t capacity,
-                             uint64_t* count) {
+                             uint32_t* subparts, double* dtimes, double* utimes, uint32_t* flags, uint32_t* cvols,
+                             uint64_t capacity, uint64_t* count) {
   if (!c) return fail(MCG_E_INVALID, "null context");
   MCG_ALIVE(c);
   const uint64_t n = c->n;
   std::vector<double> hx(n), hy(n), hz(n), htoff, hut;
-  std::vector<uint32_t> ht(n), hid(n);
+  std::vector<uint32_t> ht(n), hid(n), hcv;
   if (n && c->fp32()) {
     std::vector<float> fx(n), fy(n), fz(n);
     HIP_TRY(hipMemcpyAsync(fx.data(), c->xf.p, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -531,6 +538,10 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
       hut.resize(n);
       HIP_TRY(hipMemcpyAsync(hut.data(), c->rx.utime.p, n * 8, hipMemcpyDeviceToHost, c->stream));
     }
+    if (cvols && c->rx.num_volumes) {
+      hcv.resize(n);
+      HIP_TRY(hipMemcpyAsync(hcv.data(), c->rx.cvol.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
   uint64_t live = 0;
@@ -555,6 +566,7 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
     if (dtimes) dtimes[k] = ((f & MF_TOFF) && !htoff.empty()) ? clock + htoff[i] : clock;
     if (utimes) utimes[k] = hut.empty() ? kTimeInvalid : hut[i];
     if (flags) flags[k] = (f & MF_SCHEDULE_UNIMOL) ? MCG_MOLF_SCHEDULE_UNIMOL : 0u;
+    if (cvols) cvols[k] = hcv.empty() ? 0u : hcv[i];
     ++k;
   }
   return MCG_OK;

@@ -74,6 +74,11 @@ struct ReactState {
   DevBuf<uint32_t> kept;
   DevBuf<Proposal> aprop;
   DevBuf<uint32_t> born;       // volume products of volume-surface classes: per slot, 1 + creation tile
+  // counted volumes (Partition's counted volume indices): per molecule and
+  // snapshot slot, and per wall the index inside / outside its object
+  // (kInvalid: the wall's object is not counted)
+  uint32_t num_volumes = 0;
+  DevBuf<uint32_t> cvol, scvol, cv_in, cv_out;
   DevBuf<uint32_t> tile_wall;  // per tile of one replica: its wall
   DevBuf<uint32_t> plist, alist, alist_sorted, rlist0, rlist1, mlist, ulist;
   DevBuf<uint64_t> keys_a, keys_b;

@@ -71,8 +71,12 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   // without unimolecular classes no molecule has a reaction time: the
   // arrays are neither copied nor read
   const bool unimol = std::any_of(r.unimol.begin(), r.unimol.end(), [](const UnimolRec& u) { return u.num_pathways > 0; });
-  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p, unimol ? r.utime.p : nullptr};
-  p.Sm = RMolView<PosT>{sx, sy, sz, r.sid.p, r.stag.p, r.stoff.p, unimol ? r.sutime.p : nullptr};
+  const bool cv = r.num_volumes > 0;
+  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p, unimol ? r.utime.p : nullptr, cv ? r.cvol.p : nullptr};
+  p.Sm = RMolView<PosT>{sx,      sy,      sz, r.sid.p, r.stag.p, r.stoff.p, unimol ? r.sutime.p : nullptr,
+                        cv ? r.scvol.p : nullptr};
+  p.R.cv_in = cv ? r.cv_in.p : nullptr;
+  p.R.cv_out = cv ? r.cv_out.p : nullptr;
   p.R.unimol = r.unimol_d.p;
   const bool surf = r.total_tiles && !r.vs.empty();
   p.R.grids = surf ? r.grids_d.p : nullptr;
@@ -373,13 +377,13 @@ int count_species_into(mcg_ctx* c, uint64_t* counts, uint64_t* dcounts, bool wit
   if (c->n || tiles) {
     if (c->fp32()) {
       ReactPipeline<float> p{};
-      p.M = RMolView<float>{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p, nullptr, nullptr};
+      p.M = RMolView<float>{c->xf.p, c->yf.p, c->zf.p, c->id.p, c->tag.p, nullptr, nullptr, nullptr};
       p.R.total_tiles = tiles;
       p.tiles = r.tiles.p;
       HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, dcounts));
     } else {
       ReactPipeline<double> p{};
-      p.M = RMolView<double>{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p, nullptr, nullptr};
+      p.M = RMolView<double>{c->x.p, c->y.p, c->z.p, c->id.p, c->tag.p, nullptr, nullptr, nullptr};
       p.R.total_tiles = tiles;
       p.tiles = r.tiles.p;
       HIP_TRY(p.count_species(c->stream, (uint32_t)c->n, c->num_species, c->cfg.num_replicas, dcounts));
@@ -625,6 +629,54 @@ int mcg_count_species(mcg_ctx* c, uint64_t* counts) {
   MCG_ALIVE(c);
   if (c->num_species == 0) return fail(MCG_E_STATE, "species must be set first");
   return count_species(c, counts);
+}
+
+int mcg_set_counted_volumes(mcg_ctx* c, uint32_t num_volumes, const uint32_t* wall_inside,
+                            const uint32_t* wall_outside) {
+  if (!c || (num_volumes && (!wall_inside || !wall_outside))) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
+  if (!c->have_geometry) return fail(MCG_E_STATE, "the geometry must be set first");
+  if (c->stepped) return fail(MCG_E_STATE, "counted volumes must be set before the first step");
+  ReactState& r = c->rx;
+  if (r.num_volumes) return fail(MCG_E_STATE, "counted volumes are already set");
+  if (num_volumes == 0) return MCG_OK;
+  const size_t nw = c->walls_host.size();
+  for (size_t w = 0; w < nw; ++w) {
+    const bool none = wall_inside[w] == kInvalid && wall_outside[w] == kInvalid;
+    if (!none && (wall_inside[w] >= num_volumes || wall_outside[w] >= num_volumes))
+      return fail(MCG_E_INVALID, "counted volume index out of range (intersecting objects are not on this path)");
+  }
+  int st = ensure_react_buffers(c);
+  if (st) return st;
+  const size_t cap = c->cfg.molecule_capacity;
+  if (r.cvol.alloc(cap) || r.scvol.alloc(cap) || r.cv_in.alloc(std::max<size_t>(nw, 1)) ||
+      r.cv_out.alloc(std::max<size_t>(nw, 1)))
+    return fail(MCG_E_NOMEM, "counted volume buffers");
+  // molecules added before: outside every counted object (index 0)
+  HIP_TRY(hipMemset(r.cvol.p, 0, cap * 4));
+  if (nw) {
+    HIP_TRY(hipMemcpy(r.cv_in.p, wall_inside, nw * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(r.cv_out.p, wall_outside, nw * 4, hipMemcpyHostToDevice));
+  }
+  r.num_volumes = num_volumes;
+  return MCG_OK;
+}
+
+int mcg_count_species_by_volume(mcg_ctx* c, uint64_t* counts) {
+  if (!c || !counts) return fail(MCG_E_INVALID, "null argument");
+  MCG_ALIVE(c);
+  ReactState& r = c->rx;
+  if (r.num_volumes == 0) return fail(MCG_E_STATE, "no counted volumes are set");
+  const size_t nc = (size_t)c->cfg.num_replicas * r.num_volumes * c->num_species;
+  if (nc > 0xFFFFFFFFull) return fail(MCG_E_LIMIT, "too many counters");
+  DevBuf<uint64_t> d;
+  if (d.alloc(std::max<size_t>(nc, 1))) return fail(MCG_E_NOMEM, "count buffer");
+  HIP_TRY(launch_fill_u64(c->stream, reinterpret_cast<unsigned long long*>(d.p), 0ull, (uint32_t)nc));
+  HIP_TRY(launch_count_by_volume(c->stream, r.cvol.p, c->tag.p, (uint32_t)c->n, c->num_species, c->cfg.num_replicas,
+                                 r.num_volumes, d.p));
+  HIP_TRY(hipMemcpyAsync(counts, d.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCG_OK;
 }
 
 int mcg_count_species_noalloc(mcg_ctx* c, uint64_t* counts) {

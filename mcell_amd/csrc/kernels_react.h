@@ -21,6 +21,7 @@ struct RMolView {
   uint32_t* tag;
   double* toff;
   double* utime;  // unimolecular reaction time (absolute) or kTimeInvalid; null without unimolecular classes
+  uint32_t* cvol;  // counted volume index (Molecule::v.counted_volume_index); null without counted volumes
 };
 
 // One carried-out (or, in the snapshot mode, proposed) reaction: 64 bytes,
@@ -93,6 +94,9 @@ constexpr int kMaxDevices = 64;
 hipError_t count_scratch_address(uint64_t** out);
 
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n);
+hipError_t launch_count_by_volume(hipStream_t s, const uint32_t* cvol, const uint32_t* tag, uint32_t n,
+                                  uint32_t num_species, uint32_t num_replicas, uint32_t num_volumes,
+                                  uint64_t* counts);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v, uint32_t n);
 hipError_t launch_copy_u32(hipStream_t s, const uint32_t* src, uint32_t* dst);
 

@@ -117,6 +117,7 @@ __global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, c
   s.tag[j] = tag;
   if (tag_flags(tag) & MF_TOFF) s.toff[j] = m.toff[i];
   if (s.utime) s.utime[j] = m.utime[i];
+  if (s.cvol) s.cvol[j] = m.cvol[i];
 }
 
 // ----------------------------------------------------------------- moves --
@@ -251,6 +252,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   e.due = 0;
   e.n_reflect = 0;
   e.n_cross = 0;
+  e.cvol = src.cvol ? src.cvol[j] : kInvalid;
   Proposal p;
   bool proposed = false;
   const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
@@ -280,6 +282,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
     p.pathway = e.due - 1;  // the stream pass of the pathway draw
     p.replica = tag_replica(tag);
     p.pad = 0;
+    p.cvol = e.cvol;
     prop[j] = p;
   }
   if (proposed) {
@@ -300,6 +303,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
     t2 = store_toff(out.toff, j, sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff, t2);
     out.tag[j] = e.absorbed ? (t2 | (MF_DEFUNCT << 8)) : t2;
     if (out.utime) out.utime[j] = e.utime;
+    if (out.cvol) out.cvol[j] = e.cvol;
     if (born) R.born[j] = 0;  // the step that could use it is over
     n_reflect += e.n_reflect;
     n_cross += e.n_cross;
@@ -432,6 +436,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
       out.id[j] = idv;
       out.tag[j] = store_toff(out.toff, j, toff_past_iteration(R.iteration, toff), tag);
       if (out.utime) out.utime[j] = utime;
+      if (out.cvol) out.cvol[j] = S.cvol[j];
     } else if (unimol_now) {
       hard = true;
     } else if (sp.space_step > 0) {
@@ -582,6 +587,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
         out.id[j] = idv;
         out.tag[j] = store_toff(out.toff, j, next_toff(R.iteration, toff, t_steps), tag & ~(MF_STEP_FLAGS << 8));
         if (out.utime) out.utime[j] = utime;
+        if (out.cvol) out.cvol[j] = S.cvol[j];
       } else {
         hard = true;
         partner_hard = walls_clear;
@@ -591,6 +597,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
       out.id[j] = idv;
       out.tag[j] = store_toff(out.toff, j, toff, tag & ~(MF_STEP_FLAGS << 8));
       if (out.utime) out.utime[j] = utime;
+      if (out.cvol) out.cvol[j] = S.cvol[j];
     }
   }
   // movers queued for walls (and unimolecular events) and movers queued for
@@ -890,6 +897,11 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
       out.tag[slot] = store_toff(out.toff, slot, p.time - (double)(R.iteration + (moves_now ? 0u : 1u)),
                                  make_tag(ps, MF_SCHEDULE_UNIMOL, rep));
       if (out.utime) out.utime[slot] = kTimeInvalid;
+      // update_counted_volume_id_when_crossing_wall with the product's
+      // orientation on a counted object's wall (:2775-2781), else the
+      // volume the reactant was in
+      if (out.cvol)
+        out.cvol[slot] = (R.cv_in[wall] != kInvalid) ? (o > 0 ? R.cv_out[wall] : R.cv_in[wall]) : p.cvol;
       consumed[slot] = 0;
       if (R.born) R.born[slot] = moves_now ? ts + 1 : 0u;
       if (moves_now) mlist[atomicAdd(mcount, 1u)] = slot;
@@ -931,6 +943,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
     out.tag[slot] = store_toff(out.toff, slot, p.time - (double)(R.iteration + (moves_now ? 0u : 1u)),
                                make_tag(prods[i], MF_SCHEDULE_UNIMOL, rep));
     if (out.utime) out.utime[slot] = kTimeInvalid;
+    if (out.cvol) out.cvol[slot] = p.cvol;  // where the reaction happened
     consumed[slot] = 0;
     owner[slot] = kInvalid;
     if (kept) kept[slot] = 0;
@@ -1114,7 +1127,7 @@ template <typename PosT>
 hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uint32_t* list, const uint32_t* count,
                                      uint32_t max_movers, double barrier, bool propose_only, uint32_t* err) {
   if (max_movers == 0) return hipSuccess;
-  Snapshot<PosT> S{Sm.x, Sm.y, Sm.z, Sm.id, Sm.tag, consumed, Sm.toff, Sm.utime};
+  Snapshot<PosT> S{Sm.x, Sm.y, Sm.z, Sm.id, Sm.tag, consumed, Sm.toff, Sm.utime, Sm.cvol};
   const uint32_t po = propose_only ? 1u : 0u;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrProposals, 0u, 1u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrExd, 0u, 1u);
@@ -1218,6 +1231,34 @@ hipError_t ReactPipeline<PosT>::count_species(hipStream_t s, uint32_t n, uint32_
   if (nt)
     hipLaunchKernelGGL(k_count_tiles, dim3((uint32_t)std::min<uint64_t>((nt + 255) / 256, 2048u)), dim3(256), 0, s,
                        tiles, R.total_tiles, num_replicas, num_species, counts);
+  return hipGetLastError();
+}
+
+// Live molecules per (replica, counted volume, species); molecules whose
+// index is outside [0, num_volumes) are not counted.
+__global__ __launch_bounds__(256) void k_count_by_volume(const uint32_t* __restrict__ cvol,
+                                                         const uint32_t* __restrict__ tag, uint32_t n,
+                                                         uint32_t num_species, uint32_t num_replicas,
+                                                         uint32_t num_volumes, uint64_t* __restrict__ counts) {
+  __shared__ uint32_t h[kCountLdsBins];
+  const uint32_t bins = num_species * num_replicas * num_volumes;
+  count_clear(h, bins);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t t = tag[i];
+    if (tag_flags(t) & MF_DEFUNCT) continue;
+    const uint32_t s = tag_species(t), r = tag_replica(t), v = cvol[i];
+    if (s < num_species && r < num_replicas && v < num_volumes)
+      count_add(h, counts, bins, (r * num_volumes + v) * num_species + s);
+  }
+  count_flush(h, counts, bins);
+}
+
+hipError_t launch_count_by_volume(hipStream_t s, const uint32_t* cvol, const uint32_t* tag, uint32_t n,
+                                  uint32_t num_species, uint32_t num_replicas, uint32_t num_volumes,
+                                  uint64_t* counts) {
+  if (n)
+    hipLaunchKernelGGL(k_count_by_volume, dim3(std::min<uint32_t>(blocks_for(n).x, 2048u)), dim3(256), 0, s, cvol,
+                       tag, n, num_species, num_replicas, num_volumes, counts);
   return hipGetLastError();
 }
 

@@ -83,12 +83,16 @@ class DiffuseReactEngine:
                 eng.set_replica_params(r, species_steps(wl, r), rxn_table(wl, r) if wl.rxns else None)
         if log_capacity:
             eng.set_rxn_log(log_capacity)
+        cv = getattr(wl, "counted_volumes", None)
+        if cv is not None:
+            eng.set_counted_volumes(cv["num"], cv["wall_inside"], cv["wall_outside"])
         if not add_molecules:
             return eng
         for r in range(wl.num_replicas):
             sel = wl.replica == r
             if sel.any():
-                eng.add_volume_molecules(r, wl.ids[sel], wl.mol_species[sel], wl.positions[sel])
+                eng.add_volume_molecules(r, wl.ids[sel], wl.mol_species[sel], wl.positions[sel],
+                                         counted_volumes=None if cv is None else cv["molecules"][sel])
         sm = wl.surface_mols
         if sm is not None:
             for r in range(wl.num_replicas):
@@ -235,7 +239,7 @@ class DiffuseReactEngine:
         return out
 
     def add_volume_molecules(self, replica: int, ids: np.ndarray, species: np.ndarray, positions: np.ndarray,
-                             diffusion_times=None, unimol_times=None, flags=None):
+                             diffusion_times=None, unimol_times=None, flags=None, counted_volumes=None):
         ids = np.ascontiguousarray(ids, dtype=np.uint32)
         sp = np.ascontiguousarray(species, dtype=np.uint32)
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
@@ -243,10 +247,24 @@ class DiffuseReactEngine:
         dt = None if diffusion_times is None else np.ascontiguousarray(diffusion_times, dtype=np.float64)
         ut = None if unimol_times is None else np.ascontiguousarray(unimol_times, dtype=np.float64)
         fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint32)
-        for a in (dt, ut, fl):
+        cv = None if counted_volumes is None else np.ascontiguousarray(counted_volumes, dtype=np.uint32)
+        for a in (dt, ut, fl, cv):
             assert a is None or len(a) == len(ids)
         _lib.check(self.lib.mcg_add_volume_molecules(self.ctx, replica, _ptr(ids), _ptr(sp), _ptr(pos), _ptr(dt),
-                                                     _ptr(ut), _ptr(fl), len(ids)))
+                                                     _ptr(ut), _ptr(fl), _ptr(cv), len(ids)))
+
+    def set_counted_volumes(self, num_volumes: int, wall_inside, wall_outside):
+        """Per wall: the counted volume index inside / outside its object
+        (0xFFFFFFFF for both: not a counted object)."""
+        wi = np.ascontiguousarray(wall_inside, dtype=np.uint32)
+        wo = np.ascontiguousarray(wall_outside, dtype=np.uint32)
+        _lib.check(self.lib.mcg_set_counted_volumes(self.ctx, num_volumes, _ptr(wi), _ptr(wo)))
+        self.num_volumes = num_volumes
+
+    def count_species_by_volume(self, num_replicas: int, num_volumes: int, num_species: int) -> np.ndarray:
+        out = np.zeros((num_replicas, num_volumes, num_species), np.uint64)
+        _lib.check(self.lib.mcg_count_species_by_volume(self.ctx, _ptr(out)))
+        return out
 
     def next_molecule_ids(self) -> np.ndarray:
         out = np.zeros(self.num_replicas, np.uint32)
@@ -322,11 +340,12 @@ class DiffuseReactEngine:
         dt = np.zeros(n, np.float64)
         ut = np.zeros(n, np.float64)
         fl = np.zeros(n, np.uint32)
+        cv = np.zeros(n, np.uint32)
         cnt = ctypes.c_uint64()
         _lib.check(self.lib.mcg_get_volume_molecules(self.ctx, _ptr(ids), _ptr(sp), _ptr(rep), _ptr(pos), _ptr(sub),
-                                                     _ptr(dt), _ptr(ut), _ptr(fl), n, ctypes.byref(cnt)))
+                                                     _ptr(dt), _ptr(ut), _ptr(fl), _ptr(cv), n, ctypes.byref(cnt)))
         return {"id": ids, "species": sp, "replica": rep, "pos": pos, "subpart": sub, "diffusion_time": dt,
-                "unimol_time": ut, "flags": fl}
+                "unimol_time": ut, "flags": fl, "counted_volume": cv}
 
     def walls(self):
         cnt = ctypes.c_uint64()

@@ -50,6 +50,9 @@ class Workload:
     # tiles, orients, replica (or None)
     surface_mols: Optional[dict] = None
     spheres: Optional[np.ndarray] = None  # (k, 4) nominal centres and radii (um) of generated spheres
+    # counted volumes: {"num": n, "wall_inside": (nw,), "wall_outside": (nw,),
+    # "molecules": (num_molecules,) index of every volume molecule}, or None
+    counted_volumes: Optional[dict] = None
 
     def species_for(self, replica: int) -> List[M.Species]:
         if self.replica_D is None:
@@ -309,7 +312,7 @@ def place_on_walls(n: int, wall_ids: np.ndarray, vertices: np.ndarray, walls: np
 def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, seed: int = 3,
                     subpartition_dimension: float = 0.1, sphere_subdivisions: int = 5, rate: float = 1e8,
                     down_fraction: float = 0.5, n_t: int = 0, t_action: int = 1, products=None,
-                    more_species=(), more_volsurf=()) -> Workload:
+                    more_species=(), more_volsurf=(), counted: bool = False) -> Workload:
     """Ligand-receptor binding on a sphere: the reflective 2 um cube holding
     the icosphere of radius 0.5 um; L (D = 1e-6 cm^2/s) starts outside the
     sphere and is reflected by it, the surface receptors R sit on distinct
@@ -324,7 +327,9 @@ def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, s
     then T, then the receptors.  ``products`` replaces LR' by a product list
     of the class ((species, orientation, is surface), model.VolSurfReaction);
     ``more_species`` ((name, D), ...) follow the others and ``more_volsurf``
-    adds classes."""
+    adds classes.  counted: the sphere is a counted volume (index 1 inside,
+    0 outside), every volume molecule starting with the index of where it
+    lies."""
     cfg = M.Config(time_step=1e-6, subpartition_dimension=subpartition_dimension, seed=seed)
     box = M.create_box("box", edge_dimension=2.0)
     ball = M.create_icosphere("sphere", 0.5, sphere_subdivisions)
@@ -349,7 +354,8 @@ def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, s
     if n_t:
         pos = np.concatenate([pos, M.place_uniform_in_box(n_t, 1.0, seed + 1, part)])
         sp = np.concatenate([sp, np.full(n_t, 3, np.uint32)])
-        sc = np.array([[0, 0, 0, t_action | (t_action << 2)]], np.uint8)
+        sc = np.zeros((1, len(species)), np.uint8)
+        sc[0, 3] = t_action | (t_action << 2)
     n_vol = len(sp)
     surf = {"ids": np.arange(n_vol, n_vol + n_r, dtype=np.uint32), "species": np.ones(n_r, np.uint32),
             "walls": sw, "tiles": st, "orients": orients, "replica": np.zeros(n_r, np.uint32)}
@@ -358,10 +364,31 @@ def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, s
         steps[v.vol] = (steps[v.vol][0], steps[v.vol][1], steps[v.vol][2] | 4)
     if n_t:
         steps[3] = (steps[3][0], steps[3][1], steps[3][2] | 2)
+    cv = None
+    if counted:
+        nb = len(box.wall_list)
+        wi = np.full(len(walls), 0xFFFFFFFF, np.uint32)
+        wo = wi.copy()
+        wi[nb:] = 1
+        wo[nb:] = 0
+        cv = {"num": 2, "wall_inside": wi, "wall_outside": wo,
+              "molecules": inside_convex(pos, verts, walls[nb:]).astype(np.uint32)}
     return Workload("receptor_sphere", part, verts, walls, classes, species, steps,
                     np.arange(n_vol, dtype=np.uint32), sp, pos, np.zeros(n_vol, np.uint32),
                     iterations, config=cfg, objects=objects, volsurf_rxns=rx, surface_mols=surf,
-                    surface_classes=sc)
+                    surface_classes=sc, counted_volumes=cv)
+
+
+def inside_convex(points: np.ndarray, vertices: np.ndarray, walls: np.ndarray) -> np.ndarray:
+    """Whether each point lies inside the convex mesh whose walls face
+    outward (behind every wall plane)."""
+    tri = vertices[walls]
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    inside = np.ones(len(points), bool)
+    for k in range(0, len(points), 4096):
+        p = points[k:k + 4096]
+        inside[k:k + 4096] = np.all(np.einsum("pj,wj->pw", p, n) - np.einsum("wj,wj->w", tri[:, 0], n) < 0, axis=1)
+    return inside
 
 
 def volume_surface(iterations: int = 1000, seed: int = 3, subpartition_dimension: float = 0.1) -> Workload:

@@ -278,6 +278,22 @@ int orc_run(void* wp, uint32_t n, double* seconds) {
 uint64_t orc_iteration(void* wp) { return static_cast<World*>(wp)->iteration; }
 uint32_t orc_num_molecules(void* wp) { return (uint32_t)static_cast<World*>(wp)->mols.size(); }
 
+// counted volumes: per wall inside / outside index (INVALID_U32 for both:
+// not a counted object)
+void orc_set_counted_volumes(void* wp, uint32_t num_walls, const uint32_t* inside, const uint32_t* outside) {
+  World* w = static_cast<World*>(wp);
+  w->cv_in.assign(inside, inside + num_walls);
+  w->cv_out.assign(outside, outside + num_walls);
+}
+void orc_set_molecule_volumes(void* wp, uint32_t n, const uint32_t* ids, const uint32_t* cvols) {
+  World* w = static_cast<World*>(wp);
+  for (uint32_t i = 0; i < n; ++i) w->mols[ids[i]].cvol = cvols[i];
+}
+void orc_get_molecule_volumes(void* wp, uint32_t* out) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->mols.size(); ++i) out[i] = w->mols[i].cvol;
+}
+
 void orc_get_molecules(void* wp, uint32_t* species, uint32_t* flags, double* pos, uint32_t* subpart,
                        double* diffusion_time) {
   World* w = static_cast<World*>(wp);

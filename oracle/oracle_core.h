@@ -272,6 +272,7 @@ struct Mol {
   double diffusion_time;
   double birthday;
   double unimol_rxn_time;
+  uint32_t cvol = 0;  // counted volume index (Molecule::v.counted_volume_index)
 };
 
 enum class CollType : int { WALL_MISS = 0, WALL_FRONT = 1, WALL_BACK = 2, WALL_REDO = 3, VOLVOL = 4 };
@@ -429,6 +430,13 @@ class World {
     uint32_t born_wall = INVALID_U32, born_tile = INVALID_U32;
   };
   uint32_t born_wall = INVALID_U32, born_tile = INVALID_U32;  // of the molecule diffusing now
+
+ public:
+  // counted volumes: per wall the index inside / outside its object
+  // (INVALID_U32: not a counted object); empty = none
+  std::vector<uint32_t> cv_in, cv_out;
+
+ private:
   std::vector<DiffuseAction> new_diffuse_actions;
   std::vector<V3> snapshot_pos;
   double event_time = 0;

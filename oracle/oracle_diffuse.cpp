@@ -114,6 +114,7 @@ bool World::react_unimol(uint32_t id) {
       continue;
     }
     uint32_t nid = add_volume_molecule(ps, pos, t);
+    mols[nid].cvol = mols[id].cvol;  // the volume the reactant is in
     if (cmp_lt(t, event_time + 1.0, EPS)) new_diffuse_actions.push_back({nid});
   }
   ++unimol_reactions;
@@ -710,6 +711,10 @@ int World::wall_action(const Wall& w, uint32_t sp, CollType type) const {
 // and scale the remaining displacement and time.
 void World::cross_transparent_wall(const Collision& c, Mol& vm, V3& disp, double& t_steps, double& elapsed,
                                    uint32_t& last_hit) {
+  // update_counted_volume_id_when_crossing_wall (collision_utils.inl:1637-
+  // 1694) for objects that do not intersect: a front hit goes inside
+  if (!cv_in.empty() && cv_in[c.wall] != INVALID_U32)
+    vm.cvol = c.type == CollType::WALL_FRONT ? cv_in[c.wall] : cv_out[c.wall];
   vm.pos = c.pos;
   vm.subpart = subpart_index(vm.pos);
   double t_smash = c.time;

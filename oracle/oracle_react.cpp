@@ -584,6 +584,9 @@ void World::outcome_bimolecular(const Collision& c, int pathway, double time) {
     if (keepA && !usedA && ps == rx.a) { usedA = true; continue; }
     if (keepB && !usedB && ps == rx.b) { usedB = true; continue; }
     uint32_t nid = add_volume_molecule(ps, c.pos, time);
+    // the volume the collision point lies in: the mover's (it crossed no
+    // wall between)
+    mols[nid].cvol = mols[c.mover].cvol;
     if (cmp_lt(time, event_time + 1.0, EPS)) new_diffuse_actions.push_back({nid});
   }
   rxn_log.push_back({(uint32_t)iteration, c.mover, c.partner, c.rxn, pathway, time, c.pos});
@@ -701,6 +704,10 @@ int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, d
     }
     const uint32_t nid = add_volume_molecule(prods[q].species, c.pos, time);
     Mol& pm = mols[nid];
+    // update_counted_volume_id_when_crossing_wall with the product's
+    // orientation (:2775-2781), else the volume the mover is in
+    pm.cvol = (!cv_in.empty() && cv_in[wall] != INVALID_U32) ? (orient[q] > 0 ? cv_out[wall] : cv_in[wall])
+                                                             : mols[c.mover].cvol;
     pm.pos = volume_product_position(nid, wall, orient[q]);
     register_reactant(pm, false);
     pm.subpart = subpart_index(pm.pos);

@@ -57,6 +57,9 @@ def load():
         "orc_add_unimol_rxn": (None, [P, u32, d, P, u32]),
         "orc_unimol_reactions": (u64, [P]),
         "orc_add_volsurf_rxn": (ctypes.c_int, [P, u32, u32, ctypes.c_int, ctypes.c_int, d, u32, P, P, P]),
+        "orc_set_counted_volumes": (None, [P, u32, P, P]),
+        "orc_set_molecule_volumes": (None, [P, u32, P, P]),
+        "orc_get_molecule_volumes": (None, [P, P]),
         "orc_get_volsurf_rxn": (None, [P, ctypes.c_int, P]),
         "orc_add_surface_molecules": (ctypes.c_int, [P, u32, u32, P, P, P]),
         "orc_grid_info": (None, [P, u32, P]),
@@ -276,6 +279,22 @@ class OracleWorld:
             raise RuntimeError(self.lib.orc_error().decode())
         return secs.value
 
+    def set_counted_volumes(self, wall_inside, wall_outside):
+        wi = np.ascontiguousarray(wall_inside, np.uint32)
+        wo = np.ascontiguousarray(wall_outside, np.uint32)
+        self.lib.orc_set_counted_volumes(self.w, len(wi), _p(wi), _p(wo))
+
+    def set_molecule_volumes(self, ids, cvols):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        cv = np.ascontiguousarray(cvols, np.uint32)
+        self.lib.orc_set_molecule_volumes(self.w, len(ids), _p(ids), _p(cv))
+
+    def molecule_volumes(self):
+        n = self.lib.orc_num_molecules(self.w)
+        out = np.zeros(max(n, 1), np.uint32)
+        self.lib.orc_get_molecule_volumes(self.w, _p(out))
+        return out[:n]
+
     def molecules(self):
         n = self.lib.orc_num_molecules(self.w)
         sp = np.zeros(n, np.uint32)
@@ -391,6 +410,10 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
             end += 1
         w.add_molecules(int(sp[start]), pos[start:end])
         start = end
+    cv = getattr(wl, "counted_volumes", None)
+    if cv is not None:
+        w.set_counted_volumes(cv["wall_inside"], cv["wall_outside"])
+        w.set_molecule_volumes(ids, cv["molecules"][sel])
     sm = getattr(wl, "surface_mols", None)
     if sm is not None:
         ssel = np.nonzero(sm["replica"] == replica)[0]

@@ -205,3 +205,41 @@ def test_volume_products_match_oracle(gpu_available, oracle_lib, case):
     assert n_rx > 100 and n_p > 50
     w.close()
     eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_counted_volumes(gpu_available, oracle_lib, precision):
+    """The sphere as a counted volume (1 inside, 0 outside), T crossing it
+    through a transparent class and L binding R' on it: every molecule's
+    index equals the oracle's (FP64 form) and says on which side of the
+    sphere it lies (both forms), and the per-volume counts are those of the
+    molecules."""
+    from oracle import pyoracle as po
+    seed = 43
+    wl = W.receptor_sphere(n_l=20000, n_r=20000, iterations=30, rate=4e8, n_t=40000, counted=True,
+                           products=[(1, 1, True), (4, 0, False)], more_species=[("P", 1e-6)])
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed, precision=precision)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
+    w.set_barrier_interval(wl.barrier_interval)
+    nb = len(wl.objects[0].wall_list)
+    for chunk in range(0, wl.iterations, 10):
+        eng.step(chunk, 10)
+        g = eng.molecules()
+        o = np.argsort(g["id"], kind="stable")
+        truth = W.inside_convex(g["pos"][o], wl.vertices, wl.walls[nb:]).astype(np.uint32)
+        assert np.array_equal(g["counted_volume"][o], truth), "iteration %d: counted volumes" % (chunk + 10)
+        if precision == "fp64":
+            w.run(10)
+            om = w.molecules()
+            live = np.nonzero(((om["flags"] & 1) == 0) & ((om["species"] == 0) | (om["species"] >= 3)))[0]
+            assert np.array_equal(g["id"][o], live)
+            assert np.array_equal(g["counted_volume"][o], w.molecule_volumes()[live])
+        c = eng.count_species_by_volume(1, 2, 5)[0].astype(np.int64)
+        ref = np.zeros((2, 5), np.int64)
+        np.add.at(ref, (g["counted_volume"], g["species"]), 1)
+        assert np.array_equal(c, ref)
+        print("%s iteration %d: T inside %d outside %d, P inside %d outside %d"
+              % (precision, chunk + 10, c[1, 3], c[0, 3], c[1, 4], c[0, 4]), flush=True)
+    assert c[1, 3] > 1000 and c[0, 4] + c[1, 4] > 50
+    w.close()
+    eng.close()

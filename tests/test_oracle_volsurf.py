@@ -187,3 +187,27 @@ def test_volume_product_starts_next_to_the_wall(oracle_lib, orient):
         n_out = int(np.sum(side > 0))
         assert abs(n_out - len(side) / 2) < 4 * math.sqrt(len(side) / 4), (n_out, len(side))
     w.close()
+
+
+def test_counted_volumes_follow_the_geometry(oracle_lib):
+    """The sphere as a counted volume (1 inside, 0 outside) with T crossing
+    it through a transparent class: after 60 iterations every live volume
+    molecule's index says on which side of the sphere it is
+    (update_counted_volume_id_when_crossing_wall, collision_utils.inl:
+    1637-1694), many T having crossed."""
+    from oracle import pyoracle as po
+    wl = W.receptor_sphere(n_l=2000, n_r=500, iterations=60, n_t=20000, counted=True)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=5)
+    w.set_barrier_interval(wl.barrier_interval)
+    cv0 = w.molecule_volumes()
+    w.run(wl.iterations)
+    om = w.molecules()
+    cv = w.molecule_volumes()
+    live = np.nonzero(((om["flags"] & 1) == 0) & ((om["species"] == 0) | (om["species"] == 3)))[0]
+    nb = len(wl.objects[0].wall_list)
+    truth = W.inside_convex(om["pos"][live], wl.vertices, wl.walls[nb:]).astype(np.uint32)
+    assert np.array_equal(cv[live], truth)
+    changed = int(np.sum(cv[live] != cv0[live]))
+    print("%d of %d volume molecules changed volume" % (changed, len(live)))
+    assert changed > 100 and w.wall_stats()["transparent_crossings"] >= changed
+    w.close()
