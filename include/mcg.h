@@ -330,6 +330,17 @@ int mcg_set_counted_volumes(mcg_ctx* ctx, uint32_t num_volumes, const uint32_t* 
  * counts[(replica * num_volumes + volume) * num_species + species]. */
 int mcg_count_species_by_volume(mcg_ctx* ctx, uint64_t* counts);
 
+/* Reactions carried out since the first step (SimulationStats::
+ * inc_rxn_occurred and Partition::inc_rxn_in_volume_occured_count,
+ * diffuse_react_event.cpp:2509-2520), per replica, counted volume the mover
+ * was in (one volume without counted volumes) and pathway slot: bimolecular
+ * class i pathway k at i * 4 + k, then the unimolecular class of species s
+ * at (num_bimol + s) * 4 + k, then volume-surface class i at
+ * (num_bimol + num_species + i) * 4 + k.  dims (may be NULL) gets
+ * {replicas, volumes, slots}; counts (may be NULL) gets
+ * counts[(replica * volumes + volume) * slots + slot]. */
+int mcg_get_rxn_counts(mcg_ctx* ctx, uint64_t* counts, uint32_t* dims);
+
 /* Partition::next_molecule_id per replica (partition.h:420-470): the id the
  * replica's next product gets.  Get fills num_replicas values; set raises
  * each replica's counter to at least the given value (it never lowers it). */

@@ -39,7 +39,7 @@ EXPORTED = [
     "mcg_get_rxn_log", "mcg_count_species", "mcg_set_replica_params", "mcg_set_surface_classes",
     "mcg_set_unimol_rxns", "mcg_set_volsurf_rxns", "mcg_add_surface_molecules", "mcg_get_surface_molecules",
     "mcg_get_next_molecule_ids", "mcg_set_next_molecule_ids", "mcg_count_species_noalloc",
-    "mcg_set_counted_volumes", "mcg_count_species_by_volume",
+    "mcg_set_counted_volumes", "mcg_count_species_by_volume", "mcg_get_rxn_counts",
 ]
 
 MCG_MOLF_SCHEDULE_UNIMOL = 1
@@ -182,6 +182,7 @@ def load():
                                              ctypes.POINTER(ctypes.c_uint64)]
     lib.mcg_set_counted_volumes.argtypes = [P, ctypes.c_uint32, P, P]
     lib.mcg_count_species_by_volume.argtypes = [P, P]
+    lib.mcg_get_rxn_counts.argtypes = [P, P, P]
     lib.mcg_get_next_molecule_ids.argtypes = [P, P]
     lib.mcg_set_next_molecule_ids.argtypes = [P, P]
     lib.mcg_get_walls.argtypes = [P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]

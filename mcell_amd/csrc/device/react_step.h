@@ -49,6 +49,12 @@ struct ReactParams {
   // (kInvalid: not a counted object), or null
   const uint32_t* __restrict__ cv_in;
   const uint32_t* __restrict__ cv_out;
+  // reaction counts (SimulationStats::inc_rxn_occurred,
+  // Partition::inc_rxn_in_volume_occured_count): per (replica, counted
+  // volume, class pathway slot); slots: bimolecular class * 4 + pathway,
+  // then rc_unimol + species * 4 + pathway, then rc_vs + class * 4 + pathway
+  unsigned long long* rxn_counts;
+  uint32_t rc_slots, rc_unimol, rc_vs, rc_volumes;
   // per output slot: 1 + the tile slot a volume product was created on (it
   // does not react with that tile on its first hit there), or 0
   uint32_t* __restrict__ born;

@@ -79,6 +79,9 @@ struct ReactState {
   // (kInvalid: the wall's object is not counted)
   uint32_t num_volumes = 0;
   DevBuf<uint32_t> cvol, scvol, cv_in, cv_out;
+  // reaction counts since the first step (layout: ReactParams::rxn_counts)
+  DevBuf<unsigned long long> rxn_counts;
+  uint32_t rc_slots = 0, rc_unimol = 0, rc_vs = 0, rc_volumes = 0;
   DevBuf<uint32_t> tile_wall;  // per tile of one replica: its wall
   DevBuf<uint32_t> plist, alist, alist_sorted, rlist0, rlist1, mlist, ulist;
   DevBuf<uint64_t> keys_a, keys_b;

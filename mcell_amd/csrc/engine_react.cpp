@@ -77,6 +77,11 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
                         cv ? r.scvol.p : nullptr};
   p.R.cv_in = cv ? r.cv_in.p : nullptr;
   p.R.cv_out = cv ? r.cv_out.p : nullptr;
+  p.R.rxn_counts = r.rxn_counts.p;
+  p.R.rc_slots = r.rc_slots;
+  p.R.rc_unimol = r.rc_unimol;
+  p.R.rc_vs = r.rc_vs;
+  p.R.rc_volumes = r.rc_volumes;
   p.R.unimol = r.unimol_d.p;
   const bool surf = r.total_tiles && !r.vs.empty();
   p.R.grids = surf ? r.grids_d.p : nullptr;
@@ -334,6 +339,16 @@ int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, do
     r.next_id_dirty = false;
   }
   if (int st = sync_tiles_to_device(c)) return st;
+  if (!r.rxn_counts.p) {
+    // the classes are fixed from the first step on
+    r.rc_unimol = (uint32_t)r.rxns.size() * 4;
+    r.rc_vs = r.rc_unimol + c->num_species * 4;
+    r.rc_slots = r.rc_vs + (uint32_t)r.vs.size() * 4;
+    r.rc_volumes = std::max<uint32_t>(r.num_volumes, 1);
+    const size_t n = (size_t)c->cfg.num_replicas * r.rc_volumes * r.rc_slots;
+    if (r.rxn_counts.alloc(std::max<size_t>(n, 1))) return fail(MCG_E_NOMEM, "reaction counts");
+    HIP_TRY(hipMemset(r.rxn_counts.p, 0, std::max<size_t>(n, 1) * 8));
+  }
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   double move_ms = 0;
   int st;
@@ -675,6 +690,28 @@ int mcg_count_species_by_volume(mcg_ctx* c, uint64_t* counts) {
   HIP_TRY(launch_count_by_volume(c->stream, r.cvol.p, c->tag.p, (uint32_t)c->n, c->num_species, c->cfg.num_replicas,
                                  r.num_volumes, d.p));
   HIP_TRY(hipMemcpyAsync(counts, d.p, nc * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCG_OK;
+}
+
+int mcg_get_rxn_counts(mcg_ctx* c, uint64_t* counts, uint32_t* dims) {
+  if (!c) return fail(MCG_E_INVALID, "null context");
+  MCG_ALIVE(c);
+  ReactState& r = c->rx;
+  const uint32_t nv = std::max<uint32_t>(r.num_volumes, 1);
+  const uint32_t slots = ((uint32_t)r.rxns.size() + c->num_species + (uint32_t)r.vs.size()) * 4;
+  if (dims) {
+    dims[0] = c->cfg.num_replicas;
+    dims[1] = nv;
+    dims[2] = slots;
+  }
+  if (!counts) return MCG_OK;
+  const size_t n = (size_t)c->cfg.num_replicas * nv * slots;
+  if (!r.rxn_counts.p) {  // no step has run
+    std::fill(counts, counts + n, 0ull);
+    return MCG_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(counts, r.rxn_counts.p, n * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MCG_OK;
 }

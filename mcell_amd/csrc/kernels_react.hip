@@ -831,6 +831,14 @@ __global__ __launch_bounds__(256) void k_seg_heads(const uint64_t* __restrict__ 
 // next_id[replica] + (products of earlier reactions of that replica) + i and
 // output slots base + pre[k] + i.  Products created before the iteration's
 // end move in this iteration (diffuse_react_event.cpp:2883-2886).
+// One carried-out reaction in the counts (the volume the mover was in; 0
+// without counted volumes).
+__device__ __forceinline__ void count_reaction(const ReactParams& R, uint32_t rep, uint32_t cvol, uint32_t slot) {
+  if (!R.rxn_counts || slot >= R.rc_slots) return;
+  const uint32_t v = (R.rc_volumes > 1 && cvol < R.rc_volumes) ? cvol : 0u;
+  atomicAdd(R.rxn_counts + ((size_t)rep * R.rc_volumes + v) * R.rc_slots + slot, 1ull);
+}
+
 template <typename PosT>
 __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t* __restrict__ sorted_slots,
                                                   const uint64_t* __restrict__ keys, uint32_t n,
@@ -855,6 +863,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
     const VolSurfRec& vs = R.vs_rxns[p.rxn & ~kVolSurfBit];
     const uint32_t pw = p.pathway & kPathwayMask, drawn = p.pathway >> 8;
     const uint32_t rep = key_replica(keys[k]);
+    count_reaction(R, rep, p.cvol, R.rc_vs + (p.rxn & ~kVolSurfBit) * 4 + pw);
     const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
     const uint32_t ts = p.partner_slot & ~kTileBit;
     const uint32_t wall = R.tile_wall[ts - rep * R.total_tiles];
@@ -929,6 +938,8 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
   const uint32_t np = uni ? R.unimol[p.rxn & ~kUnimolBit].num_products[p.pathway]
                           : R.rxns[p.replica * R.rxn_stride + p.rxn].num_products[p.pathway];
   const uint32_t rep = key_replica(keys[k]);
+  count_reaction(R, rep, p.cvol,
+                 uni ? R.rc_unimol + (p.rxn & ~kUnimolBit) * 4 + p.pathway : p.rxn * 4 + p.pathway);
   const uint32_t base = hdr[kHdrNumOut];
   const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
   const bool moves_now = cmp_lt_d(p.time, (double)R.iteration + 1.0, 1e-12);

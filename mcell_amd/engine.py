@@ -261,6 +261,16 @@ class DiffuseReactEngine:
         _lib.check(self.lib.mcg_set_counted_volumes(self.ctx, num_volumes, _ptr(wi), _ptr(wo)))
         self.num_volumes = num_volumes
 
+    def rxn_counts(self) -> np.ndarray:
+        """Reactions carried out, [replica, counted volume, pathway slot]
+        (slots: bimolecular class * 4 + pathway, then unimolecular classes
+        by species, then volume-surface classes)."""
+        dims = np.zeros(3, np.uint32)
+        _lib.check(self.lib.mcg_get_rxn_counts(self.ctx, None, _ptr(dims)))
+        out = np.zeros(tuple(int(d) for d in dims), np.uint64)
+        _lib.check(self.lib.mcg_get_rxn_counts(self.ctx, _ptr(out), None))
+        return out
+
     def count_species_by_volume(self, num_replicas: int, num_volumes: int, num_species: int) -> np.ndarray:
         out = np.zeros((num_replicas, num_volumes, num_species), np.uint64)
         _lib.check(self.lib.mcg_count_species_by_volume(self.ctx, _ptr(out)))

@@ -51,6 +51,12 @@ def test_unimolecular_keep_matches_oracle_exactly(gpu_available, oracle_lib):
         print("iteration %d: A %d B %d E %d, %d reactions" % (chunk + 20, c[0], c[1], c[2], w.unimol_reactions()),
               flush=True)
     assert c[0] > 500 and c[1] > 200
+    # reaction counts by pathway (E is species 2: slots 8 + pathway) equal the oracle's events
+    u, rx, d = w.rxn_log()
+    rc = eng.rxn_counts()
+    assert rc.shape[:2] == (1, 1)
+    assert int(rc[0, 0, 8]) == int(np.sum(u[:, 3] == 0)) and int(rc[0, 0, 9]) == int(np.sum(u[:, 3] == 1))
+    assert int(rc.sum()) == w.unimol_reactions()
     w.close()
     eng.close()
 

@@ -241,5 +241,11 @@ def test_counted_volumes(gpu_available, oracle_lib, precision):
         print("%s iteration %d: T inside %d outside %d, P inside %d outside %d"
               % (precision, chunk + 10, c[1, 3], c[0, 3], c[1, 4], c[0, 4]), flush=True)
     assert c[1, 3] > 1000 and c[0, 4] + c[1, 4] > 50
+    # every reaction happened outside the sphere (the ligands start there) and is counted there
+    rc = eng.rxn_counts()
+    n_vs = int(rc[0, 0, rc.shape[2] - 4:].sum())
+    assert n_vs == int(rc.sum()) > 50 and int(rc[0, 1].sum()) == 0
+    if precision == "fp64":
+        assert n_vs == w.volsurf_reactions()
     w.close()
     eng.close()
