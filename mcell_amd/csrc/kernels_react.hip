@@ -831,12 +831,29 @@ __global__ __launch_bounds__(256) void k_seg_heads(const uint64_t* __restrict__ 
 // next_id[replica] + (products of earlier reactions of that replica) + i and
 // output slots base + pre[k] + i.  Products created before the iteration's
 // end move in this iteration (diffuse_react_event.cpp:2883-2886).
-// One carried-out reaction in the counts (the volume the mover was in; 0
-// without counted volumes).
-__device__ __forceinline__ void count_reaction(const ReactParams& R, uint32_t rep, uint32_t cvol, uint32_t slot) {
-  if (!R.rxn_counts || slot >= R.rc_slots) return;
-  const uint32_t v = (R.rc_volumes > 1 && cvol < R.rc_volumes) ? cvol : 0u;
-  atomicAdd(R.rxn_counts + ((size_t)rep * R.rc_volumes + v) * R.rc_slots + slot, 1ull);
+// The counter of one carried-out reaction (the volume the mover was in; 0
+// without counted volumes), or ~0.
+__device__ __forceinline__ uint64_t reaction_counter(const ReactParams& R, const Proposal& p, uint32_t rep) {
+  uint32_t slot;
+  if (p.rxn & kVolSurfBit) slot = R.rc_vs + (p.rxn & ~kVolSurfBit) * 4 + (p.pathway & kPathwayMask);
+  else if (p.rxn & kUnimolBit) slot = R.rc_unimol + (p.rxn & ~kUnimolBit) * 4 + p.pathway;
+  else slot = p.rxn * 4 + p.pathway;
+  if (!R.rxn_counts || slot >= R.rc_slots) return ~0ull;
+  const uint32_t v = (R.rc_volumes > 1 && p.cvol < R.rc_volumes) ? p.cvol : 0u;
+  return ((uint64_t)rep * R.rc_volumes + v) * R.rc_slots + slot;
+}
+
+// Adds one to counter idx of every lane (~0: none) with one atomic per
+// distinct counter in the wave (a phase's reactions share a few counters).
+__device__ __forceinline__ void wave_add_counters(unsigned long long* __restrict__ base, uint64_t idx) {
+  uint64_t active = __ballot(idx != ~0ull);
+  while (active) {
+    const int leader = __ffsll((long long)active) - 1;
+    const uint64_t li = __shfl((unsigned long long)idx, leader);
+    const uint64_t same = __ballot(idx == li) & active;
+    if (__lane_id() == leader) atomicAdd(base + li, (unsigned long long)__popcll(same));
+    active &= ~same;
+  }
 }
 
 template <typename PosT>
@@ -852,6 +869,8 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
                                                   uint32_t* __restrict__ tiles, uint32_t* __restrict__ tile_id,
                                                   uint32_t* __restrict__ tile_consumed, uint32_t* __restrict__ kept) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  // reaction counts, with every lane of the wave present
+  wave_add_counters(R.rxn_counts, k < n ? reaction_counter(R, prop[sorted_slots[k]], key_replica(keys[k])) : ~0ull);
   if (k >= n) return;
   const Proposal& p = prop[sorted_slots[k]];
   if (p.rxn & kVolSurfBit) {
@@ -863,7 +882,6 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
     const VolSurfRec& vs = R.vs_rxns[p.rxn & ~kVolSurfBit];
     const uint32_t pw = p.pathway & kPathwayMask, drawn = p.pathway >> 8;
     const uint32_t rep = key_replica(keys[k]);
-    count_reaction(R, rep, p.cvol, R.rc_vs + (p.rxn & ~kVolSurfBit) * 4 + pw);
     const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
     const uint32_t ts = p.partner_slot & ~kTileBit;
     const uint32_t wall = R.tile_wall[ts - rep * R.total_tiles];
@@ -938,8 +956,6 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
   const uint32_t np = uni ? R.unimol[p.rxn & ~kUnimolBit].num_products[p.pathway]
                           : R.rxns[p.replica * R.rxn_stride + p.rxn].num_products[p.pathway];
   const uint32_t rep = key_replica(keys[k]);
-  count_reaction(R, rep, p.cvol,
-                 uni ? R.rc_unimol + (p.rxn & ~kUnimolBit) * 4 + p.pathway : p.rxn * 4 + p.pathway);
   const uint32_t base = hdr[kHdrNumOut];
   const uint32_t first_id = next_id[rep] + pre[k] - pre[head[rep]];
   const bool moves_now = cmp_lt_d(p.time, (double)R.iteration + 1.0, 1e-12);
