@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 6
+#define MCG_ABI_VERSION 7
 
 typedef struct mcg_ctx mcg_ctx;
 
@@ -87,7 +87,14 @@ typedef struct mcg_config {
 #define MCG_FLAG_DRAIN_DEFER 8u
 
 /* BNG::Species fields on this path (space_step/time_step from
- * Species::update_space_and_time_step, MCell3 src/mcell_species.c:237-300). */
+ * Species::update_space_and_time_step, MCell3 src/mcell_species.c:237-300).
+ * time_step is 1 for the global step; a custom step lies in [1/8, 1e12]
+ * (MCG_E_INVALID otherwise: a molecule diffuses at most eight times in one
+ * iteration).  Steps below one iteration diffuse the molecule again within
+ * the iteration (diffuse_single_molecule, diffuse_react_event.cpp:294-319);
+ * steps above it start short and lengthen with the molecule's age until it
+ * is mature (get_max_time, :164-198).  Contexts with such species take the
+ * reactive pipeline. */
 typedef struct mcg_species {
   double space_step;   /* internal length units */
   double time_step;    /* internal time units */
@@ -104,9 +111,10 @@ typedef struct mcg_species {
  * as test_bimolecular and outcome_bimolecular use it, rxn_utils.inl:336-414,
  * diffuse_react_event.cpp:1833-1895).  Probabilities are per collision,
  * cumulative over pathways (RxnClass::cum_probs; the last entry is
- * max_fixed_p).  Every product is a new volume molecule created at the
- * collision point; both reactants are destroyed (classes that keep a reactant
- * are refused with MCG_E_UNSUPPORTED). */
+ * max_fixed_p).  A product of a reactant's species keeps that reactant
+ * (outcome_products_random, diffuse_react_event.cpp:2551-2565,2686-2727);
+ * every other product is a new volume molecule created at the collision
+ * point, and the reactants not kept are destroyed. */
 typedef struct mcg_bimol_rxn {
   uint32_t reactant_a, reactant_b; /* species ids */
   uint32_t num_pathways;           /* 1..4 */
@@ -302,15 +310,22 @@ int mcg_count_species_noalloc(mcg_ctx* ctx, uint64_t* counts);
  * at (default clock); unimol_times are absolute (TIME_INVALID = -256: no
  * lifetime drawn yet or no unimolecular class; default -256); flags are
  * MCG_MOLF_* (default MCG_MOLF_SCHEDULE_UNIMOL, a newly released molecule).
- * Molecules given a diffusion time other than the clock make a context
- * without reaction classes take the time-aware reactive pipeline.
- * counted_volumes (Molecule::v.counted_volume_index, molecule.h:76) may be
- * NULL (0, outside every counted object); with mcg_set_counted_volumes they
- * follow the molecules through transparent walls and into their products. */
+ * A diffusion time past clock + 1 belongs to a species whose time step
+ * exceeds one iteration.  Molecules given a diffusion time other than the
+ * clock make a context without reaction classes take the time-aware
+ * reactive pipeline.  counted_volumes (Molecule::v.counted_volume_index,
+ * molecule.h:76) may be NULL (0, outside every counted object); with
+ * mcg_set_counted_volumes they follow the molecules through transparent
+ * walls and into their products.  birthdays (Molecule::birthday, absolute,
+ * not after the diffusion time) matter only for species whose time step
+ * exceeds one iteration; NULL or TIME_INVALID entries give the clock (a
+ * molecule released in this iteration, molecule.h:139-141). */
 #define MCG_MOLF_SCHEDULE_UNIMOL 1u  /* MOLECULE_FLAG_SCHEDULE_UNIMOL_RXN: lifetime not drawn yet */
+#define MCG_MOLF_MATURE 2u           /* MOLECULE_FLAG_MATURE: moves with its species' full time step */
 int mcg_add_volume_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids, const uint32_t* species,
                              const double* positions, const double* diffusion_times, const double* unimol_times,
-                             const uint32_t* flags, const uint32_t* counted_volumes, uint64_t count);
+                             const uint32_t* flags, const uint32_t* counted_volumes, const double* birthdays,
+                             uint64_t count);
 
 /* Counted volumes (GeometryObject::counted_volume_index_inside / _outside,
  * geometry.h:102-107): for each wall (mcg_set_geometry order) the index of
@@ -380,10 +395,11 @@ int mcg_num_molecules(mcg_ctx* ctx, uint64_t* count);
  * changes every iteration when reactions are on), with the time fields
  * mcg_add_volume_molecules takes: an upload of what this returns into a new
  * context (start_iteration = this clock, next molecule ids copied) continues
- * the run bit for bit. */
+ * the run bit for bit.  Birthdays are TIME_INVALID where the context keeps
+ * none (no species' time step exceeds one iteration). */
 int mcg_get_volume_molecules(mcg_ctx* ctx, uint32_t* ids, uint32_t* species, uint32_t* replica, double* positions,
                              uint32_t* subparts, double* diffusion_times, double* unimol_times, uint32_t* flags,
-                             uint32_t* counted_volumes, uint64_t capacity, uint64_t* count);
+                             uint32_t* counted_volumes, double* birthdays, uint64_t capacity, uint64_t* count);
 
 /* Parity/inspection: wall constants (16 doubles per wall: normal, unit_u,
  * unit_v, distance_to_origin, uv_vert1_u, uv_vert2.u, uv_vert2.v, area, 0, 0)

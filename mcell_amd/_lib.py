@@ -43,6 +43,7 @@ EXPORTED = [
 ]
 
 MCG_MOLF_SCHEDULE_UNIMOL = 1
+MCG_MOLF_MATURE = 2
 TIME_INVALID = -256.0
 
 
@@ -174,11 +175,11 @@ def load():
     lib.mcg_destroy.argtypes = [P]
     lib.mcg_set_geometry.argtypes = [P, P, ctypes.c_uint32, P, ctypes.c_uint32, P]
     lib.mcg_set_species.argtypes = [P, ctypes.POINTER(McgSpecies), ctypes.c_uint32]
-    lib.mcg_add_volume_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, P, ctypes.c_uint64]
+    lib.mcg_add_volume_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, P, P, ctypes.c_uint64]
     lib.mcg_step.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(McgStepStats)]
     lib.mcg_sync.argtypes = [P]
     lib.mcg_num_molecules.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
-    lib.mcg_get_volume_molecules.argtypes = [P, P, P, P, P, P, P, P, P, P, ctypes.c_uint64,
+    lib.mcg_get_volume_molecules.argtypes = [P, P, P, P, P, P, P, P, P, P, P, ctypes.c_uint64,
                                              ctypes.POINTER(ctypes.c_uint64)]
     lib.mcg_set_counted_volumes.argtypes = [P, ctypes.c_uint32, P, P]
     lib.mcg_count_species_by_volume.argtypes = [P, P]

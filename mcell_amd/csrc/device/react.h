@@ -122,6 +122,7 @@ struct Snapshot {
   const double* __restrict__ toff;        // diffusion-time offsets of the snapshot's molecules
   const double* __restrict__ utime;       // unimolecular reaction times (absolute) or kTimeInvalid
   const uint32_t* __restrict__ cvol;      // counted volume indices, or null
+  const double* __restrict__ birth;       // birthdays (absolute), or null
 };
 
 // A small set of subpartitions (the reference's SubpartIndicesSet), stored

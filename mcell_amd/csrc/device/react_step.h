@@ -61,6 +61,11 @@ struct ReactParams {
   const VolSurfRec* __restrict__ vs_rxns;
   const int32_t* __restrict__ pair_vs;   // [vol species * ns + surf species], -1 = none
   unsigned long long* ctr;               // CTR_* statistics of the call
+  // molecules whose step ended before the iteration's end without a due
+  // unimolecular reaction (time steps below one iteration): their slots,
+  // for the next phase's moves
+  uint32_t* __restrict__ clist;
+  uint32_t* __restrict__ ccount;
 };
 
 // xyz2grid_tile_index (grid_utils.inl:48-118) for a point on the wall.
@@ -253,6 +258,9 @@ __device__ __forceinline__ bool next_collision(const ReactParams& R, const Snaps
   return found;
 }
 
+__device__ __forceinline__ bool cmp_eq_d(double a, double b, double eps) { return fabs(a - b) < eps; }
+__device__ __forceinline__ bool cmp_lt_d(double a, double b, double eps) { return a < b && !cmp_eq_d(a, b, eps); }
+
 // Result of one mover's step.
 struct MoverEnd {
   D3 pos;          // end position; with a proposal, the start of the segment it was found in
@@ -264,6 +272,8 @@ struct MoverEnd {
   bool exd;        // deferred: a collision needs exact_disk (the caller repeats the step with it)
   uint32_t due;    // 0, or 1 + the stream pass that draws the unimolecular pathway
   uint32_t cvol;   // counted volume (in: at the start; out: at the end or the reaction)
+  bool cont;       // the molecule diffuses again in this iteration (its next stream pass)
+  bool mature;     // the step set MOLECULE_FLAG_MATURE
 };
 
 // exact_disk is exactly 1 unless an entry of the collision point's
@@ -308,7 +318,9 @@ __device__ __forceinline__ double start_unimol_time(const UnimolRec* __restrict_
 //
 // The molecule's stream is pass n of its n-th step in the iteration (the
 // tag's pass bits; MF_SKIP_DRAW: the pathway draw of a unimolecular
-// reaction that kept it was the pass's first draw).  pass_through: the
+// reaction that kept it was the pass's first draw).  birth: the molecule's
+// birthday (read only while a species with a time step above one iteration
+// has not matured).  pass_through: the
 // step's successful bimolecular tests already carried out (the mover
 // stayed); they are passed as the reference's ray trace goes on after them
 // (collide_and_react_with_vol_mol returns false for a kept mover,
@@ -316,8 +328,9 @@ __device__ __forceinline__ double start_unimol_time(const UnimolRec* __restrict_
 template <typename PosT, int kPoolN>
 __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot<PosT>& S, const SpeciesRec& sp,
                                             const ZigguratTables& T, D3 pos, uint32_t idv, uint32_t tag, double toff,
-                                            double utime, double barrier, uint32_t pass_through, uint32_t born_tile,
-                                            Proposal& prop, MoverEnd& out, uint32_t& err, uint32_t* set_lds) {
+                                            double utime, double birth, double barrier, uint32_t pass_through,
+                                            uint32_t born_tile, Proposal& prop, MoverEnd& out, uint32_t& err,
+                                            uint32_t* set_lds) {
   // The full step tests its candidate walls in FP64 in both forms: after the
   // entry-box pre-test a mover here meets few walls, and the FP32 filter in
   // front of them cost more than it saved.  In the FP32 form the end point
@@ -330,9 +343,12 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   if (tag_flags(tag) & MF_SKIP_DRAW) (void)rng.u32();
   uint32_t succ = 0;
   double elapsed = (double)R.iteration + toff;
+  const double t_start = elapsed;
   out.near_plane = kF32;
   out.absorbed = false;
   out.exd = false;
+  out.cont = false;
+  out.mature = false;
   out.due = 0;
   out.t_steps = 0;
   out.n_reflect = 0;
@@ -346,6 +362,12 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   // get_max_time (diffuse_react_event.cpp:164-198)
   double max_time = barrier - toff;
   if (out.utime != kTimeInvalid && out.utime < elapsed + max_time) max_time = out.utime - elapsed;
+  if (!(tag_flags(tag) & MF_MATURE) && sp.time_step > 1.0) {
+    // a new molecule with a long time step lengthens its steps gradually
+    const double age_dependent_max_time = 1.0 + 0.2 * (elapsed - birth);
+    if (max_time > age_dependent_max_time) max_time = age_dependent_max_time;
+    if (age_dependent_max_time > sp.time_step) out.mature = true;
+  }
   double t_steps;
   D3 disp;
   const double r_rate_factor = clipped_displacement(rng, T, sp.space_step, sp.time_step, max_time, t_steps, disp);
@@ -556,9 +578,22 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
   if (seg == kMaxSegments) err |= ERR_SEGMENTS;
   if (!in_partition(R.P, pos)) err |= ERR_ESCAPED;
   out.pos = pos;
-  // the reaction is due within this iteration: the reference diffuses the
-  // molecule again right away and reacts (diffuse_react_event.cpp:298-306)
-  if (out.utime != kTimeInvalid && out.utime < (double)R.iteration + 1.0) out.due = pass + 2;
+  // diffuse_single_molecule's rescheduling (diffuse_react_event.cpp:294-
+  // 319): diffusion_time += max_time; the molecule is called again in this
+  // iteration when its unimolecular reaction is due before the iteration's
+  // end or its diffusion time is still before it; that call reacts first
+  // when the reaction time is not after the new diffusion time
+  // (:213-221), else it diffuses on
+  const double t_new = t_start + out.t_steps;
+  const double t_end = (double)R.iteration + 1.0;
+  if (!(sp.space_step > 0)) {
+    // a molecule that cannot diffuse is scheduled at its reaction time
+    // (:320-333)
+    if (out.utime != kTimeInvalid && out.utime < t_end) out.due = pass + 2;
+  } else if ((out.utime != kTimeInvalid && out.utime < t_end) || cmp_lt_d(t_new, t_end, 1e-12)) {
+    if (out.utime != kTimeInvalid && out.utime <= t_new) out.due = pass + 2;
+    else out.cont = true;
+  }
   return false;
 }
 

@@ -86,6 +86,9 @@ enum : uint32_t {
   MF_PASS_MASK = 7u << 3,
   MF_SKIP_DRAW = 1u << 6,
   MF_STEP_FLAGS = MF_SCHEDULE_UNIMOL | MF_PASS_MASK | MF_SKIP_DRAW,  // cleared when a step ends
+  // MOLECULE_FLAG_MATURE (molecule.h:34): a molecule of a species with a
+  // time step above one iteration moves with its full step
+  MF_MATURE = 1u << 7,
 };
 constexpr double kTimeInvalid = -256.0;   // TIME_INVALID (defines.h:144)
 constexpr double kTimeForever = 1e20;     // TIME_FOREVER (defines.h:145)
@@ -121,7 +124,7 @@ enum : uint32_t {
   ERR_QUEUE = 1u << 8,       // proposal / product capacity exceeded
   ERR_ROUND32 = 1u << 9,     // FP32 form: a point could not be rounded onto its side of every nearby plane
   ERR_COLLISIONS = 1u << 10, // more molecule collisions in one ray segment than the step's guard allows
-  ERR_PASSES = 1u << 11,     // a molecule that stays after its reactions diffused more often in one iteration than its tag counts
+  ERR_PASSES = 1u << 11,     // a molecule diffused more often in one iteration than its tag counts (8 calls)
 };
 
 // per-call statistics counters (SimulationStats::mol_wall_reflections and

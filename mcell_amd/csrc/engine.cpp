@@ -219,7 +219,7 @@ int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
   std::vector<SpeciesRec> h(ns);
   for (uint32_t i = 0; i < ns; ++i) {
     if (!(sp[i].space_step >= 0)) return fail(MCG_E_INVALID, "negative space step");
-    if (!(sp[i].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
+    if (!valid_time_step(sp[i].time_step)) return fail(MCG_E_INVALID, kTimeStepRange);
     h[i].space_step = sp[i].space_step;
     h[i].time_step = sp[i].time_step;
     h[i].flags = sp[i].flags;
@@ -235,16 +235,17 @@ int mcg_set_species(mcg_ctx* c, const mcg_species* sp, uint32_t ns) {
 
 int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, const uint32_t* species,
                              const double* pos, const double* dtimes, const double* utimes, const uint32_t* flags,
-                             const uint32_t* cvols, uint64_t count) {
+                             const uint32_t* cvols, const double* birthdays, uint64_t count) {
   if (!c || (count && (!ids || !species || !pos))) return fail(MCG_E_INVALID, "null argument");
   MCG_ALIVE(c);
   if (replica >= c->cfg.num_replicas) return fail(MCG_E_INVALID, "replica out of range");
   if (c->n + count > c->cfg.molecule_capacity) return fail(MCG_E_CAPACITY, "molecule capacity exceeded");
   const PartitionParams& P = c->P;
-  std::vector<double> hx(count), hy(count), hz(count), htoff(count, 0.0), hut(count, kTimeInvalid);
+  const double clock = (double)c->clock;
+  std::vector<double> hx(count), hy(count), hz(count), htoff(count, 0.0), hut(count, kTimeInvalid),
+      hb(count, clock);
   std::vector<uint32_t> ht(count);
   bool any_toff = false, any_time = false;
-  const double clock = (double)c->clock;
   for (uint64_t i = 0; i < count; ++i) {
     const double* p = pos + 3 * i;
     if (!(p[0] >= P.ox && p[1] >= P.oy && p[2] >= P.oz && p[0] < P.qx && p[1] < P.qy && p[2] < P.qz))
@@ -253,15 +254,21 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
     hx[i] = p[0]; hy[i] = p[1]; hz[i] = p[2];
     uint32_t f = MF_SCHEDULE_UNIMOL;
     if (flags) {
-      if (flags[i] & ~MCG_MOLF_SCHEDULE_UNIMOL) return fail(MCG_E_INVALID, "unknown molecule flag");
+      if (flags[i] & ~(MCG_MOLF_SCHEDULE_UNIMOL | MCG_MOLF_MATURE)) return fail(MCG_E_INVALID, "unknown molecule flag");
       f = (flags[i] & MCG_MOLF_SCHEDULE_UNIMOL) ? MF_SCHEDULE_UNIMOL : 0u;
       any_time |= f == 0;
+      if (flags[i] & MCG_MOLF_MATURE) f |= MF_MATURE;
+    }
+    if (birthdays && birthdays[i] != kTimeInvalid) {
+      // Molecule::birthday: not after the molecule's diffusion time
+      if (!(birthdays[i] <= (dtimes ? dtimes[i] : clock))) return fail(MCG_E_INVALID, "birthday after the diffusion time");
+      hb[i] = birthdays[i];
     }
     if (dtimes) {
       // Molecule::diffusion_time as an offset past the clock (exact: both lie
       // within one small range of whole iterations)
       if (!(dtimes[i] >= clock)) return fail(MCG_E_INVALID, "diffusion time before the engine's clock");
-      htoff[i] = dtimes[i] - clock;
+      htoff[i] = dtimes[i] >= kTimeForever ? HUGE_VAL : dtimes[i] - clock;
       if (htoff[i] != 0.0) {
         any_toff = true;
         f |= MF_TOFF;
@@ -275,9 +282,11 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
       return fail(MCG_E_INVALID, "counted volume index out of range");
     ht[i] = make_tag(species[i], f, replica);
   }
-  // offsets and drawn lifetimes live in the reactive path's arrays
+  // offsets and drawn lifetimes live in the reactive path's arrays, and
+  // so do birthdays (species time steps other than one iteration)
   if (any_toff || any_time)
     if (int st = ensure_react_buffers(c)) return st;
+  if (int st = ensure_step_buffers(c)) return st;
   if (c->rx.on && count) {
     // the device holds the id counters once steps have run
     if (!c->rx.next_id_dirty)
@@ -292,6 +301,9 @@ int mcg_add_volume_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, 
       if (cvols) std::copy(cvols, cvols + count, hcv.begin());
       HIP_TRY(hipMemcpyAsync(c->rx.cvol.p + c->n, hcv.data(), count * 4, hipMemcpyHostToDevice, c->stream));
     }
+    if (c->rx.birth.p)
+      HIP_TRY(hipMemcpyAsync(c->rx.birth.p + c->n, hb.data(), count * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
   if (c->rx.next_id_host.size() != c->cfg.num_replicas) c->rx.next_id_host.assign(c->cfg.num_replicas, 0);
@@ -444,6 +456,8 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, doub
     return fail(MCG_E_INVALID, "first_iteration " + std::to_string(first_iteration) + " is not the engine's clock " +
                                    std::to_string(c->clock));
   if (stats) std::memset(stats, 0, sizeof(*stats));
+  // species time steps other than one iteration take the reactive path
+  if (int e = ensure_step_buffers(c)) return e;
   c->stepped = true;
   int st = c->rx.on ? react_step(c, first_iteration, num_iterations, time_up_to_next_barrier, stats)
                     : free_step(c, first_iteration, num_iterations, stats);
@@ -505,11 +519,11 @@ int mcg_num_molecules(mcg_ctx* c, uint64_t* count) {
 
 int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, double* pos,
                              uint32_t* subparts, double* dtimes, double* utimes, uint32_t* flags, uint32_t* cvols,
-                             uint64_t capacity, uint64_t* count) {
+                             double* birthdays, uint64_t capacity, uint64_t* count) {
   if (!c) return fail(MCG_E_INVALID, "null context");
   MCG_ALIVE(c);
   const uint64_t n = c->n;
-  std::vector<double> hx(n), hy(n), hz(n), htoff, hut;
+  std::vector<double> hx(n), hy(n), hz(n), htoff, hut, hb;
   std::vector<uint32_t> ht(n), hid(n), hcv;
   if (n && c->fp32()) {
     std::vector<float> fx(n), fy(n), fz(n);
@@ -542,6 +556,10 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
       hcv.resize(n);
       HIP_TRY(hipMemcpyAsync(hcv.data(), c->rx.cvol.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     }
+    if (birthdays && c->rx.birth.p) {
+      hb.resize(n);
+      HIP_TRY(hipMemcpyAsync(hb.data(), c->rx.birth.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
   uint64_t live = 0;
@@ -563,10 +581,15 @@ int mcg_get_volume_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint3
           iz = (int)((hz[i] - P.oz) * P.sp_rcp);
       subparts[k] = (uint32_t)ix + (uint32_t)iy * P.n + (uint32_t)iz * P.n2;
     }
-    if (dtimes) dtimes[k] = ((f & MF_TOFF) && !htoff.empty()) ? clock + htoff[i] : clock;
+    if (dtimes) {
+      // an infinite offset: scheduled never (TIME_FOREVER)
+      const double off = ((f & MF_TOFF) && !htoff.empty()) ? htoff[i] : 0.0;
+      dtimes[k] = std::isinf(off) ? kTimeForever : clock + off;
+    }
     if (utimes) utimes[k] = hut.empty() ? kTimeInvalid : hut[i];
-    if (flags) flags[k] = (f & MF_SCHEDULE_UNIMOL) ? MCG_MOLF_SCHEDULE_UNIMOL : 0u;
+    if (flags) flags[k] = ((f & MF_SCHEDULE_UNIMOL) ? MCG_MOLF_SCHEDULE_UNIMOL : 0u) | ((f & MF_MATURE) ? MCG_MOLF_MATURE : 0u);
     if (cvols) cvols[k] = hcv.empty() ? 0u : hcv[i];
+    if (birthdays) birthdays[k] = hb.empty() ? kTimeInvalid : hb[i];
     ++k;
   }
   return MCG_OK;

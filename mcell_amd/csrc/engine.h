@@ -83,6 +83,12 @@ struct ReactState {
   DevBuf<unsigned long long> rxn_counts;
   uint32_t rc_slots = 0, rc_unimol = 0, rc_vs = 0, rc_volumes = 0;
   DevBuf<uint32_t> tile_wall;  // per tile of one replica: its wall
+  // species time steps other than one iteration: birthdays per molecule
+  // and snapshot slot (Molecule::birthday, kept while a species' time step
+  // exceeds one iteration: get_max_time's maturity rule), and the list of
+  // movers that diffuse again in the same iteration
+  DevBuf<double> birth, sbirth;
+  DevBuf<uint32_t> clist;
   DevBuf<uint32_t> plist, alist, alist_sorted, rlist0, rlist1, mlist, ulist;
   DevBuf<uint64_t> keys_a, keys_b;
   DevBuf<uint32_t> cnt, pre, head, next_id;
@@ -165,6 +171,16 @@ struct mcg_ctx {
   uint32_t num_species = 0;
   bool per_replica = false;
   mcg::SpeciesTable species_table() const { return {species.p, per_replica ? num_species : 0u}; }
+  // a species (of any replica) whose time step is not one iteration / is
+  // longer than one iteration
+  bool custom_steps() const { return any_step([](double t) { return !(t == 1.0); }); }
+  bool long_steps() const { return any_step([](double t) { return t > 1.0; }); }
+  template <typename F>
+  bool any_step(F f) const {
+    for (const mcg::SpeciesRec& s : per_replica ? species_replicas : species_host)
+      if (f(s.time_step)) return true;
+    return false;
+  }
 
   // molecules (structure of arrays): FP64 or FP32 positions by precision.
   // n counts array slots; with reactions on, consumed molecules stay as
@@ -197,11 +213,19 @@ struct mcg_ctx {
 };
 
 namespace mcg {
+// Species time steps (internal units): a molecule diffuses at most eight
+// times in one iteration (its stream passes), so steps are at least 1/8.
+inline bool valid_time_step(double t) { return t >= 0.125 && t <= 1e12; }
+constexpr const char* kTimeStepRange = "species time step outside [1/8, 1e12] iterations";
 // The reactive iteration loop (engine_react.cpp).
 int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, double time_up_to_next_barrier,
                mcg_step_stats* stats);
 // Allocates the reactive path's buffers once (reaction classes of any kind).
 int ensure_react_buffers(mcg_ctx* c);
+// Species time steps other than one iteration run on the reactive path;
+// long ones keep birthdays (allocated here, the existing molecules' set to
+// the clock).
+int ensure_step_buffers(mcg_ctx* c);
 // Live molecules per (replica, species) into counts (size num_replicas *
 // num_species); surface molecules too when with_surface.
 int count_species(mcg_ctx* c, uint64_t* counts, bool with_surface = true);

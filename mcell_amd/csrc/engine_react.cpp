@@ -72,9 +72,10 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   // arrays are neither copied nor read
   const bool unimol = std::any_of(r.unimol.begin(), r.unimol.end(), [](const UnimolRec& u) { return u.num_pathways > 0; });
   const bool cv = r.num_volumes > 0;
-  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p, unimol ? r.utime.p : nullptr, cv ? r.cvol.p : nullptr};
+  p.M = RMolView<PosT>{x, y, z, c->id.p, c->tag.p, r.toff.p, unimol ? r.utime.p : nullptr, cv ? r.cvol.p : nullptr,
+                       r.birth.p};
   p.Sm = RMolView<PosT>{sx,      sy,      sz, r.sid.p, r.stag.p, r.stoff.p, unimol ? r.sutime.p : nullptr,
-                        cv ? r.scvol.p : nullptr};
+                        cv ? r.scvol.p : nullptr, r.birth.p ? r.sbirth.p : nullptr};
   p.R.cv_in = cv ? r.cv_in.p : nullptr;
   p.R.cv_out = cv ? r.cv_out.p : nullptr;
   p.R.rxn_counts = r.rxn_counts.p;
@@ -114,6 +115,9 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.rlist1 = r.rlist1.p;
   p.mlist = r.mlist.p;
   p.ulist = r.ulist.p;
+  p.clist = r.clist.p;
+  p.R.clist = r.clist.p;
+  p.R.ccount = r.hdr.p + kHdrContinue;
   p.keys_a = r.keys_a.p;
   p.keys_b = r.keys_b.p;
   p.cnt = r.cnt.p;
@@ -224,17 +228,27 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
           st = read_hdr(c);
           if (st) return st;
         }
-        const uint32_t na = h[kHdrAccepted];
-        if (na == 0) break;
-        reactions += na;
-        const uint64_t log_base = std::min<uint64_t>(r.log_count, 0xFFFFFFFFull);
-        HIP_TRY(p.products(c->stream, na, (uint32_t)cap, r.log.p, (uint32_t)log_base, (uint32_t)r.log_cap,
-                           snapshot_src, err));
-        r.log_count += na;
-        HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
-        st = read_hdr(c);
-        if (st) return st;
-        const uint32_t nm = h[kHdrProductMovers];
+        // the next phase's movers: the products that move in this
+        // iteration, then the movers that diffuse again (the phase's moves
+        // are over, so the header's count of those is final)
+        const uint32_t na = h[kHdrAccepted], nc = h[kHdrContinue];
+        if (na == 0 && nc == 0) break;
+        HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrProductMovers, 0u, 1u));
+        if (na) {
+          reactions += na;
+          const uint64_t log_base = std::min<uint64_t>(r.log_count, 0xFFFFFFFFull);
+          HIP_TRY(p.products(c->stream, na, (uint32_t)cap, r.log.p, (uint32_t)log_base, (uint32_t)r.log_cap,
+                             snapshot_src, err));
+          r.log_count += na;
+          HIP_TRY(launch_fill_u32(c->stream, p.hdr + kHdrAccepted, 0u, 1u));
+        }
+        HIP_TRY(p.continuations(c->stream, nc));
+        uint32_t nm = nc;
+        if (na) {
+          st = read_hdr(c);
+          if (st) return st;
+          nm = h[kHdrProductMovers];
+        }
         if (nm == 0) break;
         snapshot_src = false;
         phase_movers = nm;
@@ -296,7 +310,7 @@ int ensure_react_buffers(mcg_ctx* c) {
              r.sid.alloc(cap) || r.stag.alloc(cap) || r.stoff.alloc(cap) || r.key.alloc(cap) ||
              r.rank.alloc(cap) || r.consumed.alloc(cap) || r.owner.alloc(cap) || r.prop.alloc(cap) ||
              r.plist.alloc(cap) || r.alist.alloc(cap) || r.alist_sorted.alloc(cap) || r.rlist0.alloc(cap) ||
-             r.rlist1.alloc(cap) || r.mlist.alloc(cap) || r.ulist.alloc(cap) || r.keys_a.alloc(cap) ||
+             r.rlist1.alloc(cap) || r.mlist.alloc(cap) || r.ulist.alloc(cap) || r.clist.alloc(cap) || r.keys_a.alloc(cap) ||
              r.keys_b.alloc(cap) || r.cnt.alloc(cap + 1) || r.pre.alloc(cap + 1) ||
              r.head.alloc(c->cfg.num_replicas) || r.next_id.alloc(c->cfg.num_replicas) || r.hdr.alloc(kHdrWords) ||
              r.unimol_d.alloc(ns) || r.rxns_d.alloc(1);
@@ -317,6 +331,22 @@ int ensure_react_buffers(mcg_ctx* c) {
   r.next_id_dirty = true;
   r.grid_dirty = true;
   r.on = true;
+  return MCG_OK;
+}
+
+int ensure_step_buffers(mcg_ctx* c) {
+  if (!c->custom_steps()) return MCG_OK;
+  if (int st = ensure_react_buffers(c)) return st;
+  ReactState& r = c->rx;
+  if (c->long_steps() && !r.birth.p) {
+    const size_t cap = c->cfg.molecule_capacity;
+    if (r.birth.alloc(cap) || r.sbirth.alloc(cap)) return fail(MCG_E_NOMEM, "birthday buffers");
+    // molecules added so far were released at the clock (molecule.h:139-141)
+    if (c->n) {
+      std::vector<double> b(c->n, (double)c->clock);
+      HIP_TRY(hipMemcpy(r.birth.p, b.data(), c->n * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
   return MCG_OK;
 }
 
@@ -501,8 +531,6 @@ int mcg_set_bimol_rxns(mcg_ctx* c, const mcg_bimol_rxn* rxns, uint32_t n) {
     sp[x.reactant_a].flags |= SPF_CAN_VOLVOL;
     sp[x.reactant_b].flags |= SPF_CAN_VOLVOL;
   }
-  for (uint32_t s = 0; s < ns; ++s)
-    if (!(sp[s].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
   ReactState& r = c->rx;
   int st = ensure_react_buffers(c);
   if (st) return st;
@@ -546,9 +574,6 @@ int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
     }
     u[x.reactant].num_pathways = x.num_pathways;
   }
-  for (uint32_t s = 0; s < ns; ++s)
-    if (u[s].num_pathways && !(c->species_host[s].time_step == 1.0))
-      return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
   int st = ensure_react_buffers(c);
   if (st) return st;
   if (any_keep && (st = ensure_keep_buffers(c))) return st;
@@ -568,7 +593,7 @@ int mcg_set_replica_params(mcg_ctx* c, uint32_t replica, const mcg_species* spec
   if (nrx && !rxns) return fail(MCG_E_INVALID, "the replica's reaction classes are required");
   for (uint32_t s = 0; s < ns; ++s) {
     if (!(species[s].space_step >= 0)) return fail(MCG_E_INVALID, "negative space step");
-    if (!(species[s].time_step == 1.0)) return fail(MCG_E_UNSUPPORTED, "custom time steps are not on this path yet");
+    if (!valid_time_step(species[s].time_step)) return fail(MCG_E_INVALID, kTimeStepRange);
   }
   std::vector<RxnRec> rep_rx(nrx);
   for (size_t i = 0; i < nrx; ++i) {
@@ -613,7 +638,7 @@ int mcg_set_replica_params(mcg_ctx* c, uint32_t replica, const mcg_species* spec
   if (nrx)
     HIP_TRY(hipMemcpy(r.rxns_d.p, r.rxns_replicas.data(), r.rxns_replicas.size() * sizeof(RxnRec),
                       hipMemcpyHostToDevice));
-  return MCG_OK;
+  return ensure_step_buffers(c);
 }
 
 int mcg_set_rxn_log(mcg_ctx* c, uint64_t capacity) {

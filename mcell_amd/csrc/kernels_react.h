@@ -22,6 +22,7 @@ struct RMolView {
   double* toff;
   double* utime;  // unimolecular reaction time (absolute) or kTimeInvalid; null without unimolecular classes
   uint32_t* cvol;  // counted volume index (Molecule::v.counted_volume_index); null without counted volumes
+  double* birth;   // Molecule::birthday (absolute); null unless a species' time step exceeds one iteration
 };
 
 // One carried-out (or, in the snapshot mode, proposed) reaction: 64 bytes,
@@ -49,6 +50,7 @@ enum : uint32_t {
   kHdrUnimolDue = 8,      // molecules whose unimolecular reaction falls in this phase
   kHdrErr = 9,            // error bits (ERR_*) of the call's kernels
   kHdrExd = 10,           // movers the full step deferred to its exact_disk pass
+  kHdrContinue = 11,      // movers of the phase that diffuse again in this iteration
   kHdrWords = 12,
 };
 
@@ -65,6 +67,7 @@ struct ReactPipeline {
   uint32_t *tiles, *tile_id, *tile_owner, *tile_consumed;  // surface molecules per (replica, tile)
   Proposal* prop;
   uint32_t *plist, *alist, *alist_sorted, *rlist0, *rlist1, *mlist, *ulist;
+  uint32_t* clist;  // slots of the phase's movers that diffuse again (kHdrContinue of them)
   uint64_t *keys_a, *keys_b;
   uint32_t *cnt, *pre, *head, *next_id;
   uint32_t* hdr;
@@ -81,6 +84,9 @@ struct ReactPipeline {
   hipError_t fire_unimol(hipStream_t s, uint32_t max_due);
   hipError_t products(hipStream_t s, uint32_t n_acc, uint32_t capacity, RxnEvent* log, uint32_t log_base,
                       uint32_t log_cap, bool first_phase, uint32_t* err);
+  // appends the phase's continuing movers to the next phase's mover list
+  // (after its products) and clears their count
+  hipError_t continuations(hipStream_t s, uint32_t max_cont);
   hipError_t log_proposals(hipStream_t s, uint32_t max_props, uint32_t iteration, RxnEvent* log, uint32_t* log_count,
                            uint32_t log_cap);
   hipError_t count_species(hipStream_t s, uint32_t n, uint32_t num_species, uint32_t num_replicas, uint64_t* counts);

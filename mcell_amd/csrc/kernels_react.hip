@@ -50,9 +50,6 @@ namespace mcg {
 
 static __constant__ ZigguratTables c_zig_r = MCG_ZIGGURAT_TABLES_INIT;
 
-__device__ __forceinline__ bool cmp_eq_d(double a, double b, double eps) { return fabs(a - b) < eps; }
-__device__ __forceinline__ bool cmp_lt_d(double a, double b, double eps) { return a < b && !cmp_eq_d(a, b, eps); }
-
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -118,6 +115,7 @@ __global__ __launch_bounds__(256) void k_scatter(RMolView<PosT> m, uint32_t n, c
   if (tag_flags(tag) & MF_TOFF) s.toff[j] = m.toff[i];
   if (s.utime) s.utime[j] = m.utime[i];
   if (s.cvol) s.cvol[j] = m.cvol[i];
+  if (s.birth) s.birth[j] = m.birth[i];
 }
 
 // ----------------------------------------------------------------- moves --
@@ -156,6 +154,14 @@ __device__ __forceinline__ double next_toff(uint32_t iteration, double toff, dou
   const double r = rint(t);
   if (cmp_eq_d(t, r, 1e-6)) t = r;
   return t - (double)(iteration + 1);
+}
+
+// A molecule that cannot diffuse is scheduled at its unimolecular reaction
+// time, or never (TIME_FOREVER: an infinite offset) without one
+// (diffuse_single_molecule, diffuse_react_event.cpp:320-333).  Returns the
+// offset from this iteration's start.
+__device__ __forceinline__ double immobile_toff(uint32_t iteration, double utime) {
+  return utime != kTimeInvalid ? utime - (double)iteration : __builtin_inf();
 }
 
 // Partition::get_molecules_ready_for_diffusion (partition.h:232-246): the
@@ -239,6 +245,7 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   const uint32_t idv = src.id[j];
   const double toff = load_toff(src.toff, tag, j);
   const double utime = src.utime ? src.utime[j] : kTimeInvalid;
+  const double birth = src.birth ? src.birth[j] : 0.0;
   const SpeciesRec sp = species.of(tag);
   D3 pos = {(double)src.x[j], (double)src.y[j], (double)src.z[j]};
   uint32_t err = 0;
@@ -253,14 +260,16 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   e.n_reflect = 0;
   e.n_cross = 0;
   e.cvol = src.cvol ? src.cvol[j] : kInvalid;
+  e.cont = false;
+  e.mature = false;
   Proposal p;
   bool proposed = false;
   const bool moves = sp.space_step > 0 || R.unimol[tag_species(tag)].num_pathways > 0;
   const uint32_t pass_through = R.kept ? R.kept[j] : 0u;
   const uint32_t born = R.born ? R.born[j] : 0u;
   if (moves)
-    proposed = react_mover<PosT, kPoolN>(R, S, sp, T, pos, idv, tag, toff, utime, barrier, pass_through, born, p, e,
-                                         err, set_lds);
+    proposed = react_mover<PosT, kPoolN>(R, S, sp, T, pos, idv, tag, toff, utime, birth, barrier, pass_through, born,
+                                         p, e, err, set_lds);
   deferred = e.exd;
   if (deferred) {
     due = false;
@@ -299,11 +308,27 @@ __device__ __forceinline__ bool move_one(const ReactParams& R, const Snapshot<Po
   if (!proposed || propose_only) {
     err |= store_pos(R, out.x, out.y, out.z, j, e.pos, e.near_plane);
     out.id[j] = idv;
-    uint32_t t2 = tag & ~(MF_STEP_FLAGS << 8);
-    t2 = store_toff(out.toff, j, sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : toff, t2);
+    uint32_t t2 = (tag & ~(MF_STEP_FLAGS << 8)) | (e.mature ? (MF_MATURE << 8) : 0u);
+    if (e.cont && !propose_only) {
+      // the next step of this iteration reads the stream's next pass and
+      // starts at the new diffusion time
+      const uint32_t pass = ((tag_flags(tag) & MF_PASS_MASK) >> MF_PASS_SHIFT) + 1;
+      if (pass > (MF_PASS_MASK >> MF_PASS_SHIFT)) {
+        err |= ERR_PASSES;
+      } else {
+        t2 |= (pass << MF_PASS_SHIFT) << 8;
+        R.clist[atomicAdd(R.ccount, 1u)] = j;
+      }
+      t2 = store_toff(out.toff, j, (((double)R.iteration + toff) + e.t_steps) - (double)R.iteration, t2);
+    } else {
+      t2 = store_toff(out.toff, j,
+                      sp.space_step > 0 ? next_toff(R.iteration, toff, e.t_steps) : immobile_toff(R.iteration, e.utime),
+                      t2);
+    }
     out.tag[j] = e.absorbed ? (t2 | (MF_DEFUNCT << 8)) : t2;
     if (out.utime) out.utime[j] = e.utime;
     if (out.cvol) out.cvol[j] = e.cvol;
+    if (out.birth) out.birth[j] = birth;
     if (born) R.born[j] = 0;  // the step that could use it is over
     n_reflect += e.n_reflect;
     n_cross += e.n_cross;
@@ -429,6 +454,11 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
     // lifetime is still to be drawn or ends within the step's time budget
     const bool unimol_now = R.unimol[tag_species(tag)].num_pathways > 0 &&
                             ((tag_flags(tag) & MF_SCHEDULE_UNIMOL) || utime < (double)R.iteration + barrier);
+    // time steps below one iteration (more than one step in it) and
+    // long time steps still lengthening (get_max_time's maturity rule)
+    // take the full step
+    const bool steps_apart =
+        sp.time_step < 1.0 || (sp.time_step > 1.0 && !(tag_flags(tag) & MF_MATURE));
     if (!ready_this_iteration(R.iteration, toff)) {
       // scheduled for a later iteration (Partition::get_molecules_ready_for_
       // diffusion, partition.h:232-246): stays where it is
@@ -437,7 +467,8 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
       out.tag[j] = store_toff(out.toff, j, toff_past_iteration(R.iteration, toff), tag);
       if (out.utime) out.utime[j] = utime;
       if (out.cvol) out.cvol[j] = S.cvol[j];
-    } else if (unimol_now) {
+      if (out.birth) out.birth[j] = S.birth[j];
+    } else if (unimol_now || steps_apart) {
       hard = true;
     } else if (sp.space_step > 0) {
       PhiloxStream rng;
@@ -588,6 +619,7 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
         out.tag[j] = store_toff(out.toff, j, next_toff(R.iteration, toff, t_steps), tag & ~(MF_STEP_FLAGS << 8));
         if (out.utime) out.utime[j] = utime;
         if (out.cvol) out.cvol[j] = S.cvol[j];
+        if (out.birth) out.birth[j] = S.birth[j];
       } else {
         hard = true;
         partner_hard = walls_clear;
@@ -595,9 +627,10 @@ __global__ __launch_bounds__(256, 4) void k_move_light(ReactParams R, Snapshot<P
     } else {
       store_pos(R, out.x, out.y, out.z, j, pos, false);
       out.id[j] = idv;
-      out.tag[j] = store_toff(out.toff, j, toff, tag & ~(MF_STEP_FLAGS << 8));
+      out.tag[j] = store_toff(out.toff, j, immobile_toff(R.iteration, utime), tag & ~(MF_STEP_FLAGS << 8));
       if (out.utime) out.utime[j] = utime;
       if (out.cvol) out.cvol[j] = S.cvol[j];
+      if (out.birth) out.birth[j] = S.birth[j];
     }
   }
   // movers queued for walls (and unimolecular events) and movers queued for
@@ -924,6 +957,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
       out.tag[slot] = store_toff(out.toff, slot, p.time - (double)(R.iteration + (moves_now ? 0u : 1u)),
                                  make_tag(ps, MF_SCHEDULE_UNIMOL, rep));
       if (out.utime) out.utime[slot] = kTimeInvalid;
+      if (out.birth) out.birth[slot] = p.time;
       // update_counted_volume_id_when_crossing_wall with the product's
       // orientation on a counted object's wall (:2775-2781), else the
       // volume the reactant was in
@@ -971,6 +1005,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
                                make_tag(prods[i], MF_SCHEDULE_UNIMOL, rep));
     if (out.utime) out.utime[slot] = kTimeInvalid;
     if (out.cvol) out.cvol[slot] = p.cvol;  // where the reaction happened
+    if (out.birth) out.birth[slot] = p.time;
     consumed[slot] = 0;
     owner[slot] = kInvalid;
     if (kept) kept[slot] = 0;
@@ -1154,7 +1189,7 @@ template <typename PosT>
 hipError_t ReactPipeline<PosT>::move(hipStream_t s, bool snapshot_src, const uint32_t* list, const uint32_t* count,
                                      uint32_t max_movers, double barrier, bool propose_only, uint32_t* err) {
   if (max_movers == 0) return hipSuccess;
-  Snapshot<PosT> S{Sm.x, Sm.y, Sm.z, Sm.id, Sm.tag, consumed, Sm.toff, Sm.utime, Sm.cvol};
+  Snapshot<PosT> S{Sm.x, Sm.y, Sm.z, Sm.id, Sm.tag, consumed, Sm.toff, Sm.utime, Sm.cvol, Sm.birth};
   const uint32_t po = propose_only ? 1u : 0u;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrProposals, 0u, 1u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hdr + kHdrExd, 0u, 1u);
@@ -1236,6 +1271,30 @@ hipError_t ReactPipeline<PosT>::products(hipStream_t s, uint32_t n_acc, uint32_t
                      log_cap, err, tiles, tile_id, tile_consumed, kept);
   hipLaunchKernelGGL(k_bump_ids, blocks_for(n_acc), dim3(256), 0, s, keys_b, n_acc, pre, head, next_id, hdr,
                      capacity);
+  return hipGetLastError();
+}
+
+// dst[*dst_count + i] = src[i] for i < *src_count (the counts are read on
+// the device; max bounds the grid).
+__global__ __launch_bounds__(256) void k_append_list(const uint32_t* __restrict__ src,
+                                                     const uint32_t* __restrict__ src_count,
+                                                     uint32_t* __restrict__ dst, const uint32_t* __restrict__ dst_count) {
+  const uint32_t n = *src_count, base = *dst_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[base + i] = src[i];
+}
+__global__ void k_take_count(uint32_t* __restrict__ dst_count, uint32_t* __restrict__ src_count) {
+  if (threadIdx.x == 0) {
+    *dst_count += *src_count;
+    *src_count = 0;
+  }
+}
+
+template <typename PosT>
+hipError_t ReactPipeline<PosT>::continuations(hipStream_t s, uint32_t max_cont) {
+  if (max_cont == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_append_list, dim3(std::min<uint32_t>(blocks_for(max_cont).x, 2048u)), dim3(256), 0, s, clist,
+                     hdr + kHdrContinue, mlist, hdr + kHdrProductMovers);
+  hipLaunchKernelGGL(k_take_count, dim3(1), dim3(64), 0, s, hdr + kHdrProductMovers, hdr + kHdrContinue);
   return hipGetLastError();
 }
 

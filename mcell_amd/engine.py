@@ -239,7 +239,8 @@ class DiffuseReactEngine:
         return out
 
     def add_volume_molecules(self, replica: int, ids: np.ndarray, species: np.ndarray, positions: np.ndarray,
-                             diffusion_times=None, unimol_times=None, flags=None, counted_volumes=None):
+                             diffusion_times=None, unimol_times=None, flags=None, counted_volumes=None,
+                             birthdays=None):
         ids = np.ascontiguousarray(ids, dtype=np.uint32)
         sp = np.ascontiguousarray(species, dtype=np.uint32)
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
@@ -248,10 +249,11 @@ class DiffuseReactEngine:
         ut = None if unimol_times is None else np.ascontiguousarray(unimol_times, dtype=np.float64)
         fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint32)
         cv = None if counted_volumes is None else np.ascontiguousarray(counted_volumes, dtype=np.uint32)
-        for a in (dt, ut, fl, cv):
+        bd = None if birthdays is None else np.ascontiguousarray(birthdays, dtype=np.float64)
+        for a in (dt, ut, fl, cv, bd):
             assert a is None or len(a) == len(ids)
         _lib.check(self.lib.mcg_add_volume_molecules(self.ctx, replica, _ptr(ids), _ptr(sp), _ptr(pos), _ptr(dt),
-                                                     _ptr(ut), _ptr(fl), _ptr(cv), len(ids)))
+                                                     _ptr(ut), _ptr(fl), _ptr(cv), _ptr(bd), len(ids)))
 
     def set_counted_volumes(self, num_volumes: int, wall_inside, wall_outside):
         """Per wall: the counted volume index inside / outside its object
@@ -351,11 +353,13 @@ class DiffuseReactEngine:
         ut = np.zeros(n, np.float64)
         fl = np.zeros(n, np.uint32)
         cv = np.zeros(n, np.uint32)
+        bd = np.zeros(n, np.float64)
         cnt = ctypes.c_uint64()
         _lib.check(self.lib.mcg_get_volume_molecules(self.ctx, _ptr(ids), _ptr(sp), _ptr(rep), _ptr(pos), _ptr(sub),
-                                                     _ptr(dt), _ptr(ut), _ptr(fl), _ptr(cv), n, ctypes.byref(cnt)))
+                                                     _ptr(dt), _ptr(ut), _ptr(fl), _ptr(cv), _ptr(bd), n,
+                                                     ctypes.byref(cnt)))
         return {"id": ids, "species": sp, "replica": rep, "pos": pos, "subpart": sub, "diffusion_time": dt,
-                "unimol_time": ut, "flags": fl, "counted_volume": cv}
+                "unimol_time": ut, "flags": fl, "counted_volume": cv, "birthday": bd}
 
     def walls(self):
         cnt = ctypes.c_uint64()

@@ -242,12 +242,18 @@ class Species:
     name: str
     diffusion_constant_3d: float  # cm^2/s
     target_only: bool = False
+    custom_time_step: float = 0.0  # s (0: the global time step)
 
     def steps(self, part: PartitionConfig):
-        """(space_step, time_step) in internal units (mcell_species.c:239-276)."""
+        """(space_step, time_step) in internal units (mcell_species.c:239-276;
+        a custom time step t gives sqrt(4e8 D t) / length_unit and
+        t / time_unit, :254-264)."""
         D = self.diffusion_constant_3d
         if abs(D - 0) <= EPS * max(1.0, abs(D)):
             return 0.0, 1.0
+        if self.custom_time_step > 0:
+            t = self.custom_time_step
+            return math.sqrt(4.0 * 1.0e8 * D * t) * part.rcp_length_unit, t / part.time_unit
         return math.sqrt(4.0 * 1.0e8 * D * part.time_unit) * part.rcp_length_unit, 1.0
 
 

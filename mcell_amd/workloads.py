@@ -15,8 +15,8 @@ Sizes follow the five workloads of the MCell4 diffuse-react benchmark:
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
-from typing import List, Optional
+from dataclasses import dataclass, field, replace
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -57,7 +57,8 @@ class Workload:
     def species_for(self, replica: int) -> List[M.Species]:
         if self.replica_D is None:
             return self.species
-        return [M.Species(s.name, float(d)) for s, d in zip(self.species, self.replica_D[replica])]
+        return [M.Species(s.name, float(d), s.target_only, s.custom_time_step)
+                for s, d in zip(self.species, self.replica_D[replica])]
 
     @property
     def num_molecules(self):
@@ -86,6 +87,18 @@ def rxn_table(wl: Workload, replica: Optional[int] = None):
         pb = r.pb_factor(wl.part, species[r.reactant_a], species[r.reactant_b])
         out.append((r.reactant_a, r.reactant_b, [r.rate * pb], [list(r.products)]))
     return out
+
+
+def with_time_steps(wl: Workload, factors: Sequence[float]) -> Workload:
+    """The workload with species i's time step set to factors[i] global
+    steps (a custom time step, mcell_species.c:254-264; 0 or 1: the global
+    step).  Space steps and reaction probabilities follow."""
+    sp = [M.Species(s.name, s.diffusion_constant_3d, s.target_only,
+                    f * wl.part.time_unit if f not in (0, 1) else 0.0) for s, f in zip(wl.species, factors)]
+    steps = _species_table(sp, wl.part, wl.rxns)
+    if len(wl.species_steps) == len(steps):  # keep the builder's flags
+        steps = [(ss, ts, old[2]) for (ss, ts, _), old in zip(steps, wl.species_steps)]
+    return replace(wl, species=sp, species_steps=steps, name=wl.name + "_timesteps")
 
 
 def species_steps(wl: Workload, replica: int):

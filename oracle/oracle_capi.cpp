@@ -125,8 +125,8 @@ uint64_t orc_get_subpart_walls(void* wp, uint32_t* off, uint32_t* idx) {
   return w->sp_wall_idx.size();
 }
 
-uint32_t orc_add_species(void* wp, double D, int is_vol, uint32_t flags) {
-  return static_cast<World*>(wp)->add_species(D, is_vol != 0, flags);
+uint32_t orc_add_species(void* wp, double D, int is_vol, uint32_t flags, double custom_time_step) {
+  return static_cast<World*>(wp)->add_species(D, is_vol != 0, flags, custom_time_step);
 }
 
 // out: D, space_step, time_step, flags
@@ -305,6 +305,11 @@ void orc_get_molecules(void* wp, uint32_t* species, uint32_t* flags, double* pos
     if (subpart) subpart[i] = m.subpart;
     if (diffusion_time) diffusion_time[i] = m.diffusion_time;
   }
+}
+
+void orc_get_birthdays(void* wp, double* out) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->mols.size(); ++i) out[i] = w->mols[i].birthday;
 }
 
 // hit records: u32 x5 (mol_id, iteration, wall, type, subpart_after), time

@@ -368,7 +368,8 @@ class World {
   uint32_t add_vertex_um(V3 p_um);
   void add_wall(uint32_t v0, uint32_t v1, uint32_t v2, uint32_t object_id, uint32_t surf_class);
   void finalize_walls();
-  uint32_t add_species(double D_cm2s, bool is_vol, uint32_t extra_flags);
+  // custom_time_step: seconds (0: the global time step)
+  uint32_t add_species(double D_cm2s, bool is_vol, uint32_t extra_flags, double custom_time_step = 0.0);
   int add_bimol_rxn(uint32_t a, uint32_t b, double rate, const std::vector<uint32_t>& products);
   int add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate,
                       const std::vector<VolSurfRxn::Product>& products);

@@ -377,9 +377,11 @@ void World::finalize_walls() {
   for (size_t i = 0; i < pairs.size(); ++i) sp_wall_idx[i] = pairs[i].second;
 }
 
-// MCell3 closed form for volume species (src/mcell_species.c:270-276):
-// space_step = sqrt(4e8 * D * time_unit) / length_unit, time_step = 1.
-uint32_t World::add_species(double D, bool is_vol, uint32_t extra_flags) {
+// MCell3 closed forms for volume species (src/mcell_species.c:239-276):
+// space_step = sqrt(4e8 * D * time_unit) / length_unit, time_step = 1; with a
+// custom time step t (s), space_step = sqrt(4e8 * D * t) / length_unit and
+// time_step = t / time_unit (:254-264).
+uint32_t World::add_species(double D, bool is_vol, uint32_t extra_flags, double custom_time_step) {
   Species s{};
   s.D = D;
   s.is_vol = is_vol;
@@ -387,6 +389,9 @@ uint32_t World::add_species(double D, bool is_vol, uint32_t extra_flags) {
   if (!distinguishable(D, 0, EPS)) {
     s.space_step = 0.0;
     s.time_step = 1.0;
+  } else if (custom_time_step > 0) {
+    s.space_step = std::sqrt(4.0 * 1.0e8 * D * custom_time_step) * cfg.rcp_length_unit;
+    s.time_step = custom_time_step / cfg.time_unit;
   } else {
     s.space_step = std::sqrt(4.0 * 1.0e8 * D * cfg.time_unit) * cfg.rcp_length_unit;
     s.time_step = 1.0;

@@ -50,7 +50,8 @@ def load():
         "orc_get_walls": (None, [P, P]),
         "orc_get_vertices": (None, [P, P]),
         "orc_get_subpart_walls": (u64, [P, P, P]),
-        "orc_add_species": (u32, [P, d, ctypes.c_int, u32]),
+        "orc_add_species": (u32, [P, d, ctypes.c_int, u32, d]),
+        "orc_get_birthdays": (None, [P, P]),
         "orc_get_species": (None, [P, u32, P]),
         "orc_add_bimol_rxn": (ctypes.c_int, [P, u32, u32, d, P, u32]),
         "orc_get_rxn": (None, [P, ctypes.c_int, P]),
@@ -189,8 +190,9 @@ class OracleWorld:
         self.lib.orc_get_subpart_walls(self.w, _p(off), _p(idx))
         return off, idx[:cnt]
 
-    def add_species(self, D, is_vol=True, flags=0):
-        return self.lib.orc_add_species(self.w, D, 1 if is_vol else 0, flags)
+    def add_species(self, D, is_vol=True, flags=0, custom_time_step=0.0):
+        """custom_time_step: seconds (0: the global time step)."""
+        return self.lib.orc_add_species(self.w, D, 1 if is_vol else 0, flags, custom_time_step)
 
     def species(self, s):
         out = np.zeros(4)
@@ -303,7 +305,9 @@ class OracleWorld:
         sub = np.zeros(n, np.uint32)
         dt = np.zeros(n)
         self.lib.orc_get_molecules(self.w, _p(sp), _p(fl), _p(pos), _p(sub), _p(dt))
-        return {"species": sp, "flags": fl, "pos": pos, "subpart": sub, "diffusion_time": dt}
+        bd = np.zeros(n)
+        self.lib.orc_get_birthdays(self.w, _p(bd))
+        return {"species": sp, "flags": fl, "pos": pos, "subpart": sub, "diffusion_time": dt, "birthday": bd}
 
     def hits(self):
         n = self.lib.orc_get_hits(self.w, None, None)
@@ -388,7 +392,7 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
     w.finalize_walls()
     species = wl.species_for(replica) if hasattr(wl, "species_for") else wl.species
     for s in species:
-        w.add_species(s.diffusion_constant_3d)
+        w.add_species(s.diffusion_constant_3d, custom_time_step=getattr(s, "custom_time_step", 0.0))
     for r in wl.rxns:
         w.add_bimol_rxn(r.reactant_a, r.reactant_b, r.rate, r.products)
     for u in getattr(wl, "unimol_rxns", []) or []:

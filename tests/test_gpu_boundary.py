@@ -128,7 +128,7 @@ def _round_trip(wl, seed, first, second, capacity, precision="fp64"):
     for r in range(wl.num_replicas):
         sel = m["replica"] == r
         c.add_volume_molecules(r, m["id"][sel], m["species"][sel], m["pos"][sel], m["diffusion_time"][sel],
-                               m["unimol_time"][sel], m["flags"][sel])
+                               m["unimol_time"][sel], m["flags"][sel], birthdays=m["birthday"][sel])
     c.set_next_molecule_ids(nid)
     c.step(first, second)
     cont = _by_id(c.molecules())
