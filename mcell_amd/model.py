@@ -243,6 +243,7 @@ class Species:
     diffusion_constant_3d: float  # cm^2/s
     target_only: bool = False
     custom_time_step: float = 0.0  # s (0: the global time step)
+    surface: bool = False          # a surface species (its molecules sit on wall tiles)
 
     def steps(self, part: PartitionConfig):
         """(space_step, time_step) in internal units (mcell_species.c:239-276;

@@ -57,7 +57,7 @@ class Workload:
     def species_for(self, replica: int) -> List[M.Species]:
         if self.replica_D is None:
             return self.species
-        return [M.Species(s.name, float(d), s.target_only, s.custom_time_step)
+        return [M.Species(s.name, float(d), s.target_only, s.custom_time_step, s.surface)
                 for s, d in zip(self.species, self.replica_D[replica])]
 
     @property
@@ -94,7 +94,7 @@ def with_time_steps(wl: Workload, factors: Sequence[float]) -> Workload:
     steps (a custom time step, mcell_species.c:254-264; 0 or 1: the global
     step).  Space steps and reaction probabilities follow."""
     sp = [M.Species(s.name, s.diffusion_constant_3d, s.target_only,
-                    f * wl.part.time_unit if f not in (0, 1) else 0.0) for s, f in zip(wl.species, factors)]
+                    f * wl.part.time_unit if f not in (0, 1) else 0.0, s.surface) for s, f in zip(wl.species, factors)]
     steps = _species_table(sp, wl.part, wl.rxns)
     if len(wl.species_steps) == len(steps):  # keep the builder's flags
         steps = [(ss, ts, old[2]) for (ss, ts, _), old in zip(steps, wl.species_steps)]
@@ -320,6 +320,45 @@ def place_on_walls(n: int, wall_ids: np.ndarray, vertices: np.ndarray, walls: np
     flat = np.sort(np.argpartition(key, n - 1)[:n]) if n < len(key) else np.arange(n)
     k = np.searchsorted(base, flat, side="right") - 1
     return wall_ids[k].astype(np.uint32), (flat - base[k]).astype(np.uint32)
+
+
+def surface_diffusion(n_s: int = 2000, iterations: int = 100, seed: int = 9, d_surf: float = 1e-7,
+                      sphere_subdivisions: int = 4, sheet: bool = False, n_sheet: int = 0,
+                      barrier_interval: int = 10) -> Workload:
+    """Surface molecules S diffusing on the icosphere of radius 0.5 um (a
+    closed surface of 20 * 4^s triangles) inside the reflective 2 um cube,
+    n_s of them on distinct tiles drawn uniformly by area, at the tiles'
+    centres, half oriented up; with ``sheet`` a flat square sheet of side
+    1 um (two triangles: an open mesh whose outer edges reflect) at z =
+    -0.8 um holds n_sheet more.  D_surf = d_surf cm^2/s (space step
+    sqrt(4e8 D dt) / length unit, as for volume species).  Species S only;
+    ids follow the placement order (sphere, then sheet)."""
+    cfg = M.Config(time_step=1e-6, subpartition_dimension=0.1, seed=seed)
+    box = M.create_box("box", edge_dimension=2.0)
+    ball = M.create_icosphere("sphere", 0.5, sphere_subdivisions)
+    objects = [box, ball]
+    if sheet:
+        z = -0.8
+        objects.append(M.GeometryObject("sheet", [(-0.5, -0.5, z), (0.5, -0.5, z), (0.5, 0.5, z), (-0.5, 0.5, z)],
+                                        [(0, 1, 2), (0, 2, 3)]))
+    part = M.derive_partition(cfg, M.bounding_box(objects), has_bimol_vol_rxns=False)
+    verts, walls, classes = M.vertices_internal(objects, part)
+    species = [M.Species("S", d_surf, surface=True)]
+    nb, nbl = len(box.wall_list), len(ball.wall_list)
+    sw, st = place_on_walls(n_s, np.arange(nb, nb + nbl, dtype=np.uint32), verts, walls, seed)
+    if sheet and n_sheet:
+        w2, t2 = place_on_walls(n_sheet, np.arange(nb + nbl, nb + nbl + 2, dtype=np.uint32), verts, walls, seed + 1)
+        sw, st = np.concatenate([sw, w2]), np.concatenate([st, t2])
+    n = len(sw)
+    rng = np.random.Generator(np.random.PCG64(seed + 2))
+    orients = np.where(rng.uniform(size=n) < 0.5, -1, 1).astype(np.int32)
+    surf = {"ids": np.arange(n, dtype=np.uint32), "species": np.zeros(n, np.uint32), "walls": sw, "tiles": st,
+            "orients": orients, "replica": np.zeros(n, np.uint32)}
+    steps = [(s, t, f | 16) for s, t, f in _species_table(species, part, [])]
+    e = np.zeros(0, np.uint32)
+    return Workload("surface_diffusion", part, verts, walls, classes, species, steps, e, e,
+                    np.zeros((0, 3)), e, iterations, config=cfg, objects=objects, surface_mols=surf,
+                    barrier_interval=barrier_interval)
 
 
 def receptor_sphere(n_l: int = 100000, n_r: int = 10000, iterations: int = 10, seed: int = 3,

@@ -163,11 +163,14 @@ void orc_get_volsurf_rxn(void* wp, int r, double* out) {
   out[1] = x.max_fixed_p;
 }
 
+// uv: 2 doubles per molecule (its position on the wall), or null for the
+// tiles' centres
 int orc_add_surface_molecules(void* wp, uint32_t species, uint32_t n, const uint32_t* walls, const uint32_t* tiles,
-                              const int32_t* orients) {
+                              const int32_t* orients, const double* uv) {
   World* w = static_cast<World*>(wp);
   try {
-    for (uint32_t i = 0; i < n; ++i) w->add_surface_molecule(species, walls[i], tiles[i], orients[i]);
+    for (uint32_t i = 0; i < n; ++i)
+      w->add_surface_molecule(species, walls[i], tiles[i], orients[i], uv ? uv + 2 * i : nullptr);
   } catch (const std::exception& e) {
     g_err = e.what();
     return -1;
@@ -206,6 +209,76 @@ void orc_get_tiles(void* wp, uint32_t wall, uint32_t* out) {
 }
 
 uint64_t orc_volsurf_reactions(void* wp) { return static_cast<World*>(wp)->volsurf_reactions; }
+
+// surface molecules' wall, tile and uv position (indexed by id; volume
+// molecules get 0xFFFFFFFF walls)
+void orc_get_surface_state(void* wp, uint32_t* walls, uint32_t* tiles, double* uv) {
+  World* w = static_cast<World*>(wp);
+  for (size_t i = 0; i < w->mols.size(); ++i) {
+    const Mol& m = w->mols[i];
+    const bool s = (m.flags & MF_SURFACE) != 0;
+    walls[i] = s ? m.wall : 0xFFFFFFFFu;
+    tiles[i] = s ? m.tile : 0xFFFFFFFFu;
+    uv[2 * i] = s ? m.u : 0.0;
+    uv[2 * i + 1] = s ? m.v : 0.0;
+  }
+}
+
+// out: moves between walls, retried displacements
+void orc_surface_stats(void* wp, uint64_t* out) {
+  World* w = static_cast<World*>(wp);
+  out[0] = w->surf_moves_between_walls;
+  out[1] = w->surf_retries;
+}
+
+// Per wall and edge: the neighbouring wall (0xFFFFFFFF: none) and the edge
+// transform (cos, sin, translate u, v; forward flag) of the surface net.
+int orc_surface_net(void* wp, uint32_t* nb, double* tr) {
+  World* w = static_cast<World*>(wp);
+  try {
+    w->build_surface_net();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  for (size_t i = 0; i < w->walls.size(); ++i)
+    for (int k = 0; k < 3; ++k) {
+      const WallEdge& e = w->edges[i][k];
+      nb[3 * i + k] = e.fwd == i ? e.bwd : e.fwd;
+      double* t = tr + 5 * (3 * i + k);
+      t[0] = e.cos_t; t[1] = e.sin_t; t[2] = e.tu; t[3] = e.tv; t[4] = e.fwd == i ? 1.0 : 0.0;
+    }
+  return 0;
+}
+
+// find_edge_point and traverse_surface for unit tests: out = edge, then
+// the edge point (u, v), then the wall across and the point in its frame
+void orc_edge_point(void* wp, uint32_t wall, const double* loc, const double* disp, double* out) {
+  World* w = static_cast<World*>(wp);
+  if (w->edges.size() != w->walls.size()) w->build_surface_net();
+  double eu = 0, ev = 0;
+  const int e = w->find_edge_point(wall, loc[0], loc[1], disp[0], disp[1], eu, ev);
+  out[0] = e; out[1] = eu; out[2] = ev;
+  out[3] = -1; out[4] = 0; out[5] = 0;
+  if (e < 3) {
+    double nu = 0, nv = 0;
+    const uint32_t t = w->traverse_surface(wall, eu, ev, e, nu, nv);
+    out[3] = t == 0xFFFFFFFFu ? -1.0 : (double)t;
+    out[4] = nu; out[5] = nv;
+  }
+}
+
+uint32_t orc_uv2grid(void* wp, uint32_t wall, double u, double v) {
+  World* w = static_cast<World*>(wp);
+  w->init_grid(wall);
+  return w->uv2grid(u, v, wall);
+}
+
+void orc_grid2uv(void* wp, uint32_t wall, uint32_t tile, double* out) {
+  World* w = static_cast<World*>(wp);
+  w->init_grid(wall);
+  w->grid2uv(wall, tile, out[0], out[1]);
+}
 
 void orc_get_rxn(void* wp, int r, double* out) {
   const BimolRxn& x = static_cast<World*>(wp)->rxns[r];

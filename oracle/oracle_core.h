@@ -9,6 +9,7 @@
 // multiply-add may be contracted here either.
 #pragma once
 
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -217,6 +218,14 @@ struct Grid {
   std::vector<uint32_t> tile_mol;      // molecule id per tile, or INVALID
 };
 
+// A wall edge as surface molecules cross it (Edge, src4/wall.h:32-89): the
+// two walls it joins (bwd INVALID_U32: not shared) and the uv transform from
+// the forward wall's frame to the backward wall's (rotation, translation).
+struct WallEdge {
+  uint32_t fwd = 0xFFFFFFFFu, bwd = 0xFFFFFFFFu;
+  double cos_t = 0, sin_t = 0, tu = 0, tv = 0;
+};
+
 // Volume + surface reaction class: the volume reactant hitting a tile
 // occupied by the surface reactant; the surface product replaces the
 // surface reactant on its tile and the volume reactant is destroyed.
@@ -266,6 +275,7 @@ struct Mol {
   uint32_t wall = 0xFFFFFFFFu;  // surface molecules: wall, tile, orientation (+1 up, -1 down)
   uint32_t tile = 0;
   int orient = 0;
+  double u = 0, v = 0;        // surface molecules: position in the wall's uv frame (Molecule::s.pos)
   V3 pos;
   uint32_t subpart;
   uint32_t reactant_subpart;  // subpart whose reactant sets hold this molecule
@@ -373,9 +383,22 @@ class World {
   int add_bimol_rxn(uint32_t a, uint32_t b, double rate, const std::vector<uint32_t>& products);
   int add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_surf, double rate,
                       const std::vector<VolSurfRxn::Product>& products);
-  uint32_t add_surface_molecule(uint32_t species, uint32_t wall, uint32_t tile, int orient);
+  // uv: the molecule's position on the wall (null: the tile's centre,
+  // grid2uv); surface molecules of species that diffuse are scheduled at
+  // `time` (null time: the current iteration)
+  uint32_t add_surface_molecule(uint32_t species, uint32_t wall, uint32_t tile, int orient,
+                                const double* uv = nullptr, const double* time = nullptr);
   void init_grid(uint32_t wall);
   uint32_t xyz2grid(V3 v, uint32_t wall) const;
+  // surface-molecule diffusion (oracle_surface.cpp)
+  std::vector<std::array<WallEdge, 3>> edges;  // per wall, built by build_surface_net
+  uint64_t surf_moves_between_walls = 0, surf_retries = 0;
+  void build_surface_net();
+  uint32_t uv2grid(double u, double v, uint32_t wall) const;
+  void grid2uv(uint32_t wall, uint32_t tile, double& u, double& v) const;
+  // find_edge_point / traverse_surface (public for unit tests)
+  int find_edge_point(uint32_t wall, double lu, double lv, double du, double dv, double& eu, double& ev) const;
+  uint32_t traverse_surface(uint32_t wall, double lu, double lv, int edge, double& nu, double& nv) const;
   int collide_and_react_with_surf_mol(const Collision& c, double t_steps, double r_rate_factor, double elapsed);
   // rate in s^-1 per pathway (pb = time_unit, MCell3 src/react_util.c:74-78)
   void add_unimol_rxn(uint32_t species, double rate, const std::vector<uint32_t>& products);
@@ -454,6 +477,8 @@ class World {
   bool react_unimol(uint32_t id);
   double get_max_time(Mol& m);
   void diffuse_vol_molecule(uint32_t id, double& max_time, double diffusion_start_time);
+  void diffuse_surf_molecule(uint32_t id, double& max_time);
+  uint32_t ray_trace_surf(uint32_t id, double du, double dv, double& nu, double& nv);
   int ray_trace_vol(uint32_t id, bool can_vol_react, uint32_t last_hit, V3& remaining_disp,
                     std::vector<Collision>& collisions);
   V3 volume_product_position(uint32_t id, uint32_t wall, int orient);

@@ -224,12 +224,13 @@ void World::diffuse_single_molecule(uint32_t id) {
   }
   double max_time = get_max_time(mols[id]);
   // (m0 dangles once a reaction that kept the molecule added products)
-  if (species[mols[id].species].is_vol) diffuse_vol_molecule(id, max_time, diffusion_start_time);
+  if (mols[id].flags & MF_SURFACE) diffuse_surf_molecule(id, max_time);
+  else if (species[mols[id].species].is_vol) diffuse_vol_molecule(id, max_time, diffusion_start_time);
   Mol& m = mols[id];
   if (m.flags & MF_DEFUNCT) return;
   const Species& sp = species[m.species];
   double event_end_time = (double)iteration + 1;
-  if ((m.flags & MF_SURFACE) == 0 && sp.space_step > 0) {  // can_diffuse
+  if (sp.space_step > 0) {  // can_diffuse
     m.diffusion_time += max_time;
     if ((m.unimol_rxn_time != TIME_INVALID && m.unimol_rxn_time < event_end_time) ||
         cmp_lt(m.diffusion_time, event_time + 1.0, EPS)) {  // before_this_iterations_end

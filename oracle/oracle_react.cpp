@@ -698,8 +698,13 @@ int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, d
       continue;
     }
     if (prods[q].surf) {
-      add_surface_molecule(prods[q].species, wall, tile, orient[q]);
-      mols.back().birthday = time;
+      // the product takes the receptor's tile and position
+      // (GridPos::make_with_mol_pos, diffuse_react_event.cpp:2040-2050) and,
+      // when it can diffuse, moves for the rest of the iteration
+      const double uv[2] = {mols[occ].u, mols[occ].v};
+      const uint32_t nid = add_surface_molecule(prods[q].species, wall, tile, orient[q], uv, &time);
+      if (species[prods[q].species].space_step > 0 && cmp_lt(time, event_time + 1.0, EPS))
+        new_diffuse_actions.push_back({nid});
       continue;
     }
     const uint32_t nid = add_volume_molecule(prods[q].species, c.pos, time);

@@ -493,7 +493,8 @@ int World::add_volsurf_rxn(uint32_t vol, uint32_t surf, int geom_vol, int geom_s
   return (int)vs_rxns.size() - 1;
 }
 
-uint32_t World::add_surface_molecule(uint32_t sp, uint32_t wi, uint32_t tile, int orient) {
+uint32_t World::add_surface_molecule(uint32_t sp, uint32_t wi, uint32_t tile, int orient, const double* uv,
+                                     const double* time) {
   init_grid(wi);
   Grid& g = grids[wi];
   if (tile >= g.num_tiles) throw std::runtime_error("tile index out of range");
@@ -505,11 +506,22 @@ uint32_t World::add_surface_molecule(uint32_t sp, uint32_t wi, uint32_t tile, in
   m.wall = wi;
   m.tile = tile;
   m.orient = orient;
-  m.diffusion_time = TIME_FOREVER;  // surface molecules do not move on this path
-  m.birthday = (double)iteration;
+  if (uv) {
+    m.u = uv[0];
+    m.v = uv[1];
+  } else {
+    grid2uv(wi, tile, m.u, m.v);
+  }
+  // a surface molecule that can diffuse is scheduled like a volume molecule
+  // (GridUtils::place_single_molecule_onto_grid, grid_utils.inl:2097-2160);
+  // one that cannot is never moved
+  const bool moves = species[sp].space_step > 0;
+  m.diffusion_time = moves ? (time ? *time : (double)iteration) : TIME_FOREVER;
+  m.birthday = time ? *time : (double)iteration;
   m.unimol_rxn_time = TIME_INVALID;
   mols.push_back(m);
   g.tile_mol[tile] = m.id;
+  if (moves) schedulable.push_back(m.id);
   return m.id;
 }
 

@@ -62,7 +62,13 @@ def load():
         "orc_set_molecule_volumes": (None, [P, u32, P, P]),
         "orc_get_molecule_volumes": (None, [P, P]),
         "orc_get_volsurf_rxn": (None, [P, ctypes.c_int, P]),
-        "orc_add_surface_molecules": (ctypes.c_int, [P, u32, u32, P, P, P]),
+        "orc_add_surface_molecules": (ctypes.c_int, [P, u32, u32, P, P, P, P]),
+        "orc_get_surface_state": (None, [P, P, P, P]),
+        "orc_surface_stats": (None, [P, P]),
+        "orc_surface_net": (ctypes.c_int, [P, P, P]),
+        "orc_edge_point": (None, [P, u32, P, P, P]),
+        "orc_uv2grid": (u32, [P, u32, d, d]),
+        "orc_grid2uv": (None, [P, u32, u32, P]),
         "orc_grid_info": (None, [P, u32, P]),
         "orc_xyz2grid": (ctypes.c_int64, [P, u32, P]),
         "orc_get_tiles": (None, [P, u32, P]),
@@ -220,12 +226,54 @@ class OracleWorld:
         self.lib.orc_get_volsurf_rxn(self.w, r, _p(out))
         return out
 
-    def add_surface_molecules(self, species, walls, tiles, orients):
+    def add_surface_molecules(self, species, walls, tiles, orients, uv=None):
+        """uv: (n, 2) positions on the walls, or None for the tiles' centres."""
         wa = np.ascontiguousarray(walls, np.uint32)
         ta = np.ascontiguousarray(tiles, np.uint32)
         oa = np.ascontiguousarray(orients, np.int32)
-        if self.lib.orc_add_surface_molecules(self.w, species, len(wa), _p(wa), _p(ta), _p(oa)) != 0:
+        ua = None if uv is None else np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+        if self.lib.orc_add_surface_molecules(self.w, species, len(wa), _p(wa), _p(ta), _p(oa),
+                                              None if ua is None else _p(ua)) != 0:
             raise RuntimeError(self.lib.orc_error().decode())
+
+    def surface_state(self):
+        """Per molecule id: wall, tile (0xFFFFFFFF for volume molecules) and uv."""
+        n = self.lib.orc_num_molecules(self.w)
+        wa = np.zeros(n, np.uint32)
+        ta = np.zeros(n, np.uint32)
+        uv = np.zeros((n, 2))
+        self.lib.orc_get_surface_state(self.w, _p(wa), _p(ta), _p(uv))
+        return wa, ta, uv
+
+    def surface_stats(self):
+        out = np.zeros(2, np.uint64)
+        self.lib.orc_surface_stats(self.w, _p(out))
+        return {"moves_between_walls": int(out[0]), "retries": int(out[1])}
+
+    def surface_net(self, num_walls):
+        """Per wall and edge: the wall across (-1: none) and the transform
+        (cos, sin, translate u, translate v, forward flag)."""
+        nb = np.zeros((num_walls, 3), np.uint32)
+        tr = np.zeros((num_walls, 3, 5))
+        if self.lib.orc_surface_net(self.w, _p(nb), _p(tr)) != 0:
+            raise RuntimeError(self.lib.orc_error().decode())
+        nbi = nb.astype(np.int64)
+        nbi[nb == 0xFFFFFFFF] = -1
+        return nbi, tr
+
+    def edge_point(self, wall, loc, disp):
+        out = np.zeros(6)
+        self.lib.orc_edge_point(self.w, wall, _p(np.asarray(loc, np.float64)), _p(np.asarray(disp, np.float64)),
+                                _p(out))
+        return int(out[0]), out[1:3], int(out[3]), out[4:6]
+
+    def uv2grid(self, wall, u, v):
+        return self.lib.orc_uv2grid(self.w, wall, u, v)
+
+    def grid2uv(self, wall, tile):
+        out = np.zeros(2)
+        self.lib.orc_grid2uv(self.w, wall, tile, _p(out))
+        return out
 
     def grid_info(self, wall):
         out = np.zeros(8)
@@ -392,7 +440,8 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
     w.finalize_walls()
     species = wl.species_for(replica) if hasattr(wl, "species_for") else wl.species
     for s in species:
-        w.add_species(s.diffusion_constant_3d, custom_time_step=getattr(s, "custom_time_step", 0.0))
+        w.add_species(s.diffusion_constant_3d, is_vol=not getattr(s, "surface", False),
+                      custom_time_step=getattr(s, "custom_time_step", 0.0))
     for r in wl.rxns:
         w.add_bimol_rxn(r.reactant_a, r.reactant_b, r.rate, r.products)
     for u in getattr(wl, "unimol_rxns", []) or []:
@@ -428,7 +477,8 @@ def world_from_workload(wl, draw_mode=DRAW_PHILOX, seed=None, replica=0):
             while end < len(ssel) and sm["species"][ssel[end]] == sm["species"][ssel[start]]:
                 end += 1
             k = ssel[start:end]
-            w.add_surface_molecules(int(sm["species"][k[0]]), sm["walls"][k], sm["tiles"][k], sm["orients"][k])
+            w.add_surface_molecules(int(sm["species"][k[0]]), sm["walls"][k], sm["tiles"][k], sm["orients"][k],
+                                    uv=sm["uv"][k] if "uv" in sm else None)
             start = end
     w.set_rng(draw_mode, wl.config.seed if seed is None else seed, replica)
     return w
