@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 7
+#define MCG_ABI_VERSION 8
 
 typedef struct mcg_ctx mcg_ctx;
 
@@ -106,6 +106,7 @@ typedef struct mcg_species {
 #define MCG_SPF_CAN_VOLWALL 2u
 #define MCG_SPF_CAN_VOLSURF 4u
 #define MCG_SPF_TARGET_ONLY 8u
+#define MCG_SPF_SURFACE 16u  /* a surface species; with space_step > 0 its molecules diffuse on the mesh */
 
 /* A volume+volume reaction class (BNG::RxnClass for two volume reactants
  * as test_bimolecular and outcome_bimolecular use it, rxn_utils.inl:336-414,
@@ -259,19 +260,29 @@ typedef struct mcg_volsurf_rxn {
 int mcg_set_volsurf_rxns(mcg_ctx* ctx, const mcg_volsurf_rxn* rxns, uint32_t num_rxns);
 
 /* Surface molecules of one replica (Molecule's surface part: wall, tile,
- * orientation, molecule.h:52-167).  A wall gets its tile grid (Grid::initialize,
- * wall.cpp:38-74: ceil(sqrt(area))^2 tiles) with its first molecule; a tile
- * holds at most one molecule.  Surface molecules do not diffuse on this path;
- * volume molecules react with them through mcg_set_volsurf_rxns.  ids share
- * the replica's id space with the volume molecules.  Call after
- * mcg_set_geometry and mcg_set_species. */
+ * orientation and position in the wall's uv frame, molecule.h:52-167).  A
+ * wall gets its tile grid (Grid::initialize, wall.cpp:38-74: ceil(sqrt(area))^2
+ * tiles) with its first molecule; a tile holds at most one molecule.  uv (2
+ * doubles per molecule) may be NULL: the tiles' centres (grid2uv,
+ * grid_utils.inl:233-250); diffusion_times may be NULL: the clock.  Species
+ * flagged MCG_SPF_SURFACE with a space step diffuse on the mesh after each
+ * iteration's volume moves (diffuse_surf_molecule, diffuse_react_event.cpp:
+ * 1071-1246): every wall then gets a grid, and the walls are linked across
+ * the edges two of them share (surface_net, geometry.cpp:256-353; an edge of
+ * more than two walls is refused); their time step must be the global one.
+ * Volume molecules react with surface molecules through
+ * mcg_set_volsurf_rxns.  ids share the replica's id space with the volume
+ * molecules.  Call after mcg_set_geometry and mcg_set_species. */
 int mcg_add_surface_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids, const uint32_t* species,
-                              const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, uint64_t count);
+                              const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, const double* uv,
+                              const double* diffusion_times, uint64_t count);
 
-/* The surface molecules back, in (replica, wall, tile) order.  Any output
- * pointer may be NULL; *count gets the number. */
+/* The surface molecules back, in (replica, wall, tile) order, with their uv
+ * positions and diffusion times.  Any output pointer may be NULL; *count
+ * gets the number. */
 int mcg_get_surface_molecules(mcg_ctx* ctx, uint32_t* ids, uint32_t* species, uint32_t* replica, uint32_t* walls,
-                              uint32_t* tiles, int32_t* orients, uint64_t capacity, uint64_t* count);
+                              uint32_t* tiles, int32_t* orients, double* uv, double* diffusion_times,
+                              uint64_t capacity, uint64_t* count);
 
 /* Keeps up to `capacity` reaction events of each mcg_step call (0 = off). */
 int mcg_set_rxn_log(mcg_ctx* ctx, uint64_t capacity);

@@ -66,6 +66,10 @@ struct ReactParams {
   // for the next phase's moves
   uint32_t* __restrict__ clist;
   uint32_t* __restrict__ ccount;
+  // diffusion-time offsets of surface molecules per tile slot (null unless
+  // a surface species diffuses): a surface product moves from its
+  // reaction time on
+  double* __restrict__ ttoff;
 };
 
 // xyz2grid_tile_index (grid_utils.inl:48-118) for a point on the wall.

@@ -55,6 +55,7 @@ enum : uint32_t {
   SPF_CAN_VOLWALL = 1u << 1,
   SPF_CAN_VOLSURF = 1u << 2,
   SPF_TARGET_ONLY = 1u << 3,
+  SPF_SURFACE = 1u << 4,  // a surface species: its molecules sit on wall tiles
 };
 
 // molecule tag word: species (8 bits) | flags (8 bits) | replica (16 bits)

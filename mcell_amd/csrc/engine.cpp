@@ -456,8 +456,11 @@ int mcg_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, doub
     return fail(MCG_E_INVALID, "first_iteration " + std::to_string(first_iteration) + " is not the engine's clock " +
                                    std::to_string(c->clock));
   if (stats) std::memset(stats, 0, sizeof(*stats));
-  // species time steps other than one iteration take the reactive path
+  // species time steps other than one iteration and surface molecules
+  // that diffuse take the reactive path
   if (int e = ensure_step_buffers(c)) return e;
+  if (c->surface_diffusion() && c->rx.total_tiles)
+    if (int e = ensure_react_buffers(c)) return e;
   c->stepped = true;
   int st = c->rx.on ? react_step(c, first_iteration, num_iterations, time_up_to_next_barrier, stats)
                     : free_step(c, first_iteration, num_iterations, stats);

@@ -8,6 +8,7 @@
 #include "../../include/mcg.h"
 #include "kernels.h"
 #include "kernels_react.h"
+#include "kernels_surface.h"
 
 namespace mcg {
 
@@ -129,6 +130,14 @@ struct ReactState {
   bool tiles_on_device = false;  // steps may have changed the device copy
   DevBuf<WallGrid> grids_d;
   DevBuf<uint32_t> tiles, tile_id, tile_owner, tile_consumed;
+  // surface molecules' positions in their walls' uv frames and diffusion-
+  // time offsets, per tile slot (host copies; on the device while surface
+  // species diffuse), and the surface phase's buffers
+  std::vector<double> tu_host, tv_host, ttoff_host;
+  bool surf_on = false;  // some surface species diffuses: grids on every wall, the surface phase runs
+  DevBuf<double> tu, tv, ttoff, spu, spv, sptoff;
+  DevBuf<uint32_t> sclaim, sprop, splist, slose0, slose1, sdone;
+  DevBuf<WallEdgeRec> edges_d;
   // reaction log of the last mcg_step call
   DevBuf<RxnEvent> log;
   uint64_t log_cap = 0;
@@ -175,6 +184,12 @@ struct mcg_ctx {
   // longer than one iteration
   bool custom_steps() const { return any_step([](double t) { return !(t == 1.0); }); }
   bool long_steps() const { return any_step([](double t) { return t > 1.0; }); }
+  // a surface species (of any replica) that diffuses
+  bool surface_diffusion() const {
+    for (const mcg::SpeciesRec& s : per_replica ? species_replicas : species_host)
+      if ((s.flags & mcg::SPF_SURFACE) && s.space_step > 0) return true;
+    return false;
+  }
   template <typename F>
   bool any_step(F f) const {
     for (const mcg::SpeciesRec& s : per_replica ? species_replicas : species_host)
@@ -222,6 +237,12 @@ int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, do
                mcg_step_stats* stats);
 // Allocates the reactive path's buffers once (reaction classes of any kind).
 int ensure_react_buffers(mcg_ctx* c);
+// Surface species that diffuse: grids on every wall, the surface net and
+// the surface phase's buffers (engine_react.cpp); the surface phase of
+// one iteration.
+int ensure_surface_diffusion(mcg_ctx* c);
+int download_tiles(mcg_ctx* c);
+int surface_phase(mcg_ctx* c, uint32_t iteration, double barrier, uint32_t* err);
 // Species time steps other than one iteration run on the reactive path;
 // long ones keep birthdays (allocated here, the existing molecules' set to
 // the clock).

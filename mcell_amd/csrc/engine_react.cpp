@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <map>
 #include <cmath>
 #include <cstring>
 
@@ -117,6 +119,7 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.ulist = r.ulist.p;
   p.clist = r.clist.p;
   p.R.clist = r.clist.p;
+  p.R.ttoff = r.surf_on ? r.ttoff.p : nullptr;
   p.R.ccount = r.hdr.p + kHdrContinue;
   p.keys_a = r.keys_a.p;
   p.keys_b = r.keys_b.p;
@@ -255,6 +258,9 @@ int run_iterations(mcg_ctx* c, ReactPipeline<PosT>& p, uint32_t first, uint32_t 
         steps += nm;
         HIP_TRY(p.move(c->stream, false, p.mlist, p.hdr + kHdrProductMovers, nm, barrier, false, err));
       }
+      // the surface molecules' steps follow the volume phases
+      if (r.surf_on)
+        if (int st = surface_phase(c, it, barrier, err)) return st;
     }
     // the settling loop ends on a header read; the proposal log does not
     if (propose_only) {
@@ -368,6 +374,7 @@ int react_step(mcg_ctx* c, uint32_t first_iteration, uint32_t num_iterations, do
     HIP_TRY(hipMemcpy(r.next_id.p, r.next_id_host.data(), r.next_id_host.size() * 4, hipMemcpyHostToDevice));
     r.next_id_dirty = false;
   }
+  if (int st = ensure_surface_diffusion(c)) return st;
   if (int st = sync_tiles_to_device(c)) return st;
   if (!r.rxn_counts.p) {
     // the classes are fixed from the first step on
@@ -775,6 +782,32 @@ WallGrid make_wall_grid(const WallRec64& w, uint32_t tile_base) {
   return g;
 }
 
+// The host tile arrays laid out for `total` tiles per replica (new grids
+// appended after the existing tiles), molecules kept in their slots.
+void relayout_tiles(const ReactState& r, uint32_t R, uint64_t total, std::vector<uint32_t>& nt,
+                    std::vector<uint32_t>& nid, std::vector<double>& nu, std::vector<double>& nv,
+                    std::vector<double>& nto) {
+  nt.assign(total * R, kTileEmpty);
+  nid.assign(total * R, kInvalid);
+  nu.assign(total * R, 0.0);
+  nv.assign(total * R, 0.0);
+  nto.assign(total * R, 0.0);
+  const bool pos = r.tu_host.size() == (size_t)r.total_tiles * R;
+  for (uint32_t rep = 0; rep < R; ++rep)
+    for (uint32_t t = 0; t < r.total_tiles; ++t) {
+      const size_t a = rep * total + t, b = (size_t)rep * r.total_tiles + t;
+      nt[a] = r.tiles_host[b];
+      nid[a] = r.tile_id_host[b];
+      if (pos) {
+        nu[a] = r.tu_host[b];
+        nv[a] = r.tv_host[b];
+        nto[a] = r.ttoff_host[b];
+      }
+    }
+}
+
+}  // namespace
+
 // Brings the host copy of the tiles up to date with the device's.
 int download_tiles(mcg_ctx* c) {
   ReactState& r = c->rx;
@@ -783,10 +816,13 @@ int download_tiles(mcg_ctx* c) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(r.tiles_host.data(), r.tiles.p, nt * 4, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(r.tile_id_host.data(), r.tile_id.p, nt * 4, hipMemcpyDeviceToHost));
+  if (r.surf_on && r.tu.n == nt) {
+    HIP_TRY(hipMemcpy(r.tu_host.data(), r.tu.p, nt * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(r.tv_host.data(), r.tv.p, nt * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(r.ttoff_host.data(), r.ttoff.p, nt * 8, hipMemcpyDeviceToHost));
+  }
   return MCG_OK;
 }
-
-}  // namespace
 
 int sync_tiles_to_device(mcg_ctx* c, bool allow_alloc) {
   ReactState& r = c->rx;
@@ -815,9 +851,192 @@ int sync_tiles_to_device(mcg_ctx* c, bool allow_alloc) {
   HIP_TRY(hipMemcpy(r.tile_id.p, r.tile_id_host.data(), nt * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(r.tile_owner.p, 0xFF, nt * 4));
   HIP_TRY(hipMemset(r.tile_consumed.p, 0, nt * 4));
+  if (r.surf_on) {
+    if (r.tu.n != nt) {
+      if (!allow_alloc) return fail(MCG_E_STATE, "surface molecules on new walls need tile buffers");
+      if (r.tu.alloc(nt) || r.tv.alloc(nt) || r.ttoff.alloc(nt) || r.spu.alloc(nt) || r.spv.alloc(nt) ||
+          r.sptoff.alloc(nt) || r.sclaim.alloc(nt) || r.sprop.alloc(nt) || r.splist.alloc(nt) ||
+          r.slose0.alloc(nt) || r.slose1.alloc(nt) || r.sdone.alloc(nt))
+        return fail(MCG_E_NOMEM, "surface phase buffers");
+      HIP_TRY(launch_fill_claims(c->stream, r.sclaim.p, (uint32_t)nt));
+    }
+    if (r.sdone.n == nt) {
+      // molecules keep their slots' stamps only within an iteration
+      HIP_TRY(launch_fill_done(c->stream, r.sdone.p, (uint32_t)nt));
+    }
+    HIP_TRY(hipMemcpy(r.tu.p, r.tu_host.data(), nt * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(r.tv.p, r.tv_host.data(), nt * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(r.ttoff.p, r.ttoff_host.data(), nt * 8, hipMemcpyHostToDevice));
+  }
   r.tiles_dirty = false;
   r.tiles_on_device = true;
   return MCG_OK;
+}
+
+namespace {
+
+// Edge::reinit_edge_constants (src4/wall.cpp:134-235): the uv transform
+// from the forward wall's frame to the backward wall's across the edge
+// that starts at the forward wall's vertex i.
+bool edge_constants(const WallRec64& wf, const WallRec64& wb, uint32_t i, WallEdgeRec& e) {
+  const double fv[3][3] = {{wf.p0x, wf.p0y, wf.p0z}, {wf.p1x, wf.p1y, wf.p1z}, {wf.p2x, wf.p2y, wf.p2z}};
+  const uint32_t j = (i + 1) % 3;
+  auto dotu = [](const double* a, const double* o, const WallRec64& w) {
+    return (a[0] - o[0]) * w.ux + (a[1] - o[1]) * w.uy + (a[2] - o[2]) * w.uz;
+  };
+  auto dotv = [](const double* a, const double* o, const WallRec64& w) {
+    return (a[0] - o[0]) * w.vx + (a[1] - o[1]) * w.vy + (a[2] - o[2]) * w.vz;
+  };
+  const double b0[3] = {wb.p0x, wb.p0y, wb.p0z};
+  const double ofu = dotu(fv[i], fv[0], wf), ofv = dotv(fv[i], fv[0], wf);
+  const double ffu = dotu(fv[j], fv[0], wf) - ofu, ffv = dotv(fv[j], fv[0], wf) - ofv;
+  if (!(std::fabs(ffu * ffu + ffv * ffv) >= POS_EPS)) return false;
+  const double df = 1 / std::sqrt(ffu * ffu + ffv * ffv);
+  const double efu = ffu * df, efv = ffv * df, hfu = -efv, hfv = efu;
+  const double obu = dotu(fv[i], b0, wb), obv = dotv(fv[i], b0, wb);
+  const double fbu = dotu(fv[j], b0, wb) - obu, fbv = dotv(fv[j], b0, wb) - obv;
+  if (!(std::fabs(fbu * fbu + fbv * fbv) >= POS_EPS)) return false;
+  const double db = 1 / std::sqrt(fbu * fbu + fbv * fbv);
+  const double ebu = fbu * db, ebv = fbv * db, hbu = -ebv, hbv = ebu;
+  const double m00 = efu * ebu + hfu * hbu, m01 = efv * ebu + hfv * hbu;
+  const double m10 = efu * ebv + hfu * hbv, m11 = efv * ebv + hfv * hbv;
+  e.cos_t = m00;
+  e.sin_t = m01;
+  e.tu = obu - (m00 * ofu + m01 * ofv);
+  e.tv = obv - (m10 * ofu + m11 * ofv);
+  return true;
+}
+
+// surface_net (src4/geometry.cpp:256-353) over all walls: an edge of two
+// walls traversed in opposite vertex order (compatible_edges, :50-97) links
+// them, the wall of smaller index being the forward side.  Walls are
+// matched by their vertices' coordinates, over the whole mesh (the
+// reference links walls of one object; objects that share no edge give the
+// same net).
+int build_surface_net(const std::vector<WallRec64>& walls, std::vector<WallEdgeRec>& out) {
+  WallEdgeRec none{};
+  none.fwd = none.bwd = kInvalid;
+  out.assign(3 * walls.size(), none);
+  using Key = std::array<double, 6>;
+  std::map<Key, std::vector<std::pair<uint32_t, uint32_t>>> groups;
+  auto vtx = [&](uint32_t w, uint32_t k) -> std::array<double, 3> {
+    const WallRec64& r = walls[w];
+    if (k == 0) return {r.p0x, r.p0y, r.p0z};
+    if (k == 1) return {r.p1x, r.p1y, r.p1z};
+    return {r.p2x, r.p2y, r.p2z};
+  };
+  for (uint32_t w = 0; w < walls.size(); ++w)
+    for (uint32_t j = 0; j < 3; ++j) {
+      const auto a = vtx(w, j), b = vtx(w, (j + 1) % 3);
+      Key k{a[0], a[1], a[2], b[0], b[1], b[2]}, kr{b[0], b[1], b[2], a[0], a[1], a[2]};
+      groups[std::min(k, kr)].push_back({w, j});
+    }
+  for (const auto& kv : groups) {
+    const auto& g = kv.second;
+    if (g.size() > 2) return fail(MCG_E_UNSUPPORTED, "an edge shared by more than two walls (surface diffusion)");
+    if (g.size() < 2) continue;
+    const uint32_t wa = std::min(g[0].first, g[1].first) == g[0].first ? 0 : 1;
+    const uint32_t fa = g[wa].first, ea = g[wa].second, fb = g[1 - wa].first, eb = g[1 - wa].second;
+    if (!(vtx(fa, ea) == vtx(fb, (eb + 1) % 3) && vtx(fa, (ea + 1) % 3) == vtx(fb, eb) &&
+          !(vtx(fa, (ea + 2) % 3) == vtx(fb, (eb + 2) % 3))))
+      continue;
+    WallEdgeRec e{};
+    e.fwd = fa;
+    e.bwd = fb;
+    if (!edge_constants(walls[fa], walls[fb], ea, e)) return fail(MCG_E_INVALID, "malformed wall pair at an edge");
+    out[3 * fa + ea] = e;
+    out[3 * fb + eb] = e;
+  }
+  return MCG_OK;
+}
+
+}  // namespace
+
+int ensure_surface_diffusion(mcg_ctx* c) {
+  ReactState& r = c->rx;
+  if (r.surf_on) return MCG_OK;
+  bool on = false;
+  for (const SpeciesRec& s : c->per_replica ? c->species_replicas : c->species_host)
+    if ((s.flags & SPF_SURFACE) && s.space_step > 0) {
+      if (!(s.time_step == 1.0))
+        return fail(MCG_E_UNSUPPORTED, "surface species diffuse with the global time step on this path");
+      on = true;
+    }
+  if (!on || r.total_tiles == 0) return MCG_OK;
+  if (int st = download_tiles(c)) return st;
+  // every wall gets its grid: a molecule may reach any wall of the mesh
+  const uint32_t R = c->cfg.num_replicas;
+  const uint32_t nw = (uint32_t)c->walls_host.size();
+  if (r.grids_host.size() != nw) r.grids_host.assign(nw, WallGrid{});
+  std::vector<WallGrid> g = r.grids_host;
+  uint64_t total = r.total_tiles;
+  for (uint32_t w = 0; w < nw; ++w) {
+    if (g[w].num_tiles) continue;
+    g[w] = make_wall_grid(c->walls_host[w], (uint32_t)total);
+    total += g[w].num_tiles;
+    if (total * R > 0xFFFFFFFFull) return fail(MCG_E_LIMIT, "too many tiles (tiles x replicas >= 2^32)");
+  }
+  std::vector<uint32_t> nt, nid;
+  std::vector<double> nu, nv, nto;
+  relayout_tiles(r, R, total, nt, nid, nu, nv, nto);
+  std::vector<WallEdgeRec> edges;
+  if (int st = build_surface_net(c->walls_host, edges)) return st;
+  if (r.edges_d.alloc(edges.size())) return fail(MCG_E_NOMEM, "surface net");
+  HIP_TRY(hipMemcpy(r.edges_d.p, edges.data(), edges.size() * sizeof(WallEdgeRec), hipMemcpyHostToDevice));
+  r.grids_host = g;
+  r.tiles_host.swap(nt);
+  r.tile_id_host.swap(nid);
+  r.tu_host.swap(nu);
+  r.tv_host.swap(nv);
+  r.ttoff_host.swap(nto);
+  r.total_tiles = (uint32_t)total;
+  r.tiles_dirty = true;
+  r.surf_on = true;
+  return MCG_OK;
+}
+
+int surface_phase(mcg_ctx* c, uint32_t iteration, double barrier, uint32_t* err) {
+  ReactState& r = c->rx;
+  uint32_t* h = r.hdr_host;
+  const uint32_t R = c->cfg.num_replicas;
+  SurfPhase P{};
+  P.walls = c->walls64.p;
+  P.grids = r.grids_d.p;
+  P.edges = r.edges_d.p;
+  P.tile_wall = r.tile_wall.p;
+  P.species = c->species_table();
+  P.tiles = r.tiles.p;
+  P.tile_id = r.tile_id.p;
+  P.tile_consumed = r.tile_consumed.p;
+  P.tu = r.tu.p;
+  P.tv = r.tv.p;
+  P.ttoff = r.ttoff.p;
+  P.claim = r.sclaim.p;
+  P.prop = r.sprop.p;
+  P.pu = r.spu.p;
+  P.pv = r.spv.p;
+  P.ptoff = r.sptoff.p;
+  P.plist = r.splist.p;
+  P.done = r.sdone.p;
+  P.hdr = r.hdr.p;
+  P.err = err;
+  P.total_tiles = r.total_tiles;
+  P.num_replicas = R;
+  P.seed = c->cfg.seed;
+  P.iteration = iteration;
+  P.barrier = barrier;
+  const uint32_t n_all = r.total_tiles * R;
+  uint32_t* lists[2] = {r.slose0.p, r.slose1.p};
+  const uint32_t words[2] = {kHdrSurfLose0, kHdrSurfLose1};
+  HIP_TRY(surf_round(c->stream, P, nullptr, nullptr, n_all, lists[0], r.hdr.p + words[0]));
+  for (uint32_t round = 0;; ++round) {
+    if (int st = read_hdr(c)) return st;
+    const uint32_t nl = h[words[round & 1]];
+    if (nl == 0) return MCG_OK;
+    if (round > 100000) return fail(MCG_E_LIMIT, "surface moves did not settle");
+    HIP_TRY(surf_round(c->stream, P, lists[round & 1], r.hdr.p + words[round & 1], nl, lists[(round + 1) & 1],
+                       r.hdr.p + words[(round + 1) & 1]));
+  }
 }
 
 }  // namespace mcg
@@ -908,7 +1127,8 @@ int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
 }
 
 int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, const uint32_t* species,
-                              const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, uint64_t count) {
+                              const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, const double* uv,
+                              const double* dtimes, uint64_t count) {
   if (!c || (count && (!ids || !species || !walls || !tiles || !orients))) return fail(MCG_E_INVALID, "null argument");
   MCG_ALIVE(c);
   if (!c->have_geometry) return fail(MCG_E_STATE, "surface molecules need the geometry first");
@@ -922,6 +1142,7 @@ int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids,
     if (walls[i] >= nw) return fail(MCG_E_INVALID, "wall index out of range");
     if (species[i] >= c->num_species) return fail(MCG_E_INVALID, "species id out of range");
     if (orients[i] != 1 && orients[i] != -1) return fail(MCG_E_INVALID, "surface molecules are oriented +1 or -1");
+    if (dtimes && !(dtimes[i] >= (double)c->clock)) return fail(MCG_E_INVALID, "diffusion time before the engine's clock");
   }
   if (int st = download_tiles(c)) return st;
   // grids for walls without one, appended after the existing tiles
@@ -937,17 +1158,22 @@ int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids,
   }
   for (uint64_t i = 0; i < count; ++i)
     if (tiles[i] >= g[walls[i]].num_tiles) return fail(MCG_E_INVALID, "tile index out of range");
-  std::vector<uint32_t> nt(total * R, kTileEmpty), nid(total * R, kInvalid);
-  for (uint32_t rep = 0; rep < R; ++rep)
-    for (uint32_t t = 0; t < r.total_tiles; ++t) {
-      nt[rep * total + t] = r.tiles_host[(size_t)rep * r.total_tiles + t];
-      nid[rep * total + t] = r.tile_id_host[(size_t)rep * r.total_tiles + t];
-    }
+  std::vector<uint32_t> nt, nid;
+  std::vector<double> nu, nv, nto;
+  relayout_tiles(r, R, total, nt, nid, nu, nv, nto);
   for (uint64_t i = 0; i < count; ++i) {
     const size_t slot = replica * total + g[walls[i]].tile_base + tiles[i];
     if (nt[slot] != kTileEmpty) return fail(MCG_E_INVALID, "tile already occupied");
     nt[slot] = make_tile(species[i], orients[i]);
     nid[slot] = ids[i];
+    // the position on the wall: given, else the tile's centre (grid2uv)
+    if (uv) {
+      nu[slot] = uv[2 * i];
+      nv[slot] = uv[2 * i + 1];
+    } else {
+      grid2uv(c->walls_host[walls[i]], g[walls[i]], tiles[i], nu[slot], nv[slot]);
+    }
+    nto[slot] = dtimes ? dtimes[i] - (double)c->clock : 0.0;
   }
   // the id counters (the device holds them once reactive steps have run)
   if (r.on && !r.next_id_dirty && r.next_id_host.size() == R)
@@ -958,13 +1184,17 @@ int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids,
   r.grids_host = g;
   r.tiles_host.swap(nt);
   r.tile_id_host.swap(nid);
+  r.tu_host.swap(nu);
+  r.tv_host.swap(nv);
+  r.ttoff_host.swap(nto);
   r.total_tiles = (uint32_t)total;
   r.tiles_dirty = true;
   return MCG_OK;
 }
 
 int mcg_get_surface_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, uint32_t* walls,
-                              uint32_t* tiles, int32_t* orients, uint64_t capacity, uint64_t* count) {
+                              uint32_t* tiles, int32_t* orients, double* uv, double* dtimes, uint64_t capacity,
+                              uint64_t* count) {
   if (!c) return fail(MCG_E_INVALID, "null context");
   MCG_ALIVE(c);
   ReactState& r = c->rx;
@@ -984,6 +1214,11 @@ int mcg_get_surface_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint
           if (walls) walls[k] = w;
           if (tiles) tiles[k] = t;
           if (orients) orients[k] = tile_orient(tw);
+          if (uv) {
+            uv[2 * k] = r.tu_host[slot];
+            uv[2 * k + 1] = r.tv_host[slot];
+          }
+          if (dtimes) dtimes[k] = (double)c->clock + r.ttoff_host[slot];
         }
         ++k;
       }

@@ -51,7 +51,10 @@ enum : uint32_t {
   kHdrErr = 9,            // error bits (ERR_*) of the call's kernels
   kHdrExd = 10,           // movers the full step deferred to its exact_disk pass
   kHdrContinue = 11,      // movers of the phase that diffuse again in this iteration
-  kHdrWords = 12,
+  kHdrSurfProps = 12,     // surface phase: moves to other tiles proposed in the round
+  kHdrSurfLose0 = 13,     // surface phase: movers whose tile went to a smaller id (two lists)
+  kHdrSurfLose1 = 14,
+  kHdrWords = 16,
 };
 
 template <typename PosT>
