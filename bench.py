@@ -121,6 +121,14 @@ def build_workload(args):
                            "outside the sphere, 10,000 T crossing it through a transparent surface class, "
                            "0.1 um subpartitions, dt 1e-6 s", "molecules": wl.num_molecules,
                "surface_molecules": int(len(wl.surface_mols["ids"])), "walls": int(len(wl.walls)), "replicas": 1}
+    elif args.workload == "volume_surface_diffusing":
+        wl = W.with_diffusing_surface(W.volume_surface(iterations=args.steps), {1: 1e-8, 2: 1e-8})
+        cfg = {"workload": "the volume-surface case with diffusing receptors: L' + R' -> LR' (1e8 /M/s), R' and LR' "
+                           "diffusing on the icosphere r 0.5 um (5,120 triangles, D 1e-8 cm^2/s), 20,000 receptors, "
+                           "100,000 L (D 1e-6 cm^2/s) outside the sphere, 10,000 T crossing it through a "
+                           "transparent surface class, 0.1 um subpartitions, dt 1e-6 s",
+               "molecules": wl.num_molecules, "surface_molecules": int(len(wl.surface_mols["ids"])),
+               "walls": int(len(wl.walls)), "replicas": 1}
     elif args.workload == "crowded_spheres":
         wl = W.crowded_spheres(iterations=args.steps)
         cfg = {"workload": "64 perturbed icospheres (20,480 triangles each, radii U[0.1, 0.3] um, +-2% radial "
@@ -134,6 +142,13 @@ def build_workload(args):
         cfg = {"workload": "A + B -> C (1e8 /M/s) in a reflective 1 um cube (12 walls), 50,000 A + 50,000 B "
                            "(D 1e-6 cm^2/s), C with D 5e-7, 0.05 um subpartitions, counts every 10 iterations, "
                            "dt 1e-6 s", "molecules": wl.num_molecules, "walls": int(len(wl.walls)), "replicas": 1}
+    elif args.workload == "bimol_box_timesteps":
+        wl = W.with_time_steps(W.bimol_box(iterations=args.steps), [0.5, 1, 3])
+        cfg = {"workload": "A + B -> C (1e8 /M/s) in a reflective 1 um cube (12 walls), 50,000 A + 50,000 B "
+                           "(D 1e-6 cm^2/s) with species time steps of 1/2 (A, two steps per iteration), 1 (B) "
+                           "and 3 iterations (C, D 5e-7, maturing from its reaction time), 0.05 um "
+                           "subpartitions, counts every 10 iterations, dt 1e-6 s",
+               "molecules": wl.num_molecules, "walls": int(len(wl.walls)), "replicas": 1}
     elif args.workload == "free_diffusion_box":
         wl = W.free_diffusion_box(iterations=args.steps)
         cfg = {"workload": "free diffusion, 10,000 A in a reflective 1 um cube, 0.1 um subpartitions",
@@ -219,6 +234,10 @@ def cpu_baseline(seconds, workload):
     elif workload == "volume_surface":
         wl = W.volume_surface()
         what = "the whole volume-surface case (110,000 volume molecules, 20,000 receptors, 5,132 walls)"
+    elif workload == "volume_surface_diffusing":
+        wl = W.with_diffusing_surface(W.volume_surface(), {1: 1e-8, 2: 1e-8})
+        what = ("the whole volume-surface case with diffusing receptors (110,000 volume molecules, 20,000 "
+                "receptors, 5,132 walls)")
     elif workload == "crowded_spheres":
         wl = W.crowded_spheres(n_molecules=200_000)
         what = ("the crowded geometry (1,310,732 walls, 200,000 S') with 200,000 volume molecules A:B:E = 5:4:1 "
@@ -227,6 +246,10 @@ def cpu_baseline(seconds, workload):
         wl = W.replicas_sphere_in_cube(num_replicas=1)
         what = ("instance 0 of the replica batch (A + B -> C, %d molecules, sphere-in-cube mesh, 5,132 walls), "
                 "one single-threaded process" % wl.num_molecules)
+    elif workload == "bimol_box_timesteps":
+        wl = W.with_time_steps(W.bimol_box(), [0.5, 1, 3])
+        what = ("A + B -> C, 50,000 A + 50,000 B in the reflective 1 um cube, time steps 1/2, 1 and 3 "
+                "iterations")
     else:
         wl = W.bimol_box()
         what = "A + B -> C, 50,000 A + 50,000 B in the reflective 1 um cube"
@@ -370,7 +393,8 @@ def traffic_passes(args, wl_subpart, main_prec, iterations):
     names = ("k_move_light<float>", "k_move_drain<float>") if main_prec == "fp32" else \
         ("k_move_light<double>", "k_move_drain<double>")
     wl_name = args.workload
-    if wl_name not in ("replicas_sphere", "bimol_box", "crowded_spheres", "volume_surface"):
+    if wl_name not in ("replicas_sphere", "bimol_box", "crowded_spheres", "volume_surface",
+                       "volume_surface_diffusing", "bimol_box_timesteps"):
         names = ("k_light32", "k_mid32", "k_hard32") if main_prec == "fp32" else ("k_light64", "k_mid64", "k_hard64")
     out = {}
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as tmp:

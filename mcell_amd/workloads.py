@@ -101,6 +101,19 @@ def with_time_steps(wl: Workload, factors: Sequence[float]) -> Workload:
     return replace(wl, species=sp, species_steps=steps, name=wl.name + "_timesteps")
 
 
+def with_diffusing_surface(wl: Workload, diffusion: dict) -> Workload:
+    """The workload with the surface species {index: D (cm^2/s)} diffusing
+    on the mesh (space step as for volume species, MCell3
+    src/mcell_species.c:270-276)."""
+    sp = list(wl.species)
+    steps = list(wl.species_steps)
+    for k, d in diffusion.items():
+        sp[k] = M.Species(sp[k].name, d, surface=True)
+        ss, ts = sp[k].steps(wl.part)
+        steps[k] = (ss, ts, steps[k][2] | 16)
+    return replace(wl, species=sp, species_steps=steps, name=wl.name + "_surface_diffusion")
+
+
 def species_steps(wl: Workload, replica: int):
     """(space_step, time_step, flags) of one replica's species."""
     return _species_table(wl.species_for(replica), wl.part, wl.rxns)

@@ -69,21 +69,10 @@ def test_sparse_surface_diffusion_matches_oracle(gpu_available, oracle_lib, shee
     eng.close()
 
 
-def _diffusing_receptors(wl, d_surf):
-    """The receptor workload with R and LR diffusing on the sphere."""
-    sp = list(wl.species)
-    steps = list(wl.species_steps)
-    for k in (1, 2):
-        sp[k] = M.Species(sp[k].name, d_surf, surface=True)
-        ss, ts = sp[k].steps(wl.part)
-        steps[k] = (ss, ts, steps[k][2] | 16)
-    return replace(wl, species=sp, species_steps=steps)
-
-
 def test_receptors_that_diffuse_match_oracle(gpu_available, oracle_lib):
     from oracle import pyoracle as po
     seed = 53
-    wl = _diffusing_receptors(W.receptor_sphere(n_l=20000, n_r=1500, iterations=60, rate=4e8), 1e-8)
+    wl = W.with_diffusing_surface(W.receptor_sphere(n_l=20000, n_r=1500, iterations=60, rate=4e8), {1: 1e-8, 2: 1e-8})
     eng = DiffuseReactEngine.from_workload(wl, seed=seed)
     w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
     w.set_barrier_interval(wl.barrier_interval)
