@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 8
+#define MCG_ABI_VERSION 9
 
 typedef struct mcg_ctx mcg_ctx;
 
@@ -269,20 +269,27 @@ int mcg_set_volsurf_rxns(mcg_ctx* ctx, const mcg_volsurf_rxn* rxns, uint32_t num
  * iteration's volume moves (diffuse_surf_molecule, diffuse_react_event.cpp:
  * 1071-1246): every wall then gets a grid, and the walls are linked across
  * the edges two of them share (surface_net, geometry.cpp:256-353; an edge of
- * more than two walls is refused); their time step must be the global one.
- * Volume molecules react with surface molecules through
- * mcg_set_volsurf_rxns.  ids share the replica's id space with the volume
- * molecules.  Call after mcg_set_geometry and mcg_set_species. */
+ * more than two walls is refused).  Their species' time steps follow the
+ * volume molecules' rules: a step below one iteration is taken again in the
+ * same iteration, a long step is shortened for a young molecule until it is
+ * mature (get_max_time, diffuse_react_event.cpp:164-198); flags may be NULL
+ * (none) or carry MCG_MOLF_MATURE only, birthdays (absolute, not after the
+ * diffusion time) may be NULL or TIME_INVALID (-256): the clock.  Volume
+ * molecules react with surface molecules through mcg_set_volsurf_rxns.  ids
+ * share the replica's id space with the volume molecules.  Call after
+ * mcg_set_geometry and mcg_set_species. */
 int mcg_add_surface_molecules(mcg_ctx* ctx, uint32_t replica, const uint32_t* ids, const uint32_t* species,
                               const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, const double* uv,
-                              const double* diffusion_times, uint64_t count);
+                              const double* diffusion_times, const uint32_t* flags, const double* birthdays,
+                              uint64_t count);
 
 /* The surface molecules back, in (replica, wall, tile) order, with their uv
- * positions and diffusion times.  Any output pointer may be NULL; *count
- * gets the number. */
+ * positions, diffusion times, flags and birthdays (the birthdays of products
+ * made while no surface species diffuses are not kept).  Any output pointer
+ * may be NULL; *count gets the number. */
 int mcg_get_surface_molecules(mcg_ctx* ctx, uint32_t* ids, uint32_t* species, uint32_t* replica, uint32_t* walls,
                               uint32_t* tiles, int32_t* orients, double* uv, double* diffusion_times,
-                              uint64_t capacity, uint64_t* count);
+                              uint32_t* flags, double* birthdays, uint64_t capacity, uint64_t* count);
 
 /* Keeps up to `capacity` reaction events of each mcg_step call (0 = off). */
 int mcg_set_rxn_log(mcg_ctx* ctx, uint64_t capacity);

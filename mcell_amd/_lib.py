@@ -199,8 +199,8 @@ def load():
     lib.mcg_set_unimol_rxns.argtypes = [P, ctypes.POINTER(McgUnimolRxn), ctypes.c_uint32]
     lib.mcg_set_replica_params.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(McgSpecies), P]
     lib.mcg_set_volsurf_rxns.argtypes = [P, ctypes.POINTER(McgVolSurfRxn), ctypes.c_uint32]
-    lib.mcg_add_surface_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, P, ctypes.c_uint64]
-    lib.mcg_get_surface_molecules.argtypes = [P, P, P, P, P, P, P, P, P, ctypes.c_uint64,
+    lib.mcg_add_surface_molecules.argtypes = [P, ctypes.c_uint32, P, P, P, P, P, P, P, P, P, ctypes.c_uint64]
+    lib.mcg_get_surface_molecules.argtypes = [P, P, P, P, P, P, P, P, P, P, P, ctypes.c_uint64,
                                               ctypes.POINTER(ctypes.c_uint64)]
     for name in EXPORTED:
         if not name.startswith("mcg_last_error") and name != "mcg_abi_version":

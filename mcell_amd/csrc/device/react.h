@@ -75,7 +75,13 @@ constexpr uint32_t kPathwayMask = 0xFFu;
 constexpr uint32_t kVolSurfBit = 0x40000000u;  // Proposal::rxn of a volume-surface reaction: bit | class
 constexpr uint32_t kTileBit = 0x80000000u;     // Proposal::partner_slot naming a tile: bit | tile slot
 constexpr uint32_t kTileEmpty = 0xFFFFFFFFu;
-// tile word: species (8 bits) | orientation + 1 (2 bits) << 8
+// tile word: species (8 bits) | orientation + 1 (2 bits) << 8 | mature
+// (MOLECULE_FLAG_MATURE: a surface molecule with a long time step whose
+// steps no longer lengthen) << 10 | delayed << 11 (within an iteration's
+// surface phase: the molecule's diffusion time did not fall on the
+// iteration's start, so its steps come after the others')
+constexpr uint32_t kTileMature = 1u << 10;
+constexpr uint32_t kTileDelayed = 1u << 11;
 __device__ __host__ __forceinline__ uint32_t tile_species(uint32_t w) { return w & 0xFFu; }
 __device__ __host__ __forceinline__ int tile_orient(uint32_t w) { return (int)((w >> 8) & 3u) - 1; }
 __device__ __host__ __forceinline__ uint32_t make_tile(uint32_t species, int orient) {

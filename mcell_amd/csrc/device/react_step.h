@@ -70,6 +70,7 @@ struct ReactParams {
   // a surface species diffuses): a surface product moves from its
   // reaction time on
   double* __restrict__ ttoff;
+  double* __restrict__ tbirth;  // their birthdays (absolute), alongside ttoff
 };
 
 // xyz2grid_tile_index (grid_utils.inl:48-118) for a point on the wall.

@@ -130,13 +130,14 @@ struct ReactState {
   bool tiles_on_device = false;  // steps may have changed the device copy
   DevBuf<WallGrid> grids_d;
   DevBuf<uint32_t> tiles, tile_id, tile_owner, tile_consumed;
-  // surface molecules' positions in their walls' uv frames and diffusion-
-  // time offsets, per tile slot (host copies; on the device while surface
-  // species diffuse), and the surface phase's buffers
-  std::vector<double> tu_host, tv_host, ttoff_host;
+  // surface molecules' positions in their walls' uv frames, diffusion-
+  // time offsets and birthdays, per tile slot (host copies; on the device
+  // while surface species diffuse), and the surface phase's buffers
+  std::vector<double> tu_host, tv_host, ttoff_host, tbirth_host;
   bool surf_on = false;  // some surface species diffuses: grids on every wall, the surface phase runs
-  DevBuf<double> tu, tv, ttoff, spu, spv, sptoff;
-  DevBuf<uint32_t> sclaim, sprop, splist, slose0, slose1, sdone;
+  DevBuf<double> tu, tv, ttoff, tbirth, spu, spv;
+  DevBuf<unsigned long long> sclaim;
+  DevBuf<uint32_t> sprop, splist, slose0, slose1, sdone, scont0, scont1;
   DevBuf<WallEdgeRec> edges_d;
   // reaction log of the last mcg_step call
   DevBuf<RxnEvent> log;

@@ -120,6 +120,7 @@ ReactPipeline<PosT> make_pipeline(mcg_ctx* c, PosT* x, PosT* y, PosT* z, PosT* s
   p.clist = r.clist.p;
   p.R.clist = r.clist.p;
   p.R.ttoff = r.surf_on ? r.ttoff.p : nullptr;
+  p.R.tbirth = r.surf_on ? r.tbirth.p : nullptr;
   p.R.ccount = r.hdr.p + kHdrContinue;
   p.keys_a = r.keys_a.p;
   p.keys_b = r.keys_b.p;
@@ -565,6 +566,8 @@ int mcg_set_unimol_rxns(mcg_ctx* c, const mcg_unimol_rxn* rxns, uint32_t n) {
   for (uint32_t i = 0; i < n; ++i) {
     const mcg_unimol_rxn& x = rxns[i];
     if (x.reactant >= ns) return fail(MCG_E_INVALID, "reactant species out of range");
+    if (c->species_host[x.reactant].flags & SPF_SURFACE)
+      return fail(MCG_E_UNSUPPORTED, "unimolecular classes of surface species are not on this path");
     if (u[x.reactant].num_pathways) return fail(MCG_E_INVALID, "one unimolecular class per species");
     if (x.num_pathways < 1 || x.num_pathways > (uint32_t)kMaxPathways) return fail(MCG_E_INVALID, "1..4 pathways");
     double last = 0;
@@ -786,12 +789,13 @@ WallGrid make_wall_grid(const WallRec64& w, uint32_t tile_base) {
 // appended after the existing tiles), molecules kept in their slots.
 void relayout_tiles(const ReactState& r, uint32_t R, uint64_t total, std::vector<uint32_t>& nt,
                     std::vector<uint32_t>& nid, std::vector<double>& nu, std::vector<double>& nv,
-                    std::vector<double>& nto) {
+                    std::vector<double>& nto, std::vector<double>& nb) {
   nt.assign(total * R, kTileEmpty);
   nid.assign(total * R, kInvalid);
   nu.assign(total * R, 0.0);
   nv.assign(total * R, 0.0);
   nto.assign(total * R, 0.0);
+  nb.assign(total * R, 0.0);
   const bool pos = r.tu_host.size() == (size_t)r.total_tiles * R;
   for (uint32_t rep = 0; rep < R; ++rep)
     for (uint32_t t = 0; t < r.total_tiles; ++t) {
@@ -802,6 +806,7 @@ void relayout_tiles(const ReactState& r, uint32_t R, uint64_t total, std::vector
         nu[a] = r.tu_host[b];
         nv[a] = r.tv_host[b];
         nto[a] = r.ttoff_host[b];
+        nb[a] = r.tbirth_host[b];
       }
     }
 }
@@ -820,6 +825,7 @@ int download_tiles(mcg_ctx* c) {
     HIP_TRY(hipMemcpy(r.tu_host.data(), r.tu.p, nt * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(r.tv_host.data(), r.tv.p, nt * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(r.ttoff_host.data(), r.ttoff.p, nt * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(r.tbirth_host.data(), r.tbirth.p, nt * 8, hipMemcpyDeviceToHost));
   }
   return MCG_OK;
 }
@@ -854,9 +860,10 @@ int sync_tiles_to_device(mcg_ctx* c, bool allow_alloc) {
   if (r.surf_on) {
     if (r.tu.n != nt) {
       if (!allow_alloc) return fail(MCG_E_STATE, "surface molecules on new walls need tile buffers");
-      if (r.tu.alloc(nt) || r.tv.alloc(nt) || r.ttoff.alloc(nt) || r.spu.alloc(nt) || r.spv.alloc(nt) ||
-          r.sptoff.alloc(nt) || r.sclaim.alloc(nt) || r.sprop.alloc(nt) || r.splist.alloc(nt) ||
-          r.slose0.alloc(nt) || r.slose1.alloc(nt) || r.sdone.alloc(nt))
+      if (r.tu.alloc(nt) || r.tv.alloc(nt) || r.ttoff.alloc(nt) || r.tbirth.alloc(nt) || r.spu.alloc(nt) ||
+          r.spv.alloc(nt) || r.sclaim.alloc(nt) || r.sprop.alloc(nt) || r.splist.alloc(nt) ||
+          r.slose0.alloc(nt) || r.slose1.alloc(nt) || r.sdone.alloc(nt) || r.scont0.alloc(nt) ||
+          r.scont1.alloc(nt))
         return fail(MCG_E_NOMEM, "surface phase buffers");
       HIP_TRY(launch_fill_claims(c->stream, r.sclaim.p, (uint32_t)nt));
     }
@@ -867,6 +874,7 @@ int sync_tiles_to_device(mcg_ctx* c, bool allow_alloc) {
     HIP_TRY(hipMemcpy(r.tu.p, r.tu_host.data(), nt * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(r.tv.p, r.tv_host.data(), nt * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(r.ttoff.p, r.ttoff_host.data(), nt * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(r.tbirth.p, r.tbirth_host.data(), nt * 8, hipMemcpyHostToDevice));
   }
   r.tiles_dirty = false;
   r.tiles_on_device = true;
@@ -957,11 +965,7 @@ int ensure_surface_diffusion(mcg_ctx* c) {
   if (r.surf_on) return MCG_OK;
   bool on = false;
   for (const SpeciesRec& s : c->per_replica ? c->species_replicas : c->species_host)
-    if ((s.flags & SPF_SURFACE) && s.space_step > 0) {
-      if (!(s.time_step == 1.0))
-        return fail(MCG_E_UNSUPPORTED, "surface species diffuse with the global time step on this path");
-      on = true;
-    }
+    if ((s.flags & SPF_SURFACE) && s.space_step > 0) on = true;
   if (!on || r.total_tiles == 0) return MCG_OK;
   if (int st = download_tiles(c)) return st;
   // every wall gets its grid: a molecule may reach any wall of the mesh
@@ -977,8 +981,8 @@ int ensure_surface_diffusion(mcg_ctx* c) {
     if (total * R > 0xFFFFFFFFull) return fail(MCG_E_LIMIT, "too many tiles (tiles x replicas >= 2^32)");
   }
   std::vector<uint32_t> nt, nid;
-  std::vector<double> nu, nv, nto;
-  relayout_tiles(r, R, total, nt, nid, nu, nv, nto);
+  std::vector<double> nu, nv, nto, nb;
+  relayout_tiles(r, R, total, nt, nid, nu, nv, nto, nb);
   std::vector<WallEdgeRec> edges;
   if (int st = build_surface_net(c->walls_host, edges)) return st;
   if (r.edges_d.alloc(edges.size())) return fail(MCG_E_NOMEM, "surface net");
@@ -989,6 +993,7 @@ int ensure_surface_diffusion(mcg_ctx* c) {
   r.tu_host.swap(nu);
   r.tv_host.swap(nv);
   r.ttoff_host.swap(nto);
+  r.tbirth_host.swap(nb);
   r.total_tiles = (uint32_t)total;
   r.tiles_dirty = true;
   r.surf_on = true;
@@ -1011,11 +1016,11 @@ int surface_phase(mcg_ctx* c, uint32_t iteration, double barrier, uint32_t* err)
   P.tu = r.tu.p;
   P.tv = r.tv.p;
   P.ttoff = r.ttoff.p;
+  P.birth = r.tbirth.p;
   P.claim = r.sclaim.p;
   P.prop = r.sprop.p;
   P.pu = r.spu.p;
   P.pv = r.spv.p;
-  P.ptoff = r.sptoff.p;
   P.plist = r.splist.p;
   P.done = r.sdone.p;
   P.hdr = r.hdr.p;
@@ -1024,19 +1029,39 @@ int surface_phase(mcg_ctx* c, uint32_t iteration, double barrier, uint32_t* err)
   P.num_replicas = R;
   P.seed = c->cfg.seed;
   P.iteration = iteration;
+  // never 0xFFFFFFF0, the high bits of a slot no iteration has stamped
+  P.stamp_base = ((iteration + 1u) % 0x0FFFFFFFu) << 4;
   P.barrier = barrier;
   const uint32_t n_all = r.total_tiles * R;
   uint32_t* lists[2] = {r.slose0.p, r.slose1.p};
   const uint32_t words[2] = {kHdrSurfLose0, kHdrSurfLose1};
-  HIP_TRY(surf_round(c->stream, P, nullptr, nullptr, n_all, lists[0], r.hdr.p + words[0]));
-  for (uint32_t round = 0;; ++round) {
-    if (int st = read_hdr(c)) return st;
-    const uint32_t nl = h[words[round & 1]];
-    if (nl == 0) return MCG_OK;
-    if (round > 100000) return fail(MCG_E_LIMIT, "surface moves did not settle");
-    HIP_TRY(surf_round(c->stream, P, lists[round & 1], r.hdr.p + words[round & 1], nl, lists[(round + 1) & 1],
-                       r.hdr.p + words[(round + 1) & 1]));
+  uint32_t* conts[2] = {r.scont0.p, r.scont1.p};
+  const uint32_t cwords[2] = {kHdrSurfCont0, kHdrSurfCont1};
+  // pass k: the molecules' k-th steps of the iteration; the first over every
+  // slot, the others over the molecules the previous pass listed
+  const uint32_t* in_list = nullptr;
+  const uint32_t* in_count = nullptr;
+  uint32_t n_in = n_all;
+  for (uint32_t pass = 0; pass < kSurfPasses; ++pass) {
+    P.pass = pass;
+    P.clist = conts[pass & 1];
+    P.ccount = r.hdr.p + cwords[pass & 1];
+    HIP_TRY(launch_zero_word(c->stream, P.ccount));
+    HIP_TRY(surf_round(c->stream, P, in_list, in_count, n_in, lists[0], r.hdr.p + words[0]));
+    for (uint32_t round = 0;; ++round) {
+      if (int st = read_hdr(c)) return st;
+      const uint32_t nl = h[words[round & 1]];
+      if (nl == 0) break;
+      if (round > 100000) return fail(MCG_E_LIMIT, "surface moves did not settle");
+      HIP_TRY(surf_round(c->stream, P, lists[round & 1], r.hdr.p + words[round & 1], nl, lists[(round + 1) & 1],
+                         r.hdr.p + words[(round + 1) & 1]));
+    }
+    n_in = h[cwords[pass & 1]];
+    if (n_in == 0) return MCG_OK;
+    in_list = conts[pass & 1];
+    in_count = r.hdr.p + cwords[pass & 1];
   }
+  return MCG_OK;  // a molecule listed past the last pass raised ERR_PASSES
 }
 
 }  // namespace mcg
@@ -1128,7 +1153,7 @@ int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
 
 int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids, const uint32_t* species,
                               const uint32_t* walls, const uint32_t* tiles, const int32_t* orients, const double* uv,
-                              const double* dtimes, uint64_t count) {
+                              const double* dtimes, const uint32_t* flags, const double* birthdays, uint64_t count) {
   if (!c || (count && (!ids || !species || !walls || !tiles || !orients))) return fail(MCG_E_INVALID, "null argument");
   MCG_ALIVE(c);
   if (!c->have_geometry) return fail(MCG_E_STATE, "surface molecules need the geometry first");
@@ -1143,6 +1168,9 @@ int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids,
     if (species[i] >= c->num_species) return fail(MCG_E_INVALID, "species id out of range");
     if (orients[i] != 1 && orients[i] != -1) return fail(MCG_E_INVALID, "surface molecules are oriented +1 or -1");
     if (dtimes && !(dtimes[i] >= (double)c->clock)) return fail(MCG_E_INVALID, "diffusion time before the engine's clock");
+    if (flags && (flags[i] & ~MCG_MOLF_MATURE)) return fail(MCG_E_INVALID, "surface molecules take the mature flag only");
+    if (birthdays && birthdays[i] != kTimeInvalid && !(birthdays[i] <= (dtimes ? dtimes[i] : (double)c->clock)))
+      return fail(MCG_E_INVALID, "birthday after the diffusion time");
   }
   if (int st = download_tiles(c)) return st;
   // grids for walls without one, appended after the existing tiles
@@ -1159,12 +1187,12 @@ int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids,
   for (uint64_t i = 0; i < count; ++i)
     if (tiles[i] >= g[walls[i]].num_tiles) return fail(MCG_E_INVALID, "tile index out of range");
   std::vector<uint32_t> nt, nid;
-  std::vector<double> nu, nv, nto;
-  relayout_tiles(r, R, total, nt, nid, nu, nv, nto);
+  std::vector<double> nu, nv, nto, nb;
+  relayout_tiles(r, R, total, nt, nid, nu, nv, nto, nb);
   for (uint64_t i = 0; i < count; ++i) {
     const size_t slot = replica * total + g[walls[i]].tile_base + tiles[i];
     if (nt[slot] != kTileEmpty) return fail(MCG_E_INVALID, "tile already occupied");
-    nt[slot] = make_tile(species[i], orients[i]);
+    nt[slot] = make_tile(species[i], orients[i]) | ((flags && (flags[i] & MCG_MOLF_MATURE)) ? kTileMature : 0u);
     nid[slot] = ids[i];
     // the position on the wall: given, else the tile's centre (grid2uv)
     if (uv) {
@@ -1174,6 +1202,7 @@ int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids,
       grid2uv(c->walls_host[walls[i]], g[walls[i]], tiles[i], nu[slot], nv[slot]);
     }
     nto[slot] = dtimes ? dtimes[i] - (double)c->clock : 0.0;
+    nb[slot] = (birthdays && birthdays[i] != kTimeInvalid) ? birthdays[i] : (double)c->clock;
   }
   // the id counters (the device holds them once reactive steps have run)
   if (r.on && !r.next_id_dirty && r.next_id_host.size() == R)
@@ -1187,14 +1216,15 @@ int mcg_add_surface_molecules(mcg_ctx* c, uint32_t replica, const uint32_t* ids,
   r.tu_host.swap(nu);
   r.tv_host.swap(nv);
   r.ttoff_host.swap(nto);
+  r.tbirth_host.swap(nb);
   r.total_tiles = (uint32_t)total;
   r.tiles_dirty = true;
   return MCG_OK;
 }
 
 int mcg_get_surface_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint32_t* replica, uint32_t* walls,
-                              uint32_t* tiles, int32_t* orients, double* uv, double* dtimes, uint64_t capacity,
-                              uint64_t* count) {
+                              uint32_t* tiles, int32_t* orients, double* uv, double* dtimes, uint32_t* flags,
+                              double* birthdays, uint64_t capacity, uint64_t* count) {
   if (!c) return fail(MCG_E_INVALID, "null context");
   MCG_ALIVE(c);
   ReactState& r = c->rx;
@@ -1219,6 +1249,8 @@ int mcg_get_surface_molecules(mcg_ctx* c, uint32_t* ids, uint32_t* species, uint
             uv[2 * k + 1] = r.tv_host[slot];
           }
           if (dtimes) dtimes[k] = (double)c->clock + r.ttoff_host[slot];
+          if (flags) flags[k] = (tw & kTileMature) ? MCG_MOLF_MATURE : 0u;
+          if (birthdays) birthdays[k] = r.tbirth_host[slot];
         }
         ++k;
       }

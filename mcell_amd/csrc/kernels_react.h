@@ -54,7 +54,9 @@ enum : uint32_t {
   kHdrSurfProps = 12,     // surface phase: moves to other tiles proposed in the round
   kHdrSurfLose0 = 13,     // surface phase: movers whose tile went to a smaller id (two lists)
   kHdrSurfLose1 = 14,
-  kHdrWords = 16,
+  kHdrSurfCont0 = 15,     // surface phase: molecules that step again in this iteration (two lists)
+  kHdrSurfCont1 = 16,
+  kHdrWords = 18,
 };
 
 template <typename PosT>

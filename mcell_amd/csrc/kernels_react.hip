@@ -896,7 +896,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
       const int o = vs.product_orient[pw][q] != 0 ? vs.product_orient[pw][q] * flip : (((drawn >> q) & 1u) ? 1 : -1);
       const uint32_t ps = vs.products[pw][q];
       if (vs.keep_surf[pw] == q + 1) {
-        tiles[ts] = make_tile(ps, o);
+        tiles[ts] = make_tile(ps, o) | (tiles[ts] & kTileMature);
         continue;
       }
       const uint32_t id = first_id + i;
@@ -915,7 +915,10 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
         // it takes the receptor's position on the tile (GridPos::
         // make_with_mol_pos, diffuse_react_event.cpp:2040-2050) and diffuses
         // from its reaction time on
-        if (R.ttoff) R.ttoff[ts] = p.time - (double)R.iteration;
+        if (R.ttoff) {
+          R.ttoff[ts] = p.time - (double)R.iteration;
+          R.tbirth[ts] = p.time;
+        }
         out.tag[slot] = make_tag(0, MF_DEFUNCT, rep);
         consumed[slot] = kInvalid;
         if (R.born) R.born[slot] = 0;

@@ -22,27 +22,39 @@ struct SurfPhase {
   double* tu;                                // molecule positions in the wall's uv frame
   double* tv;
   double* ttoff;                             // diffusion-time offsets from the iteration's start
-  uint32_t* claim;                           // per slot: smallest id proposing to move there
+  double* birth;                             // birthdays (absolute)
+  unsigned long long* claim;                 // per slot: smallest order key proposing to move there
   uint32_t* prop;                            // per mover slot: the slot it proposes
-  double* pu;                                // the proposed position and offset
+  double* pu;                                // the proposed position
   double* pv;
-  double* ptoff;
   uint32_t* plist;                           // proposing slots of the round
-  uint32_t* done;                            // per slot: iteration + 1 once its molecule's step is over
+  // per slot: the iteration's stamp once its molecule's step of a pass is
+  // over, stamp_base | kDoneFinal when it has no further step in the
+  // iteration, stamp_base | (pass + 1) when it steps again in the next pass
+  uint32_t* done;
+  uint32_t* clist;                           // slots of the molecules that step again (next pass)
+  uint32_t* ccount;
   uint32_t* hdr;
   uint32_t* err;
   uint32_t total_tiles, num_replicas;
   uint64_t seed;
   uint32_t iteration;
+  uint32_t pass;                             // the pass (a molecule's n-th step in the iteration)
+  uint32_t stamp_base;                       // ((iteration + 1) mod 2^28) << 4
   double barrier;                            // time up to the next barrier from the iteration's start
 };
 
-// One round of the phase over `list` (every slot when null; else the
-// movers of the previous round that lost a tile), then the claims'
-// settlement: winners move, losers are listed for the next round.
+constexpr uint32_t kDoneFinal = 15u;
+constexpr uint32_t kSurfPasses = 8;          // steps per iteration (time steps of at least 1/8)
+
+// One round of a pass over `list` (every slot when null: the first pass's
+// first round; else the molecules stepping again in this pass, or the
+// movers of the previous round that lost a tile or waited), then the
+// claims' settlement: winners move, losers are listed for the next round.
 hipError_t surf_round(hipStream_t s, const SurfPhase& P, const uint32_t* list, const uint32_t* count,
                       uint32_t max_movers, uint32_t* lose_list, uint32_t* lose_count);
 hipError_t launch_fill_done(hipStream_t s, uint32_t* done, uint32_t n);
-hipError_t launch_fill_claims(hipStream_t s, uint32_t* claim, uint32_t n);
+hipError_t launch_zero_word(hipStream_t s, uint32_t* word);
+hipError_t launch_fill_claims(hipStream_t s, unsigned long long* claim, uint32_t n);
 
 }  // namespace mcg

@@ -196,33 +196,38 @@ class DiffuseReactEngine:
         _lib.check(self.lib.mcg_set_volsurf_rxns(self.ctx, arr, len(rxns)))
 
     def add_surface_molecules(self, replica: int, ids, species, walls, tiles, orients, uv=None,
-                              diffusion_times=None):
+                              diffusion_times=None, flags=None, birthdays=None):
         """uv: (n, 2) positions in the walls' frames (None: the tiles'
-        centres); diffusion_times: absolute (None: the clock)."""
+        centres); diffusion_times and birthdays: absolute (None: the clock);
+        flags: MCG_MOLF_MATURE or 0 (None: 0)."""
         a = [np.ascontiguousarray(x, dtype=np.uint32) for x in (ids, species, walls, tiles)]
         o = np.ascontiguousarray(orients, dtype=np.int32)
         assert all(len(x) == len(o) for x in a)
         u = None if uv is None else np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
         dt = None if diffusion_times is None else np.ascontiguousarray(diffusion_times, dtype=np.float64)
-        assert u is None or len(u) == len(o)
-        assert dt is None or len(dt) == len(o)
+        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint32)
+        bd = None if birthdays is None else np.ascontiguousarray(birthdays, dtype=np.float64)
+        assert all(x is None or len(x) == len(o) for x in (u, dt, fl, bd))
         _lib.check(self.lib.mcg_add_surface_molecules(self.ctx, replica, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
-                                                      _ptr(a[3]), _ptr(o), _ptr(u), _ptr(dt), len(o)))
+                                                      _ptr(a[3]), _ptr(o), _ptr(u), _ptr(dt), _ptr(fl), _ptr(bd),
+                                                      len(o)))
 
     def surface_molecules(self):
         """The surface molecules in (replica, wall, tile) order."""
         cnt = ctypes.c_uint64()
-        _lib.check(self.lib.mcg_get_surface_molecules(self.ctx, None, None, None, None, None, None, None, None, 0,
-                                                      ctypes.byref(cnt)))
+        _lib.check(self.lib.mcg_get_surface_molecules(self.ctx, None, None, None, None, None, None, None, None,
+                                                      None, None, 0, ctypes.byref(cnt)))
         n = cnt.value
-        out = {k: np.zeros(n, np.uint32) for k in ("id", "species", "replica", "wall", "tile")}
+        out = {k: np.zeros(n, np.uint32) for k in ("id", "species", "replica", "wall", "tile", "flags")}
         out["orient"] = np.zeros(n, np.int32)
         out["uv"] = np.zeros((n, 2), np.float64)
         out["diffusion_time"] = np.zeros(n, np.float64)
+        out["birthday"] = np.zeros(n, np.float64)
         _lib.check(self.lib.mcg_get_surface_molecules(self.ctx, _ptr(out["id"]), _ptr(out["species"]),
                                                       _ptr(out["replica"]), _ptr(out["wall"]), _ptr(out["tile"]),
                                                       _ptr(out["orient"]), _ptr(out["uv"]),
-                                                      _ptr(out["diffusion_time"]), n, ctypes.byref(cnt)))
+                                                      _ptr(out["diffusion_time"]), _ptr(out["flags"]),
+                                                      _ptr(out["birthday"]), n, ctypes.byref(cnt)))
         return out
 
     def set_rxn_log(self, capacity: int):

@@ -31,7 +31,7 @@ def test_abi_version_and_struct_sizes(tmp_path):
     import shutil
     import subprocess
     lib = _lib.load()
-    assert lib.mcg_abi_version() == 8
+    assert lib.mcg_abi_version() == 9
     sizes = (ctypes.sizeof(_lib.McgConfig), ctypes.sizeof(_lib.McgSpecies), ctypes.sizeof(_lib.McgStepStats),
              ctypes.sizeof(_lib.McgBimolRxn), ctypes.sizeof(_lib.McgRxnEvent), ctypes.sizeof(_lib.McgUnimolRxn),
              ctypes.sizeof(_lib.McgVolSurfRxn))

@@ -16,7 +16,14 @@ iteration the runs are equal.
 * At size: the sheet's 2-D mean squared displacement is 4 D t; at high
   density on the sphere every molecule keeps a tile of its own, the one
   under its position, and none is lost.
-* A download and upload in the middle of a run continues it bit for bit.
+* Species time steps (diffuse_single_molecule's rescheduling and
+  get_max_time's maturity ramp, diffuse_react_event.cpp:164-198, 287-318):
+  steps of 1/2 and 1/3 iteration taken again in the same iteration, a step
+  of 3 iterations lengthening from one iteration as the molecule ages,
+  against the oracle (walls, tiles, uv, diffusion times, birthdays and the
+  mature flag).
+* A download and upload in the middle of a run continues it bit for bit,
+  also with the maturity ramp under way.
 """
 import math
 from dataclasses import replace
@@ -24,6 +31,7 @@ from dataclasses import replace
 import numpy as np
 import pytest
 
+from mcell_amd import _lib
 from mcell_amd import model as M
 from mcell_amd import workloads as W
 from mcell_amd.engine import DiffuseReactEngine
@@ -188,5 +196,73 @@ def test_surface_round_trip_continues_exactly(gpu_available):
     cont = _surface_by_id(c)
     for k in ("id", "wall", "tile", "orient", "uv", "diffusion_time"):
         assert np.array_equal(whole[k], cont[k]), k
+    a.close()
+    c.close()
+
+
+def _surface_time_steps(iterations, barrier_interval=7, n_s=600, n_sheet=200):
+    """S, S2, S3 on the sphere and the sheet (ids round-robin over the
+    species) with time steps of 1/2, 3 and 1/3 iterations."""
+    wl = W.surface_diffusion(n_s=n_s, iterations=iterations, sheet=True, n_sheet=n_sheet, d_surf=2e-7,
+                             barrier_interval=barrier_interval)
+    sp = [wl.species[0], M.Species("S2", 3e-7, surface=True), M.Species("S3", 1e-7, surface=True)]
+    steps = [(ss, ts, f | 16) for ss, ts, f in W._species_table(sp, wl.part, [])]
+    sm = dict(wl.surface_mols)
+    sm["species"] = (np.arange(len(sm["ids"])) % 3).astype(np.uint32)
+    wl = replace(wl, species=sp, species_steps=steps, surface_mols=sm)
+    return W.with_time_steps(wl, [0.5, 3.0, 1.0 / 3.0])
+
+
+def test_surface_time_steps_match_oracle(gpu_available, oracle_lib):
+    from oracle import pyoracle as po
+    seed = 57
+    wl = _surface_time_steps(iterations=42)
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
+    w.set_barrier_interval(wl.barrier_interval)
+    for chunk in range(0, wl.iterations, 14):
+        eng.step(chunk, 14)
+        w.run(14)
+        n = _compare_surface(eng, w, "iteration %d" % (chunk + 14))
+        g = _surface_by_id(eng)
+        om = w.molecules()
+        ids = g["id"]
+        dt = np.abs(g["diffusion_time"] - om["diffusion_time"][ids]).max(initial=0)
+        assert dt <= 1e-12 * (chunk + 14), "diffusion times differ by %g" % dt
+        assert np.array_equal(g["birthday"], om["birthday"][ids])
+        mature = (om["flags"][ids] & 2) != 0
+        assert np.array_equal((g["flags"] & _lib.MCG_MOLF_MATURE) != 0, mature)
+        print("iteration %d: %d surface molecules, %d mature, %s" % (chunk + 14, n, mature.sum(), w.surface_stats()),
+              flush=True)
+    # the long-step species matured during the run (age 1 + 0.2 t > 3 from t = 10)
+    assert mature[g["species"] == 1].all() and not mature[g["species"] != 1].any()
+    # every molecule's diffusion time lies past the run's end, within its step
+    assert np.all(g["diffusion_time"] >= wl.iterations - 1e-9)
+    assert np.all(g["diffusion_time"][g["species"] != 1] <= wl.iterations + 1e-9)
+    w.close()
+    eng.close()
+
+
+def test_surface_time_steps_round_trip(gpu_available):
+    wl = _surface_time_steps(iterations=20, barrier_interval=50)
+    first, second = 5, 15  # the long-step species still young at the split
+    a = DiffuseReactEngine.from_workload(wl, seed=61)
+    a.step(0, first + second)
+    whole = _surface_by_id(a)
+    b = DiffuseReactEngine.from_workload(wl, seed=61)
+    b.step(0, first)
+    m = b.surface_molecules()
+    nid = b.next_molecule_ids()
+    b.close()
+    assert not (m["flags"] & _lib.MCG_MOLF_MATURE).any() and np.any(m["diffusion_time"] > first)
+    c = DiffuseReactEngine.from_workload(wl, seed=61, add_molecules=False, start_iteration=first)
+    c.add_surface_molecules(0, m["id"], m["species"], m["wall"], m["tile"], m["orient"], uv=m["uv"],
+                            diffusion_times=m["diffusion_time"], flags=m["flags"], birthdays=m["birthday"])
+    c.set_next_molecule_ids(nid)
+    c.step(first, second)
+    cont = _surface_by_id(c)
+    for k in ("id", "wall", "tile", "orient", "uv", "diffusion_time", "flags", "birthday"):
+        assert np.array_equal(whole[k], cont[k]), k
+    assert (whole["flags"][whole["species"] == 1] & _lib.MCG_MOLF_MATURE).all()
     a.close()
     c.close()

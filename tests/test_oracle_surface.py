@@ -14,6 +14,11 @@ forms.
 * pick_surf_displacement's law (diffusion_utils.inl:60-93): on the flat
   sheet, away from its border, the 2-D mean squared displacement is 4 D t
   (5 sigma); walls are crossed at the diagonal.
+* Species time steps on the sheet: 4 D t over the time each molecule has
+  diffused, for steps of 1/2 iteration (two per iteration) and of 3
+  iterations (lengthening from one as the molecule ages: get_max_time,
+  diffuse_react_event.cpp:164-198), whose diffusion times follow the
+  ramp's closed form.
 * Reflection at an open mesh's border (ray_trace_surf, diffuse_react_event.
   cpp:1685-1720): molecules started next to the sheet's outer edges stay on
   it, every one on its own tile, the tile the one uv2grid gives for its
@@ -200,6 +205,75 @@ def test_sheet_msd_is_4dt(oracle_lib):
     assert abs(msd - expect) < 5 * expect / math.sqrt(n), (msd, expect)
     assert np.allclose(xyz[:, 2], xyz0[0, 2])  # on the sheet's plane
     assert w.surface_stats()["moves_between_walls"] > 100
+    w.close()
+
+
+def _ramp_times(t_end, step, barrier):
+    """Diffusion times of a molecule born at 0 with a long time step, step
+    by step (MCell3's age-dependent clipping: max_time = min(step, time to
+    the barrier, 1 + 0.2 (t - birthday)) until 1 + 0.2 (t - birthday) >
+    step)."""
+    t, mature, out = 0.0, False, []
+    while t < t_end:
+        m = min(step, barrier - t % barrier if t % barrier else barrier)
+        if not mature:
+            age = 1.0 + 0.2 * t
+            m = min(m, age)
+            mature = age > step
+        t += m
+        out.append(t)
+    return out
+
+
+def test_sheet_time_steps_msd_is_4dt(oracle_lib):
+    """250 molecules of each of two species within 35 length units of the
+    middle of the sheet's diagonal (a fifth of the tiles; the border 15
+    away), time steps 1/2 and 3 iterations, 60 iterations without a barrier
+    in between: each species' 2-D mean squared displacement is 4 D (its
+    diffusion time), and the long step's diffusion time is the ramp's."""
+    from mcell_amd import model as M
+    from dataclasses import replace
+    n, iters, d = 250, 60, 1e-7
+    wl = W.surface_diffusion(n_s=0, iterations=iters, sheet=True, n_sheet=0, d_surf=d, barrier_interval=100)
+    sp = [wl.species[0], M.Species("S2", d, surface=True)]
+    steps = [(ss, ts, f | 16) for ss, ts, f in W._species_table(sp, wl.part, [])]
+    wl = W.with_time_steps(replace(wl, species=sp, species_steps=steps), [0.5, 3.0])
+    w = _world(wl)
+    nw = len(wl.walls)
+    rng = np.random.default_rng(19)
+    used, tiles, uvs = set(), [], []
+    _, _, _, p1, p2 = _frame(wl, nw - 2)
+    while len(tiles) < 2 * n:
+        uv = 0.5 * p2 + rng.uniform(-35, 35, size=2)
+        if not _inside(uv, p1, p2, 1e-3):
+            continue
+        t = w.uv2grid(nw - 2, uv[0], uv[1])
+        if t in used:
+            continue
+        used.add(t)
+        tiles.append(t)
+        uvs.append(uv)
+    for k in range(2):
+        w.add_surface_molecules(k, [nw - 2] * n, tiles[k * n:(k + 1) * n], np.ones(n, np.int32),
+                                uv=np.array(uvs[k * n:(k + 1) * n]))
+    xyz0 = np.array([_xyz(wl, nw - 2, p) for p in uvs])
+    w.run(iters)
+    wa, ta, uv = w.surface_state()
+    m = w.molecules()
+    xyz = np.array([_xyz(wl, int(a), b) for a, b in zip(wa, uv)])
+    d2 = ((xyz - xyz0) ** 2).sum(axis=1)
+    unit = 4 * d * wl.part.time_unit * 1e8 * wl.part.rcp_length_unit ** 2
+    ramp = _ramp_times(iters, 3.0, 100)
+    for k in range(2):
+        sel = slice(k * n, (k + 1) * n)
+        t = m["diffusion_time"][sel]
+        if k == 0:
+            assert np.all(t == iters)
+        else:
+            assert np.allclose(t, ramp[-1], rtol=0, atol=1e-9) and ramp[-2] < iters
+            assert np.all((m["flags"][sel] & 2) != 0)  # mature
+        expect = unit * t.mean()
+        assert abs(d2[sel].mean() - expect) < 5 * expect / math.sqrt(n), (k, d2[sel].mean(), expect)
     w.close()
 
 
