@@ -238,8 +238,12 @@ int mcg_set_unimol_rxns(mcg_ctx* ctx, const mcg_unimol_rxn* rxns, uint32_t num_r
  * and bit q of surf_products[k] set for a surface species
  * (outcome_products_random, diffuse_react_event.cpp:2446-2933).  A given
  * orientation is flipped when the surface reactant's orientation differs
- * from geom_surf (:2636-2658).  The volume reactant is consumed (classes
- * that keep it are refused).  A surface product of S's species keeps S (its
+ * from geom_surf (:2636-2658).  The first volume product of the volume
+ * reactant's species keeps it: it goes on with its step, through the wall
+ * when that product's orientation differs from geom_vol (RX_FLIP, :940-971)
+ * and on its side otherwise; else the volume reactant is consumed.  Within
+ * its step a kept molecule does not react again with the tile it emptied.
+ * A surface product of S's species keeps S (its
  * orientation becomes the product's); at most one other surface product,
  * which takes S's tile (no neighbour-tile search on this path).  A volume
  * product starts at the hit point moved off the wall to the side its

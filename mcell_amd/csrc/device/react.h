@@ -67,7 +67,8 @@ struct VolSurfRec {
   int32_t product_orient[kMaxPathways][kMaxProducts];
   uint32_t surf_mask[kMaxPathways];  // bit q: product q is a surface species
   uint32_t keep_surf[kMaxPathways];  // 0, or 1 + the product index that is the kept surface reactant
-  uint32_t num_new[kMaxPathways];    // new molecules (the kept reactant excluded)
+  uint32_t num_new[kMaxPathways];    // new molecules (the kept reactants excluded)
+  uint32_t keep_vol[kMaxPathways];   // 0, or 1 + the product index that is the kept volume reactant
 };
 // Proposal::pathway of a volume-surface reaction: pathway | drawn
 // orientation bits << 8 (bit q: product q drawn up)

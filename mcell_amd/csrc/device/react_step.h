@@ -502,14 +502,19 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
       break;
     }
     // collide_and_react_with_surf_mol (diffuse_react_event.cpp:505-527, 845-975):
-    // the surface molecule on the tile the hit point falls in
+    // the surface molecule on the tile the hit point falls in.  A tile this
+    // mover consumed in a reaction it stayed through is met again, up to
+    // that reaction, by the repeat of its step.
+    bool flip_through = false;
     if ((sp.flags & SPF_CAN_VOLSURF) && R.grids && R.grids[hit.wall].num_tiles) {
       const WallGrid& g = R.grids[hit.wall];
       const uint32_t slot = rep * R.total_tiles + g.tile_base + xyz2grid(R.walls[hit.wall], g, hit.pos);
       const uint32_t tw = R.tiles[slot];
-      if (tw != kTileEmpty && !R.tile_consumed[slot] && born_tile == slot + 1) {
+      const uint32_t tc = R.tile_consumed[slot];
+      const bool live = tc == 0 || (tc == idv + 1 && succ < pass_through);
+      if (tw != kTileEmpty && live && born_tile == slot + 1) {
         born_tile = 0;  // created on this tile: no rebinding on the first hit (:878-885)
-      } else if (tw != kTileEmpty && !R.tile_consumed[slot]) {
+      } else if (tw != kTileEmpty && live) {
         const int32_t ri = R.pair_vs[spc * R.num_species + tile_species(tw)];
         if (ri >= 0) {
           const VolSurfRec& vs = R.vs_rxns[ri];
@@ -523,7 +528,24 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
             rr.num_pathways = vs.num_pathways;
             for (int k = 0; k < kMaxPathways; ++k) rr.cum_probs[k] = vs.cum_probs[k];
             const int pw = test_bimolecular(rr, r_rate_factor / g.binding_factor, rng);
-            if (pw >= 0) {
+            if (pw >= 0 && succ < pass_through) {
+              // carried out already and the mover stayed (outcome_products_
+              // random, :2690-2720): the orientation draws again, then the
+              // mover passes through the wall (RX_FLIP, :940-971) when its
+              // product is oriented other than the rule's volume reactant,
+              // else it goes on as after no reaction
+              ++succ;
+              uint32_t drawn = 0;
+              for (uint32_t q = 0; q < vs.num_products[pw]; ++q)
+                if (vs.product_orient[pw][q] == 0 && (rng.u32() & 1u)) drawn |= 1u << q;
+              if (vs.keep_vol[pw]) {  // (always: a reaction that consumed the mover ended its step)
+                const uint32_t kq = vs.keep_vol[pw] - 1;
+                const int flip = (vs.geom_surf != 0 && tile_orient(tw) != vs.geom_surf) ? -1 : 1;
+                const int o = vs.product_orient[pw][kq] != 0 ? vs.product_orient[pw][kq] * flip
+                                                             : (((drawn >> kq) & 1u) ? 1 : -1);
+                flip_through = o != vs.geom_vol;
+              }
+            } else if (pw >= 0) {
               prop.time = elapsed + t_steps * hit.t;
               prop.x = hit.pos.x;
               prop.y = hit.pos.y;
@@ -548,8 +570,9 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
         }
       }
     }
-    // collide_and_react_with_walls (diffuse_react_event.cpp:529-561, 991-1067)
-    const uint32_t act = R.SC.of(hit.wall, spc, hit.type == WALL_FRONT);
+    // collide_and_react_with_walls (diffuse_react_event.cpp:529-561, 991-1067);
+    // a flip crosses the wall as a transparent one would
+    const uint32_t act = flip_through ? SC_TRANSPARENT : R.SC.of(hit.wall, spc, hit.type == WALL_FRONT);
     if (act == SC_ABSORB) {
       (void)rng.dbl();  // test_intersect's two draws (rxn_utils.inl:593-626)
       (void)rng.dbl();
@@ -572,7 +595,7 @@ __device__ __forceinline__ bool react_mover(const ReactParams& R, const Snapshot
         out.cvol = hit.type == WALL_FRONT ? R.cv_in[hit.wall] : R.cv_out[hit.wall];
       if (t_steps < 1e-12) t_steps = 1e-12;
       disp = {disp.x * s, disp.y * s, disp.z * s};
-      ++out.n_cross;
+      if (!flip_through) ++out.n_cross;
       continue;
     }
     // reflect_from_wall (collision_utils.inl:1711-1747)

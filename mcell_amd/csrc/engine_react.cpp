@@ -1112,14 +1112,18 @@ int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
           v.keep_surf[k] = q + 1;
           continue;
         }
-        if (!surf && ps == x.vol_species)
-          return fail(MCG_E_UNSUPPORTED, "volume-surface classes that keep the volume reactant are not on this path");
+        // and the first volume product of the volume reactant's species is
+        // the volume reactant, which goes on with its step
+        if (!surf && ps == x.vol_species && !v.keep_vol[k]) {
+          v.keep_vol[k] = q + 1;
+          continue;
+        }
         if (surf) ++new_surf;
       }
       if (new_surf > (v.keep_surf[k] ? 0u : 1u))
         return fail(MCG_E_UNSUPPORTED, "surface products beyond the reactant's tile need a neighbour-tile search");
       v.num_products[k] = np;
-      v.num_new[k] = np - (v.keep_surf[k] ? 1u : 0u);
+      v.num_new[k] = np - (v.keep_surf[k] ? 1u : 0u) - (v.keep_vol[k] ? 1u : 0u);
     }
     v.num_pathways = x.num_pathways;
     v.max_fixed_p = v.cum_probs[x.num_pathways - 1];
@@ -1133,10 +1137,14 @@ int mcg_set_volsurf_rxns(mcg_ctx* c, const mcg_volsurf_rxn* rxns, uint32_t n) {
   int st = ensure_react_buffers(c);
   if (st) return st;
   if (r.vs_d.alloc(n) || r.pair_vs_d.alloc(pair.size())) return fail(MCG_E_NOMEM, "reaction buffers");
-  bool vol_products = false;
+  bool vol_products = false, keep_vol = false;
   for (const VolSurfRec& v : h)
-    for (uint32_t k = 0; k < v.num_pathways; ++k)
-      for (uint32_t q = 0; q < v.num_products[k]; ++q) vol_products |= !((v.surf_mask[k] >> q) & 1u);
+    for (uint32_t k = 0; k < v.num_pathways; ++k) {
+      keep_vol |= v.keep_vol[k] != 0;
+      for (uint32_t q = 0; q < v.num_products[k]; ++q)
+        vol_products |= !((v.surf_mask[k] >> q) & 1u) && v.keep_vol[k] != q + 1;
+    }
+  if (keep_vol && (st = ensure_keep_buffers(c))) return st;
   if (vol_products && !r.born.p) {
     const size_t cap = c->cfg.molecule_capacity;
     if (r.born.alloc(cap)) return fail(MCG_E_NOMEM, "reaction buffers");

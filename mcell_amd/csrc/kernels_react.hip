@@ -671,6 +671,8 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
       if (!tile) {
         const uint32_t k = rxns[p.replica * rxn_stride + p.rxn].keep[p.pathway];
         keep = (p.pad & kPadMoverIsB) ? ((k >> 1) & 1u) | ((k & 1u) << 1) : k;
+      } else if (vs_rxns[p.rxn & ~kVolSurfBit].keep_vol[p.pathway & kPathwayMask]) {
+        keep = 1u;
       }
       if (keep & 1u) {
         // the mover stays: its step is repeated through this reaction and
@@ -683,8 +685,10 @@ __global__ __launch_bounds__(256) void k_accept(const uint32_t* __restrict__ pli
         out_tag[j] = out_tag[j] | (MF_DEFUNCT << 8);
       }
       if (tile) {
-        // the surface reactant stays when a product of its species keeps it
-        if (!vs_rxns[p.rxn & ~kVolSurfBit].keep_surf[p.pathway & kPathwayMask]) tile_consumed[ps] = 1;
+        // the surface reactant stays when a product of its species keeps it;
+        // a consumed tile names its consumer (a kept mover's repeated step
+        // passes through it)
+        if (!vs_rxns[p.rxn & ~kVolSurfBit].keep_surf[p.pathway & kPathwayMask]) tile_consumed[ps] = p.mover_id + 1;
       } else if (!(keep & 2u)) {
         consumed[ps] = p.mover_id + 1;
         out_tag[ps] = out_tag[ps] | (MF_DEFUNCT << 8);
@@ -899,6 +903,7 @@ __global__ __launch_bounds__(256) void k_products(ReactParams R, const uint32_t*
         tiles[ts] = make_tile(ps, o) | (tiles[ts] & kTileMature);
         continue;
       }
+      if (vs.keep_vol[pw] == q + 1) continue;  // the mover, which goes on with its step
       const uint32_t id = first_id + i;
       const uint32_t slot = hdr[kHdrNumOut] + pre[k] + i;
       ++i;

@@ -313,8 +313,14 @@ void World::diffuse_vol_molecule(uint32_t id, double& max_time, double diffusion
         // collide_and_react_with_surf_mol first (diffuse_react_event.cpp:505-527)
         if ((species[mols[id].species].flags & SPF_CAN_VOLSURF) && c.wall < grids.size() &&
             grids[c.wall].num_tiles) {
-          if (collide_and_react_with_surf_mol(c, t_steps, r_rate_factor, elapsed) == 1) {
+          const int res = collide_and_react_with_surf_mol(c, t_steps, r_rate_factor, elapsed);
+          if (res == 1) {
             was_defunct = true;
+            break;
+          }
+          if (res == 0) {
+            // the mover stayed and passes through the wall (:947-971)
+            cross_transparent_wall(c, mols[id], disp, t_steps, elapsed, last_hit);
             break;
           }
         }

@@ -633,7 +633,9 @@ uint32_t World::xyz2grid(V3 v, uint32_t wi) const {
 
 // collide_and_react_with_surf_mol (diffuse_react_event.cpp:845-975) for the
 // classes of VolSurfRxn; 1 = the volume molecule was destroyed (or, in the
-// snapshot mode, proposed a reaction), -1 = nothing happened.
+// snapshot mode, proposed a reaction), 0 = it reacted, stayed and passes
+// through the wall (RX_FLIP), -1 = nothing happened or it reacted and stayed
+// on its side.
 int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, double r_rate_factor,
                                            double elapsed) {
   const Grid& g = grids[c.wall];
@@ -675,6 +677,7 @@ int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, d
   // surface product takes the reactant's tile; volume products start next
   // to the wall (update_vol_mol_after_rxn_with_surf_mol, :2291-2320).
   const std::vector<VolSurfRxn::Product>& prods = rx.products;
+  const uint32_t vol_species = vm.species;
   std::vector<int> orient(prods.size());
   for (size_t q = 0; q < prods.size(); ++q)
     if (prods[q].orient == 0) orient[q] = (rng().u32() & 1u) ? 1 : -1;
@@ -687,6 +690,14 @@ int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, d
       keep = q;
       break;
     }
+  // the first volume product of the mover's species is the mover itself
+  // (is_simple_cplx_product_on_both_sides_of_rxn_w_identical_compartments)
+  size_t keep_vol = prods.size();
+  for (size_t q = 0; q < prods.size(); ++q)
+    if (!prods[q].surf && prods[q].species == vol_species) {
+      keep_vol = q;
+      break;
+    }
   const uint32_t wall = sm.wall, tile = sm.tile;
   if (keep == prods.size()) {
     mols[occ].flags |= MF_DEFUNCT;
@@ -697,6 +708,7 @@ int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, d
       mols[occ].orient = orient[q];
       continue;
     }
+    if (q == keep_vol) continue;
     if (prods[q].surf) {
       // the product takes the receptor's tile and position
       // (GridPos::make_with_mol_pos, diffuse_react_event.cpp:2040-2050) and,
@@ -720,11 +732,17 @@ int World::collide_and_react_with_surf_mol(const Collision& c, double t_steps, d
     register_reactant(pm, true);
     if (cmp_lt(time, event_time + 1.0, EPS)) new_diffuse_actions.push_back({nid, wall, tile});
   }
+  ++volsurf_reactions;
+  rxn_log.push_back({(uint32_t)iteration, c.mover, occ, (int32_t)code, pw, time, c.pos});
+  if (keep_vol < prods.size()) {
+    // the mover stays (:2690-2720): a product orientation other than the
+    // rule's makes it pass through the wall (RX_FLIP, :940-971), else its
+    // ray trace goes on as after no reaction
+    return orient[keep_vol] != rx.geom_vol ? 0 : -1;
+  }
   Mol& mv = mols[c.mover];
   mv.flags |= MF_DEFUNCT;
   register_reactant(mv, false);
-  ++volsurf_reactions;
-  rxn_log.push_back({(uint32_t)iteration, c.mover, occ, (int32_t)code, pw, time, c.pos});
   return 1;
 }
 

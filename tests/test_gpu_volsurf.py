@@ -249,3 +249,36 @@ def test_counted_volumes(gpu_available, oracle_lib, precision):
         assert n_vs == w.volsurf_reactions()
     w.close()
     eng.close()
+
+
+@pytest.mark.parametrize("side", [1, -1])
+def test_kept_ligand_matches_oracle(gpu_available, oracle_lib, side):
+    """L' + R' -> L? + LR' (the ligand stays: outcome_products_random,
+    diffuse_react_event.cpp:2690-2720): with L' it goes on outside, with L,
+    it passes through the sphere (RX_FLIP, :940-971), its step repeated
+    through the reaction.  Every L and every tile equal the oracle's each
+    iteration, and the L inside the sphere are as many as the LR."""
+    from oracle import pyoracle as po
+    seed = 43
+    wl = W.receptor_sphere(n_l=60000, n_r=30000, iterations=12, rate=4e8, down_fraction=0.0,
+                           products=[(0, side, False), (2, 1, True)])
+    eng = DiffuseReactEngine.from_workload(wl, seed=seed)
+    w = po.world_from_workload(wl, draw_mode=po.DRAW_PHILOX, seed=seed)
+    w.set_barrier_interval(wl.barrier_interval)
+    for it in range(wl.iterations):
+        eng.step(it, 1)
+        w.run(1)
+        g = eng.molecules()
+        o = np.argsort(g["id"], kind="stable")
+        om = w.molecules()
+        live = np.nonzero(((om["flags"] & 1) == 0) & (om["species"] == 0))[0]
+        assert np.array_equal(g["id"][o], live), "iteration %d: the same volume molecules" % it
+        d = np.abs(g["pos"][o] - om["pos"][live]).max(initial=0)
+        assert d <= 1e-12 * 100, "iteration %d: max |dpos| %g" % (it, d)
+        assert np.array_equal(_engine_surface(eng), _oracle_surface(w, wl)), "iteration %d: surface" % it
+    c = eng.count_species(1, 3)[0]
+    inside = int(np.sum(W.inside_convex(g["pos"], wl.vertices, wl.walls[12:])))
+    print("side %d: L %d (inside %d) R %d LR %d" % (side, c[0], inside, c[1], c[2]), flush=True)
+    assert c[0] == 60000 and c[2] > 100 and inside == (0 if side == 1 else c[2])
+    w.close()
+    eng.close()
