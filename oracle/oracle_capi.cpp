@@ -2,6 +2,8 @@
 // A C interface over the oracle so that tests (ctypes) and the CPU-baseline
 // leg of bench.py can drive it.
 #include <cstring>
+#include <iomanip>
+#include <sstream>
 #include <chrono>
 #include <stdexcept>
 #include <string>
@@ -511,6 +513,19 @@ void orc_test_bimolecular_u(double max_fixed_p, const double* cum_probs, uint32_
 
 uint32_t orc_subpart_index(void* wp, const double* pos) {
   return static_cast<World*>(wp)->subpart_index({pos[0], pos[1], pos[2]});
+}
+
+// How C++ iostreams print a count (the checker of the count writer): form 0,
+// a default stream (CountItem::write_as_dat, count_buffer.cpp:30-32); form 1,
+// std::scientific at precision 8 (write_gdat_value's base, :35-61).
+int orc_format_count(double d, int form, char* out, int cap) {
+  std::ostringstream os;
+  if (form == 1) os << std::scientific << std::setprecision(8);
+  os << d;
+  const std::string s = os.str();
+  if ((int)s.size() >= cap) return -1;
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return (int)s.size();
 }
 
 }  // extern "C"

@@ -418,6 +418,17 @@ class OracleWorld:
         return self.lib.orc_subpart_index(self.w, _p(np.asarray(pos, np.float64)))
 
 
+def format_count(d, form):
+    """C++ iostream's text for d: form 0 a default stream, 1 std::scientific
+    at precision 8."""
+    lib = load()
+    buf = ctypes.create_string_buffer(64)
+    lib.orc_format_count.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    n = lib.orc_format_count(d, form, buf, 64)
+    assert n >= 0
+    return buf.value.decode()
+
+
 def test_bimolecular_u(max_fixed_p, cum_probs, scaling, u):
     """The oracle's test_bimolecular for given draws u: pathway or -1."""
     lib = load()
