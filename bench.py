@@ -129,6 +129,16 @@ def build_workload(args):
                            "transparent surface class, 0.1 um subpartitions, dt 1e-6 s",
                "molecules": wl.num_molecules, "surface_molecules": int(len(wl.surface_mols["ids"])),
                "walls": int(len(wl.walls)), "replicas": 1}
+    elif args.workload == "volume_surface_diffusing_timesteps":
+        wl = W.with_time_steps(W.with_diffusing_surface(W.volume_surface(iterations=args.steps), {1: 1e-8, 2: 1e-8}),
+                               [1, 0.5, 0.5, 1])
+        cfg = {"workload": "the volume-surface case with diffusing receptors taking time steps of 1/2 iteration: "
+                           "L' + R' -> LR' (1e8 /M/s), R' and LR' diffusing on the icosphere r 0.5 um (5,120 "
+                           "triangles, D 1e-8 cm^2/s, two steps per iteration), 20,000 receptors, 100,000 L "
+                           "(D 1e-6 cm^2/s) outside the sphere, 10,000 T crossing it through a transparent surface "
+                           "class, 0.1 um subpartitions, dt 1e-6 s",
+               "molecules": wl.num_molecules, "surface_molecules": int(len(wl.surface_mols["ids"])),
+               "walls": int(len(wl.walls)), "replicas": 1}
     elif args.workload == "crowded_spheres":
         wl = W.crowded_spheres(iterations=args.steps)
         cfg = {"workload": "64 perturbed icospheres (20,480 triangles each, radii U[0.1, 0.3] um, +-2% radial "
@@ -238,6 +248,10 @@ def cpu_baseline(seconds, workload):
         wl = W.with_diffusing_surface(W.volume_surface(), {1: 1e-8, 2: 1e-8})
         what = ("the whole volume-surface case with diffusing receptors (110,000 volume molecules, 20,000 "
                 "receptors, 5,132 walls)")
+    elif workload == "volume_surface_diffusing_timesteps":
+        wl = W.with_time_steps(W.with_diffusing_surface(W.volume_surface(), {1: 1e-8, 2: 1e-8}), [1, 0.5, 0.5, 1])
+        what = ("the whole volume-surface case with diffusing receptors at time steps of 1/2 iteration (110,000 "
+                "volume molecules, 20,000 receptors, 5,132 walls)")
     elif workload == "crowded_spheres":
         wl = W.crowded_spheres(n_molecules=200_000)
         what = ("the crowded geometry (1,310,732 walls, 200,000 S') with 200,000 volume molecules A:B:E = 5:4:1 "
@@ -394,7 +408,7 @@ def traffic_passes(args, wl_subpart, main_prec, iterations):
         ("k_move_light<double>", "k_move_drain<double>")
     wl_name = args.workload
     if wl_name not in ("replicas_sphere", "bimol_box", "crowded_spheres", "volume_surface",
-                       "volume_surface_diffusing", "bimol_box_timesteps"):
+                       "volume_surface_diffusing", "volume_surface_diffusing_timesteps", "bimol_box_timesteps"):
         names = ("k_light32", "k_mid32", "k_hard32") if main_prec == "fp32" else ("k_light64", "k_mid64", "k_hard64")
     out = {}
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
